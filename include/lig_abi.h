@@ -1,0 +1,192 @@
+/*
+ * lig_abi.h -- C-ABI of the MI355X (gfx950) gsplat2d hot path.
+ *
+ * Every entry point takes raw device pointers (row-major, contiguous, FP32
+ * unless stated), integer sizes and the caller's HIP stream (hipStream_t passed
+ * as void*; NULL = the default stream).  The library never allocates device
+ * memory: outputs and workspaces are the caller's.  Outputs need no
+ * initialisation -- each call overwrites its outputs completely (the reference
+ * zero-fills them with torch::zeros and then writes; the result is the same).
+ * Calls are asynchronous on `stream` and reentrant; calls on one stream are
+ * ordered, calls on different streams that share buffers are the caller's
+ * business.  Return value: LIG_OK or a negative LIG_ERR_* code; lig_strerror()
+ * names it.  Layouts: float2/float3 arrays are [N,2] / [N,3] row-major, images
+ * are HWC [H,W,3], tile bins are [T,2] int32.
+ *
+ * Each function names the reference interface it replaces (paths relative to
+ * the reference repository, gsplat2d/gsplat2d/cuda/csrc/ abbreviated csrc/).
+ */
+#ifndef LIG_ABI_H
+#define LIG_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LIG_OK 0
+#define LIG_ERR_INVALID_ARGUMENT (-1)
+#define LIG_ERR_WORKSPACE (-2)
+#define LIG_ERR_LAUNCH (-3)
+#define LIG_ERR_UNSUPPORTED (-4)
+
+/* Human-readable name of a status code. */
+const char *lig_strerror(int status);
+
+/* ABI version (major*10000 + minor*100 + patch). */
+int lig_abi_version(void);
+
+/* ---- projection ---------------------------------------------------------- */
+
+/* Replaces compute_cov2d_bounds_tensor (csrc/bindings.cu:41-66, kernel
+ * csrc/bindings.cu:19-39).  cov2d [n,3] (a,b,c upper triangle), opacities [n]
+ * or NULL (= 1) -> conics [n,3], extents [n,2].  det == 0 gives zeros. */
+int lig_compute_cov2d_bounds(int n, const float *cov2d, const float *opacities,
+                             float *conics, float *extents, void *stream);
+
+/* Replaces project_gaussians_forward_tensor (csrc/bindings.cu:68-117, kernel
+ * csrc/forward.cu:13-53).  -> xys [n,2], extents [n,2], conics [n,3],
+ * num_tiles_hit [n] int32.  block_width in [1, 16]. */
+int lig_project_gaussians_forward(int n, const float *cov2d, const float *means2d,
+                                  const float *opacities, int img_height, int img_width,
+                                  int block_width, float *xys, float *extents, float *conics,
+                                  int32_t *num_tiles_hit, void *stream);
+
+/* Replaces project_gaussians_forward_cholesky_tensor (csrc/bindings.cu:153-202,
+ * kernel csrc/forward.cu:55-105).  cholesky [n,3] = (l11, l21, l22). */
+int lig_project_gaussians_forward_cholesky(int n, const float *cholesky, const float *means2d,
+                                           const float *opacities, int img_height,
+                                           int img_width, int block_width, float *xys,
+                                           float *extents, float *conics,
+                                           int32_t *num_tiles_hit, void *stream);
+
+/* Replaces project_gaussians_backward_tensor (csrc/bindings.cu:119-151, kernel
+ * csrc/backward.cu:185-204).  -> v_cov2d [n,3], v_mean2d [n,2]. */
+int lig_project_gaussians_backward(int n, const float *extents, const float *conics,
+                                   const float *v_xy, const float *v_conic, float *v_cov2d,
+                                   float *v_mean2d, void *stream);
+
+/* Replaces project_gaussians_backward_cholesky_tensor (csrc/bindings.cu:204-237,
+ * kernel csrc/backward.cu:206-247).  -> v_cholesky [n,3], v_mean2d [n,2]. */
+int lig_project_gaussians_backward_cholesky(int n, const float *extents, const float *cholesky,
+                                            const float *conics, const float *v_xy,
+                                            const float *v_conic, float *v_cholesky,
+                                            float *v_mean2d, void *stream);
+
+/* ---- tile intersection ----------------------------------------------------- */
+
+/* Replaces torch.cumsum(num_tiles_hit, dtype=int32) of compute_cumulative_intersects
+ * (gsplat2d/gsplat2d/utils.py:61-67): inclusive int32 scan.  The workspace is
+ * lig_scan_workspace_size(n) bytes. */
+size_t lig_scan_workspace_size(int n);
+int lig_inclusive_scan_i32(int n, const int32_t *in, int32_t *out, void *workspace,
+                           size_t workspace_bytes, void *stream);
+
+/* Replaces map_gaussian_to_intersects_tensor (csrc/bindings.cu:239-281, kernel
+ * csrc/forward.cu:109-143).  -> isect_ids [num_intersects] int64
+ * ((tile << 32) | sign-extended depth bits), gaussian_ids [num_intersects] int32.
+ * Slots of Gaussians whose extent is 0 are written as (0, 0), as the
+ * reference's zero fill leaves them. */
+int lig_map_gaussian_to_intersects(int n, int num_intersects, const float *xys,
+                                   const float *depths, const float *extents,
+                                   const int32_t *cum_tiles_hit, int tiles_x, int tiles_y,
+                                   int block_width, int64_t *isect_ids, int32_t *gaussian_ids,
+                                   void *stream);
+
+/* Replaces torch.sort(isect_ids) + torch.gather(gaussian_ids, sorted_indices)
+ * (gsplat2d/gsplat2d/utils.py:97-98): stable LSD radix sort by signed 64-bit
+ * key carrying int32 values.  Workspace: lig_sort_pairs_workspace_size(m, 64). */
+size_t lig_sort_pairs_workspace_size(int m, int key_bits);
+int lig_sort_isect_pairs(int m, const int64_t *keys_in, const int32_t *vals_in,
+                         int64_t *keys_out, int32_t *vals_out, void *workspace,
+                         size_t workspace_bytes, void *stream);
+
+/* Replaces get_tile_bin_edges_tensor (csrc/bindings.cu:283-301, kernel
+ * csrc/forward.cu:148-170).  -> tile_bins [tiles_x*tiles_y, 2], empty tiles (0,0). */
+int lig_get_tile_bin_edges(int num_intersects, const int64_t *isect_ids_sorted, int tiles_x,
+                           int tiles_y, int32_t *tile_bins, void *stream);
+
+/* Fused replacement of bin_and_sort_gaussians (gsplat2d/gsplat2d/utils.py:70-100)
+ * for the rasterisation path, where every depth is 0 (gsplat2d/gsplat2d/
+ * rasterize.py:158): emits 32-bit tile keys, radix-sorts only the
+ * ceil(log2(tiles)) key bits that vary, and builds the bins.  cum_tiles_hit is
+ * the inclusive scan of num_tiles_hit and num_intersects its last element.
+ * -> gaussian_ids_sorted [num_intersects] int32 in (tile, emission) order,
+ * tile_bins [tiles,2].  Workspace: lig_bin_workspace_size(num_intersects, tiles). */
+size_t lig_bin_workspace_size(int num_intersects, int num_tiles);
+int lig_bin_and_sort_gaussians(int n, int num_intersects, const float *xys,
+                               const float *extents, const int32_t *cum_tiles_hit,
+                               int img_height, int img_width, int block_width,
+                               int32_t *gaussian_ids_sorted, int32_t *tile_bins,
+                               void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---- rasterisation ------------------------------------------------------- */
+
+/* No reference counterpart: builds the per-Gaussian records (48 bytes each,
+ * lig_packed_gaussians_bytes(n)) that both rasteriser kernels read, from
+ * xys [n,2], conics [n,3], colors [n,3] and opacities [n] or NULL.
+ * lig_rasterize_forward calls it itself; a backward that is not handed the
+ * forward's buffer calls it first. */
+size_t lig_packed_gaussians_bytes(int n);
+int lig_pack_gaussians(int n, const float *xys, const float *conics, const float *colors,
+                       const float *opacities, void *packed, void *stream);
+
+/* Replaces rasterize_forward_tensor (csrc/bindings.cu:303-401, kernels
+ * csrc/forward.cu:173-279 and csrc/gradient_aware_rasterizer.cu:351-495).
+ * colors [n,3]; opacities [n] or NULL.  -> out_img [H,W,3], out_wsum [H,W],
+ * final_idx [H,W] int32; when out_dx != NULL (compute_upscale_gradients) also
+ * out_dx/out_dy/out_dxy [H,W,3].  `packed` is a caller buffer of
+ * lig_packed_gaussians_bytes(n) bytes that the call fills with the per-Gaussian
+ * records the kernels read; pass the same buffer, unchanged, to the backward. */
+int lig_rasterize_forward(int n, int img_height, int img_width, int block_width,
+                          const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
+                          const float *xys, const float *conics, const float *colors,
+                          const float *opacities, void *packed, float *out_img,
+                          float *out_wsum, int32_t *final_idx, float *out_dx, float *out_dy,
+                          float *out_dxy, void *stream);
+
+/* Replaces rasterize_backward_tensor (csrc/bindings.cu:403-506, kernels
+ * csrc/backward.cu:8-183 and csrc/gradient_aware_rasterizer.cu:118-349).
+ * v_output [H,W,3], v_wsum [H,W]; v_output_dx/dy/dxy [H,W,3] or all NULL (plain).
+ * -> v_xy [n,2], v_xy_abs [n,2], v_conic [n,3], v_colors [n,3], v_opacity [n]
+ * (written only when has_opacities).  `packed` holds the records of
+ * lig_pack_gaussians for the same Gaussians. */
+int lig_rasterize_backward(int n, int img_height, int img_width, int block_width,
+                           const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
+                           const void *packed, const int32_t *final_idx,
+                           const float *v_output, const float *v_wsum,
+                           const float *v_output_dx, const float *v_output_dy,
+                           const float *v_output_dxy, int has_opacities, float *v_xy,
+                           float *v_xy_abs, float *v_conic, float *v_colors,
+                           float *v_opacity, void *stream);
+
+/* ---- gradient-aware upscaler ---------------------------------------------- */
+
+/* Replaces gradient_aware_upscale_forward_tensor (csrc/bindings.cu:508-559,
+ * kernel csrc/gradient_aware_upscaler.cu:181-246).  render/dx/dy/dxy
+ * [src_h,src_w,3] -> output [dst_h,dst_w,3]; roi = (x1,y1,x2,y2), x2>=x1, y2>=y1. */
+int lig_gradient_aware_upscale_forward(int src_h, int src_w, int dst_h, int dst_w,
+                                       float roi_x1, float roi_y1, float roi_x2,
+                                       float roi_y2, const float *render, const float *dx,
+                                       const float *dy, const float *dxy, float *output,
+                                       void *stream);
+
+/* Replaces gradient_aware_upscale_backward_tensor (csrc/bindings.cu:561-605,
+ * kernel csrc/gradient_aware_upscaler.cu:248-371) with a deterministic
+ * two-pass separable gather (no atomics).  -> grad_render/dx/dy/dxy
+ * [src_h,src_w,3].  Workspace: lig_upscale_backward_workspace_size(...). */
+size_t lig_upscale_backward_workspace_size(int src_h, int src_w, int dst_h, int dst_w);
+int lig_gradient_aware_upscale_backward(int src_h, int src_w, int dst_h, int dst_w,
+                                        float roi_x1, float roi_y1, float roi_x2,
+                                        float roi_y2, const float *grad_output,
+                                        float *grad_render, float *grad_dx, float *grad_dy,
+                                        float *grad_dxy, void *workspace,
+                                        size_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIG_ABI_H */

@@ -1,0 +1,120 @@
+"""ctypes binding of ``liblig_hip.so`` (the C-ABI declared in include/lig_abi.h).
+
+The library is loaded after ``torch`` so that it shares torch's HIP runtime
+(both name ``libamdhip64.so.7``).  There is no fallback: a missing library or a
+tensor that is not on a HIP device raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG_DIR, "liblig_hip.so")
+
+_c_int = ctypes.c_int
+_c_float = ctypes.c_float
+_c_size = ctypes.c_size_t
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "lig_strerror": (ctypes.c_char_p, [_c_int]),
+    "lig_abi_version": (_c_int, []),
+    "lig_compute_cov2d_bounds": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    "lig_project_gaussians_forward": (
+        _c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "lig_project_gaussians_forward_cholesky": (
+        _c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "lig_project_gaussians_backward": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lig_project_gaussians_backward_cholesky": (
+        _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lig_scan_workspace_size": (_c_size, [_c_int]),
+    "lig_inclusive_scan_i32": (_c_int, [_c_int, _vp, _vp, _vp, _c_size, _vp]),
+    "lig_map_gaussian_to_intersects": (
+        _c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
+    "lig_sort_pairs_workspace_size": (_c_size, [_c_int, _c_int]),
+    "lig_sort_isect_pairs": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "lig_get_tile_bin_edges": (_c_int, [_c_int, _vp, _c_int, _c_int, _vp, _vp]),
+    "lig_bin_workspace_size": (_c_size, [_c_int, _c_int]),
+    "lig_bin_and_sort_gaussians": (
+        _c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _c_size,
+                 _vp]),
+    "lig_packed_gaussians_bytes": (_c_size, [_c_int]),
+    "lig_pack_gaussians": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lig_rasterize_forward": (
+        _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                 _vp, _vp, _vp, _vp, _vp]),
+    "lig_rasterize_backward": (
+        _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                 _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lig_gradient_aware_upscale_forward": (
+        _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,
+                 _vp, _vp, _vp, _vp, _vp]),
+    "lig_upscale_backward_workspace_size": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
+    "lig_gradient_aware_upscale_backward": (
+        _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,
+                 _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+_lib = None
+
+
+class LigError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m lig_amd.build` "
+                "(there is no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def call(name: str, *args) -> None:
+    """Call an int-returning entry point and raise LigError on failure."""
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        msg = load().lig_strerror(rc).decode()
+        raise LigError(f"{name} failed: {msg} (status {rc})")
+
+
+def size_query(name: str, *args) -> int:
+    return int(getattr(load(), name)(*args))
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_of(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def check_device(name: str, t, dtype=None):
+    """Reference's CHECK_INPUT (bindings.h:10-15): a HIP tensor, contiguous."""
+    if t is None:
+        return
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+    if dtype is not None and t.dtype != dtype:
+        raise RuntimeError(f"{name} must have dtype {dtype}, got {t.dtype}")

@@ -1,0 +1,498 @@
+// Tile intersection: int32 device scan, intersection emission, stable LSD radix
+// sort, and tile bin edges.
+//
+// Ordering contract: the sorted order is a STABLE sort of the emission order,
+// which is gid-major (Gaussian g's records are written at [cum[g-1], cum[g])
+// in row-major tile order).  With the all-zero depths of the rasterisation
+// path this is the (tile id, Gaussian id) ascending order.
+//
+// Reference behaviour followed (paths relative to the reference repository):
+//   gsplat2d/gsplat2d/utils.py:61-100 (cumsum, .item, torch.sort, gather),
+//   gsplat2d/gsplat2d/cuda/csrc/forward.cu:109-170, bindings.cu:239-301.
+#include "lig_device.h"
+
+namespace lig {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Device-wide int32 scan: reduce per block, scan the block sums (recursively),
+// then scan each block with its offset.  2048 items per 256-thread block.
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Exclusive scan of one int per thread across a 256-thread block.  `wsum` is
+// 4 ints of LDS.  Returns the exclusive prefix; *total gets the block sum.
+__device__ __forceinline__ int block_exclusive_scan(int v, int *wsum, int *total) {
+    const int lane = threadIdx.x & (LIG_WAVE - 1);
+    const int wave = threadIdx.x / LIG_WAVE;
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < LIG_WAVE; off <<= 1) {
+        const int y = __shfl_up(x, off, LIG_WAVE);
+        if (lane >= off) x += y;
+    }
+    if (lane == LIG_WAVE - 1) wsum[wave] = x;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / LIG_WAVE; ++w) {
+        const int s = wsum[w];
+        before += w < wave ? s : 0;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const int32_t *__restrict__ in,
+                                                                   int n,
+                                                                   int32_t *__restrict__ sums) {
+    __shared__ int wsum[kScanThreads / LIG_WAVE];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const size_t i = base + (size_t)k * kScanThreads + threadIdx.x;
+        s += i < (size_t)n ? in[i] : 0;
+    }
+    int total;
+    block_exclusive_scan(s, wsum, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Scans one tile of 2048 items (in place allowed) and adds offsets[block].
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(
+    const int32_t *in, int n, int inclusive, const int32_t *__restrict__ offsets,
+    int32_t *out) {
+    __shared__ int tile[kScanTile];
+    __shared__ int wsum[kScanThreads / LIG_WAVE];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int j = k * kScanThreads + threadIdx.x;
+        const size_t i = base + j;
+        tile[j] = i < (size_t)n ? in[i] : 0;
+    }
+    __syncthreads();
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = tile[threadIdx.x * kScanItems + k];
+        s += v[k];
+    }
+    int total;
+    int run = block_exclusive_scan(s, wsum, &total);
+    run += offsets ? offsets[blockIdx.x] : 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int incl = run + v[k];
+        tile[threadIdx.x * kScanItems + k] = inclusive ? incl : run;
+        run = incl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int j = k * kScanThreads + threadIdx.x;
+        const size_t i = base + j;
+        if (i < (size_t)n) out[i] = tile[j];
+    }
+}
+
+size_t scan_ws_elems(size_t n) {
+    if (n <= (size_t)kScanTile) return 0;
+    const size_t nb = (n + kScanTile - 1) / kScanTile;
+    return nb + scan_ws_elems(nb);
+}
+
+// Scans n int32 (exclusive unless `inclusive`); `ws` holds scan_ws_elems(n).
+void scan_i32(const int32_t *in, int32_t *out, size_t n, bool inclusive, int32_t *ws,
+              hipStream_t st) {
+    if (n == 0) return;
+    if (n <= (size_t)kScanTile) {
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(1), dim3(kScanThreads), 0, st, in, (int)n,
+                           (int)inclusive, (const int32_t *)nullptr, out);
+        return;
+    }
+    const size_t nb = (n + kScanTile - 1) / kScanTile;
+    int32_t *sums = ws;
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in,
+                       (int)n, sums);
+    scan_i32(sums, sums, nb, false, ws + nb, st);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in,
+                       (int)n, (int)inclusive, (const int32_t *)sums, out);
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (key, uint32 value) pairs, 8-bit digits.
+// Each 256-thread block owns 4096 consecutive pairs: a histogram kernel
+// counts digits per block (digit-major table), an exclusive scan turns the
+// table into output offsets, and a scatter kernel ranks each key stably
+// (wave ballots for lane order, LDS for wave and round order).
+// ---------------------------------------------------------------------------
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 16;
+constexpr int kSortTile = kSortThreads * kSortRounds;
+constexpr int kSortWaves = kSortThreads / LIG_WAVE;
+
+template <typename K>
+__device__ __forceinline__ unsigned digit_of(K key, int shift, K flip) {
+    return (unsigned)(((key ^ flip) >> shift) & (K)0xFF);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const K *__restrict__ keys,
+                                                                  int n, int shift, K flip,
+                                                                  int nblocks,
+                                                                  int32_t *__restrict__ hist) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kSortRounds; ++r) {
+        const size_t i = base + (size_t)r * kSortThreads + threadIdx.x;
+        if (i < (size_t)n) atomicAdd(&h[digit_of(keys[i], shift, flip)], 1);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
+    const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+    K *__restrict__ keys_out, uint32_t *__restrict__ vals_out,
+    const int32_t *__restrict__ offsets, int n, int shift, K flip, int nblocks) {
+    __shared__ int run[256];
+    __shared__ int wcnt[kSortWaves][256];
+    const int tid = threadIdx.x;
+    const int lane = tid & (LIG_WAVE - 1);
+    const int wave = tid / LIG_WAVE;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (LIG_WAVE - lane));
+    run[tid] = offsets[(size_t)tid * nblocks + blockIdx.x];
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRounds; ++r) {
+        const size_t i = base + (size_t)r * kSortThreads + tid;
+        if (base + (size_t)r * kSortThreads >= (size_t)n) break;  // uniform across the block
+        const bool valid = i < (size_t)n;
+        K key = 0;
+        uint32_t val = 0;
+        if (valid) {
+            key = keys_in[i];
+            val = vals_in[i];
+        }
+        const unsigned d = valid ? digit_of(key, shift, flip) : 0u;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) wcnt[w][tid] = 0;
+        __syncthreads();
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int bit = (d >> b) & 1;
+            const uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const int rank = __popcll(m & lt_mask);
+        if (valid && rank == 0) wcnt[wave][d] = __popcll(m);
+        __syncthreads();
+        if (valid) {
+            int pos = run[d] + rank;
+#pragma unroll
+            for (int w = 0; w < kSortWaves; ++w) pos += w < wave ? wcnt[w][d] : 0;
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) add += wcnt[w][tid];
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+template <typename K>
+__global__ void copy_pairs_kernel(const K *__restrict__ ki, const uint32_t *__restrict__ vi,
+                                  K *__restrict__ ko, uint32_t *__restrict__ vo, int n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (size_t)n) {
+        ko[i] = ki[i];
+        vo[i] = vi[i];
+    }
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int sort_passes(int key_bits) { return (key_bits + 7) / 8; }
+
+template <typename K>
+size_t sort_ws_bytes(size_t m, int key_bits) {
+    const size_t nblocks = (m + kSortTile - 1) / kSortTile;
+    const size_t nh = 256 * (nblocks ? nblocks : 1);
+    return align_up(m * sizeof(K)) + align_up(m * 4) + align_up(nh * 4) +
+           align_up(scan_ws_elems(nh) * 4) + 256;
+}
+
+// Sorts m pairs by the low `key_bits` bits of the key (all 64 with sign
+// flip when `signed64`).  Input is read only; the result lands in out.
+template <typename K>
+int radix_sort(const K *keys_in, const uint32_t *vals_in, K *keys_out, uint32_t *vals_out,
+               size_t m, int key_bits, K flip, char *ws, hipStream_t st) {
+    if (m == 0) return LIG_OK;
+    const int passes = sort_passes(key_bits);
+    if (passes == 0) {
+        hipLaunchKernelGGL(copy_pairs_kernel<K>, dim3(ceil_div(m, 256)), dim3(256), 0, st,
+                           keys_in, vals_in, keys_out, vals_out, (int)m);
+        return launch_status();
+    }
+    const int nblocks = (int)((m + kSortTile - 1) / kSortTile);
+    char *p = ws;
+    K *ktmp = reinterpret_cast<K *>(p);
+    p += align_up(m * sizeof(K));
+    uint32_t *vtmp = reinterpret_cast<uint32_t *>(p);
+    p += align_up(m * 4);
+    int32_t *hist = reinterpret_cast<int32_t *>(p);
+    p += align_up((size_t)256 * nblocks * 4);
+    int32_t *scan_ws = reinterpret_cast<int32_t *>(p);
+    const K *ksrc = keys_in;
+    const uint32_t *vsrc = vals_in;
+    for (int pass = 0; pass < passes; ++pass) {
+        const bool to_out = ((passes - 1 - pass) % 2) == 0;
+        K *kdst = to_out ? keys_out : ktmp;
+        uint32_t *vdst = to_out ? vals_out : vtmp;
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(radix_hist_kernel<K>, dim3(nblocks), dim3(kSortThreads), 0, st, ksrc,
+                           (int)m, shift, flip, nblocks, hist);
+        scan_i32(hist, hist, (size_t)256 * nblocks, false, scan_ws, st);
+        hipLaunchKernelGGL(radix_scatter_kernel<K>, dim3(nblocks), dim3(kSortThreads), 0, st,
+                           ksrc, vsrc, kdst, vdst, (const int32_t *)hist, (int)m, shift, flip,
+                           nblocks);
+        ksrc = kdst;
+        vsrc = vdst;
+    }
+    return launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// Emission and bin edges.
+// ---------------------------------------------------------------------------
+constexpr int kEmitThreads = 256;
+
+// Reference layout: int64 key (tile << 32) | sign-extended depth bits.
+__global__ __launch_bounds__(kEmitThreads) void emit_isect_i64_kernel(
+    int n, int m, const float *__restrict__ xys, const float *__restrict__ depths,
+    const float *__restrict__ extents, const int32_t *__restrict__ cum, int tiles_x,
+    int tiles_y, int block_width, int64_t *__restrict__ isect_ids,
+    int32_t *__restrict__ gids) {
+    const int g = blockIdx.x * kEmitThreads + threadIdx.x;
+    if (g >= n) return;
+    int cur = g == 0 ? 0 : cum[g - 1];
+    const int end = min(cum[g], m);
+    const float ex = extents[2 * g], ey = extents[2 * g + 1];
+    if (ex <= 0.0f || ey <= 0.0f) {
+        for (; cur < end; ++cur) {
+            isect_ids[cur] = 0;
+            gids[cur] = 0;
+        }
+        return;
+    }
+    int x0, y0, x1, y1;
+    tile_bbox(xys[2 * g], xys[2 * g + 1], ex, ey, tiles_x, tiles_y, block_width, x0, y0, x1, y1);
+    const int64_t depth_id = (int64_t)__float_as_int(depths[g]);
+    for (int y = y0; y < y1; ++y)
+        for (int x = x0; x < x1; ++x) {
+            if (cur >= end) return;
+            const int64_t tile = (int64_t)y * tiles_x + x;
+            isect_ids[cur] = (int64_t)((uint64_t)tile << 32) | depth_id;
+            gids[cur] = g;
+            ++cur;
+        }
+}
+
+// Rasterisation layout: 32-bit tile key, gid value.
+__global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
+    int n, int m, const float *__restrict__ xys, const float *__restrict__ extents,
+    const int32_t *__restrict__ cum, int tiles_x, int tiles_y, int block_width,
+    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    const int g = blockIdx.x * kEmitThreads + threadIdx.x;
+    if (g >= n) return;
+    int cur = g == 0 ? 0 : cum[g - 1];
+    const int end = min(cum[g], m);
+    if (cur >= end) return;
+    const float ex = extents[2 * g], ey = extents[2 * g + 1];
+    if (ex <= 0.0f || ey <= 0.0f) {
+        for (; cur < end; ++cur) {
+            keys[cur] = 0;
+            vals[cur] = 0;
+        }
+        return;
+    }
+    int x0, y0, x1, y1;
+    tile_bbox(xys[2 * g], xys[2 * g + 1], ex, ey, tiles_x, tiles_y, block_width, x0, y0, x1, y1);
+    for (int y = y0; y < y1; ++y) {
+        const uint32_t row = (uint32_t)y * (uint32_t)tiles_x;
+        for (int x = x0; x < x1; ++x) {
+            if (cur >= end) return;
+            keys[cur] = row + (uint32_t)x;
+            vals[cur] = (uint32_t)g;
+            ++cur;
+        }
+    }
+}
+
+__global__ void fill_i32_kernel(int32_t *__restrict__ p, size_t count, int32_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
+// Bin edges from tile ids sorted ascending (forward.cu:148-170 semantics on
+// a zero-filled bins array).
+template <typename K, int SHIFT>
+__global__ void bin_edges_kernel(int m, const K *__restrict__ sorted, int num_tiles,
+                                 int32_t *__restrict__ bins) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int cur = (int)(sorted[i] >> SHIFT);
+    if (cur < 0 || cur >= num_tiles) return;
+    if (i == 0) bins[2 * cur] = 0;
+    if (i == m - 1) bins[2 * cur + 1] = m;
+    if (i == 0) return;
+    const int prev = (int)(sorted[i - 1] >> SHIFT);
+    if (prev != cur) {
+        if (prev >= 0 && prev < num_tiles) bins[2 * prev + 1] = i;
+        bins[2 * cur] = i;
+    }
+}
+
+int bits_for(int num_tiles) {
+    int b = 0;
+    while (b < 31 && (1 << b) < num_tiles) ++b;
+    return b;
+}
+
+}  // namespace
+}  // namespace lig
+
+using namespace lig;
+
+extern "C" size_t lig_scan_workspace_size(int n) {
+    return n > 0 ? scan_ws_elems((size_t)n) * 4 + 256 : 0;
+}
+
+extern "C" int lig_inclusive_scan_i32(int n, const int32_t *in, int32_t *out, void *workspace,
+                                      size_t workspace_bytes, void *stream) {
+    if (n < 0 || (n > 0 && (!in || !out))) return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    if (workspace_bytes < scan_ws_elems((size_t)n) * 4 ||
+        (scan_ws_elems((size_t)n) > 0 && !workspace))
+        return LIG_ERR_WORKSPACE;
+    scan_i32(in, out, (size_t)n, true, reinterpret_cast<int32_t *>(workspace),
+             as_stream(stream));
+    return launch_status();
+}
+
+extern "C" int lig_map_gaussian_to_intersects(int n, int num_intersects, const float *xys,
+                                              const float *depths, const float *extents,
+                                              const int32_t *cum_tiles_hit, int tiles_x,
+                                              int tiles_y, int block_width, int64_t *isect_ids,
+                                              int32_t *gaussian_ids, void *stream) {
+    if (n < 0 || num_intersects < 0 || tiles_x <= 0 || tiles_y <= 0 || block_width < 1)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0 || num_intersects == 0) return LIG_OK;
+    if (!xys || !depths || !extents || !cum_tiles_hit || !isect_ids || !gaussian_ids)
+        return LIG_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(emit_isect_i64_kernel, dim3(ceil_div(n, kEmitThreads)),
+                       dim3(kEmitThreads), 0, as_stream(stream), n, num_intersects, xys, depths,
+                       extents, cum_tiles_hit, tiles_x, tiles_y, block_width, isect_ids,
+                       gaussian_ids);
+    return launch_status();
+}
+
+extern "C" size_t lig_sort_pairs_workspace_size(int m, int key_bits) {
+    if (m <= 0) return 0;
+    return key_bits > 32 ? sort_ws_bytes<uint64_t>((size_t)m, key_bits)
+                         : sort_ws_bytes<uint32_t>((size_t)m, key_bits);
+}
+
+extern "C" int lig_sort_isect_pairs(int m, const int64_t *keys_in, const int32_t *vals_in,
+                                    int64_t *keys_out, int32_t *vals_out, void *workspace,
+                                    size_t workspace_bytes, void *stream) {
+    if (m < 0 || (m > 0 && (!keys_in || !vals_in || !keys_out || !vals_out)))
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (m == 0) return LIG_OK;
+    if (!workspace || workspace_bytes < sort_ws_bytes<uint64_t>((size_t)m, 64))
+        return LIG_ERR_WORKSPACE;
+    return radix_sort<uint64_t>(reinterpret_cast<const uint64_t *>(keys_in),
+                                reinterpret_cast<const uint32_t *>(vals_in),
+                                reinterpret_cast<uint64_t *>(keys_out),
+                                reinterpret_cast<uint32_t *>(vals_out), (size_t)m, 64,
+                                (uint64_t)1 << 63, reinterpret_cast<char *>(workspace),
+                                as_stream(stream));
+}
+
+extern "C" int lig_get_tile_bin_edges(int num_intersects, const int64_t *isect_ids_sorted,
+                                      int tiles_x, int tiles_y, int32_t *tile_bins,
+                                      void *stream) {
+    if (num_intersects < 0 || tiles_x <= 0 || tiles_y <= 0 || !tile_bins)
+        return LIG_ERR_INVALID_ARGUMENT;
+    const size_t nt = (size_t)tiles_x * tiles_y;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * nt, 256)), dim3(256), 0, st, tile_bins,
+                       2 * nt, 0);
+    if (num_intersects > 0) {
+        if (!isect_ids_sorted) return LIG_ERR_INVALID_ARGUMENT;
+        hipLaunchKernelGGL((bin_edges_kernel<int64_t, 32>), dim3(ceil_div(num_intersects, 256)),
+                           dim3(256), 0, st, num_intersects, isect_ids_sorted, (int)nt,
+                           tile_bins);
+    }
+    return launch_status();
+}
+
+// Workspace layout of the fused path: keys, vals (emission), sorted keys,
+// then the sort's own workspace.
+extern "C" size_t lig_bin_workspace_size(int num_intersects, int num_tiles) {
+    const size_t m = num_intersects > 0 ? (size_t)num_intersects : 0;
+    return 3 * align_up(m * 4) + sort_ws_bytes<uint32_t>(m, bits_for(num_tiles)) + 256;
+}
+
+extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float *xys,
+                                          const float *extents, const int32_t *cum_tiles_hit,
+                                          int img_height, int img_width, int block_width,
+                                          int32_t *gaussian_ids_sorted, int32_t *tile_bins,
+                                          void *workspace, size_t workspace_bytes,
+                                          void *stream) {
+    if (n < 0 || num_intersects < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 ||
+        block_width > 16 || !tile_bins)
+        return LIG_ERR_INVALID_ARGUMENT;
+    const int tx = (img_width + block_width - 1) / block_width;
+    const int ty = (img_height + block_width - 1) / block_width;
+    const int nt = tx * ty;
+    if (workspace_bytes < lig_bin_workspace_size(num_intersects, nt)) return LIG_ERR_WORKSPACE;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * (size_t)nt, 256)), dim3(256), 0, st,
+                       tile_bins, 2 * (size_t)nt, 0);
+    if (num_intersects == 0 || n == 0) return launch_status();
+    if (!xys || !extents || !cum_tiles_hit || !gaussian_ids_sorted || !workspace)
+        return LIG_ERR_INVALID_ARGUMENT;
+    const size_t m = (size_t)num_intersects;
+    char *p = reinterpret_cast<char *>(workspace);
+    uint32_t *keys = reinterpret_cast<uint32_t *>(p);
+    p += align_up(m * 4);
+    uint32_t *vals = reinterpret_cast<uint32_t *>(p);
+    p += align_up(m * 4);
+    uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(p);
+    p += align_up(m * 4);
+    hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(ceil_div(n, kEmitThreads)), dim3(kEmitThreads),
+                       0, st, n, num_intersects, xys, extents, cum_tiles_hit, tx, ty,
+                       block_width, keys, vals);
+    int rc = radix_sort<uint32_t>(keys, vals, keys_sorted,
+                                  reinterpret_cast<uint32_t *>(gaussian_ids_sorted), m,
+                                  bits_for(nt), 0u, p, st);
+    if (rc != LIG_OK) return rc;
+    hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(m, 256)), dim3(256), 0, st,
+                       num_intersects, keys_sorted, nt, tile_bins);
+    return launch_status();
+}

@@ -1,0 +1,85 @@
+// Shared device/host helpers for the gfx950 gsplat2d kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lig_abi.h"
+
+#define LIG_WAVE 64
+
+namespace lig {
+
+// Status of the last launch on this thread, mapped to the ABI codes.
+inline int launch_status() {
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LIG_OK : LIG_ERR_LAUNCH;
+}
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// IEEE FP32 helpers with a fixed meaning regardless of compiler flags:
+// division and square root are computed in FP64 and rounded once (correctly
+// rounded FP32 results, since FP64 carries more than 2*24+2 bits), the cast
+// to int truncates and saturates with NaN -> 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float div_rn(float a, float b) {
+    return (float)((double)a / (double)b);
+}
+__device__ __forceinline__ float sqrt_rn(float a) { return (float)__builtin_sqrt((double)a); }
+__device__ __forceinline__ float log_rd(float a) { return (float)log((double)a); }
+__device__ __forceinline__ int f2i_sat(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int)x;
+}
+
+// Tile-space bounding box of a Gaussian: inclusive min, exclusive max, clamped
+// to the tile grid (helpers.cuh:12-45 semantics).
+__device__ __forceinline__ void tile_bbox(float cx, float cy, float ex, float ey, int tiles_x,
+                                          int tiles_y, int block_width, int &x0, int &y0,
+                                          int &x1, int &y1) {
+#pragma clang fp contract(off)
+    const float fb = (float)block_width;
+    const float tcx = div_rn(cx, fb), tcy = div_rn(cy, fb);
+    const float tex = div_rn(ex, fb), tey = div_rn(ey, fb);
+    x0 = min(max(0, f2i_sat(tcx - tex)), tiles_x);
+    x1 = min(max(0, f2i_sat(tcx + tex + 1.0f)), tiles_x);
+    y0 = min(max(0, f2i_sat(tcy - tey)), tiles_y);
+    y1 = min(max(0, f2i_sat(tcy + tey + 1.0f)), tiles_y);
+}
+
+// Map a linear workgroup id to a work item so that each XCD (workgroups are
+// dispatched round-robin over the 8 XCDs) receives one contiguous range of
+// items: neighbouring tiles then share the XCD's L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
+    const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Per-Gaussian record the rasteriser kernels read (48 bytes, 16-byte aligned):
+//   p0 = (mean.x, mean.y, A, B)   p1 = (C, opacity, red, green)
+//   p2 = (blue, a, b, c)
+// with (a, b, c) the conic and A = a*log2(e)/2, B = b*log2(e), C = c*log2(e)/2
+// so that sigma*log2(e) = dx*(A*dx + B*dy) + C*dy*dy and alpha = o*2^-(that).
+struct __align__(16) GaussPack {
+    float4 p0, p1, p2;
+};
+
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kAlphaMin = 1.0f / 255.0f;
+
+// sigma * log2(e) of the pair, evaluated with explicit FMAs so that every
+// kernel that calls it computes bit-identical values.  d = mean - pixel.
+__device__ __forceinline__ float pair_s2(float A, float B, float C, float dx, float dy) {
+    const float q = __builtin_fmaf(A, dx, B * dy);
+    return __builtin_fmaf(dx, q, (C * dy) * dy);
+}
+
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+}  // namespace lig

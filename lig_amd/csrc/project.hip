@@ -1,0 +1,256 @@
+// Projection of 2-D Gaussians: covariance (or Cholesky factor) -> conic,
+// SnugBox extent and tile count; and the backward of both forms.
+//
+// Numerics contract: IEEE FP32 in the reference's operation order with FP
+// contraction disabled, division and square root correctly rounded and the
+// logarithm taken in FP64 and rounded once.  Results are therefore
+// bit-reproducible on any IEEE host, which is how they are checked.
+//
+// Reference behaviour followed (paths relative to the reference repository):
+//   gsplat2d/gsplat2d/cuda/csrc/helpers.cuh:47-93, forward.cu:13-105,
+//   backward.cu:185-247, bindings.cu:19-66.
+#include "lig_device.h"
+
+namespace lig {
+namespace {
+
+constexpr int kThreads = 256;
+
+// SnugBox bounds (helpers.cuh:47-81).  Returns false when det == 0.
+__device__ __forceinline__ bool cov_bounds(float ca, float cb, float cc, float opacity,
+                                           float &ka, float &kb, float &kc, float &ex,
+                                           float &ey) {
+#pragma clang fp contract(off)
+    const float det = ca * cc - cb * cb;
+    if (det == 0.0f) return false;
+    const float inv_det = div_rn(1.0f, det);
+    ka = cc * inv_det;
+    kb = -cb * inv_det;
+    kc = ca * inv_det;
+    const float t = 2.0f * log_rd(255.0f * opacity);
+    if (t <= 0.0f) {
+        ex = 0.0f;
+        ey = 0.0f;
+        return true;
+    }
+    ex = ceilf(sqrt_rn(div_rn(t * kc, inv_det)));
+    ey = ceilf(sqrt_rn(div_rn(t * ka, inv_det)));
+    return true;
+}
+
+// One Gaussian of the forward projection; writes every output slot (zeros
+// where the reference leaves its zero fill).
+__device__ __forceinline__ void project_one(int i, float ca, float cb, float cc,
+                                            const float *__restrict__ means2d,
+                                            const float *__restrict__ opacities, int tiles_x,
+                                            int tiles_y, int block_width,
+                                            float *__restrict__ xys,
+                                            float *__restrict__ extents,
+                                            float *__restrict__ conics,
+                                            int32_t *__restrict__ num_tiles_hit) {
+    float ka = 0.0f, kb = 0.0f, kc = 0.0f, ex = 0.0f, ey = 0.0f;
+    const float o = opacities ? opacities[i] : 1.0f;
+    const bool ok = cov_bounds(ca, cb, cc, o, ka, kb, kc, ex, ey);
+    int area = 0;
+    const float mx = means2d[2 * i], my = means2d[2 * i + 1];
+    if (ok) {
+        int x0, y0, x1, y1;
+        tile_bbox(mx, my, ex, ey, tiles_x, tiles_y, block_width, x0, y0, x1, y1);
+        area = (int)((unsigned)(x1 - x0) * (unsigned)(y1 - y0));
+    }
+    const bool hit = area > 0;
+    conics[3 * i + 0] = ka;
+    conics[3 * i + 1] = kb;
+    conics[3 * i + 2] = kc;
+    extents[2 * i + 0] = hit ? ex : 0.0f;
+    extents[2 * i + 1] = hit ? ey : 0.0f;
+    xys[2 * i + 0] = hit ? mx : 0.0f;
+    xys[2 * i + 1] = hit ? my : 0.0f;
+    num_tiles_hit[i] = hit ? area : 0;
+}
+
+__global__ __launch_bounds__(kThreads) void project_cov_kernel(
+    int n, const float *__restrict__ cov2d, const float *__restrict__ means2d,
+    const float *__restrict__ opacities, int tiles_x, int tiles_y, int block_width,
+    float *__restrict__ xys, float *__restrict__ extents, float *__restrict__ conics,
+    int32_t *__restrict__ num_tiles_hit) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    project_one(i, cov2d[3 * i], cov2d[3 * i + 1], cov2d[3 * i + 2], means2d, opacities,
+                tiles_x, tiles_y, block_width, xys, extents, conics, num_tiles_hit);
+}
+
+__global__ __launch_bounds__(kThreads) void project_chol_kernel(
+    int n, const float *__restrict__ chol, const float *__restrict__ means2d,
+    const float *__restrict__ opacities, int tiles_x, int tiles_y, int block_width,
+    float *__restrict__ xys, float *__restrict__ extents, float *__restrict__ conics,
+    int32_t *__restrict__ num_tiles_hit) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const float l11 = chol[3 * i], l21 = chol[3 * i + 1], l22 = chol[3 * i + 2];
+    const float ca = l11 * l11;
+    const float cb = l11 * l21;
+    const float cc = l21 * l21 + l22 * l22;
+    project_one(i, ca, cb, cc, means2d, opacities, tiles_x, tiles_y, block_width, xys,
+                extents, conics, num_tiles_hit);
+}
+
+__global__ __launch_bounds__(kThreads) void cov2d_bounds_kernel(
+    int n, const float *__restrict__ cov2d, const float *__restrict__ opacities,
+    float *__restrict__ conics, float *__restrict__ extents) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    float ka = 0.0f, kb = 0.0f, kc = 0.0f, ex = 0.0f, ey = 0.0f;
+    const float o = opacities ? opacities[i] : 1.0f;
+    cov_bounds(cov2d[3 * i], cov2d[3 * i + 1], cov2d[3 * i + 2], o, ka, kb, kc, ex, ey);
+    conics[3 * i + 0] = ka;
+    conics[3 * i + 1] = kb;
+    conics[3 * i + 2] = kc;
+    extents[2 * i + 0] = ex;
+    extents[2 * i + 1] = ey;
+}
+
+// v_Sigma = -X G X (helpers.cuh:84-93), 2x2 products in glm's column-major
+// order, left to right.
+__device__ __forceinline__ void conic_vjp(float a, float b, float c, float va, float vb,
+                                          float vc, float &o0, float &o1, float &o2) {
+#pragma clang fp contract(off)
+    const float X[2][2] = {{a, b}, {b, c}};
+    const float hb = vb / 2.0f;
+    const float G[2][2] = {{va, hb}, {hb, vc}};
+    const float N[2][2] = {{-a, -b}, {-b, -c}};
+    float P[2][2], V[2][2];
+#pragma unroll
+    for (int col = 0; col < 2; ++col)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) P[col][r] = N[0][r] * G[col][0] + N[1][r] * G[col][1];
+#pragma unroll
+    for (int col = 0; col < 2; ++col)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) V[col][r] = P[0][r] * X[col][0] + P[1][r] * X[col][1];
+    o0 = V[0][0];
+    o1 = V[1][0] + V[0][1];
+    o2 = V[1][1];
+}
+
+__global__ __launch_bounds__(kThreads) void project_bwd_kernel(
+    int n, const float *__restrict__ extents, const float *__restrict__ conics,
+    const float *__restrict__ v_xy, const float *__restrict__ v_conic,
+    const float *__restrict__ chol, float *__restrict__ v_par, float *__restrict__ v_mean) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    float m0 = 0.0f, m1 = 0.0f, p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
+    if (extents[2 * i] > 0.0f && extents[2 * i + 1] > 0.0f) {
+        m0 = v_xy[2 * i];
+        m1 = v_xy[2 * i + 1];
+        float va, vb, vc;
+        conic_vjp(conics[3 * i], conics[3 * i + 1], conics[3 * i + 2], v_conic[3 * i],
+                  v_conic[3 * i + 1], v_conic[3 * i + 2], va, vb, vc);
+        if (chol) {
+            const float l11 = chol[3 * i], l21 = chol[3 * i + 1], l22 = chol[3 * i + 2];
+            const float v11 = 2.0f * l11 * va + l21 * vb;
+            const float v21 = l11 * vb + 2.0f * l21 * vc;
+            const float v22 = 2.0f * l22 * vc;
+            const bool fin = __builtin_isfinite(v11) && __builtin_isfinite(v21) &&
+                             __builtin_isfinite(v22);
+            p0 = fin ? v11 : 0.0f;
+            p1 = fin ? v21 : 0.0f;
+            p2 = fin ? v22 : 0.0f;
+        } else {
+            p0 = va;
+            p1 = vb;
+            p2 = vc;
+        }
+    }
+    v_mean[2 * i] = m0;
+    v_mean[2 * i + 1] = m1;
+    v_par[3 * i] = p0;
+    v_par[3 * i + 1] = p1;
+    v_par[3 * i + 2] = p2;
+}
+
+bool dims_ok(int img_height, int img_width, int block_width) {
+    return img_height > 0 && img_width > 0 && block_width >= 1 && block_width <= 16;
+}
+
+}  // namespace
+}  // namespace lig
+
+using namespace lig;
+
+extern "C" int lig_compute_cov2d_bounds(int n, const float *cov2d, const float *opacities,
+                                        float *conics, float *extents, void *stream) {
+    if (n < 0 || (n > 0 && (!cov2d || !conics || !extents))) return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    hipLaunchKernelGGL(cov2d_bounds_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0,
+                       as_stream(stream), n, cov2d, opacities, conics, extents);
+    return launch_status();
+}
+
+static int project_fwd(bool cholesky, int n, const float *par, const float *means2d,
+                       const float *opacities, int img_height, int img_width, int block_width,
+                       float *xys, float *extents, float *conics, int32_t *num_tiles_hit,
+                       void *stream) {
+    if (n < 0 || !dims_ok(img_height, img_width, block_width)) return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    if (!par || !means2d || !xys || !extents || !conics || !num_tiles_hit)
+        return LIG_ERR_INVALID_ARGUMENT;
+    const int tx = (img_width + block_width - 1) / block_width;
+    const int ty = (img_height + block_width - 1) / block_width;
+    auto kern = cholesky ? project_chol_kernel : project_cov_kernel;
+    hipLaunchKernelGGL(kern, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, as_stream(stream),
+                       n, par, means2d, opacities, tx, ty, block_width, xys, extents, conics,
+                       num_tiles_hit);
+    return launch_status();
+}
+
+extern "C" int lig_project_gaussians_forward(int n, const float *cov2d, const float *means2d,
+                                             const float *opacities, int img_height,
+                                             int img_width, int block_width, float *xys,
+                                             float *extents, float *conics,
+                                             int32_t *num_tiles_hit, void *stream) {
+    return project_fwd(false, n, cov2d, means2d, opacities, img_height, img_width, block_width,
+                       xys, extents, conics, num_tiles_hit, stream);
+}
+
+extern "C" int lig_project_gaussians_forward_cholesky(int n, const float *cholesky,
+                                                      const float *means2d,
+                                                      const float *opacities, int img_height,
+                                                      int img_width, int block_width,
+                                                      float *xys, float *extents,
+                                                      float *conics, int32_t *num_tiles_hit,
+                                                      void *stream) {
+    return project_fwd(true, n, cholesky, means2d, opacities, img_height, img_width,
+                       block_width, xys, extents, conics, num_tiles_hit, stream);
+}
+
+static int project_bwd(int n, const float *extents, const float *chol, const float *conics,
+                       const float *v_xy, const float *v_conic, float *v_par, float *v_mean,
+                       void *stream) {
+    if (n < 0) return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    if (!extents || !conics || !v_xy || !v_conic || !v_par || !v_mean)
+        return LIG_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(project_bwd_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0,
+                       as_stream(stream), n, extents, conics, v_xy, v_conic, chol, v_par,
+                       v_mean);
+    return launch_status();
+}
+
+extern "C" int lig_project_gaussians_backward(int n, const float *extents, const float *conics,
+                                              const float *v_xy, const float *v_conic,
+                                              float *v_cov2d, float *v_mean2d, void *stream) {
+    return project_bwd(n, extents, nullptr, conics, v_xy, v_conic, v_cov2d, v_mean2d, stream);
+}
+
+extern "C" int lig_project_gaussians_backward_cholesky(int n, const float *extents,
+                                                       const float *cholesky,
+                                                       const float *conics, const float *v_xy,
+                                                       const float *v_conic, float *v_cholesky,
+                                                       float *v_mean2d, void *stream) {
+    if (!cholesky && n > 0) return LIG_ERR_INVALID_ARGUMENT;
+    return project_bwd(n, extents, cholesky, conics, v_xy, v_conic, v_cholesky, v_mean2d,
+                       stream);
+}

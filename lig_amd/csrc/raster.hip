@@ -1,0 +1,458 @@
+// Tile-based sum rasteriser: forward (plain and gradient-aware) and backward.
+//
+// Work decomposition (one 64-lane wave per tile, workgroup = one wave):
+//   * forward: lane = pixel.  A tile of bw x bw pixels is spread over the 64
+//     lanes, PX = ceil(bw*bw/64) pixels per lane (4 at bw = 16).  The tile's
+//     Gaussians are fetched 64 at a time (one per lane) into LDS and every
+//     lane walks the batch, so each LDS read feeds PX pixels.  FP32
+//     accumulation in bin order, as the reference.
+//   * backward: lane = Gaussian.  Each lane takes one Gaussian of the tile's
+//     bin and walks the tile's pixels (their upstream gradients staged in LDS
+//     and read as wave-uniform values), accumulating that Gaussian's gradient
+//     in registers.  No cross-lane reduction is needed; each Gaussian then
+//     receives one set of atomics per tile.
+//   * tiles are assigned XCD-contiguously (see xcd_remap).
+//
+// Pair evaluation: sigma*log2(e) from the pre-scaled conic with explicit FMAs
+// (pair_s2) and alpha = min(clamp, o * 2^-(sigma*log2 e)); a pair contributes
+// unless sigma < 0 or alpha < 1/255.  Clamps: 0.999 in both forwards and the
+// gradient-aware backward, 0.99 in the plain backward; no backward
+// differentiates the clamp.
+//
+// Reference behaviour followed (paths relative to the reference repository):
+//   gsplat2d/gsplat2d/cuda/csrc/forward.cu:173-279, backward.cu:8-183,
+//   gradient_aware_rasterizer.cu:118-495, bindings.cu:303-506.
+#include "lig_device.h"
+
+namespace lig {
+namespace {
+
+constexpr int kThreads = LIG_WAVE;
+constexpr int kMaxPix = 256;  // 16 x 16
+
+__global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restrict__ xys,
+                                                   const float *__restrict__ conics,
+                                                   const float *__restrict__ colors,
+                                                   const float *__restrict__ opacities,
+                                                   GaussPack *__restrict__ pack) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float a = conics[3 * i], b = conics[3 * i + 1], c = conics[3 * i + 2];
+    const float o = opacities ? opacities[i] : 1.0f;
+    GaussPack g;
+    g.p0 = make_float4(xys[2 * i], xys[2 * i + 1], a * (0.5f * kLog2e), b * kLog2e);
+    g.p1 = make_float4(c * (0.5f * kLog2e), o, colors[3 * i], colors[3 * i + 1]);
+    g.p2 = make_float4(colors[3 * i + 2], a, b, c);
+    pack[i] = g;
+}
+
+template <int PX, bool GA>
+__global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
+    int tiles_x, int bw, int img_w, int img_h, const int32_t *__restrict__ gids,
+    const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
+    float *__restrict__ out_img, float *__restrict__ out_wsum, int32_t *__restrict__ final_idx,
+    float *__restrict__ out_dx, float *__restrict__ out_dy, float *__restrict__ out_dxy) {
+    __shared__ float4 s0[kThreads];
+    __shared__ float4 s1[kThreads];
+    __shared__ float4 s2[kThreads];
+
+    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
+    const int npix = bw * bw;
+
+    float px[PX], py[PX];
+    float acc_r[PX], acc_g[PX], acc_b[PX], acc_w[PX];
+    float dxr[GA ? PX : 1], dxg[GA ? PX : 1], dxb[GA ? PX : 1];
+    float dyr[GA ? PX : 1], dyg[GA ? PX : 1], dyb[GA ? PX : 1];
+    float dzr[GA ? PX : 1], dzg[GA ? PX : 1], dzb[GA ? PX : 1];
+    int fin[PX];
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        const int p = lane + LIG_WAVE * k;
+        const int ly = p / bw;
+        const int lx = p - ly * bw;
+        px[k] = (float)(tx * bw + lx) + 0.5f;
+        py[k] = (float)(ty * bw + ly) + 0.5f;
+        acc_r[k] = acc_g[k] = acc_b[k] = acc_w[k] = 0.0f;
+        fin[k] = 0;
+        if constexpr (GA) {
+            dxr[k] = dxg[k] = dxb[k] = 0.0f;
+            dyr[k] = dyg[k] = dyb[k] = 0.0f;
+            dzr[k] = dzg[k] = dzb[k] = 0.0f;
+        }
+    }
+
+    const int2 range = bins[tile];
+    for (int base = range.x; base < range.y; base += kThreads) {
+        const int cnt = min(kThreads, range.y - base);
+        if (lane < cnt) {
+            const GaussPack gp = pack[gids[base + lane]];
+            s0[lane] = gp.p0;
+            s1[lane] = gp.p1;
+            s2[lane] = gp.p2;
+        }
+        __syncthreads();
+        for (int t = 0; t < cnt; ++t) {
+            const float4 a0 = s0[t];
+            const float4 a1 = s1[t];
+            const float4 a2 = s2[t];
+            const int kidx = base + t;
+#pragma unroll
+            for (int k = 0; k < PX; ++k) {
+                const float dx = a0.x - px[k];
+                const float dy = a0.y - py[k];
+                const float s = pair_s2(a0.z, a0.w, a1.x, dx, dy);
+                const float e = fast_exp2(-s);
+                float al = fminf(a1.y * e, 0.999f);
+                const bool ok = !(s < 0.0f) && !(al < kAlphaMin);
+                al = ok ? al : 0.0f;
+                acc_r[k] = __builtin_fmaf(al, a1.z, acc_r[k]);
+                acc_g[k] = __builtin_fmaf(al, a1.w, acc_g[k]);
+                acc_b[k] = __builtin_fmaf(al, a2.x, acc_b[k]);
+                acc_w[k] += al;
+                fin[k] = ok ? kidx : fin[k];
+                if constexpr (GA) {
+                    // d' = p - mean = -(dx, dy): sigma_x = -u, sigma_y = -v
+                    const float u = __builtin_fmaf(a2.y, dx, a2.z * dy);
+                    const float v = __builtin_fmaf(a2.z, dx, a2.w * dy);
+                    const float ax = al * u;
+                    const float ay = al * v;
+                    const float axy = al * __builtin_fmaf(u, v, -a2.z);
+                    dxr[k] = __builtin_fmaf(a1.z, ax, dxr[k]);
+                    dxg[k] = __builtin_fmaf(a1.w, ax, dxg[k]);
+                    dxb[k] = __builtin_fmaf(a2.x, ax, dxb[k]);
+                    dyr[k] = __builtin_fmaf(a1.z, ay, dyr[k]);
+                    dyg[k] = __builtin_fmaf(a1.w, ay, dyg[k]);
+                    dyb[k] = __builtin_fmaf(a2.x, ay, dyb[k]);
+                    dzr[k] = __builtin_fmaf(a1.z, axy, dzr[k]);
+                    dzg[k] = __builtin_fmaf(a1.w, axy, dzg[k]);
+                    dzb[k] = __builtin_fmaf(a2.x, axy, dzb[k]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        const int p = lane + LIG_WAVE * k;
+        const int ly = p / bw;
+        const int lx = p - ly * bw;
+        const int i = ty * bw + ly;
+        const int j = tx * bw + lx;
+        if (p < npix && i < img_h && j < img_w) {
+            const size_t pid = (size_t)i * img_w + j;
+            out_img[3 * pid + 0] = acc_r[k];
+            out_img[3 * pid + 1] = acc_g[k];
+            out_img[3 * pid + 2] = acc_b[k];
+            out_wsum[pid] = acc_w[k];
+            final_idx[pid] = fin[k];
+            if constexpr (GA) {
+                out_dx[3 * pid + 0] = dxr[k];
+                out_dx[3 * pid + 1] = dxg[k];
+                out_dx[3 * pid + 2] = dxb[k];
+                out_dy[3 * pid + 0] = dyr[k];
+                out_dy[3 * pid + 1] = dyg[k];
+                out_dy[3 * pid + 2] = dyb[k];
+                out_dxy[3 * pid + 0] = dzr[k];
+                out_dxy[3 * pid + 1] = dzg[k];
+                out_dxy[3 * pid + 2] = dzb[k];
+            }
+        }
+    }
+}
+
+// Upstream gradients of one pixel as staged in LDS.
+struct PixPlain {
+    float4 v;  // (d/dI r, g, b, d/dW)
+};
+
+template <bool GA>
+__global__ __launch_bounds__(kThreads) void raster_bwd_kernel(
+    int tiles_x, int bw, int img_w, int img_h, const int32_t *__restrict__ gids,
+    const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
+    const int32_t *__restrict__ final_idx, const float *__restrict__ v_out,
+    const float *__restrict__ v_wsum, const float *__restrict__ v_dx,
+    const float *__restrict__ v_dy, const float *__restrict__ v_dxy, int has_opac,
+    float *__restrict__ g_xy, float *__restrict__ g_xy_abs, float *__restrict__ g_conic,
+    float *__restrict__ g_rgb, float *__restrict__ g_opac) {
+    __shared__ float4 sv[kMaxPix];
+    __shared__ int sfin[kMaxPix];
+    __shared__ float4 sa[GA ? kMaxPix : 1];  // (vdx r, g, b, vdy r)
+    __shared__ float4 sb[GA ? kMaxPix : 1];  // (vdy g, b, vdxy r, g)
+    __shared__ float sc[GA ? kMaxPix : 1];   // vdxy b
+
+    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    const int2 range = bins[tile];
+    if (range.x >= range.y) return;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
+    const int npix = bw * bw;
+    const int x_org = tx * bw, y_org = ty * bw;
+
+    for (int p = lane; p < npix; p += kThreads) {
+        const int ly = p / bw;
+        const int lx = p - ly * bw;
+        const int i = y_org + ly;
+        const int j = x_org + lx;
+        if (i < img_h && j < img_w) {
+            const size_t pid = (size_t)i * img_w + j;
+            sv[p] = make_float4(v_out[3 * pid], v_out[3 * pid + 1], v_out[3 * pid + 2],
+                                v_wsum ? v_wsum[pid] : 0.0f);
+            sfin[p] = final_idx[pid];
+            if constexpr (GA) {
+                sa[p] = make_float4(v_dx[3 * pid], v_dx[3 * pid + 1], v_dx[3 * pid + 2],
+                                    v_dy[3 * pid]);
+                sb[p] = make_float4(v_dy[3 * pid + 1], v_dy[3 * pid + 2], v_dxy[3 * pid],
+                                    v_dxy[3 * pid + 1]);
+                sc[p] = v_dxy[3 * pid + 2];
+            }
+        } else {
+            sv[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            sfin[p] = INT32_MIN;
+            if constexpr (GA) {
+                sa[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                sb[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                sc[p] = 0.0f;
+            }
+        }
+    }
+    __syncthreads();
+
+    constexpr float kClamp = GA ? 0.999f : 0.99f;
+    for (int base = range.x; base < range.y; base += kThreads) {
+        const int kidx = base + lane;
+        const bool act = kidx < range.y;
+        const int g = act ? gids[kidx] : 0;
+        const GaussPack gp = pack[g];
+        const float mx = gp.p0.x, my = gp.p0.y, A = gp.p0.z, B = gp.p0.w;
+        const float C = gp.p1.x, op = gp.p1.y, cr = gp.p1.z, cg = gp.p1.w;
+        const float cb = gp.p2.x, ka = gp.p2.y, kb = gp.p2.z, kc = gp.p2.w;
+
+        float a_r = 0.f, a_g = 0.f, a_b = 0.f;
+        float a_c0 = 0.f, a_c1 = 0.f, a_c2 = 0.f;
+        float a_x = 0.f, a_y = 0.f, a_ax = 0.f, a_ay = 0.f, a_o = 0.f;
+
+        for (int p = 0; p < npix; ++p) {
+            const int fp = sfin[p];
+            if (fp < base) continue;  // wave-uniform: nothing in this batch reaches p
+            const int ly = p / bw;
+            const int lx = p - ly * bw;
+            const float ppx = (float)(x_org + lx) + 0.5f;
+            const float ppy = (float)(y_org + ly) + 0.5f;
+            const float dx = mx - ppx;
+            const float dy = my - ppy;
+            const float s = pair_s2(A, B, C, dx, dy);
+            const float vis = fast_exp2(-s);
+            const float al0 = fminf(op * vis, kClamp);
+            const bool ok = act && kidx <= fp && !(s < 0.0f) && !(al0 < kAlphaMin);
+            const float al = ok ? al0 : 0.0f;
+            const float vs_ = ok ? vis : 0.0f;
+            const float4 v = sv[p];
+            if constexpr (!GA) {
+                const float v_alpha =
+                    __builtin_fmaf(cr, v.x, __builtin_fmaf(cg, v.y, __builtin_fmaf(cb, v.z, v.w)));
+                a_r = __builtin_fmaf(al, v.x, a_r);
+                a_g = __builtin_fmaf(al, v.y, a_g);
+                a_b = __builtin_fmaf(al, v.z, a_b);
+                a_o = __builtin_fmaf(vs_, v_alpha, a_o);
+                const float vsig = -al * v_alpha;
+                const float hdx = 0.5f * vsig * dx;
+                a_c0 = __builtin_fmaf(hdx, dx, a_c0);
+                a_c1 = __builtin_fmaf(vsig * dx, dy, a_c1);
+                a_c2 = __builtin_fmaf(0.5f * vsig * dy, dy, a_c2);
+                const float gx = vsig * __builtin_fmaf(ka, dx, kb * dy);
+                const float gy = vsig * __builtin_fmaf(kb, dx, kc * dy);
+                a_x += gx;
+                a_y += gy;
+                a_ax += fabsf(gx);
+                a_ay += fabsf(gy);
+            } else {
+                const float4 wa = sa[p];
+                const float4 wb = sb[p];
+                const float wc = sc[p];
+                // upstream: vI = v.xyz, vW = v.w, vdx = wa.xyz, vdy = (wa.w, wb.x, wb.y),
+                // vdxy = (wb.z, wb.w, wc).  d' = p - mean.
+                const float ex = -dx, ey = -dy;
+                const float sx = __builtin_fmaf(ka, ex, kb * ey);
+                const float sy = __builtin_fmaf(kb, ex, kc * ey);
+                const float alx = -al * sx;
+                const float aly = -al * sy;
+                const float P = __builtin_fmaf(sx, sy, -kb);
+                const float alxy = al * P;
+                a_r += v.x * al + wa.x * alx + wa.w * aly + wb.z * alxy;
+                a_g += v.y * al + wa.y * alx + wb.x * aly + wb.w * alxy;
+                a_b += v.z * al + wa.z * alx + wb.y * aly + wc * alxy;
+                const float va = __builtin_fmaf(v.x, cr, __builtin_fmaf(v.y, cg, __builtin_fmaf(v.z, cb, v.w)));
+                const float vax = __builtin_fmaf(wa.x, cr, __builtin_fmaf(wa.y, cg, wa.z * cb));
+                const float vay = __builtin_fmaf(wa.w, cr, __builtin_fmaf(wb.x, cg, wb.y * cb));
+                const float vaxy = __builtin_fmaf(wb.z, cr, __builtin_fmaf(wb.w, cg, wc * cb));
+                const float sx2 = sx * sx, sy2 = sy * sy, sxsy = sx * sy;
+                const float gx = al * (va * sx + vax * (ka - sx2) + vay * (kb - sxsy) +
+                                       vaxy * (sx2 * sy - 2.0f * kb * sx - ka * sy));
+                const float gy = al * (va * sy + vax * (kb - sxsy) + vay * (kc - sy2) +
+                                       vaxy * (sx * sy2 - 2.0f * kb * sy - kc * sx));
+                const float ex2 = ex * ex, ey2 = ey * ey, exy = ex * ey;
+                a_c0 += al * (va * (-0.5f * ex2) + vax * (0.5f * ex2 * sx - ex) +
+                              vay * (0.5f * ex2 * sy) + vaxy * (-0.5f * ex2 * P + ex * sy));
+                a_c1 += al * (va * (-exy) + vax * (exy * sx - ey) + vay * (exy * sy - ex) +
+                              vaxy * (-exy * P + ey * sy + ex * sx - 1.0f));
+                a_c2 += al * (va * (-0.5f * ey2) + vax * (0.5f * ey2 * sx) +
+                              vay * (0.5f * ey2 * sy - ey) + vaxy * (-0.5f * ey2 * P + ey * sx));
+                a_x += gx;
+                a_y += gy;
+                a_ax += fabsf(gx);
+                a_ay += fabsf(gy);
+                a_o = __builtin_fmaf(vs_, va, a_o);
+            }
+        }
+        if (act) {
+            unsafeAtomicAdd(&g_rgb[3 * g + 0], a_r);
+            unsafeAtomicAdd(&g_rgb[3 * g + 1], a_g);
+            unsafeAtomicAdd(&g_rgb[3 * g + 2], a_b);
+            unsafeAtomicAdd(&g_conic[3 * g + 0], a_c0);
+            unsafeAtomicAdd(&g_conic[3 * g + 1], a_c1);
+            unsafeAtomicAdd(&g_conic[3 * g + 2], a_c2);
+            unsafeAtomicAdd(&g_xy[2 * g + 0], a_x);
+            unsafeAtomicAdd(&g_xy[2 * g + 1], a_y);
+            unsafeAtomicAdd(&g_xy_abs[2 * g + 0], a_ax);
+            unsafeAtomicAdd(&g_xy_abs[2 * g + 1], a_ay);
+            if (has_opac) unsafeAtomicAdd(&g_opac[g], a_o);
+        }
+    }
+}
+
+__global__ void zero_f32_kernel(float *__restrict__ p, size_t count) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0.0f;
+}
+
+template <bool GA>
+void launch_fwd(int px, dim3 grid, hipStream_t st, int tiles_x, int bw, int img_w, int img_h,
+                const int32_t *gids, const int2 *bins, const GaussPack *pack, float *img,
+                float *wsum, int32_t *fin, float *dx, float *dy, float *dxy) {
+    switch (px) {
+        case 1:
+            hipLaunchKernelGGL((raster_fwd_kernel<1, GA>), grid, dim3(kThreads), 0, st, tiles_x,
+                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy);
+            break;
+        case 2:
+            hipLaunchKernelGGL((raster_fwd_kernel<2, GA>), grid, dim3(kThreads), 0, st, tiles_x,
+                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy);
+            break;
+        case 3:
+            hipLaunchKernelGGL((raster_fwd_kernel<3, GA>), grid, dim3(kThreads), 0, st, tiles_x,
+                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy);
+            break;
+        default:
+            hipLaunchKernelGGL((raster_fwd_kernel<4, GA>), grid, dim3(kThreads), 0, st, tiles_x,
+                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy);
+            break;
+    }
+}
+
+}  // namespace
+}  // namespace lig
+
+using namespace lig;
+
+extern "C" size_t lig_packed_gaussians_bytes(int n) {
+    return n > 0 ? (size_t)n * sizeof(GaussPack) : 0;
+}
+
+extern "C" int lig_pack_gaussians(int n, const float *xys, const float *conics,
+                                  const float *colors, const float *opacities, void *packed,
+                                  void *stream) {
+    if (n < 0 || (n > 0 && (!xys || !conics || !colors || !packed)))
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), n,
+                       xys, conics, colors, opacities, reinterpret_cast<GaussPack *>(packed));
+    return launch_status();
+}
+
+extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int block_width,
+                                     const int32_t *gaussian_ids_sorted,
+                                     const int32_t *tile_bins, const float *xys,
+                                     const float *conics, const float *colors,
+                                     const float *opacities, void *packed, float *out_img,
+                                     float *out_wsum, int32_t *final_idx, float *out_dx,
+                                     float *out_dy, float *out_dxy, void *stream) {
+    if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (!tile_bins || !out_img || !out_wsum || !final_idx) return LIG_ERR_INVALID_ARGUMENT;
+    const bool ga = out_dx != nullptr;
+    if (ga && (!out_dy || !out_dxy)) return LIG_ERR_INVALID_ARGUMENT;
+    if (n > 0 && (!gaussian_ids_sorted || !xys || !conics || !colors || !packed))
+        return LIG_ERR_INVALID_ARGUMENT;
+    hipStream_t st = as_stream(stream);
+    GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
+    if (n > 0)
+        hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
+                           colors, opacities, pk);
+    const int tiles_x = (img_width + block_width - 1) / block_width;
+    const int tiles_y = (img_height + block_width - 1) / block_width;
+    const dim3 grid((unsigned)(tiles_x * tiles_y));
+    const int px = (block_width * block_width + LIG_WAVE - 1) / LIG_WAVE;
+    const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
+    if (ga)
+        launch_fwd<true>(px, grid, st, tiles_x, block_width, img_width, img_height,
+                         gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, out_dx,
+                         out_dy, out_dxy);
+    else
+        launch_fwd<false>(px, grid, st, tiles_x, block_width, img_width, img_height,
+                          gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, nullptr,
+                          nullptr, nullptr);
+    return launch_status();
+}
+
+extern "C" int lig_rasterize_backward(int n, int img_height, int img_width, int block_width,
+                                      const int32_t *gaussian_ids_sorted,
+                                      const int32_t *tile_bins, const void *packed,
+                                      const int32_t *final_idx, const float *v_output,
+                                      const float *v_wsum, const float *v_output_dx,
+                                      const float *v_output_dy, const float *v_output_dxy,
+                                      int has_opacities, float *v_xy, float *v_xy_abs,
+                                      float *v_conic, float *v_colors, float *v_opacity,
+                                      void *stream) {
+    if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
+        return LIG_ERR_INVALID_ARGUMENT;
+    const bool ga = v_output_dx != nullptr;
+    if (ga && (!v_output_dy || !v_output_dxy)) return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    if (!gaussian_ids_sorted || !tile_bins || !packed || !final_idx || !v_output || !v_xy ||
+        !v_xy_abs || !v_conic || !v_colors || (has_opacities && !v_opacity))
+        return LIG_ERR_INVALID_ARGUMENT;
+    hipStream_t st = as_stream(stream);
+    const size_t nn = (size_t)n;
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(2 * nn, 256)), dim3(256), 0, st, v_xy, 2 * nn);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(2 * nn, 256)), dim3(256), 0, st, v_xy_abs,
+                       2 * nn);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(3 * nn, 256)), dim3(256), 0, st, v_conic,
+                       3 * nn);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(3 * nn, 256)), dim3(256), 0, st, v_colors,
+                       3 * nn);
+    if (has_opacities)
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, v_opacity,
+                           nn);
+    const int tiles_x = (img_width + block_width - 1) / block_width;
+    const int tiles_y = (img_height + block_width - 1) / block_width;
+    const dim3 grid((unsigned)(tiles_x * tiles_y));
+    const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
+    const GaussPack *pk = reinterpret_cast<const GaussPack *>(packed);
+    if (ga)
+        hipLaunchKernelGGL(raster_bwd_kernel<true>, grid, dim3(kThreads), 0, st, tiles_x,
+                           block_width, img_width, img_height, gaussian_ids_sorted, bins, pk,
+                           final_idx, v_output, v_wsum, v_output_dx, v_output_dy, v_output_dxy,
+                           has_opacities, v_xy, v_xy_abs, v_conic, v_colors, v_opacity);
+    else
+        hipLaunchKernelGGL(raster_bwd_kernel<false>, grid, dim3(kThreads), 0, st, tiles_x,
+                           block_width, img_width, img_height, gaussian_ids_sorted, bins, pk,
+                           final_idx, v_output, v_wsum, (const float *)nullptr,
+                           (const float *)nullptr, (const float *)nullptr, has_opacities, v_xy,
+                           v_xy_abs, v_conic, v_colors, v_opacity);
+    return launch_status();
+}

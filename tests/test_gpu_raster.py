@@ -1,0 +1,166 @@
+"""GPU parity of the rasteriser (plain and gradient-aware, forward and backward)
+and the upscaler against the CPU oracle.
+
+Tolerances (stated per output):
+  * images / wsum / dx / dy / dxy: max |gpu - oracle_fp64| <= 1e-5 over pixels
+    with no ambiguous pair (a pair within relative 1e-5 of the 1/255 cut);
+  * final_idx: exact on the same pixels;
+  * gradients: norm-wise relative error <= 1e-4 against the reference-gradient
+    restatement (FP32 pair math, oracle mode "ref32") and against the FP64
+    oracle, over Gaussians with no ambiguous pair.
+"""
+import numpy as np
+import pytest
+import torch
+
+from tests.scenes import random_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dev, dtype=torch.float32):
+    return None if a is None else torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _prepare(oracle_lib, s, bw):
+    H, W = s["H"], s["W"]
+    xys, ext, con, nt = oracle_lib.project_gaussians_forward(
+        s["par"], s["means"], H, W, bw, s["opac"], cholesky=s["cholesky"])
+    m, cum, isect, gids, ks, gs, bins = oracle_lib.bin_and_sort(xys, ext, nt, H, W, bw)
+    return dict(xys=xys, ext=ext, con=con, nt=nt, m=m, cum=cum, gs=gs, bins=bins)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+RASTER_CASES = [
+    # n, H, W, bw, opacities, seed, scale
+    (200, 64, 64, 16, True, 0, (0.5, 3.0)),
+    (600, 96, 80, 16, False, 1, (0.5, 3.0)),
+    (400, 64, 64, 16, True, 2, (2.0, 9.0)),
+    (300, 45, 70, 8, True, 3, (0.5, 4.0)),
+    (250, 40, 40, 5, False, 4, (0.5, 3.0)),
+    (120, 33, 50, 12, True, 5, (1.0, 5.0)),
+]
+
+
+@pytest.mark.parametrize("ga", [False, True])
+@pytest.mark.parametrize("n,H,W,bw,op,seed,scale", RASTER_CASES)
+def test_raster_forward_backward(device, oracle_lib, n, H, W, bw, op, seed, scale, ga):
+    from lig_amd.gsplat2d import cuda as C
+    print(f"raster case n={n} H={H} W={W} bw={bw} op={op} seed={seed} ga={ga}", flush=True)
+    s = random_scene(n, H, W, seed, opacities=op, scale=scale, margin=10.0)
+    p = _prepare(oracle_lib, s, bw)
+    dev = device
+    gs, bins = _t(p["gs"], dev, torch.int32), _t(p["bins"], dev, torch.int32)
+    xys, con, col = _t(p["xys"], dev), _t(p["con"], dev), _t(s["colors"], dev)
+    opac = _t(s["opac"], dev)
+    img, wsum, dx, dy, dxy, fin, packed = C.rasterize_forward_packed(
+        H, W, bw, gs, bins, xys, con, col, opac, ga)
+    ref = oracle_lib.rasterize_forward(p["gs"], p["bins"], p["xys"], p["con"], s["colors"],
+                                       s["opac"], H, W, bw, gradient_aware=ga)
+    clean = ref["ambig"] == 0
+    assert clean.mean() > 0.9
+    np.testing.assert_array_equal(_np(fin)[clean], ref["final_idx"][clean])
+    assert np.abs(_np(img) - ref["img"])[clean].max() <= 1e-5
+    assert np.abs(_np(wsum) - ref["wsum"])[clean].max() <= 1e-5
+    if ga:
+        for a, b in ((dx, "dx"), (dy, "dy"), (dxy, "dxy")):
+            assert np.abs(_np(a) - ref[b])[clean].max() <= 1e-5, b
+
+    rng = np.random.default_rng(seed + 100)
+    v_out = rng.normal(size=(H, W, 3)).astype(np.float32)
+    v_w = rng.normal(size=(H, W)).astype(np.float32)
+    v_d = [rng.normal(size=(H, W, 3)).astype(np.float32) for _ in range(3)] if ga else [None] * 3
+    fin_np = _np(fin)
+    got = C.rasterize_backward_packed(H, W, bw, gs, bins, packed, n, fin, _t(v_out, dev),
+                                      _t(v_w, dev), op, *[_t(v, dev) for v in v_d])
+    names = ("v_xy", "v_xy_abs", "v_conic", "v_rgb", "v_opacity")
+    r64 = oracle_lib.rasterize_backward(p["gs"], p["bins"], p["xys"], p["con"], s["colors"],
+                                        s["opac"], fin_np, v_out, v_w, H, W, bw, *v_d,
+                                        mode="fp64")
+    r32 = oracle_lib.rasterize_backward(p["gs"], p["bins"], p["xys"], p["con"], s["colors"],
+                                        s["opac"], fin_np, v_out, v_w, H, W, bw, *v_d,
+                                        mode="ref32")
+    keep = r64["ambig"] == 0
+    assert keep.mean() > 0.8
+    for name, g in zip(names, got):
+        if g is None:
+            assert name == "v_opacity" and not op
+            continue
+        a = _np(g)[keep].astype(np.float64)
+        e64 = _rel(a, r64[name][keep])
+        e32 = _rel(a, r32[name][keep])
+        print(f"  {name}: rel err vs fp64 {e64:.2e}, vs ref32 {e32:.2e}")
+        assert e32 <= 1e-4, (name, e32)
+        assert e64 <= 1e-4, (name, e64)
+
+
+def test_raster_api_end_to_end(device, oracle_lib):
+    """project -> rasterize -> loss -> backward through the autograd wrappers."""
+    import lig_amd.gsplat2d as g
+    n, H, W = 300, 64, 80
+    s = random_scene(n, H, W, 9, opacities=True, margin=5.0)
+    chol = _t(s["par"], device).requires_grad_(True)
+    means = _t(s["means"], device).requires_grad_(True)
+    col = _t(s["colors"], device).requires_grad_(True)
+    opac = _t(s["opac"], device).requires_grad_(True)
+    xys, ext, con, nt = g.project_gaussians_cholesky(chol, means, H, W, 16, opac)
+    img, wsum, dx, dy, dxy = g.rasterize_gaussians(xys, ext, con, nt, col, opac, H, W, 16)
+    assert dx is None and dy is None and dxy is None
+    loss = (img * torch.linspace(0, 1, 3, device=device)).sum() + 0.3 * wsum.sum()
+    loss.backward()
+    for t in (chol, means, col, opac):
+        assert t.grad is not None and torch.isfinite(t.grad).all()
+    assert xys.absgrad.shape == (n, 2)
+
+    # nothing on screen: the ones image (rasterize.py:80-90) and zero gradients
+    far = _t(s["means"] + 1e5, device).requires_grad_(True)
+    xys, ext, con, nt = g.project_gaussians_cholesky(chol, far, H, W, 16, opac)
+    img, wsum, dx, dy, dxy = g.rasterize_gaussians(xys, ext, con, nt, col, opac, H, W, 16,
+                                                   compute_upscale_gradients=True)
+    assert torch.all(img == 1) and torch.all(wsum == 0) and torch.all(dx == 0)
+    img.sum().backward()
+    assert torch.all(far.grad == 0)
+
+
+UPSCALE_CASES = [
+    (16, 16, 64, 64, None),
+    (20, 30, 80, 120, None),
+    (17, 13, 50, 41, None),
+    (32, 32, 16, 16, None),
+    (24, 24, 48, 48, (4.0, 2.0, 20.0, 18.0)),
+    (24, 24, 37, 29, (2.5, 3.5, 21.0, 22.7)),
+]
+
+
+@pytest.mark.parametrize("sh,sw,dh,dw,roi", UPSCALE_CASES)
+def test_upscale_forward_backward(device, oracle_lib, sh, sw, dh, dw, roi):
+    import lig_amd.gsplat2d as g
+    rng = np.random.default_rng(sh * 1000 + dh)
+    f, fx, fy, fxy = (rng.normal(size=(sh, sw, 3)).astype(np.float32) for _ in range(4))
+    ts = [_t(a, device).requires_grad_(True) for a in (f, fx, fy, fxy)]
+    out = g.gradient_aware_upscale(*ts, dh, dw, roi)
+    ref = oracle_lib.upscale_forward(f, fx, fy, fxy, dh, dw, roi)
+    assert np.abs(_np(out) - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max())
+    gout = rng.normal(size=(dh, dw, 3)).astype(np.float32)
+    out.backward(_t(gout, device))
+    rg = oracle_lib.upscale_backward(gout, sh, sw, roi)
+    for t, r in zip(ts, rg):
+        assert np.abs(_np(t.grad) - r).max() <= 1e-5 * max(1.0, np.abs(r).max())
+
+
+def test_upscale_backward_deterministic(device):
+    from lig_amd.gsplat2d import cuda as C
+    rng = np.random.default_rng(1)
+    g = _t(rng.normal(size=(256, 256, 3)).astype(np.float32), device)
+    src = torch.zeros(64, 64, 3, device=device)
+    a = C.gradient_aware_upscale_backward(g, src, src, src, src, 256, 256, (0., 0., 64., 64.))
+    b = C.gradient_aware_upscale_backward(g, src, src, src, src, 256, 256, (0., 0., 64., 64.))
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
