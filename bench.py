@@ -1,0 +1,246 @@
+#!/usr/bin/env python
+"""Fit throughput of the two-level LIG workload on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on):
+8192 x 8192 RGB target (seeded blobs + 1/f noise + edges + tiled cell
+texture); level 1 fits the 4x bicubic-downsampled 2048^2 image with 1M
+Gaussians, level 2 fits the normalised residual of the upscaled level-1
+render at 8192^2 with 2M Gaussians.  Cholesky diagonal ~U(0.5, 3) * 7.2 px
+(about 40 tiles per Gaussian), 16x16 tiles, sum rasterisation, clamp + L2
+loss, Adam lr 1e-3.  One step = one optimiser step of level 1 plus one of
+level 2; value = steps per second.
+
+Also reported: render Gpix/s (level-2 forward: project + bin + rasterise),
+the dominant kernel's roofline (HIP events around its C-ABI call on the
+stream it runs on; algorithmic FLOPs from exact pair counts), and the CPU
+oracle timed on a bounded crop of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+CONFIGS = {
+    "c3": dict(H=8192, W=8192, seed=2, cells=True, n1=1_000_000, n2=2_000_000, scale=7.2,
+               factor=4,
+               workload="two-level LIG fit, 8192x8192 RGB: L1 2048^2 x 1M + L2 8192^2 x 2M "
+                        "Gaussians, chol diag U(0.5,3)*7.2 px, 16x16 tiles, L2 loss, Adam"),
+    "c2": dict(H=2048, W=2048, seed=1, cells=False, n1=100_000, n2=300_000, scale=1.0, factor=4,
+               workload="two-level LIG fit, 2048x2048 RGB: L1 512^2 x 100k + L2 2048^2 x 300k "
+                        "Gaussians, chol diag U(0.5,3) px, 16x16 tiles, L2 loss, Adam"),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def raster_flops(E, C, backward):
+    """Algorithmic FLOPs of the sum rasteriser (FMA = 2; exp, min, compare = 1):
+    17 per pair inside the SnugBox box, plus 7 (forward) or 37 (backward) per
+    contributing pair."""
+    return 17 * E + (37 if backward else 7) * C
+
+
+def raster_bytes(I, T, P, N, backward):
+    return 36 * I + 8 * T + 16 * P + (40 * N if backward else 0)
+
+
+def cpu_baseline(level, target, crop, oracle, torch):
+    """Oracle (FP64 CPU port) fit step on a crop x crop window of `level`:
+    project, bin, rasterise forward, loss gradient, rasterise backward,
+    projection backward.  Returns seconds for the crop."""
+    import numpy as np
+    H, W = level.H, level.W
+    y0, x0 = (H - crop) // 2, (W - crop) // 2
+    with torch.no_grad():
+        xys, ext, con, nt = level.project()
+        xs, ys = xys[:, 0], xys[:, 1]
+        ex, ey = ext[:, 0], ext[:, 1]
+        sel = (nt > 0) & (xs + ex >= x0) & (xs - ex <= x0 + crop) & (ys + ey >= y0) & \
+              (ys - ey <= y0 + crop)
+        idx = torch.nonzero(sel).flatten()
+        means = (level.means[idx] - torch.tensor([x0, y0], device=xys.device,
+                                                 dtype=torch.float32)).cpu().numpy()
+        chol = level.cholesky[idx].cpu().numpy()
+        rgbs = level.rgbs[idx].cpu().numpy()
+        tgt = target[y0:y0 + crop, x0:x0 + crop].cpu().numpy().astype(np.float64)
+    t0 = time.perf_counter()
+    pxy, pext, pcon, pnt = oracle.project_gaussians_forward(chol, means, crop, crop, 16, None,
+                                                            cholesky=True)
+    m, cum, isect, gids, ks, gs, bins = oracle.bin_and_sort(pxy, pext, pnt, crop, crop, 16)
+    fwd = oracle.rasterize_forward(gs, bins, pxy, pcon, rgbs, None, crop, crop, 16)
+    img = fwd["img"]
+    inside = (img > 0) & (img < 1)
+    v_out = (2.0 / img.size) * (np.clip(img, 0, 1) - tgt) * inside
+    bwd = oracle.rasterize_backward(gs, bins, pxy, pcon, rgbs, None, fwd["final_idx"], v_out,
+                                    np.zeros((crop, crop), np.float32), crop, crop, 16)
+    oracle.project_gaussians_backward(pext, pcon, bwd["v_xy"], bwd["v_conic"], cholesky=chol)
+    return time.perf_counter() - t0, int(idx.numel()), m
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--render-reps", type=int, default=5)
+    ap.add_argument("--cpu-baseline", type=int, default=1, help="0 to skip the CPU oracle leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        sys.exit(2)
+
+    import torch
+
+    import lig_amd._lib as L
+    from lig_amd import data, fit
+    from lig_amd.gsplat2d import cuda as C
+    from lig_amd.profiling import KernelTimer
+
+    if not os.path.exists(L.LIB_PATH):
+        import __graft_entry__
+        __graft_entry__.build()
+    L.load()
+    cfg = CONFIGS[args.config]
+    dev = torch.device("cuda:0")
+    torch.manual_seed(cfg["seed"])
+
+    t0 = time.perf_counter()
+    gt = data.make_image(cfg["H"], cfg["W"], cfg["seed"], cells=cfg["cells"], device=dev)
+    f = cfg["factor"]
+    l1 = fit.Level(cfg["n1"], cfg["H"] // f, cfg["W"] // f, cfg["seed"] * 10 + 1, cfg["scale"],
+                   device=dev)
+    l2 = fit.Level(cfg["n2"], cfg["H"], cfg["W"], cfg["seed"] * 10 + 2, cfg["scale"], device=dev)
+    t1 = fit.downsample(gt, f)
+    with torch.no_grad():
+        up = torch.clamp(fit.upscale_render(l1, cfg["H"], cfg["W"]), 0, 1)
+        t2, lo, hi = fit.residual_target(gt, up)
+    del up
+    torch.cuda.synchronize()
+    log(f"setup {time.perf_counter() - t0:.1f}s")
+
+    for i in range(args.warmup):
+        l1.step(t1)
+        l2.step(t2)
+        torch.cuda.synchronize()
+        log(f"warmup {i + 1}/{args.warmup}")
+
+    lvl_ms = {"l1": 0.0, "l2": 0.0}
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd_t = KernelTimer("lig_rasterize_forward", dev)
+    bwd_t = KernelTimer("lig_rasterize_backward", dev)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for i in range(args.steps):
+        ev[0].record()
+        l1.step(t1)
+        ev[1].record()
+        with fwd_t, bwd_t:
+            l2.step(t2)
+        ev[2].record()
+        ev[2].synchronize()
+        lvl_ms["l1"] += ev[0].elapsed_time(ev[1])
+        lvl_ms["l2"] += ev[1].elapsed_time(ev[2])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    steps_per_s = args.steps / elapsed
+    log(f"timed {args.steps} steps in {elapsed:.3f}s")
+
+    # render throughput: level-2 forward only
+    with torch.no_grad():
+        for _ in range(2):
+            l2.render()
+        torch.cuda.synchronize()
+        r0 = time.perf_counter()
+        for _ in range(args.render_reps):
+            l2.render()
+        torch.cuda.synchronize()
+        render_s = (time.perf_counter() - r0) / args.render_reps
+        xys, ext, con, nt = l2.project()
+        E, Cc = C.count_pairs(xys, ext, con, None, nt, l2.H, l2.W)
+        I = int(nt.sum().item())
+        mean_tiles = float(nt.float().mean().item())
+    T = (l2.H // 16) * (l2.W // 16)
+    P = l2.H * l2.W
+    fwd_ms, bwd_ms = fwd_t.mean_ms(), bwd_t.mean_ms()
+    kernels = {
+        "raster_fwd": dict(ms=fwd_ms, flops=raster_flops(E, Cc, False),
+                           bytes=raster_bytes(I, T, P, l2.means.shape[0], False)),
+        "raster_bwd": dict(ms=bwd_ms, flops=raster_flops(E, Cc, True),
+                           bytes=raster_bytes(I, T, P, l2.means.shape[0], True)),
+    }
+    dom_name = max(kernels, key=lambda k: kernels[k]["ms"])
+    dom = kernels[dom_name]
+    achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": dom_name, "achieved": round(achieved, 3),
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "kernel_ms": round(dom["ms"], 3), "algorithmic_flops": dom["flops"],
+                "algorithmic_bytes": dom["bytes"],
+                "hbm_gbs_algorithmic": round(dom["bytes"] / (dom["ms"] * 1e-3) / 1e9, 1)}
+
+    cpu = None
+    if args.cpu_baseline:
+        import oracle
+        oracle.build()
+        threads = oracle.num_threads()
+        crops = {"l1": 512, "l2": 1024}
+        est = 0.0
+        parts = []
+        for name, lev, tgt in (("l1", l1, t1), ("l2", l2, t2)):
+            cr = min(crops[name], lev.H, lev.W)
+            secs, ng, m = cpu_baseline(lev, tgt, cr, oracle, torch)
+            scale = (lev.H * lev.W) / (cr * cr)
+            est += secs * scale
+            parts.append(f"{name}: {cr}x{cr} crop ({ng} Gaussians, {m} intersections) "
+                         f"{secs:.2f}s x{scale:.0f}")
+            log(f"cpu baseline {parts[-1]}")
+        cpu = {"value": round(1.0 / est, 6), "unit": "fit steps/s", "cores": threads,
+               "kind": "port",
+               "sample": "oracle (C, FP64 raster, OpenMP) project+bin+raster fwd/bwd+proj bwd "
+                         "on a central crop of each level, scaled by image area / crop area; "
+                         + "; ".join(parts)}
+
+    out = {
+        "metric": "fit steps/s + render Gpix/s @8192², 2M Gaussians; p50/p99 crop latency",
+        "value": round(steps_per_s, 4),
+        "unit": "fit steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / steps_per_s, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, random-init Gaussians)",
+        "config": {"workload": cfg["workload"], "config": args.config,
+                   "level1": f"{cfg['H'] // f}x{cfg['W'] // f} x {cfg['n1']}",
+                   "level2": f"{cfg['H']}x{cfg['W']} x {cfg['n2']}"},
+        "level_ms": {k: round(v / args.steps, 3) for k, v in lvl_ms.items()},
+        "render_gpix_s": round(P / render_s / 1e9, 4),
+        "render_ms": round(render_s * 1e3, 3),
+        "workload_counts_l2": {"N": l2.means.shape[0], "I": I, "T": T, "P": P, "E": E, "C": Cc,
+                               "mean_tiles_per_gaussian": round(mean_tiles, 2),
+                               "tile_pairs": I * 256},
+        "kernel_ms_l2": {k: round(v["ms"], 3) for k, v in kernels.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -162,6 +162,15 @@ int lig_rasterize_backward(int n, int img_height, int img_width, int block_width
                            float *v_xy_abs, float *v_conic, float *v_colors,
                            float *v_opacity, void *stream);
 
+/* No reference counterpart: workload counters for roofline accounting.
+ * counts[0] = E, pixel/Gaussian pairs whose pixel centre lies inside the
+ * Gaussian's clipped SnugBox box |p - mean| <= extent (Gaussians with
+ * num_tiles_hit > 0); counts[1] = C, the pairs of E with sigma >= 0 and
+ * alpha >= 1/255 (evaluated in FP64).  Independent of the tile size. */
+int lig_count_pairs(int n, int img_height, int img_width, const float *xys,
+                    const float *extents, const float *conics, const float *opacities,
+                    const int32_t *num_tiles_hit, unsigned long long *counts, void *stream);
+
 /* ---- gradient-aware upscaler ---------------------------------------------- */
 
 /* Replaces gradient_aware_upscale_forward_tensor (csrc/bindings.cu:508-559,

@@ -50,6 +50,7 @@ _SIGNATURES = {
     "lig_rasterize_backward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                  _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lig_count_pairs": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_gradient_aware_upscale_forward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,
                  _vp, _vp, _vp, _vp, _vp]),
@@ -85,9 +86,20 @@ def load():
     return _lib
 
 
+# Optional per-entry-point timing: name -> callable(name, phase) invoked with
+# phase "begin" / "end" around the call on the calling thread (see
+# lig_amd.profiling).  Empty in normal use.
+hooks = {}
+
+
 def call(name: str, *args) -> None:
     """Call an int-returning entry point and raise LigError on failure."""
+    hook = hooks.get(name)
+    if hook is not None:
+        hook(name, "begin")
     rc = getattr(load(), name)(*args)
+    if hook is not None:
+        hook(name, "end")
     if rc != 0:
         msg = load().lig_strerror(rc).decode()
         raise LigError(f"{name} failed: {msg} (status {rc})")

@@ -330,6 +330,51 @@ __global__ void zero_f32_kernel(float *__restrict__ p, size_t count) {
     if (i < count) p[i] = 0.0f;
 }
 
+// Workload counters for the roofline: E = pairs whose pixel centre lies in the
+// Gaussian's clipped SnugBox box |p - mean| <= extent, C = pairs of E with
+// sigma >= 0 and alpha >= 1/255 (FP64, clamp 0.999).  One thread per Gaussian.
+__global__ __launch_bounds__(256) void count_pairs_kernel(
+    int n, int img_h, int img_w, const float *__restrict__ xys, const float *__restrict__ extents,
+    const float *__restrict__ conics, const float *__restrict__ opacities,
+    const int32_t *__restrict__ ntiles, unsigned long long *__restrict__ out) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long e = 0, c = 0;
+    if (g < n && ntiles[g] > 0) {
+        const double mx = xys[2 * g], my = xys[2 * g + 1];
+        const double ex = extents[2 * g], ey = extents[2 * g + 1];
+        int j0 = (int)ceil(mx - ex - 0.5), j1 = (int)floor(mx + ex - 0.5);
+        int i0 = (int)ceil(my - ey - 0.5), i1 = (int)floor(my + ey - 0.5);
+        j0 = max(j0, 0);
+        i0 = max(i0, 0);
+        j1 = min(j1, img_w - 1);
+        i1 = min(i1, img_h - 1);
+        const double a = conics[3 * g], b = conics[3 * g + 1], cc = conics[3 * g + 2];
+        const double o = opacities ? opacities[g] : 1.0;
+        for (int i = i0; i <= i1; ++i)
+            for (int j = j0; j <= j1; ++j) {
+                const double dx = mx - (j + 0.5), dy = my - (i + 0.5);
+                const double sg = 0.5 * (a * dx * dx + cc * dy * dy) + b * dx * dy;
+                double al = o * exp(-sg);
+                al = al > 0.999 ? 0.999 : al;
+                ++e;
+                c += (sg < 0.0 || al < 1.0 / 255.0) ? 0 : 1;
+            }
+    }
+    // wave reduction, then one atomic per wave
+    for (int off = LIG_WAVE / 2; off > 0; off >>= 1) {
+        e += __shfl_down(e, off, LIG_WAVE);
+        c += __shfl_down(c, off, LIG_WAVE);
+    }
+    if ((threadIdx.x & (LIG_WAVE - 1)) == 0) {
+        atomicAdd(&out[0], e);
+        atomicAdd(&out[1], c);
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, int count) {
+    if ((int)threadIdx.x < count) p[threadIdx.x] = 0ull;
+}
+
 template <bool GA>
 void launch_fwd(int px, dim3 grid, hipStream_t st, int tiles_x, int bw, int img_w, int img_h,
                 const int32_t *gids, const int2 *bins, const GaussPack *pack, float *img,
@@ -454,5 +499,20 @@ extern "C" int lig_rasterize_backward(int n, int img_height, int img_width, int 
                            final_idx, v_output, v_wsum, (const float *)nullptr,
                            (const float *)nullptr, (const float *)nullptr, has_opacities, v_xy,
                            v_xy_abs, v_conic, v_colors, v_opacity);
+    return launch_status();
+}
+
+extern "C" int lig_count_pairs(int n, int img_height, int img_width, const float *xys,
+                               const float *extents, const float *conics,
+                               const float *opacities, const int32_t *num_tiles_hit,
+                               unsigned long long *counts, void *stream) {
+    if (n < 0 || img_height <= 0 || img_width <= 0 || !counts) return LIG_ERR_INVALID_ARGUMENT;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, counts, 2);
+    if (n == 0) return launch_status();
+    if (!xys || !extents || !conics || !num_tiles_hit) return LIG_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(count_pairs_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n,
+                       img_height, img_width, xys, extents, conics, opacities, num_tiles_hit,
+                       counts);
     return launch_status();
 }

@@ -1,0 +1,208 @@
+"""Fit harness: the callers of the gsplat2d path.
+
+`Level` is one resolution level parameterised by Cholesky factors (the form
+the workloads use): project_gaussians_cholesky -> rasterize_gaussians ->
+clamp(0, 1) -> MSE -> backward -> Adam.  `Gaussian2D` and `LIG` mirror the
+reference's model classes (gaussianlig.py:9-111: covariance parameters,
+rgbs initialised to zero, the level/point split), and `two_level_fit`
+follows the two-level flow of train.py:92-158 (downsampled level-1 target,
+normalised residual target for level 2, de-normalised composition).
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional
+
+import torch
+import torch.nn as nn
+import torch.nn.functional as F
+
+from . import gsplat2d as g
+from .data import init_gaussians
+
+
+def _adam(params, lr):
+    try:
+        return torch.optim.Adam(params, lr=lr, fused=True)
+    except (RuntimeError, TypeError):
+        return torch.optim.Adam(params, lr=lr, foreach=True)
+
+
+class Level(nn.Module):
+    """One level: n Gaussians on an H x W canvas, Cholesky parameterisation."""
+
+    def __init__(self, n: int, H: int, W: int, seed: int, scale: float = 1.0, lr: float = 1e-3,
+                 block_width: int = 16, device="cuda"):
+        super().__init__()
+        means, chol, rgbs = init_gaussians(n, H, W, seed, scale, device)
+        self.means = nn.Parameter(means)
+        self.cholesky = nn.Parameter(chol)
+        self.rgbs = nn.Parameter(rgbs)
+        self.H, self.W, self.block_width = H, W, block_width
+        self.optimizer = _adam([self.rgbs, self.means, self.cholesky], lr)
+
+    def project(self):
+        return g.project_gaussians_cholesky(self.cholesky, self.means, self.H, self.W,
+                                            self.block_width)
+
+    def render(self, gradient_aware: bool = False):
+        """Returns the raw [H, W, 3] sum image (and dx, dy, dxy when asked)."""
+        xys, ext, con, nt = self.project()
+        out = g.rasterize_gaussians(xys, ext, con, nt, self.rgbs, None, self.H, self.W,
+                                    self.block_width, compute_upscale_gradients=gradient_aware)
+        return out if gradient_aware else out[0]
+
+    def step(self, target_hwc: torch.Tensor) -> torch.Tensor:
+        """One optimiser step against an [H, W, 3] target; returns the loss
+        tensor (no host synchronisation beyond the binning's count)."""
+        img = torch.clamp(self.render(), 0, 1)
+        loss = F.mse_loss(img, target_hwc)
+        loss.backward()
+        self.optimizer.step()
+        self.optimizer.zero_grad(set_to_none=True)
+        return loss.detach()
+
+
+def psnr(img: torch.Tensor, target: torch.Tensor) -> float:
+    mse = F.mse_loss(img.float(), target.float()).item()
+    return 10.0 * math.log10(1.0 / max(mse, 1e-20))
+
+
+def downsample(img_hwc: torch.Tensor, factor: int, mode: str = "bicubic") -> torch.Tensor:
+    x = img_hwc.permute(2, 0, 1)[None]
+    H, W = img_hwc.shape[0] // factor, img_hwc.shape[1] // factor
+    kw = dict(antialias=True, align_corners=False) if mode == "bicubic" else {}
+    y = F.interpolate(x, size=(H, W), mode=mode, **kw)
+    return y[0].permute(1, 2, 0).contiguous()
+
+
+def upscale_render(level: Level, H: int, W: int) -> torch.Tensor:
+    """Level render upscaled to H x W with the gradient-aware Hermite upscaler."""
+    img, wsum, dx, dy, dxy = level.render(gradient_aware=True)
+    return g.gradient_aware_upscale(img, dx, dy, dxy, H, W)
+
+
+def residual_target(gt_hwc: torch.Tensor, prev_up: torch.Tensor):
+    """train.py:113-135: residual + 0.5, min-max normalised; returns
+    (target, store_min, store_max)."""
+    t = gt_hwc - prev_up + 0.5
+    lo, hi = t.min(), t.max()
+    return (t - lo) / (hi - lo), lo, hi
+
+
+def compose(level2_img: torch.Tensor, prev_up: torch.Tensor, lo, hi) -> torch.Tensor:
+    """train.py:150-154 / 187-196: de-normalise the level-2 render and add it
+    to the upscaled level-1 estimate."""
+    return torch.clamp(level2_img * (hi - lo) + prev_up - 0.5 + lo, 0, 1)
+
+
+def two_level_fit(gt_hwc: torch.Tensor, n1: int, n2: int, steps: int, seed: int = 0,
+                  scale: float = 1.0, lr: float = 1e-3, factor: int = 4, device="cuda"):
+    """Plain two-level fit; returns (level1, level2, final PSNR)."""
+    H, W = gt_hwc.shape[:2]
+    l1 = Level(n1, H // factor, W // factor, seed, scale, lr, device=device)
+    t1 = downsample(gt_hwc, factor)
+    for _ in range(steps):
+        l1.step(t1)
+    with torch.no_grad():
+        up = upscale_render(l1, H, W)
+        t2, lo, hi = residual_target(gt_hwc, torch.clamp(up, 0, 1))
+    l2 = Level(n2, H, W, seed + 1, scale, lr, device=device)
+    for _ in range(steps):
+        l2.step(t2)
+    with torch.no_grad():
+        out = compose(torch.clamp(l2.render(), 0, 1), torch.clamp(up, 0, 1), lo, hi)
+    return l1, l2, psnr(out, gt_hwc)
+
+
+# ---------------------------------------------------------------------------
+# Mirrors of the reference's model classes (gaussianlig.py:9-111)
+# ---------------------------------------------------------------------------
+
+class Gaussian2D(nn.Module):
+    """Covariance-parameterised level exactly as gaussianlig.py:39-111: means
+    uniform, cov2d ~U(0,1)^3 (may be indefinite), rgbs = 0, Adam + StepLR."""
+
+    def __init__(self, loss_type="L2", **kwargs):
+        super().__init__()
+        self.loss_type = loss_type
+        self.init_num_points = kwargs["num_points"]
+        self.H, self.W = kwargs["H"], kwargs["W"]
+        self.B_SIZE = 16
+        self.device = kwargs["device"]
+        self.last_size = (self.H, self.W)
+        w_init = torch.rand(self.init_num_points, 1, device=self.device) * self.W
+        h_init = torch.rand(self.init_num_points, 1, device=self.device) * self.H
+        self.means = nn.Parameter(torch.cat((w_init, h_init), dim=1))
+        self.cov2d = nn.Parameter(torch.rand(self.init_num_points, 3, device=self.device))
+        self.rgbs = nn.Parameter(torch.zeros(self.init_num_points, 3, device=self.device))
+        self.optimizer = torch.optim.Adam([self.rgbs, self.means, self.cov2d], lr=kwargs["lr"])
+        self.scheduler = torch.optim.lr_scheduler.StepLR(self.optimizer, step_size=70000,
+                                                         gamma=0.7)
+
+    def forward(self):
+        xys, extents, conics, num_tiles_hit = g.project_gaussians(self.cov2d, self.means,
+                                                                  self.H, self.W, self.B_SIZE)
+        out_img = g.rasterize_gaussians(xys, extents, conics, num_tiles_hit, self.rgbs, None,
+                                        self.H, self.W, self.B_SIZE)[0][..., :3]
+        out_img = torch.clamp(out_img, 0, 1)
+        out_img = out_img.view(-1, self.H, self.W, 3).permute(0, 3, 1, 2).contiguous()
+        return {"render": out_img}
+
+    def train_iter(self, gt_image):
+        image = self.forward()["render"]
+        loss = F.mse_loss(image, gt_image)
+        loss.backward()
+        with torch.no_grad():
+            mse_loss = F.mse_loss(image, gt_image)
+            p = 10 * math.log10(1.0 / mse_loss.item())
+        self.optimizer.step()
+        self.optimizer.zero_grad(set_to_none=True)
+        self.scheduler.step()
+        return loss, p
+
+
+class LIG(nn.Module):
+    """Level split of gaussianlig.py:9-37: level s is H * 2^(s+1-n) and gets
+    num_points * 4^(s+1-n) * allo_ratio points; the last level takes the rest."""
+
+    def __init__(self, loss_type="L2", **kwargs):
+        super().__init__()
+        self.loss_type = loss_type
+        self.init_num_points = kwargs["num_points"]
+        self.H, self.W = kwargs["H"], kwargs["W"]
+        self.n_scales = kwargs["n_scales"]
+        self.allo_ratio = kwargs["allo_ratio"]
+        self.level_models: List[Gaussian2D] = []
+        self.store_min: List[torch.Tensor] = []
+        self.store_max: List[torch.Tensor] = []
+        for s in range(self.n_scales):
+            H = int(self.H * pow(2.0, -self.n_scales + s + 1))
+            W = int(self.W * pow(2.0, -self.n_scales + s + 1))
+            if s != self.n_scales - 1:
+                num_points = int(kwargs["num_points"] * pow(2.0, (-self.n_scales + s + 1) * 2)
+                                 * self.allo_ratio)
+            else:
+                num_points = kwargs["num_points"]
+                for i in range(s):
+                    num_points -= int(kwargs["num_points"] *
+                                      pow(2.0, (-self.n_scales + i + 1) * 2) * self.allo_ratio)
+            self.level_models.append(Gaussian2D(loss_type="L2", num_points=num_points, H=H, W=W,
+                                                device=kwargs["device"], lr=kwargs["lr"]))
+
+
+def save_checkpoint(path: str, levels: List[nn.Module], store_min: Optional[list] = None,
+                    store_max: Optional[list] = None) -> None:
+    """train.py:172-176 format: {'state_dict': [per level], 'store_max', 'store_min'}."""
+    torch.save({"state_dict": [m.state_dict() for m in levels],
+                "store_max": store_max or [], "store_min": store_min or []}, path)
+
+
+def load_checkpoint(path: str, levels: List[nn.Module], map_location=None):
+    """train.py:51-67: per-level state dicts by key; returns (store_min, store_max)."""
+    ck = torch.load(path, map_location=map_location, weights_only=True)
+    for level, sd in zip(levels, ck["state_dict"]):
+        model_dict = level.state_dict()
+        model_dict.update({k: v for k, v in sd.items() if k in model_dict})
+        level.load_state_dict(model_dict)
+    return ck["store_min"], ck["store_max"]

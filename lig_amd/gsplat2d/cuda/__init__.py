@@ -322,3 +322,18 @@ def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_s
                L.ptr(v_output_dxy), int(bool(has_opacities)), L.ptr(v_xy), L.ptr(v_xy_abs),
                L.ptr(v_conic), L.ptr(v_colors), L.ptr(v_opacity), L.stream_of(like))
     return v_xy, v_xy_abs, v_conic, v_colors, v_opacity
+
+
+def count_pairs(xys, extents, conics, opacities, num_tiles_hit, img_height, img_width):
+    """(E, C) workload counters (see lig_count_pairs in include/lig_abi.h)."""
+    for n_, t in (("xys", xys), ("extents", extents), ("conics", conics),
+                  ("opacities", opacities)):
+        _check(n_, t)
+    _check("num_tiles_hit", num_tiles_hit, _i32)
+    out = torch.empty(2, dtype=torch.int64, device=xys.device)
+    with torch.cuda.device(xys.device):
+        L.call("lig_count_pairs", int(xys.size(0)), int(img_height), int(img_width), L.ptr(xys),
+               L.ptr(extents), L.ptr(conics), L.ptr(opacities), L.ptr(num_tiles_hit),
+               L.ptr(out), L.stream_of(xys))
+    e, c = out.tolist()
+    return int(e), int(c)
