@@ -114,13 +114,17 @@ int lig_get_tile_bin_edges(int num_intersects, const int64_t *isect_ids_sorted, 
  * ceil(log2(tiles)) key bits that vary, and builds the bins.  cum_tiles_hit is
  * the inclusive scan of num_tiles_hit and num_intersects its last element.
  * -> gaussian_ids_sorted [num_intersects] int32 in (tile, emission) order,
- * tile_bins [tiles,2].  Workspace: lig_bin_workspace_size(num_intersects, tiles). */
+ * tile_bins [tiles,2], and, when emission_ids_sorted is not NULL, the emission
+ * slot of each sorted intersection (Gaussian g's slots are [cum[g-1], cum[g]);
+ * the deterministic backward needs them).
+ * Workspace: lig_bin_workspace_size(num_intersects, tiles). */
 size_t lig_bin_workspace_size(int num_intersects, int num_tiles);
 int lig_bin_and_sort_gaussians(int n, int num_intersects, const float *xys,
                                const float *extents, const int32_t *cum_tiles_hit,
                                int img_height, int img_width, int block_width,
                                int32_t *gaussian_ids_sorted, int32_t *tile_bins,
-                               void *workspace, size_t workspace_bytes, void *stream);
+                               int32_t *emission_ids_sorted, void *workspace,
+                               size_t workspace_bytes, void *stream);
 
 /* ---- rasterisation ------------------------------------------------------- */
 
@@ -149,17 +153,27 @@ int lig_rasterize_forward(int n, int img_height, int img_width, int block_width,
 
 /* Replaces rasterize_backward_tensor (csrc/bindings.cu:403-506, kernels
  * csrc/backward.cu:8-183 and csrc/gradient_aware_rasterizer.cu:118-349).
- * v_output [H,W,3], v_wsum [H,W]; v_output_dx/dy/dxy [H,W,3] or all NULL (plain).
- * -> v_xy [n,2], v_xy_abs [n,2], v_conic [n,3], v_colors [n,3], v_opacity [n]
- * (written only when has_opacities).  `packed` holds the records of
- * lig_pack_gaussians for the same Gaussians. */
+ * v_output [H,W,3], v_wsum [H,W] or NULL (= zeros); v_output_dx/dy/dxy
+ * [H,W,3] or all NULL (plain).  -> v_xy [n,2], v_xy_abs [n,2], v_conic [n,3],
+ * v_colors [n,3], v_opacity [n] (written only when has_opacities).  `packed`
+ * holds the records of lig_pack_gaussians for the same Gaussians.
+ * Two reductions:
+ *   emission_ids_sorted == NULL: each Gaussian's per-tile sum is added with
+ *     float atomics (one set per Gaussian per tile); summation order varies.
+ *   emission_ids_sorted != NULL (from lig_bin_and_sort_gaussians, with its
+ *     cum_tiles_hit and num_intersects): per-tile sums are written to the
+ *     workspace (lig_rasterize_backward_workspace_size(num_intersects) bytes)
+ *     and reduced per Gaussian in slot order: bit-reproducible, no atomics. */
+size_t lig_rasterize_backward_workspace_size(int num_intersects);
 int lig_rasterize_backward(int n, int img_height, int img_width, int block_width,
                            const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
                            const void *packed, const int32_t *final_idx,
                            const float *v_output, const float *v_wsum,
                            const float *v_output_dx, const float *v_output_dy,
-                           const float *v_output_dxy, int has_opacities, float *v_xy,
-                           float *v_xy_abs, float *v_conic, float *v_colors,
+                           const float *v_output_dxy, int has_opacities,
+                           const int32_t *emission_ids_sorted, const int32_t *cum_tiles_hit,
+                           int num_intersects, void *workspace, size_t workspace_bytes,
+                           float *v_xy, float *v_xy_abs, float *v_conic, float *v_colors,
                            float *v_opacity, void *stream);
 
 /* No reference counterpart: workload counters for roofline accounting.

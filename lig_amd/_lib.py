@@ -40,16 +40,17 @@ _SIGNATURES = {
     "lig_get_tile_bin_edges": (_c_int, [_c_int, _vp, _c_int, _c_int, _vp, _vp]),
     "lig_bin_workspace_size": (_c_size, [_c_int, _c_int]),
     "lig_bin_and_sort_gaussians": (
-        _c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _c_size,
-                 _vp]),
+        _c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                 _c_size, _vp]),
     "lig_packed_gaussians_bytes": (_c_size, [_c_int]),
     "lig_pack_gaussians": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_rasterize_forward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                  _vp, _vp, _vp, _vp, _vp]),
+    "lig_rasterize_backward_workspace_size": (_c_size, [_c_int]),
     "lig_rasterize_backward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                 _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+                 _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_count_pairs": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_gradient_aware_upscale_forward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,

@@ -312,11 +312,14 @@ __global__ __launch_bounds__(kEmitThreads) void emit_isect_i64_kernel(
         }
 }
 
-// Rasterisation layout: 32-bit tile key, gid value.
+// Rasterisation layout: 32-bit tile key; the value is the emission slot (so
+// the sorted order can be mapped back to each Gaussian's contiguous slot
+// range), and slot_gid records the Gaussian id of each slot (0 for the slots
+// of zero-extent Gaussians, as the reference's zero fill leaves them).
 __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
     int n, int m, const float *__restrict__ xys, const float *__restrict__ extents,
     const int32_t *__restrict__ cum, int tiles_x, int tiles_y, int block_width,
-    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    uint32_t *__restrict__ keys, uint32_t *__restrict__ slots, int32_t *__restrict__ slot_gid) {
     const int g = blockIdx.x * kEmitThreads + threadIdx.x;
     if (g >= n) return;
     int cur = g == 0 ? 0 : cum[g - 1];
@@ -326,7 +329,8 @@ __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
     if (ex <= 0.0f || ey <= 0.0f) {
         for (; cur < end; ++cur) {
             keys[cur] = 0;
-            vals[cur] = 0;
+            slots[cur] = (uint32_t)cur;
+            slot_gid[cur] = 0;
         }
         return;
     }
@@ -337,9 +341,30 @@ __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
         for (int x = x0; x < x1; ++x) {
             if (cur >= end) return;
             keys[cur] = row + (uint32_t)x;
-            vals[cur] = (uint32_t)g;
+            slots[cur] = (uint32_t)cur;
+            slot_gid[cur] = g;
             ++cur;
         }
+    }
+}
+
+// Bin edges from the sorted tile keys plus the sorted Gaussian ids.
+__global__ void finish_bins_kernel(int m, const uint32_t *__restrict__ keys_sorted,
+                                   const uint32_t *__restrict__ slots_sorted,
+                                   const int32_t *__restrict__ slot_gid, int num_tiles,
+                                   int32_t *__restrict__ bins, int32_t *__restrict__ gids_sorted) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    gids_sorted[i] = slot_gid[slots_sorted[i]];
+    const int cur = (int)keys_sorted[i];
+    if (cur >= num_tiles) return;
+    if (i == 0) bins[2 * cur] = 0;
+    if (i == m - 1) bins[2 * cur + 1] = m;
+    if (i == 0) return;
+    const int prev = (int)keys_sorted[i - 1];
+    if (prev != cur) {
+        if (prev < num_tiles) bins[2 * prev + 1] = i;
+        bins[2 * cur] = i;
     }
 }
 
@@ -451,19 +476,20 @@ extern "C" int lig_get_tile_bin_edges(int num_intersects, const int64_t *isect_i
     return launch_status();
 }
 
-// Workspace layout of the fused path: keys, vals (emission), sorted keys,
-// then the sort's own workspace.
+// Workspace layout of the fused path: tile keys, emission slots and slot
+// gids (emission order), sorted keys, sorted slots (unless the caller keeps
+// them), then the sort's own workspace.
 extern "C" size_t lig_bin_workspace_size(int num_intersects, int num_tiles) {
     const size_t m = num_intersects > 0 ? (size_t)num_intersects : 0;
-    return 3 * align_up(m * 4) + sort_ws_bytes<uint32_t>(m, bits_for(num_tiles)) + 256;
+    return 5 * align_up(m * 4) + sort_ws_bytes<uint32_t>(m, bits_for(num_tiles)) + 256;
 }
 
 extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float *xys,
                                           const float *extents, const int32_t *cum_tiles_hit,
                                           int img_height, int img_width, int block_width,
                                           int32_t *gaussian_ids_sorted, int32_t *tile_bins,
-                                          void *workspace, size_t workspace_bytes,
-                                          void *stream) {
+                                          int32_t *emission_ids_sorted, void *workspace,
+                                          size_t workspace_bytes, void *stream) {
     if (n < 0 || num_intersects < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 ||
         block_width > 16 || !tile_bins)
         return LIG_ERR_INVALID_ARGUMENT;
@@ -479,20 +505,26 @@ extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float
         return LIG_ERR_INVALID_ARGUMENT;
     const size_t m = (size_t)num_intersects;
     char *p = reinterpret_cast<char *>(workspace);
-    uint32_t *keys = reinterpret_cast<uint32_t *>(p);
-    p += align_up(m * 4);
-    uint32_t *vals = reinterpret_cast<uint32_t *>(p);
-    p += align_up(m * 4);
-    uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(p);
-    p += align_up(m * 4);
+    auto take = [&p, m]() {
+        char *q = p;
+        p += align_up(m * 4);
+        return q;
+    };
+    uint32_t *keys = reinterpret_cast<uint32_t *>(take());
+    uint32_t *slots = reinterpret_cast<uint32_t *>(take());
+    int32_t *slot_gid = reinterpret_cast<int32_t *>(take());
+    uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(take());
+    uint32_t *slots_sorted = reinterpret_cast<uint32_t *>(take());
+    if (emission_ids_sorted) slots_sorted = reinterpret_cast<uint32_t *>(emission_ids_sorted);
     hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(ceil_div(n, kEmitThreads)), dim3(kEmitThreads),
                        0, st, n, num_intersects, xys, extents, cum_tiles_hit, tx, ty,
-                       block_width, keys, vals);
-    int rc = radix_sort<uint32_t>(keys, vals, keys_sorted,
-                                  reinterpret_cast<uint32_t *>(gaussian_ids_sorted), m,
-                                  bits_for(nt), 0u, p, st);
+                       block_width, keys, slots, slot_gid);
+    int rc = radix_sort<uint32_t>(keys, slots, keys_sorted, slots_sorted, m, bits_for(nt), 0u, p,
+                                  st);
     if (rc != LIG_OK) return rc;
-    hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(m, 256)), dim3(256), 0, st,
-                       num_intersects, keys_sorted, nt, tile_bins);
+    hipLaunchKernelGGL(finish_bins_kernel, dim3(ceil_div(m, 256)), dim3(256), 0, st,
+                       num_intersects, (const uint32_t *)keys_sorted,
+                       (const uint32_t *)slots_sorted, (const int32_t *)slot_gid, nt, tile_bins,
+                       gaussian_ids_sorted);
     return launch_status();
 }

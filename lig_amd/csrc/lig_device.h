@@ -62,22 +62,28 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
 }
 
 // Per-Gaussian record the rasteriser kernels read (48 bytes, 16-byte aligned):
-//   p0 = (mean.x, mean.y, A, B)   p1 = (C, opacity, red, green)
+//   p0 = (mean.x, mean.y, a', b')   p1 = (c', opacity, red, green)
 //   p2 = (blue, a, b, c)
-// with (a, b, c) the conic and A = a*log2(e)/2, B = b*log2(e), C = c*log2(e)/2
-// so that sigma*log2(e) = dx*(A*dx + B*dy) + C*dy*dy and alpha = o*2^-(that).
+// with (a, b, c) the conic and (a', b', c') = K * (a, b, c), K = log2(e)/2, so
+// that with d = mean - pixel, u' = a'dx + b'dy, v' = b'dx + c'dy:
+//   sigma * log2(e) = dx*u' + dy*v'   and   alpha = o * 2^-(dx*u' + dy*v').
 struct __align__(16) GaussPack {
     float4 p0, p1, p2;
 };
 
 constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kHalfLog2e = 0.5f * kLog2e;
 constexpr float kAlphaMin = 1.0f / 255.0f;
 
-// sigma * log2(e) of the pair, evaluated with explicit FMAs so that every
-// kernel that calls it computes bit-identical values.  d = mean - pixel.
-__device__ __forceinline__ float pair_s2(float A, float B, float C, float dx, float dy) {
-    const float q = __builtin_fmaf(A, dx, B * dy);
-    return __builtin_fmaf(dx, q, (C * dy) * dy);
+// sigma*log2(e) of a pair and the scaled conic products u', v', evaluated with
+// explicit FMAs so that every kernel computes bit-identical values for the
+// same pair.  ap_dx = a'*dx and bp_dx = b'*dx are passed in so callers can
+// share them between pixels of one column.
+__device__ __forceinline__ float pair_s2(float ap_dx, float bp_dx, float bp, float cp,
+                                         float dx, float dy, float &u, float &v) {
+    u = __builtin_fmaf(bp, dy, ap_dx);
+    v = __builtin_fmaf(cp, dy, bp_dx);
+    return __builtin_fmaf(dx, u, dy * v);
 }
 
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }

@@ -40,8 +40,8 @@ __global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restric
     const float a = conics[3 * i], b = conics[3 * i + 1], c = conics[3 * i + 2];
     const float o = opacities ? opacities[i] : 1.0f;
     GaussPack g;
-    g.p0 = make_float4(xys[2 * i], xys[2 * i + 1], a * (0.5f * kLog2e), b * kLog2e);
-    g.p1 = make_float4(c * (0.5f * kLog2e), o, colors[3 * i], colors[3 * i + 1]);
+    g.p0 = make_float4(xys[2 * i], xys[2 * i + 1], a * kHalfLog2e, b * kHalfLog2e);
+    g.p1 = make_float4(c * kHalfLog2e, o, colors[3 * i], colors[3 * i + 1]);
     g.p2 = make_float4(colors[3 * i + 2], a, b, c);
     pack[i] = g;
 }
@@ -101,9 +101,12 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
             const int kidx = base + t;
 #pragma unroll
             for (int k = 0; k < PX; ++k) {
+                // pixels of one lane share a column when bw == 16 (px[k] equal),
+                // so the compiler can share dx, a'dx and b'dx between them
                 const float dx = a0.x - px[k];
                 const float dy = a0.y - py[k];
-                const float s = pair_s2(a0.z, a0.w, a1.x, dx, dy);
+                float u_, v_;
+                const float s = pair_s2(a0.z * dx, a0.w * dx, a0.w, a1.x, dx, dy, u_, v_);
                 const float e = fast_exp2(-s);
                 float al = fminf(a1.y * e, 0.999f);
                 const bool ok = !(s < 0.0f) && !(al < kAlphaMin);
@@ -164,36 +167,17 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
     }
 }
 
-// Upstream gradients of one pixel as staged in LDS.
-struct PixPlain {
-    float4 v;  // (d/dI r, g, b, d/dW)
-};
-
+// Stage one tile's upstream gradients in LDS: sv = (dL/dI rgb, dL/dW), sfin =
+// final index (INT32_MIN outside the image).  GA adds dL/d(dx, dy, dxy).
 template <bool GA>
-__global__ __launch_bounds__(kThreads) void raster_bwd_kernel(
-    int tiles_x, int bw, int img_w, int img_h, const int32_t *__restrict__ gids,
-    const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
-    const int32_t *__restrict__ final_idx, const float *__restrict__ v_out,
-    const float *__restrict__ v_wsum, const float *__restrict__ v_dx,
-    const float *__restrict__ v_dy, const float *__restrict__ v_dxy, int has_opac,
-    float *__restrict__ g_xy, float *__restrict__ g_xy_abs, float *__restrict__ g_conic,
-    float *__restrict__ g_rgb, float *__restrict__ g_opac) {
-    __shared__ float4 sv[kMaxPix];
-    __shared__ int sfin[kMaxPix];
-    __shared__ float4 sa[GA ? kMaxPix : 1];  // (vdx r, g, b, vdy r)
-    __shared__ float4 sb[GA ? kMaxPix : 1];  // (vdy g, b, vdxy r, g)
-    __shared__ float sc[GA ? kMaxPix : 1];   // vdxy b
-
-    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
-    const int lane = threadIdx.x;
-    const int2 range = bins[tile];
-    if (range.x >= range.y) return;
-    const int ty = tile / tiles_x;
-    const int tx = tile - ty * tiles_x;
+__device__ __forceinline__ void stage_tile_grads(
+    int bw, int x_org, int y_org, int img_w, int img_h, const int32_t *__restrict__ final_idx,
+    const float *__restrict__ v_out, const float *__restrict__ v_wsum,
+    const float *__restrict__ v_dx, const float *__restrict__ v_dy,
+    const float *__restrict__ v_dxy, float4 *sv, int *sfin, float4 *sa, float4 *sb,
+    float *sc) {
     const int npix = bw * bw;
-    const int x_org = tx * bw, y_org = ty * bw;
-
-    for (int p = lane; p < npix; p += kThreads) {
+    for (int p = threadIdx.x; p < npix; p += kThreads) {
         const int ly = p / bw;
         const int lx = p - ly * bw;
         const int i = y_org + ly;
@@ -221,61 +205,255 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_kernel(
         }
     }
     __syncthreads();
+}
 
-    constexpr float kClamp = GA ? 0.999f : 0.99f;
+// Per-intersection gradient record (deterministic path): 48 bytes,
+//   r0 = (v_r, v_g, v_b, v_conic.a)  r1 = (v_conic.b, v_conic.c, v_x, v_y)
+//   r2 = (|v_x|, |v_y|, v_opacity, gid bits)
+// written at the intersection's emission slot, so that each Gaussian's
+// records sit contiguously at [cum[g-1], cum[g]) in tile row-major order.
+struct __align__(16) GradRec {
+    float4 r0, r1, r2;
+};
+
+__device__ __forceinline__ void emit_grads(bool det, GradRec *__restrict__ rec, int slot, int g,
+                                           float vr, float vg, float vb, float c0, float c1,
+                                           float c2, float gx, float gy, float ax, float ay,
+                                           float vo, bool has_opac, float *__restrict__ g_xy,
+                                           float *__restrict__ g_xy_abs,
+                                           float *__restrict__ g_conic,
+                                           float *__restrict__ g_rgb,
+                                           float *__restrict__ g_opac) {
+    if (det) {
+        GradRec r;
+        r.r0 = make_float4(vr, vg, vb, c0);
+        r.r1 = make_float4(c1, c2, gx, gy);
+        r.r2 = make_float4(ax, ay, vo, __int_as_float(g));
+        rec[slot] = r;
+    } else {
+        unsafeAtomicAdd(&g_rgb[3 * g + 0], vr);
+        unsafeAtomicAdd(&g_rgb[3 * g + 1], vg);
+        unsafeAtomicAdd(&g_rgb[3 * g + 2], vb);
+        unsafeAtomicAdd(&g_conic[3 * g + 0], c0);
+        unsafeAtomicAdd(&g_conic[3 * g + 1], c1);
+        unsafeAtomicAdd(&g_conic[3 * g + 2], c2);
+        unsafeAtomicAdd(&g_xy[2 * g + 0], gx);
+        unsafeAtomicAdd(&g_xy[2 * g + 1], gy);
+        unsafeAtomicAdd(&g_xy_abs[2 * g + 0], ax);
+        unsafeAtomicAdd(&g_xy_abs[2 * g + 1], ay);
+        if (has_opac) unsafeAtomicAdd(&g_opac[g], vo);
+    }
+}
+
+// Sum each Gaussian's records in slot order (tile row-major) and write its
+// gradients.  Slots of a zero-extent Gaussian hold records of Gaussian 0 (the
+// reference bins them as (tile 0, gid 0)); their sums go to `phantom` and are
+// added to Gaussian 0 by add_phantom_kernel.
+__global__ __launch_bounds__(256) void reduce_records_kernel(
+    int n, int m, const int32_t *__restrict__ cum, const GradRec *__restrict__ rec,
+    int has_opac, float *__restrict__ g_xy, float *__restrict__ g_xy_abs,
+    float *__restrict__ g_conic, float *__restrict__ g_rgb, float *__restrict__ g_opac,
+    float *__restrict__ phantom) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= n) return;
+    const int lo = max(0, g == 0 ? 0 : cum[g - 1]);
+    const int hi = min(cum[g], m);
+    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+    float s2x = 0.f, s2y = 0.f, s2z = 0.f;
+    bool foreign = false;
+    for (int e = lo; e < hi; ++e) {
+        const GradRec r = rec[e];
+        foreign = __float_as_int(r.r2.w) != g;
+        s0.x += r.r0.x; s0.y += r.r0.y; s0.z += r.r0.z; s0.w += r.r0.w;
+        s1.x += r.r1.x; s1.y += r.r1.y; s1.z += r.r1.z; s1.w += r.r1.w;
+        s2x += r.r2.x; s2y += r.r2.y; s2z += r.r2.z;
+    }
+    if (foreign) {
+        const float v[11] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w, s2x, s2y, s2z};
+#pragma unroll
+        for (int q = 0; q < 11; ++q) unsafeAtomicAdd(&phantom[q], v[q]);
+        s0 = make_float4(0.f, 0.f, 0.f, 0.f);
+        s1 = s0;
+        s2x = s2y = s2z = 0.f;
+    }
+    g_rgb[3 * g + 0] = s0.x;
+    g_rgb[3 * g + 1] = s0.y;
+    g_rgb[3 * g + 2] = s0.z;
+    g_conic[3 * g + 0] = s0.w;
+    g_conic[3 * g + 1] = s1.x;
+    g_conic[3 * g + 2] = s1.y;
+    g_xy[2 * g + 0] = s1.z;
+    g_xy[2 * g + 1] = s1.w;
+    g_xy_abs[2 * g + 0] = s2x;
+    g_xy_abs[2 * g + 1] = s2y;
+    if (has_opac) g_opac[g] = s2z;
+}
+
+__global__ void add_phantom_kernel(const float *__restrict__ phantom, int has_opac,
+                                   float *__restrict__ g_xy, float *__restrict__ g_xy_abs,
+                                   float *__restrict__ g_conic, float *__restrict__ g_rgb,
+                                   float *__restrict__ g_opac) {
+    if (threadIdx.x != 0) return;
+    g_rgb[0] += phantom[0];
+    g_rgb[1] += phantom[1];
+    g_rgb[2] += phantom[2];
+    g_conic[0] += phantom[3];
+    g_conic[1] += phantom[4];
+    g_conic[2] += phantom[5];
+    g_xy[0] += phantom[6];
+    g_xy[1] += phantom[7];
+    g_xy_abs[0] += phantom[8];
+    g_xy_abs[1] += phantom[9];
+    if (has_opac) g_opac[0] += phantom[10];
+}
+
+// Plain backward (backward.cu:101-150 per pair).  Per lane and pixel:
+//   v_rgb += alpha * dL/dI,  v_alpha = c . dL/dI + dL/dW,  v_sigma = -alpha v_alpha,
+//   v_conic += (v_sigma dx^2 / 2, v_sigma dx dy, v_sigma dy^2 / 2),
+//   v_xy += v_sigma (a dx + b dy, b dx + c dy),  v_xy_abs += |that|,
+//   v_opacity += vis * v_alpha.
+// v_xy and v_xy_abs are accumulated in the scaled conic (u', v' = K u, K v,
+// shared with the alpha evaluation) and divided by K once at the end; v_conic
+// accumulates the moments v_sigma dx^2, v_sigma dx dy, v_sigma dy^2.
+template <bool HAS_OPAC, bool DET>
+__global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
+    int tiles_x, int bw, int img_w, int img_h, const int32_t *__restrict__ gids,
+    const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
+    const GaussPack *__restrict__ pack, const int32_t *__restrict__ final_idx,
+    const float *__restrict__ v_out, const float *__restrict__ v_wsum,
+    GradRec *__restrict__ rec, float *__restrict__ g_xy, float *__restrict__ g_xy_abs,
+    float *__restrict__ g_conic, float *__restrict__ g_rgb, float *__restrict__ g_opac) {
+    __shared__ float4 sv[kMaxPix];
+    __shared__ int sfin[kMaxPix];
+
+    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    const int2 range = bins[tile];
+    if (range.x >= range.y) return;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
+    const int x_org = tx * bw, y_org = ty * bw;
+    stage_tile_grads<false>(bw, x_org, y_org, img_w, img_h, final_idx, v_out, v_wsum, nullptr,
+                            nullptr, nullptr, sv, sfin, nullptr, nullptr, nullptr);
+
+    constexpr float kClamp = 0.99f;
+    constexpr float kInvK = 1.0f / kHalfLog2e;
     for (int base = range.x; base < range.y; base += kThreads) {
-        const int kidx = base + lane;
-        const bool act = kidx < range.y;
-        const int g = act ? gids[kidx] : 0;
+        const bool act = base + lane < range.y;
+        const int kidx = act ? base + lane : INT32_MAX;
+        const int g = act ? gids[base + lane] : 0;
         const GaussPack gp = pack[g];
-        const float mx = gp.p0.x, my = gp.p0.y, A = gp.p0.z, B = gp.p0.w;
-        const float C = gp.p1.x, op = gp.p1.y, cr = gp.p1.z, cg = gp.p1.w;
+        const float mx = gp.p0.x, my = gp.p0.y, ap = gp.p0.z, bp = gp.p0.w;
+        const float cp = gp.p1.x, op = gp.p1.y, cr = gp.p1.z, cg = gp.p1.w;
+        const float cb = gp.p2.x;
+
+        float a_r = 0.f, a_g = 0.f, a_b = 0.f;
+        float sxx = 0.f, sxy = 0.f, syy = 0.f;
+        float sgx = 0.f, sgy = 0.f, sax = 0.f, say = 0.f, a_o = 0.f;
+
+        for (int ly = 0; ly < bw; ++ly) {
+            const float dy = my - ((float)(y_org + ly) + 0.5f);
+            for (int lx = 0; lx < bw; ++lx) {
+                const int p = ly * bw + lx;
+                const int fp = sfin[p];
+                if (fp < base) continue;  // wave-uniform: nothing in this batch reaches p
+                const float dx = mx - ((float)(x_org + lx) + 0.5f);
+                float u, v;
+                const float s = pair_s2(ap * dx, bp * dx, bp, cp, dx, dy, u, v);
+                const float vis = fast_exp2(-s);
+                const float al0 = fminf(HAS_OPAC ? op * vis : vis, kClamp);
+                const bool ok = kidx <= fp && !(s < 0.0f) && !(al0 < kAlphaMin);
+                const float al = ok ? al0 : 0.0f;
+                const float4 w = sv[p];
+                const float va =
+                    __builtin_fmaf(cr, w.x, __builtin_fmaf(cg, w.y, __builtin_fmaf(cb, w.z, w.w)));
+                a_r = __builtin_fmaf(al, w.x, a_r);
+                a_g = __builtin_fmaf(al, w.y, a_g);
+                a_b = __builtin_fmaf(al, w.z, a_b);
+                if constexpr (HAS_OPAC) a_o = __builtin_fmaf(ok ? vis : 0.0f, va, a_o);
+                const float vsig = -al * va;
+                const float t1 = vsig * dx;
+                const float t2 = vsig * dy;
+                sxx = __builtin_fmaf(t1, dx, sxx);
+                sxy = __builtin_fmaf(t1, dy, sxy);
+                syy = __builtin_fmaf(t2, dy, syy);
+                const float gx = vsig * u;
+                const float gy = vsig * v;
+                sgx += gx;
+                sgy += gy;
+                sax += fabsf(gx);
+                say += fabsf(gy);
+            }
+        }
+        if (act)
+            emit_grads(DET, rec, DET ? slots[base + lane] : 0, g, a_r, a_g, a_b, 0.5f * sxx, sxy,
+                       0.5f * syy, sgx * kInvK, sgy * kInvK, sax * kInvK, say * kInvK, a_o,
+                       HAS_OPAC, g_xy, g_xy_abs, g_conic, g_rgb, g_opac);
+    }
+}
+
+// Gradient-aware backward (gradient_aware_rasterizer.cu:201-315 per pair),
+// d' = pixel - mean, sigma_x = a d'x + b d'y, sigma_y = b d'x + c d'y.
+template <bool DET>
+__global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
+    int tiles_x, int bw, int img_w, int img_h, const int32_t *__restrict__ gids,
+    const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
+    const GaussPack *__restrict__ pack,
+    const int32_t *__restrict__ final_idx, const float *__restrict__ v_out,
+    const float *__restrict__ v_wsum, const float *__restrict__ v_dx,
+    const float *__restrict__ v_dy, const float *__restrict__ v_dxy, int has_opac,
+    GradRec *__restrict__ rec, float *__restrict__ g_xy, float *__restrict__ g_xy_abs,
+    float *__restrict__ g_conic, float *__restrict__ g_rgb, float *__restrict__ g_opac) {
+    __shared__ float4 sv[kMaxPix];
+    __shared__ int sfin[kMaxPix];
+    __shared__ float4 sa[kMaxPix];  // (vdx r, g, b, vdy r)
+    __shared__ float4 sb[kMaxPix];  // (vdy g, b, vdxy r, g)
+    __shared__ float sc[kMaxPix];   // vdxy b
+
+    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    const int2 range = bins[tile];
+    if (range.x >= range.y) return;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
+    const int x_org = tx * bw, y_org = ty * bw;
+    stage_tile_grads<true>(bw, x_org, y_org, img_w, img_h, final_idx, v_out, v_wsum, v_dx,
+                           v_dy, v_dxy, sv, sfin, sa, sb, sc);
+
+    constexpr float kClamp = 0.999f;
+    for (int base = range.x; base < range.y; base += kThreads) {
+        const bool act = base + lane < range.y;
+        const int kidx = act ? base + lane : INT32_MAX;
+        const int g = act ? gids[base + lane] : 0;
+        const GaussPack gp = pack[g];
+        const float mx = gp.p0.x, my = gp.p0.y, ap = gp.p0.z, bp = gp.p0.w;
+        const float cp = gp.p1.x, op = gp.p1.y, cr = gp.p1.z, cg = gp.p1.w;
         const float cb = gp.p2.x, ka = gp.p2.y, kb = gp.p2.z, kc = gp.p2.w;
 
         float a_r = 0.f, a_g = 0.f, a_b = 0.f;
         float a_c0 = 0.f, a_c1 = 0.f, a_c2 = 0.f;
         float a_x = 0.f, a_y = 0.f, a_ax = 0.f, a_ay = 0.f, a_o = 0.f;
 
-        for (int p = 0; p < npix; ++p) {
-            const int fp = sfin[p];
-            if (fp < base) continue;  // wave-uniform: nothing in this batch reaches p
-            const int ly = p / bw;
-            const int lx = p - ly * bw;
-            const float ppx = (float)(x_org + lx) + 0.5f;
-            const float ppy = (float)(y_org + ly) + 0.5f;
-            const float dx = mx - ppx;
-            const float dy = my - ppy;
-            const float s = pair_s2(A, B, C, dx, dy);
-            const float vis = fast_exp2(-s);
-            const float al0 = fminf(op * vis, kClamp);
-            const bool ok = act && kidx <= fp && !(s < 0.0f) && !(al0 < kAlphaMin);
-            const float al = ok ? al0 : 0.0f;
-            const float vs_ = ok ? vis : 0.0f;
-            const float4 v = sv[p];
-            if constexpr (!GA) {
-                const float v_alpha =
-                    __builtin_fmaf(cr, v.x, __builtin_fmaf(cg, v.y, __builtin_fmaf(cb, v.z, v.w)));
-                a_r = __builtin_fmaf(al, v.x, a_r);
-                a_g = __builtin_fmaf(al, v.y, a_g);
-                a_b = __builtin_fmaf(al, v.z, a_b);
-                a_o = __builtin_fmaf(vs_, v_alpha, a_o);
-                const float vsig = -al * v_alpha;
-                const float hdx = 0.5f * vsig * dx;
-                a_c0 = __builtin_fmaf(hdx, dx, a_c0);
-                a_c1 = __builtin_fmaf(vsig * dx, dy, a_c1);
-                a_c2 = __builtin_fmaf(0.5f * vsig * dy, dy, a_c2);
-                const float gx = vsig * __builtin_fmaf(ka, dx, kb * dy);
-                const float gy = vsig * __builtin_fmaf(kb, dx, kc * dy);
-                a_x += gx;
-                a_y += gy;
-                a_ax += fabsf(gx);
-                a_ay += fabsf(gy);
-            } else {
+        for (int ly = 0; ly < bw; ++ly) {
+            const float dy = my - ((float)(y_org + ly) + 0.5f);
+            for (int lx = 0; lx < bw; ++lx) {
+                const int p = ly * bw + lx;
+                const int fp = sfin[p];
+                if (fp < base) continue;
+                const float dx = mx - ((float)(x_org + lx) + 0.5f);
+                float u, v;
+                const float s = pair_s2(ap * dx, bp * dx, bp, cp, dx, dy, u, v);
+                const float vis = fast_exp2(-s);
+                const float al0 = fminf(op * vis, kClamp);
+                const bool ok = kidx <= fp && !(s < 0.0f) && !(al0 < kAlphaMin);
+                const float al = ok ? al0 : 0.0f;
+                const float vs_ = ok ? vis : 0.0f;
+                const float4 w = sv[p];
                 const float4 wa = sa[p];
                 const float4 wb = sb[p];
                 const float wc = sc[p];
-                // upstream: vI = v.xyz, vW = v.w, vdx = wa.xyz, vdy = (wa.w, wb.x, wb.y),
-                // vdxy = (wb.z, wb.w, wc).  d' = p - mean.
+                // upstream: vI = w.xyz, vW = w.w, vdx = wa.xyz, vdy = (wa.w, wb.x, wb.y),
+                // vdxy = (wb.z, wb.w, wc).
                 const float ex = -dx, ey = -dy;
                 const float sx = __builtin_fmaf(ka, ex, kb * ey);
                 const float sy = __builtin_fmaf(kb, ex, kc * ey);
@@ -283,10 +461,10 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_kernel(
                 const float aly = -al * sy;
                 const float P = __builtin_fmaf(sx, sy, -kb);
                 const float alxy = al * P;
-                a_r += v.x * al + wa.x * alx + wa.w * aly + wb.z * alxy;
-                a_g += v.y * al + wa.y * alx + wb.x * aly + wb.w * alxy;
-                a_b += v.z * al + wa.z * alx + wb.y * aly + wc * alxy;
-                const float va = __builtin_fmaf(v.x, cr, __builtin_fmaf(v.y, cg, __builtin_fmaf(v.z, cb, v.w)));
+                a_r += w.x * al + wa.x * alx + wa.w * aly + wb.z * alxy;
+                a_g += w.y * al + wa.y * alx + wb.x * aly + wb.w * alxy;
+                a_b += w.z * al + wa.z * alx + wb.y * aly + wc * alxy;
+                const float va = __builtin_fmaf(w.x, cr, __builtin_fmaf(w.y, cg, __builtin_fmaf(w.z, cb, w.w)));
                 const float vax = __builtin_fmaf(wa.x, cr, __builtin_fmaf(wa.y, cg, wa.z * cb));
                 const float vay = __builtin_fmaf(wa.w, cr, __builtin_fmaf(wb.x, cg, wb.y * cb));
                 const float vaxy = __builtin_fmaf(wb.z, cr, __builtin_fmaf(wb.w, cg, wc * cb));
@@ -309,19 +487,10 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_kernel(
                 a_o = __builtin_fmaf(vs_, va, a_o);
             }
         }
-        if (act) {
-            unsafeAtomicAdd(&g_rgb[3 * g + 0], a_r);
-            unsafeAtomicAdd(&g_rgb[3 * g + 1], a_g);
-            unsafeAtomicAdd(&g_rgb[3 * g + 2], a_b);
-            unsafeAtomicAdd(&g_conic[3 * g + 0], a_c0);
-            unsafeAtomicAdd(&g_conic[3 * g + 1], a_c1);
-            unsafeAtomicAdd(&g_conic[3 * g + 2], a_c2);
-            unsafeAtomicAdd(&g_xy[2 * g + 0], a_x);
-            unsafeAtomicAdd(&g_xy[2 * g + 1], a_y);
-            unsafeAtomicAdd(&g_xy_abs[2 * g + 0], a_ax);
-            unsafeAtomicAdd(&g_xy_abs[2 * g + 1], a_ay);
-            if (has_opac) unsafeAtomicAdd(&g_opac[g], a_o);
-        }
+        if (act)
+            emit_grads(DET, rec, DET ? slots[base + lane] : 0, g, a_r, a_g, a_b, a_c0, a_c1,
+                       a_c2, a_x, a_y, a_ax, a_ay, a_o, has_opac != 0, g_xy, g_xy_abs, g_conic,
+                       g_rgb, g_opac);
     }
 }
 
@@ -454,15 +623,21 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
     return launch_status();
 }
 
+extern "C" size_t lig_rasterize_backward_workspace_size(int num_intersects) {
+    return num_intersects > 0 ? (size_t)num_intersects * sizeof(GradRec) + 256 : 256;
+}
+
 extern "C" int lig_rasterize_backward(int n, int img_height, int img_width, int block_width,
                                       const int32_t *gaussian_ids_sorted,
                                       const int32_t *tile_bins, const void *packed,
                                       const int32_t *final_idx, const float *v_output,
                                       const float *v_wsum, const float *v_output_dx,
                                       const float *v_output_dy, const float *v_output_dxy,
-                                      int has_opacities, float *v_xy, float *v_xy_abs,
-                                      float *v_conic, float *v_colors, float *v_opacity,
-                                      void *stream) {
+                                      int has_opacities, const int32_t *emission_ids_sorted,
+                                      const int32_t *cum_tiles_hit, int num_intersects,
+                                      void *workspace, size_t workspace_bytes, float *v_xy,
+                                      float *v_xy_abs, float *v_conic, float *v_colors,
+                                      float *v_opacity, void *stream) {
     if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
         return LIG_ERR_INVALID_ARGUMENT;
     const bool ga = v_output_dx != nullptr;
@@ -471,34 +646,77 @@ extern "C" int lig_rasterize_backward(int n, int img_height, int img_width, int 
     if (!gaussian_ids_sorted || !tile_bins || !packed || !final_idx || !v_output || !v_xy ||
         !v_xy_abs || !v_conic || !v_colors || (has_opacities && !v_opacity))
         return LIG_ERR_INVALID_ARGUMENT;
+    const bool det = emission_ids_sorted != nullptr;
+    if (det) {
+        if (!cum_tiles_hit || num_intersects < 0) return LIG_ERR_INVALID_ARGUMENT;
+        if (!workspace || workspace_bytes < lig_rasterize_backward_workspace_size(num_intersects))
+            return LIG_ERR_WORKSPACE;
+    }
     hipStream_t st = as_stream(stream);
     const size_t nn = (size_t)n;
-    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(2 * nn, 256)), dim3(256), 0, st, v_xy, 2 * nn);
-    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(2 * nn, 256)), dim3(256), 0, st, v_xy_abs,
-                       2 * nn);
-    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(3 * nn, 256)), dim3(256), 0, st, v_conic,
-                       3 * nn);
-    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(3 * nn, 256)), dim3(256), 0, st, v_colors,
-                       3 * nn);
-    if (has_opacities)
-        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, v_opacity,
-                           nn);
+    GradRec *rec = nullptr;
+    float *phantom = nullptr;
+    if (det) {
+        phantom = reinterpret_cast<float *>(workspace);
+        rec = reinterpret_cast<GradRec *>(reinterpret_cast<char *>(workspace) + 256);
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, st, phantom, (size_t)16);
+    } else {
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(2 * nn, 256)), dim3(256), 0, st, v_xy,
+                           2 * nn);
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(2 * nn, 256)), dim3(256), 0, st,
+                           v_xy_abs, 2 * nn);
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(3 * nn, 256)), dim3(256), 0, st,
+                           v_conic, 3 * nn);
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(3 * nn, 256)), dim3(256), 0, st,
+                           v_colors, 3 * nn);
+        if (has_opacities)
+            hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st,
+                               v_opacity, nn);
+    }
     const int tiles_x = (img_width + block_width - 1) / block_width;
     const int tiles_y = (img_height + block_width - 1) / block_width;
     const dim3 grid((unsigned)(tiles_x * tiles_y));
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
     const GaussPack *pk = reinterpret_cast<const GaussPack *>(packed);
-    if (ga)
-        hipLaunchKernelGGL(raster_bwd_kernel<true>, grid, dim3(kThreads), 0, st, tiles_x,
-                           block_width, img_width, img_height, gaussian_ids_sorted, bins, pk,
-                           final_idx, v_output, v_wsum, v_output_dx, v_output_dy, v_output_dxy,
+    const int32_t *slots = emission_ids_sorted;
+#define LIG_BWD_ARGS_PLAIN                                                                    \
+    tiles_x, block_width, img_width, img_height, gaussian_ids_sorted, slots, bins, pk, final_idx, \
+        v_output, v_wsum, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity
+#define LIG_BWD_ARGS_GA                                                                      \
+    tiles_x, block_width, img_width, img_height, gaussian_ids_sorted, slots, bins, pk, final_idx, \
+        v_output, v_wsum, v_output_dx, v_output_dy, v_output_dxy, has_opacities, rec, v_xy,    \
+        v_xy_abs, v_conic, v_colors, v_opacity
+    if (ga) {
+        if (det)
+            hipLaunchKernelGGL(raster_bwd_ga_kernel<true>, grid, dim3(kThreads), 0, st,
+                               LIG_BWD_ARGS_GA);
+        else
+            hipLaunchKernelGGL(raster_bwd_ga_kernel<false>, grid, dim3(kThreads), 0, st,
+                               LIG_BWD_ARGS_GA);
+    } else if (has_opacities) {
+        if (det)
+            hipLaunchKernelGGL((raster_bwd_plain_kernel<true, true>), grid, dim3(kThreads), 0,
+                               st, LIG_BWD_ARGS_PLAIN);
+        else
+            hipLaunchKernelGGL((raster_bwd_plain_kernel<true, false>), grid, dim3(kThreads), 0,
+                               st, LIG_BWD_ARGS_PLAIN);
+    } else {
+        if (det)
+            hipLaunchKernelGGL((raster_bwd_plain_kernel<false, true>), grid, dim3(kThreads), 0,
+                               st, LIG_BWD_ARGS_PLAIN);
+        else
+            hipLaunchKernelGGL((raster_bwd_plain_kernel<false, false>), grid, dim3(kThreads), 0,
+                               st, LIG_BWD_ARGS_PLAIN);
+    }
+#undef LIG_BWD_ARGS_PLAIN
+#undef LIG_BWD_ARGS_GA
+    if (det) {
+        hipLaunchKernelGGL(reduce_records_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, n,
+                           num_intersects, cum_tiles_hit, (const GradRec *)rec, has_opacities, v_xy, v_xy_abs,
+                           v_conic, v_colors, v_opacity, phantom);
+        hipLaunchKernelGGL(add_phantom_kernel, dim3(1), dim3(64), 0, st, (const float *)phantom,
                            has_opacities, v_xy, v_xy_abs, v_conic, v_colors, v_opacity);
-    else
-        hipLaunchKernelGGL(raster_bwd_kernel<false>, grid, dim3(kThreads), 0, st, tiles_x,
-                           block_width, img_width, img_height, gaussian_ids_sorted, bins, pk,
-                           final_idx, v_output, v_wsum, (const float *)nullptr,
-                           (const float *)nullptr, (const float *)nullptr, has_opacities, v_xy,
-                           v_xy_abs, v_conic, v_colors, v_opacity);
+    }
     return launch_status();
 }
 

@@ -232,9 +232,10 @@ def sort_isect_pairs(isect_ids, gaussian_ids):
 
 
 def bin_and_sort_fast(num_points, num_intersects, xys, extents, cum_tiles_hit, img_height,
-                      img_width, block_width):
+                      img_width, block_width, with_emission_ids=False):
     """Fused emit + tile-key radix sort + bins for the all-zero-depth path.
-    -> (gaussian_ids_sorted [I] int32, tile_bins [T,2] int32)."""
+    -> (gaussian_ids_sorted [I] int32, tile_bins [T,2] int32), plus the emission
+    slot of each sorted intersection [I] int32 when with_emission_ids."""
     _check("xys", xys)
     _check("extents", extents)
     _check("cum_tiles_hit", cum_tiles_hit, _i32)
@@ -242,13 +243,14 @@ def bin_and_sort_fast(num_points, num_intersects, xys, extents, cum_tiles_hit, i
     ty = (img_height + block_width - 1) // block_width
     gids = _empty((num_intersects,), xys, _i32)
     bins = _empty((tx * ty, 2), xys, _i32)
+    eids = _empty((num_intersects,), xys, _i32) if with_emission_ids else None
     ws = _workspace(L.size_query("lig_bin_workspace_size", int(num_intersects), tx * ty), xys)
     with torch.cuda.device(xys.device):
         L.call("lig_bin_and_sort_gaussians", int(num_points), int(num_intersects), L.ptr(xys),
                L.ptr(extents), L.ptr(cum_tiles_hit), int(img_height), int(img_width),
-               int(block_width), L.ptr(gids), L.ptr(bins), L.ptr(ws), ws.numel(),
+               int(block_width), L.ptr(gids), L.ptr(bins), L.ptr(eids), L.ptr(ws), ws.numel(),
                L.stream_of(xys))
-    return gids, bins
+    return (gids, bins, eids) if with_emission_ids else (gids, bins)
 
 
 def pack_gaussians(xys, conics, colors, opacities):
@@ -295,19 +297,26 @@ def rasterize_forward_packed(img_height, img_width, block_width, gaussian_ids_so
 def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_sorted, tile_bins,
                               packed, num_points, final_idx, v_output, v_render_wsum,
                               has_opacities, v_output_dx=None, v_output_dy=None,
-                              v_output_dxy=None):
-    """Backward over packed records; v_render_wsum may be None (zeros)."""
+                              v_output_dxy=None, emission_ids_sorted=None, cum_tiles_hit=None):
+    """Backward over packed records; v_render_wsum may be None (zeros).  With
+    emission_ids_sorted and cum_tiles_hit (from bin_and_sort_fast) the
+    per-Gaussian sums are reduced deterministically; without, by atomics."""
     v_output, v_render_wsum, v_output_dx, v_output_dy, v_output_dxy = map(
         _contig, (v_output, v_render_wsum, v_output_dx, v_output_dy, v_output_dxy))
     for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),
                       ("tile_bins", tile_bins, _i32), ("final_idx", final_idx, _i32),
                       ("v_output", v_output, _f32), ("v_render_wsum", v_render_wsum, _f32),
                       ("v_output_dx", v_output_dx, _f32), ("v_output_dy", v_output_dy, _f32),
-                      ("v_output_dxy", v_output_dxy, _f32)):
+                      ("v_output_dxy", v_output_dxy, _f32),
+                      ("emission_ids_sorted", emission_ids_sorted, _i32),
+                      ("cum_tiles_hit", cum_tiles_hit, _i32)):
         _check(n_, t, dt)
     ga = v_output_dx is not None
     if ga and (v_output_dy is None or v_output_dxy is None):
         raise RuntimeError("v_output_dx, v_output_dy and v_output_dxy go together")
+    det = emission_ids_sorted is not None
+    if det and cum_tiles_hit is None:
+        raise RuntimeError("emission_ids_sorted needs cum_tiles_hit")
     n = int(num_points)
     like = v_output
     v_xy = _empty((n, 2), like)
@@ -315,12 +324,17 @@ def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_s
     v_conic = _empty((n, 3), like)
     v_colors = _empty((n, 3), like)
     v_opacity = _empty((n,), like) if has_opacities else None
+    m = int(emission_ids_sorted.numel()) if det else 0
+    ws = _workspace(L.size_query("lig_rasterize_backward_workspace_size", m), like) if det \
+        else None
     with torch.cuda.device(like.device):
         L.call("lig_rasterize_backward", n, int(img_height), int(img_width), int(block_width),
                L.ptr(gaussian_ids_sorted), L.ptr(tile_bins), L.ptr(packed), L.ptr(final_idx),
                L.ptr(v_output), L.ptr(v_render_wsum), L.ptr(v_output_dx), L.ptr(v_output_dy),
-               L.ptr(v_output_dxy), int(bool(has_opacities)), L.ptr(v_xy), L.ptr(v_xy_abs),
-               L.ptr(v_conic), L.ptr(v_colors), L.ptr(v_opacity), L.stream_of(like))
+               L.ptr(v_output_dxy), int(bool(has_opacities)), L.ptr(emission_ids_sorted),
+               L.ptr(cum_tiles_hit), m, L.ptr(ws), ws.numel() if det else 0, L.ptr(v_xy),
+               L.ptr(v_xy_abs), L.ptr(v_conic), L.ptr(v_colors), L.ptr(v_opacity),
+               L.stream_of(like))
     return v_xy, v_xy_abs, v_conic, v_colors, v_opacity
 
 

@@ -63,7 +63,7 @@ class _RasterizeGaussians(Function):
         num_points = xys.size(0)
         dev = xys.device
         num_intersects, cum_tiles_hit = compute_cumulative_intersects(num_tiles_hit)
-        packed = None
+        packed = emission_ids = None
         if num_intersects < 1:
             out_img = torch.ones(img_height, img_width, colors.shape[-1], device=dev)
             out_wsum = torch.zeros(img_height, img_width, device=dev)
@@ -74,9 +74,9 @@ class _RasterizeGaussians(Function):
             tile_bins = torch.zeros(0, 2, device=dev)
             final_idx = torch.zeros(img_height, img_width, device=dev)
         else:
-            gaussian_ids_sorted, tile_bins = _C.bin_and_sort_fast(
+            gaussian_ids_sorted, tile_bins, emission_ids = _C.bin_and_sort_fast(
                 num_points, num_intersects, xys, extents, cum_tiles_hit, img_height, img_width,
-                block_width)
+                block_width, with_emission_ids=True)
             (out_img, out_wsum, out_dx, out_dy, out_dxy, final_idx,
              packed) = _C.rasterize_forward_packed(img_height, img_width, block_width,
                                                    gaussian_ids_sorted, tile_bins, xys, conics,
@@ -88,6 +88,8 @@ class _RasterizeGaussians(Function):
         ctx.block_width = block_width
         ctx.compute_upscale_gradients = compute_upscale_gradients
         ctx.packed = packed
+        ctx.emission_ids = emission_ids
+        ctx.cum_tiles_hit = cum_tiles_hit
         ctx.save_for_backward(gaussian_ids_sorted, tile_bins, xys, conics, colors, opacities,
                               final_idx)
         return out_img, out_wsum, out_dx, out_dy, out_dxy
@@ -120,7 +122,8 @@ class _RasterizeGaussians(Function):
             v_xy, v_xy_abs, v_conic, v_colors, v_opacity = _C.rasterize_backward_packed(
                 H, W, ctx.block_width, gaussian_ids_sorted, tile_bins, packed, xys.size(0),
                 final_idx, v_out_img, v_out_wsum, opacities is not None,
-                v_out_dx if ga else None, v_out_dy if ga else None, v_out_dxy if ga else None)
-        ctx.packed = None
+                v_out_dx if ga else None, v_out_dy if ga else None, v_out_dxy if ga else None,
+                emission_ids_sorted=ctx.emission_ids, cum_tiles_hit=ctx.cum_tiles_hit)
+        ctx.packed = ctx.emission_ids = ctx.cum_tiles_hit = None
         xys.absgrad = v_xy_abs
         return (v_xy, None, v_conic, None, v_colors, v_opacity, None, None, None, None)
