@@ -46,3 +46,46 @@ class KernelTimer:
     def mean_ms(self) -> float:
         t = self.times_ms()
         return sum(t) / len(t) if t else float("nan")
+
+
+def _short(name: str) -> str:
+    """Kernel name without namespaces, argument list and template noise."""
+    base = name.replace("(anonymous namespace)::", "").split("(")[0]
+    for pre in ("void ", "lig::", "at::native::"):
+        base = base.replace(pre, "")
+    return base.strip()
+
+
+def summarize_rocpd(db_path: str, top: int = 25, by_grid: bool = True) -> str:
+    """Per-kernel statistics from a rocprofv3 --kernel-trace database, grouped
+    by (kernel, grid size) so launches of one kernel at different image sizes
+    are reported separately.  Durations in microseconds."""
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    rows = con.execute("select name, duration, grid_x, grid_y, grid_z from kernels").fetchall()
+    groups = {}
+    for name, dur, gx, gy, gz in rows:
+        key = (_short(name), gx * gy * gz if by_grid else 0)
+        groups.setdefault(key, []).append(dur / 1e3)
+    total = sum(sum(v) for v in groups.values())
+    lines = [f"{'kernel':<48} {'threads':>10} {'calls':>6} {'avg_us':>10} {'min_us':>10} "
+             f"{'max_us':>10} {'total_ms':>9} {'pct':>6}"]
+    for (k, grid), v in sorted(groups.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        lines.append(f"{k[:48]:<48} {grid:>10} {len(v):>6} {sum(v) / len(v):>10.1f} "
+                     f"{min(v):>10.1f} {max(v):>10.1f} {sum(v) / 1e3:>9.3f} "
+                     f"{100 * sum(v) / total:>5.1f}%")
+    lines.append(f"total kernel time {total / 1e3:.3f} ms over {len(rows)} dispatches")
+    return "\n".join(lines)
+
+
+if __name__ == "__main__":
+    import argparse
+    import glob
+    ap = argparse.ArgumentParser(description="summarise a rocprofv3 kernel-trace database")
+    ap.add_argument("path", help="rocprofv3 output directory or *_results.db file")
+    ap.add_argument("--top", type=int, default=25)
+    a = ap.parse_args()
+    dbs = [a.path] if a.path.endswith(".db") else sorted(
+        glob.glob(f"{a.path}/**/*_results.db", recursive=True))
+    for db in dbs:
+        print(summarize_rocpd(db, a.top))
