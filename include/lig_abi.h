@@ -128,7 +128,7 @@ int lig_bin_and_sort_gaussians(int n, int num_intersects, const float *xys,
 
 /* ---- rasterisation ------------------------------------------------------- */
 
-/* No reference counterpart: builds the per-Gaussian records (48 bytes each,
+/* No reference counterpart: builds the per-Gaussian records (64 bytes each,
  * lig_packed_gaussians_bytes(n)) that both rasteriser kernels read, from
  * xys [n,2], conics [n,3], colors [n,3] and opacities [n] or NULL.
  * lig_rasterize_forward calls it itself; a backward that is not handed the

@@ -127,52 +127,61 @@ void scan_i32(const int32_t *in, int32_t *out, size_t n, bool inclusive, int32_t
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort of (key, uint32 value) pairs, 8-bit digits.
+// Stable LSD radix sort of (key, uint32 value) pairs, RB-bit digits (RB <= 10).
 // Each 256-thread block owns 4096 consecutive pairs: a histogram kernel
 // counts digits per block (digit-major table), an exclusive scan turns the
 // table into output offsets, and a scatter kernel ranks each key stably
-// (wave ballots for lane order, LDS for wave and round order).
+// (wave ballots for lane order, LDS counters for wave and round order).
+// A NULL value array stands for the identity (value = input index).
 // ---------------------------------------------------------------------------
 constexpr int kSortThreads = 256;
 constexpr int kSortRounds = 16;
 constexpr int kSortTile = kSortThreads * kSortRounds;
 constexpr int kSortWaves = kSortThreads / LIG_WAVE;
 
-template <typename K>
+template <typename K, int RB>
 __device__ __forceinline__ unsigned digit_of(K key, int shift, K flip) {
-    return (unsigned)(((key ^ flip) >> shift) & (K)0xFF);
+    return (unsigned)(((key ^ flip) >> shift) & (K)((1u << RB) - 1));
 }
 
-template <typename K>
+template <typename K, int RB>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const K *__restrict__ keys,
                                                                   int n, int shift, K flip,
                                                                   int nblocks,
                                                                   int32_t *__restrict__ hist) {
-    __shared__ int h[256];
-    h[threadIdx.x] = 0;
+    constexpr int NB = 1 << RB;
+    __shared__ int h[NB];
+    for (int d = threadIdx.x; d < NB; d += kSortThreads) h[d] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * kSortTile;
 #pragma unroll 4
     for (int r = 0; r < kSortRounds; ++r) {
         const size_t i = base + (size_t)r * kSortThreads + threadIdx.x;
-        if (i < (size_t)n) atomicAdd(&h[digit_of(keys[i], shift, flip)], 1);
+        if (i < (size_t)n) atomicAdd(&h[digit_of<K, RB>(keys[i], shift, flip)], 1);
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < NB; d += kSortThreads)
+        hist[(size_t)d * nblocks + blockIdx.x] = h[d];
 }
 
-template <typename K>
+template <typename K, int RB>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
     K *__restrict__ keys_out, uint32_t *__restrict__ vals_out,
     const int32_t *__restrict__ offsets, int n, int shift, K flip, int nblocks) {
-    __shared__ int run[256];
-    __shared__ int wcnt[kSortWaves][256];
+    constexpr int NB = 1 << RB;
+    __shared__ int run[NB];
+    __shared__ int wcnt[kSortWaves][NB];
     const int tid = threadIdx.x;
     const int lane = tid & (LIG_WAVE - 1);
     const int wave = tid / LIG_WAVE;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (LIG_WAVE - lane));
-    run[tid] = offsets[(size_t)tid * nblocks + blockIdx.x];
+    for (int d = tid; d < NB; d += kSortThreads) {
+        run[d] = offsets[(size_t)d * nblocks + blockIdx.x];
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) wcnt[w][d] = 0;
+    }
+    __syncthreads();
     const size_t base = (size_t)blockIdx.x * kSortTile;
     for (int r = 0; r < kSortRounds; ++r) {
         const size_t i = base + (size_t)r * kSortThreads + tid;
@@ -182,21 +191,20 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
         uint32_t val = 0;
         if (valid) {
             key = keys_in[i];
-            val = vals_in[i];
+            val = vals_in ? vals_in[i] : (uint32_t)i;
         }
-        const unsigned d = valid ? digit_of(key, shift, flip) : 0u;
-#pragma unroll
-        for (int w = 0; w < kSortWaves; ++w) wcnt[w][tid] = 0;
-        __syncthreads();
+        const unsigned d = valid ? digit_of<K, RB>(key, shift, flip) : 0u;
         uint64_t m = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < RB; ++b) {
             const int bit = (d >> b) & 1;
             const uint64_t bb = __ballot(bit);
             m &= bit ? bb : ~bb;
         }
         const int rank = __popcll(m & lt_mask);
-        if (valid && rank == 0) wcnt[wave][d] = __popcll(m);
+        const bool leader = valid && rank == 0;
+        const int cnt = __popcll(m);
+        if (leader) wcnt[wave][d] = cnt;
         __syncthreads();
         if (valid) {
             int pos = run[d] + rank;
@@ -206,10 +214,11 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             vals_out[pos] = val;
         }
         __syncthreads();
-        int add = 0;
-#pragma unroll
-        for (int w = 0; w < kSortWaves; ++w) add += wcnt[w][tid];
-        run[tid] += add;
+        // leaders advance the digit's running offset and clear their counter
+        if (leader) {
+            atomicAdd(&run[d], cnt);
+            wcnt[wave][d] = 0;
+        }
         __syncthreads();
     }
 }
@@ -220,29 +229,55 @@ __global__ void copy_pairs_kernel(const K *__restrict__ ki, const uint32_t *__re
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < (size_t)n) {
         ko[i] = ki[i];
-        vo[i] = vi[i];
+        vo[i] = vi ? vi[i] : (uint32_t)i;
     }
 }
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int sort_passes(int key_bits) { return (key_bits + 7) / 8; }
+// Digit plan for `key_bits` bits: the fewest passes of at most 10 bits, bits
+// split evenly (18 -> 2 x 9, 14 -> 2 x 7, 10 -> 1 x 10); 64-bit keys use 8 x 8.
+void digit_plan(int key_bits, int *passes, int *rb) {
+    if (key_bits > 32) {
+        *passes = (key_bits + 7) / 8;
+        *rb = 8;
+        return;
+    }
+    *passes = (key_bits + 9) / 10;
+    *rb = *passes ? (key_bits + *passes - 1) / *passes : 0;
+    if (*rb < 4 && *passes) *rb = 4;
+}
 
 template <typename K>
 size_t sort_ws_bytes(size_t m, int key_bits) {
+    int passes, rb;
+    digit_plan(key_bits, &passes, &rb);
     const size_t nblocks = (m + kSortTile - 1) / kSortTile;
-    const size_t nh = 256 * (nblocks ? nblocks : 1);
+    const size_t nh = ((size_t)1 << (rb > 0 ? rb : 1)) * (nblocks ? nblocks : 1);
     return align_up(m * sizeof(K)) + align_up(m * 4) + align_up(nh * 4) +
            align_up(scan_ws_elems(nh) * 4) + 256;
 }
 
-// Sorts m pairs by the low `key_bits` bits of the key (all 64 with sign
-// flip when `signed64`).  Input is read only; the result lands in out.
+template <typename K, int RB>
+void sort_pass(const K *ksrc, const uint32_t *vsrc, K *kdst, uint32_t *vdst, size_t m, int shift,
+               K flip, int nblocks, int32_t *hist, int32_t *scan_ws, hipStream_t st) {
+    hipLaunchKernelGGL((radix_hist_kernel<K, RB>), dim3(nblocks), dim3(kSortThreads), 0, st, ksrc,
+                       (int)m, shift, flip, nblocks, hist);
+    scan_i32(hist, hist, ((size_t)1 << RB) * nblocks, false, scan_ws, st);
+    hipLaunchKernelGGL((radix_scatter_kernel<K, RB>), dim3(nblocks), dim3(kSortThreads), 0, st,
+                       ksrc, vsrc, kdst, vdst, (const int32_t *)hist, (int)m, shift, flip,
+                       nblocks);
+}
+
+// Sorts m pairs by the low `key_bits` bits of the key (XOR `flip` first: the
+// sign bit for signed 64-bit keys).  Input is read only; vals_in may be NULL
+// (identity values); the result lands in keys_out / vals_out.
 template <typename K>
 int radix_sort(const K *keys_in, const uint32_t *vals_in, K *keys_out, uint32_t *vals_out,
                size_t m, int key_bits, K flip, char *ws, hipStream_t st) {
     if (m == 0) return LIG_OK;
-    const int passes = sort_passes(key_bits);
+    int passes, rb;
+    digit_plan(key_bits, &passes, &rb);
     if (passes == 0) {
         hipLaunchKernelGGL(copy_pairs_kernel<K>, dim3(ceil_div(m, 256)), dim3(256), 0, st,
                            keys_in, vals_in, keys_out, vals_out, (int)m);
@@ -255,7 +290,7 @@ int radix_sort(const K *keys_in, const uint32_t *vals_in, K *keys_out, uint32_t 
     uint32_t *vtmp = reinterpret_cast<uint32_t *>(p);
     p += align_up(m * 4);
     int32_t *hist = reinterpret_cast<int32_t *>(p);
-    p += align_up((size_t)256 * nblocks * 4);
+    p += align_up(((size_t)1 << rb) * nblocks * 4);
     int32_t *scan_ws = reinterpret_cast<int32_t *>(p);
     const K *ksrc = keys_in;
     const uint32_t *vsrc = vals_in;
@@ -263,13 +298,16 @@ int radix_sort(const K *keys_in, const uint32_t *vals_in, K *keys_out, uint32_t 
         const bool to_out = ((passes - 1 - pass) % 2) == 0;
         K *kdst = to_out ? keys_out : ktmp;
         uint32_t *vdst = to_out ? vals_out : vtmp;
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(radix_hist_kernel<K>, dim3(nblocks), dim3(kSortThreads), 0, st, ksrc,
-                           (int)m, shift, flip, nblocks, hist);
-        scan_i32(hist, hist, (size_t)256 * nblocks, false, scan_ws, st);
-        hipLaunchKernelGGL(radix_scatter_kernel<K>, dim3(nblocks), dim3(kSortThreads), 0, st,
-                           ksrc, vsrc, kdst, vdst, (const int32_t *)hist, (int)m, shift, flip,
-                           nblocks);
+        const int shift = rb * pass;
+        switch (rb) {
+            case 4: sort_pass<K, 4>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 5: sort_pass<K, 5>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 6: sort_pass<K, 6>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 7: sort_pass<K, 7>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 8: sort_pass<K, 8>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 9: sort_pass<K, 9>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
+            default: sort_pass<K, 10>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
+        }
         ksrc = kdst;
         vsrc = vdst;
     }
@@ -312,38 +350,53 @@ __global__ __launch_bounds__(kEmitThreads) void emit_isect_i64_kernel(
         }
 }
 
-// Rasterisation layout: 32-bit tile key; the value is the emission slot (so
-// the sorted order can be mapped back to each Gaussian's contiguous slot
-// range), and slot_gid records the Gaussian id of each slot (0 for the slots
-// of zero-extent Gaussians, as the reference's zero fill leaves them).
+// Rasterisation layout: 32-bit tile key per emission slot, plus slot_gid, the
+// Gaussian id of each slot (0 for the slots of zero-extent Gaussians, as the
+// reference's zero fill leaves them).  The sort's values are the slots
+// themselves (implicit identity), so the sorted order maps back to each
+// Gaussian's contiguous slot range.  Each wave computes the tile rectangles
+// of its 64 Gaussians, then writes their slots one Gaussian at a time with
+// all 64 lanes, so the stores are coalesced.
 __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
     int n, int m, const float *__restrict__ xys, const float *__restrict__ extents,
     const int32_t *__restrict__ cum, int tiles_x, int tiles_y, int block_width,
-    uint32_t *__restrict__ keys, uint32_t *__restrict__ slots, int32_t *__restrict__ slot_gid) {
-    const int g = blockIdx.x * kEmitThreads + threadIdx.x;
-    if (g >= n) return;
-    int cur = g == 0 ? 0 : cum[g - 1];
-    const int end = min(cum[g], m);
-    if (cur >= end) return;
-    const float ex = extents[2 * g], ey = extents[2 * g + 1];
-    if (ex <= 0.0f || ey <= 0.0f) {
-        for (; cur < end; ++cur) {
-            keys[cur] = 0;
-            slots[cur] = (uint32_t)cur;
-            slot_gid[cur] = 0;
+    uint32_t *__restrict__ keys, int32_t *__restrict__ slot_gid) {
+    const int lane = threadIdx.x & (LIG_WAVE - 1);
+    const int g0 = blockIdx.x * kEmitThreads + (threadIdx.x - lane);
+    const int g = g0 + lane;
+    int start = 0, end = 0, x0 = 0, y0 = 0, w = 1, zero = 0;
+    if (g < n) {
+        start = g == 0 ? 0 : cum[g - 1];
+        end = min(cum[g], m);
+        const float ex = extents[2 * g], ey = extents[2 * g + 1];
+        if (ex <= 0.0f || ey <= 0.0f) {
+            zero = 1;
+        } else {
+            int x1, y1;
+            tile_bbox(xys[2 * g], xys[2 * g + 1], ex, ey, tiles_x, tiles_y, block_width, x0, y0,
+                      x1, y1);
+            w = max(x1 - x0, 1);
         }
-        return;
     }
-    int x0, y0, x1, y1;
-    tile_bbox(xys[2 * g], xys[2 * g + 1], ex, ey, tiles_x, tiles_y, block_width, x0, y0, x1, y1);
-    for (int y = y0; y < y1; ++y) {
-        const uint32_t row = (uint32_t)y * (uint32_t)tiles_x;
-        for (int x = x0; x < x1; ++x) {
-            if (cur >= end) return;
-            keys[cur] = row + (uint32_t)x;
-            slots[cur] = (uint32_t)cur;
-            slot_gid[cur] = g;
-            ++cur;
+    for (int j = 0; j < LIG_WAVE; ++j) {
+        if (g0 + j >= n) break;  // wave-uniform
+        const int s0 = __builtin_amdgcn_readlane(start, j);
+        const int cnt = __builtin_amdgcn_readlane(end, j) - s0;
+        if (cnt <= 0) continue;
+        const int rx0 = __builtin_amdgcn_readlane(x0, j);
+        const int ry0 = __builtin_amdgcn_readlane(y0, j);
+        const int rw = __builtin_amdgcn_readlane(w, j);
+        const int rz = __builtin_amdgcn_readlane(zero, j);
+        for (int k = lane; k < cnt; k += LIG_WAVE) {
+            uint32_t key = 0;
+            int gid = 0;
+            if (!rz) {
+                const int row = k / rw;
+                key = (uint32_t)(ry0 + row) * (uint32_t)tiles_x + (uint32_t)(rx0 + k - row * rw);
+                gid = g0 + j;
+            }
+            keys[s0 + k] = key;
+            slot_gid[s0 + k] = gid;
         }
     }
 }
@@ -476,12 +529,12 @@ extern "C" int lig_get_tile_bin_edges(int num_intersects, const int64_t *isect_i
     return launch_status();
 }
 
-// Workspace layout of the fused path: tile keys, emission slots and slot
-// gids (emission order), sorted keys, sorted slots (unless the caller keeps
-// them), then the sort's own workspace.
+// Workspace layout of the fused path: tile keys and slot gids (emission
+// order), sorted keys, sorted slots (unless the caller keeps them), then the
+// sort's own workspace.
 extern "C" size_t lig_bin_workspace_size(int num_intersects, int num_tiles) {
     const size_t m = num_intersects > 0 ? (size_t)num_intersects : 0;
-    return 5 * align_up(m * 4) + sort_ws_bytes<uint32_t>(m, bits_for(num_tiles)) + 256;
+    return 4 * align_up(m * 4) + sort_ws_bytes<uint32_t>(m, bits_for(num_tiles)) + 256;
 }
 
 extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float *xys,
@@ -511,16 +564,15 @@ extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float
         return q;
     };
     uint32_t *keys = reinterpret_cast<uint32_t *>(take());
-    uint32_t *slots = reinterpret_cast<uint32_t *>(take());
     int32_t *slot_gid = reinterpret_cast<int32_t *>(take());
     uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(take());
     uint32_t *slots_sorted = reinterpret_cast<uint32_t *>(take());
     if (emission_ids_sorted) slots_sorted = reinterpret_cast<uint32_t *>(emission_ids_sorted);
     hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(ceil_div(n, kEmitThreads)), dim3(kEmitThreads),
                        0, st, n, num_intersects, xys, extents, cum_tiles_hit, tx, ty,
-                       block_width, keys, slots, slot_gid);
-    int rc = radix_sort<uint32_t>(keys, slots, keys_sorted, slots_sorted, m, bits_for(nt), 0u, p,
-                                  st);
+                       block_width, keys, slot_gid);
+    int rc = radix_sort<uint32_t>(keys, nullptr, keys_sorted, slots_sorted, m, bits_for(nt), 0u,
+                                  p, st);
     if (rc != LIG_OK) return rc;
     hipLaunchKernelGGL(finish_bins_kernel, dim3(ceil_div(m, 256)), dim3(256), 0, st,
                        num_intersects, (const uint32_t *)keys_sorted,

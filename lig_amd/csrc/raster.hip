@@ -43,6 +43,24 @@ __global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restric
     g.p0 = make_float4(xys[2 * i], xys[2 * i + 1], a * kHalfLog2e, b * kHalfLog2e);
     g.p1 = make_float4(c * kHalfLog2e, o, colors[3 * i], colors[3 * i + 1]);
     g.p2 = make_float4(colors[3 * i + 2], a, b, c);
+    // Cull box: for a positive definite conic, alpha >= 1/255 needs
+    // d^T Q d <= t = 2 ln(255 o), whose x-extent is sqrt(t * Sigma_xx) with
+    // Sigma_xx = c / det.  Half a pixel of margin covers FP32 rounding of the
+    // pair evaluation many times over, so culled pairs are exactly the ones the
+    // alpha test would reject.  Indefinite conics are never culled (sigma < 0
+    // there is a rejection rule of its own and the region is unbounded).
+    const float det = a * c - b * b;
+    float cx = __builtin_inff(), cy = __builtin_inff();
+    if (a > 0.0f && c > 0.0f && det > 0.0f && __builtin_isfinite(det) && o == o) {
+        const float t = 2.0f * __logf(255.0f * o);
+        if (!(t > 0.0f)) {
+            cx = cy = -1.0f;
+        } else {
+            cx = __builtin_sqrtf(t * c / det) + 0.5f;
+            cy = __builtin_sqrtf(t * a / det) + 0.5f;
+        }
+    }
+    g.p3 = make_float4(cx, cy, 0.0f, 0.0f);
     pack[i] = g;
 }
 
@@ -163,6 +181,109 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
                 out_dxy[3 * pid + 1] = dzg[k];
                 out_dxy[3 * pid + 2] = dzb[k];
             }
+        }
+    }
+}
+
+// Forward for 16x16 tiles.  Lane l owns pixel (l % 8, l / 8) of each 8x8
+// quadrant q (q = 0..3: x half q & 1, y half q >> 1), so the four pixels of a
+// lane share two columns and two rows (dx, a'dx, b'dx and dy are computed
+// once per Gaussian), and a Gaussian's cull box is tested against each
+// quadrant once, in the staging lane: quadrants it cannot reach are skipped
+// by a wave-uniform branch.  Pair evaluation and accumulation order are those
+// of raster_fwd_kernel, so the outputs are identical.
+template <bool HAS_OPAC>
+__global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
+    int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
+    const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
+    float *__restrict__ out_img, float *__restrict__ out_wsum,
+    int32_t *__restrict__ final_idx) {
+    __shared__ float4 s0[kThreads];
+    __shared__ float4 s1[kThreads];
+    __shared__ float2 s2[kThreads];  // (blue, quadrant mask bits)
+
+    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
+    const int x_org = tx * 16, y_org = ty * 16;
+    const int qx0 = lane & 7, qy0 = lane >> 3;
+    const float X0 = (float)(x_org + qx0) + 0.5f, X1 = X0 + 8.0f;
+    const float Y0 = (float)(y_org + qy0) + 0.5f, Y1 = Y0 + 8.0f;
+
+    float ar[4], ag[4], ab[4], aw[4];
+    int fin[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        ar[q] = ag[q] = ab[q] = aw[q] = 0.0f;
+        fin[q] = 0;
+    }
+
+    const int2 range = bins[tile];
+    for (int base = range.x; base < range.y; base += kThreads) {
+        const int cnt = min(kThreads, range.y - base);
+        if (lane < cnt) {
+            const GaussPack gp = pack[gids[base + lane]];
+            s0[lane] = gp.p0;
+            s1[lane] = gp.p1;
+            // quadrant q spans pixel centres [org + 8h + 0.5, org + 8h + 7.5]
+            const float xl = gp.p0.x - gp.p3.x, xh = gp.p0.x + gp.p3.x;
+            const float yl = gp.p0.y - gp.p3.y, yh = gp.p0.y + gp.p3.y;
+            const float bx0 = (float)x_org + 0.5f, by0 = (float)y_org + 0.5f;
+            // negated tests keep NaN boxes (NaN means) unculled, as the generic kernel
+            const bool cx0 = !(xl > bx0 + 7.0f) && !(xh < bx0);
+            const bool cx1 = !(xl > bx0 + 15.0f) && !(xh < bx0 + 8.0f);
+            const bool cy0 = !(yl > by0 + 7.0f) && !(yh < by0);
+            const bool cy1 = !(yl > by0 + 15.0f) && !(yh < by0 + 8.0f);
+            const int m = (cx0 && cy0 ? 1 : 0) | (cx1 && cy0 ? 2 : 0) | (cx0 && cy1 ? 4 : 0) |
+                          (cx1 && cy1 ? 8 : 0);
+            s2[lane] = make_float2(gp.p2.x, __int_as_float(m));
+        }
+        __syncthreads();
+        for (int t = 0; t < cnt; ++t) {
+            const float2 a2 = s2[t];
+            const int mask = __builtin_amdgcn_readfirstlane(__float_as_int(a2.y));
+            if (mask == 0) continue;
+            const float4 a0 = s0[t];
+            const float4 a1 = s1[t];
+            const int kidx = base + t;
+            const float dx0 = a0.x - X0, dx1 = a0.x - X1;
+            const float dy0 = a0.y - Y0, dy1 = a0.y - Y1;
+            const float apdx0 = a0.z * dx0, bpdx0 = a0.w * dx0;
+            const float apdx1 = a0.z * dx1, bpdx1 = a0.w * dx1;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (!(mask & (1 << q))) continue;
+                const float dx = (q & 1) ? dx1 : dx0;
+                const float dy = (q & 2) ? dy1 : dy0;
+                float u_, v_;
+                const float s = pair_s2((q & 1) ? apdx1 : apdx0, (q & 1) ? bpdx1 : bpdx0, a0.w,
+                                        a1.x, dx, dy, u_, v_);
+                const float e = fast_exp2(-s);
+                float al = fminf(HAS_OPAC ? a1.y * e : e, 0.999f);
+                const bool ok = !(s < 0.0f) && !(al < kAlphaMin);
+                al = ok ? al : 0.0f;
+                ar[q] = __builtin_fmaf(al, a1.z, ar[q]);
+                ag[q] = __builtin_fmaf(al, a1.w, ag[q]);
+                ab[q] = __builtin_fmaf(al, a2.x, ab[q]);
+                aw[q] += al;
+                fin[q] = ok ? kidx : fin[q];
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = y_org + (q >> 1) * 8 + qy0;
+        const int j = x_org + (q & 1) * 8 + qx0;
+        if (i < img_h && j < img_w) {
+            const size_t pid = (size_t)i * img_w + j;
+            out_img[3 * pid + 0] = ar[q];
+            out_img[3 * pid + 1] = ag[q];
+            out_img[3 * pid + 2] = ab[q];
+            out_wsum[pid] = aw[q];
+            final_idx[pid] = fin[q];
         }
     }
 }
@@ -616,6 +737,14 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
         launch_fwd<true>(px, grid, st, tiles_x, block_width, img_width, img_height,
                          gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, out_dx,
                          out_dy, out_dxy);
+    else if (block_width == 16 && opacities)
+        hipLaunchKernelGGL(raster_fwd16_kernel<true>, grid, dim3(kThreads), 0, st, tiles_x,
+                           img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
+                           out_wsum, final_idx);
+    else if (block_width == 16)
+        hipLaunchKernelGGL(raster_fwd16_kernel<false>, grid, dim3(kThreads), 0, st, tiles_x,
+                           img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
+                           out_wsum, final_idx);
     else
         launch_fwd<false>(px, grid, st, tiles_x, block_width, img_width, img_height,
                           gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, nullptr,

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     assert lib.lig_abi_version() >= 100
     assert lib.lig_strerror(-1) == b"invalid argument"
     # size queries are pure host functions
-    assert L.size_query("lig_packed_gaussians_bytes", 10) == 480
+    assert L.size_query("lig_packed_gaussians_bytes", 10) == 640
     assert L.size_query("lig_bin_workspace_size", 0, 16) > 0
 
 

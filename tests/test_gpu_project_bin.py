@@ -24,6 +24,9 @@ CASES = [
     (300, 64, 96, 8, False, False, 2),
     (400, 50, 50, 5, True, False, 3),
     (1000, 256, 256, 16, False, False, 4),
+    (20000, 512, 512, 16, False, True, 5),      # 1024 tiles: one 10-bit pass
+    (30000, 2048, 2048, 16, True, True, 6),     # 16384 tiles: two 7-bit passes
+    (40000, 4100, 8190, 16, False, True, 7),    # 131072 tiles: two 9-bit passes
 ]
 
 
