@@ -185,6 +185,21 @@ int lig_count_pairs(int n, int img_height, int img_width, const float *xys,
                     const float *extents, const float *conics, const float *opacities,
                     const int32_t *num_tiles_hit, unsigned long long *counts, void *stream);
 
+/* ---- fused loss ---------------------------------------------------------- */
+
+/* Fused clamp + mean-squared-error of the fit step, replacing the caller's
+ * torch.clamp(img, 0, 1) followed by F.mse_loss (gsplat2d caller
+ * gaussianlig.py:95-103, utils.py:21-22).  n = number of floats.
+ * forward: loss[0] = mean((clamp(img,0,1) - target)^2), deterministic;
+ * workspace lig_l2_loss_workspace_size(n).  backward: grad_img =
+ * (0 <= img <= 1) ? (2/n)(img - target) * grad_loss[0] : 0 (grad_loss is a
+ * device scalar). */
+size_t lig_l2_loss_workspace_size(long long n);
+int lig_l2_clamped_loss_forward(long long n, const float *img, const float *target, float *loss,
+                                void *workspace, size_t workspace_bytes, void *stream);
+int lig_l2_clamped_loss_backward(long long n, const float *img, const float *target,
+                                 const float *grad_loss, float *grad_img, void *stream);
+
 /* ---- gradient-aware upscaler ---------------------------------------------- */
 
 /* Replaces gradient_aware_upscale_forward_tensor (csrc/bindings.cu:508-559,

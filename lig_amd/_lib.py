@@ -51,6 +51,10 @@ _SIGNATURES = {
     "lig_rasterize_backward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                  _c_int, _vp, _vp, _c_int, _vp, _c_size, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lig_l2_loss_workspace_size": (_c_size, [ctypes.c_longlong]),
+    "lig_l2_clamped_loss_forward": (
+        _c_int, [ctypes.c_longlong, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "lig_l2_clamped_loss_backward": (_c_int, [ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp]),
     "lig_count_pairs": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_gradient_aware_upscale_forward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,

@@ -80,12 +80,13 @@ constexpr float kAlphaMin = 1.0f / 255.0f;
 
 // sigma*log2(e) of a pair and the scaled conic products u', v', evaluated with
 // explicit FMAs so that every kernel computes bit-identical values for the
-// same pair.  ap_dx = a'*dx and bp_dx = b'*dx are passed in so callers can
-// share them between pixels of one column.
-__device__ __forceinline__ float pair_s2(float ap_dx, float bp_dx, float bp, float cp,
-                                         float dx, float dy, float &u, float &v) {
-    u = __builtin_fmaf(bp, dy, ap_dx);
-    v = __builtin_fmaf(cp, dy, bp_dx);
+// same pair:  u' = fma(a', dx, b'dy),  v' = fma(b', dx, c'dy),
+// s = fma(dx, u', dy * v').  bp_dy = b'*dy and cp_dy = c'*dy are passed in so
+// callers can hoist them out of loops over a row.
+__device__ __forceinline__ float pair_s2(float ap, float bp, float bp_dy, float cp_dy, float dx,
+                                         float dy, float &u, float &v) {
+    u = __builtin_fmaf(ap, dx, bp_dy);
+    v = __builtin_fmaf(bp, dx, cp_dy);
     return __builtin_fmaf(dx, u, dy * v);
 }
 

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
                 const float dx = a0.x - px[k];
                 const float dy = a0.y - py[k];
                 float u_, v_;
-                const float s = pair_s2(a0.z * dx, a0.w * dx, a0.w, a1.x, dx, dy, u_, v_);
+                const float s = pair_s2(a0.z, a0.w, a0.w * dy, a1.x * dy, dx, dy, u_, v_);
                 const float e = fast_exp2(-s);
                 float al = fminf(a1.y * e, 0.999f);
                 const bool ok = !(s < 0.0f) && !(al < kAlphaMin);
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
 
 // Forward for 16x16 tiles.  Lane l owns pixel (l % 8, l / 8) of each 8x8
 // quadrant q (q = 0..3: x half q & 1, y half q >> 1), so the four pixels of a
-// lane share two columns and two rows (dx, a'dx, b'dx and dy are computed
+// lane share two columns and two rows (dx, dy, b'dy and c'dy are computed
 // once per Gaussian), and a Gaussian's cull box is tested against each
 // quadrant once, in the staging lane: quadrants it cannot reach are skipped
 // by a wave-uniform branch.  Pair evaluation and accumulation order are those
@@ -249,16 +249,16 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
             const int kidx = base + t;
             const float dx0 = a0.x - X0, dx1 = a0.x - X1;
             const float dy0 = a0.y - Y0, dy1 = a0.y - Y1;
-            const float apdx0 = a0.z * dx0, bpdx0 = a0.w * dx0;
-            const float apdx1 = a0.z * dx1, bpdx1 = a0.w * dx1;
+            const float bpdy0 = a0.w * dy0, cpdy0 = a1.x * dy0;
+            const float bpdy1 = a0.w * dy1, cpdy1 = a1.x * dy1;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (!(mask & (1 << q))) continue;
                 const float dx = (q & 1) ? dx1 : dx0;
                 const float dy = (q & 2) ? dy1 : dy0;
                 float u_, v_;
-                const float s = pair_s2((q & 1) ? apdx1 : apdx0, (q & 1) ? bpdx1 : bpdx0, a0.w,
-                                        a1.x, dx, dy, u_, v_);
+                const float s = pair_s2(a0.z, a0.w, (q & 2) ? bpdy1 : bpdy0,
+                                        (q & 2) ? cpdy1 : cpdy0, dx, dy, u_, v_);
                 const float e = fast_exp2(-s);
                 float al = fminf(HAS_OPAC ? a1.y * e : e, 0.999f);
                 const bool ok = !(s < 0.0f) && !(al < kAlphaMin);
@@ -436,7 +436,7 @@ __global__ void add_phantom_kernel(const float *__restrict__ phantom, int has_op
 // v_xy and v_xy_abs are accumulated in the scaled conic (u', v' = K u, K v,
 // shared with the alpha evaluation) and divided by K once at the end; v_conic
 // accumulates the moments v_sigma dx^2, v_sigma dx dy, v_sigma dy^2.
-template <bool HAS_OPAC, bool DET>
+template <bool HAS_OPAC, bool DET, bool BW16>
 __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
     int tiles_x, int bw, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
@@ -472,15 +472,19 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
         float sxx = 0.f, sxy = 0.f, syy = 0.f;
         float sgx = 0.f, sgy = 0.f, sax = 0.f, say = 0.f, a_o = 0.f;
 
-        for (int ly = 0; ly < bw; ++ly) {
+        const int tw = BW16 ? 16 : bw;
+        for (int ly = 0; ly < tw; ++ly) {
             const float dy = my - ((float)(y_org + ly) + 0.5f);
-            for (int lx = 0; lx < bw; ++lx) {
-                const int p = ly * bw + lx;
+            const float bpdy = bp * dy, cpdy = cp * dy;
+            float pxf = (float)x_org + 0.5f;  // exact: integer + 0.5, incremented by 1
+#pragma unroll
+            for (int lx = 0; lx < tw; ++lx, pxf += 1.0f) {
+                const int p = ly * tw + lx;
                 const int fp = sfin[p];
                 if (fp < base) continue;  // wave-uniform: nothing in this batch reaches p
-                const float dx = mx - ((float)(x_org + lx) + 0.5f);
+                const float dx = mx - pxf;
                 float u, v;
-                const float s = pair_s2(ap * dx, bp * dx, bp, cp, dx, dy, u, v);
+                const float s = pair_s2(ap, bp, bpdy, cpdy, dx, dy, u, v);
                 const float vis = fast_exp2(-s);
                 const float al0 = fminf(HAS_OPAC ? op * vis : vis, kClamp);
                 const bool ok = kidx <= fp && !(s < 0.0f) && !(al0 < kAlphaMin);
@@ -557,13 +561,15 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
 
         for (int ly = 0; ly < bw; ++ly) {
             const float dy = my - ((float)(y_org + ly) + 0.5f);
-            for (int lx = 0; lx < bw; ++lx) {
+            const float bpdy = bp * dy, cpdy = cp * dy;
+            float pxf = (float)x_org + 0.5f;
+            for (int lx = 0; lx < bw; ++lx, pxf += 1.0f) {
                 const int p = ly * bw + lx;
                 const int fp = sfin[p];
                 if (fp < base) continue;
-                const float dx = mx - ((float)(x_org + lx) + 0.5f);
+                const float dx = mx - pxf;
                 float u, v;
-                const float s = pair_s2(ap * dx, bp * dx, bp, cp, dx, dy, u, v);
+                const float s = pair_s2(ap, bp, bpdy, cpdy, dx, dy, u, v);
                 const float vis = fast_exp2(-s);
                 const float al0 = fminf(op * vis, kClamp);
                 const bool ok = kidx <= fp && !(s < 0.0f) && !(al0 < kAlphaMin);
@@ -822,20 +828,19 @@ extern "C" int lig_rasterize_backward(int n, int img_height, int img_width, int 
         else
             hipLaunchKernelGGL(raster_bwd_ga_kernel<false>, grid, dim3(kThreads), 0, st,
                                LIG_BWD_ARGS_GA);
-    } else if (has_opacities) {
-        if (det)
-            hipLaunchKernelGGL((raster_bwd_plain_kernel<true, true>), grid, dim3(kThreads), 0,
-                               st, LIG_BWD_ARGS_PLAIN);
-        else
-            hipLaunchKernelGGL((raster_bwd_plain_kernel<true, false>), grid, dim3(kThreads), 0,
-                               st, LIG_BWD_ARGS_PLAIN);
     } else {
-        if (det)
-            hipLaunchKernelGGL((raster_bwd_plain_kernel<false, true>), grid, dim3(kThreads), 0,
-                               st, LIG_BWD_ARGS_PLAIN);
-        else
-            hipLaunchKernelGGL((raster_bwd_plain_kernel<false, false>), grid, dim3(kThreads), 0,
-                               st, LIG_BWD_ARGS_PLAIN);
+        // plain: (opacities, deterministic, 16x16) -> one of eight instantiations
+        const int sel = (has_opacities ? 4 : 0) | (det ? 2 : 0) | (block_width == 16 ? 1 : 0);
+        switch (sel) {
+            case 0: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            case 1: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, false, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            case 2: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            case 3: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, true, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            case 4: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            case 5: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, false, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            case 6: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            default: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, true, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+        }
     }
 #undef LIG_BWD_ARGS_PLAIN
 #undef LIG_BWD_ARGS_GA

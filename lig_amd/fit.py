@@ -19,6 +19,7 @@ import torch.nn.functional as F
 
 from . import gsplat2d as g
 from .data import init_gaussians
+from .loss import l2_loss_clamped
 
 
 def _adam(params, lr):
@@ -29,7 +30,8 @@ def _adam(params, lr):
 
 
 class Level(nn.Module):
-    """One level: n Gaussians on an H x W canvas, Cholesky parameterisation."""
+    """One level: n Gaussians on an H x W canvas, Cholesky parameterisation.
+    step(): render -> fused clamp + L2 loss -> backward -> Adam."""
 
     def __init__(self, n: int, H: int, W: int, seed: int, scale: float = 1.0, lr: float = 1e-3,
                  block_width: int = 16, device="cuda"):
@@ -55,8 +57,7 @@ class Level(nn.Module):
     def step(self, target_hwc: torch.Tensor) -> torch.Tensor:
         """One optimiser step against an [H, W, 3] target; returns the loss
         tensor (no host synchronisation beyond the binning's count)."""
-        img = torch.clamp(self.render(), 0, 1)
-        loss = F.mse_loss(img, target_hwc)
+        loss = l2_loss_clamped(self.render(), target_hwc)
         loss.backward()
         self.optimizer.step()
         self.optimizer.zero_grad(set_to_none=True)

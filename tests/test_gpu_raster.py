@@ -177,3 +177,23 @@ def test_upscale_backward_deterministic(device):
     b = C.gradient_aware_upscale_backward(g, src, src, src, src, 256, 256, (0., 0., 64., 64.))
     for x, y in zip(a, b):
         assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("shape", [(64, 80, 3), (1, 7), (513, 257, 3)])
+def test_fused_l2_loss_matches_torch(device, shape):
+    """Fused clamp + MSE equals F.mse_loss(clamp(img, 0, 1), target) and its
+    gradient (torch on the CPU in float64 as the reference)."""
+    import torch.nn.functional as F
+    from lig_amd.loss import l2_loss_clamped
+    gen = torch.Generator().manual_seed(sum(shape))
+    img = (torch.rand(shape, generator=gen) * 1.4 - 0.2)
+    img.view(-1)[:5] = torch.tensor([0.0, 1.0, -0.0, 1.0000001, -1e-8])  # clamp bounds
+    tgt = torch.rand(shape, generator=gen)
+    x = img.to(device).requires_grad_(True)
+    loss = l2_loss_clamped(x, tgt.to(device))
+    (loss * 3.0).backward()
+    xr = img.double().requires_grad_(True)
+    lr = F.mse_loss(torch.clamp(xr, 0, 1), tgt.double())
+    (lr * 3.0).backward()
+    assert abs(loss.item() - lr.item()) <= 1e-6 * max(1.0, lr.item())
+    np.testing.assert_allclose(x.grad.cpu().numpy(), xr.grad.numpy(), rtol=1e-5, atol=1e-12)
