@@ -164,36 +164,77 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const K *__res
         hist[(size_t)d * nblocks + blockIdx.x] = h[d];
 }
 
+// Scatter of one pass.  The block's 4096 pairs are first ranked stably into
+// LDS in digit order (wave ballots give the rank among equal digits of a
+// wave, LDS counters the offset of earlier waves and rounds), then written
+// out in LDS order, so that each digit's run goes to consecutive global
+// addresses (coalesced stores instead of one scattered store per pair).
 template <typename K, int RB>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
     K *__restrict__ keys_out, uint32_t *__restrict__ vals_out,
     const int32_t *__restrict__ offsets, int n, int shift, K flip, int nblocks) {
     constexpr int NB = 1 << RB;
-    __shared__ int run[NB];
+    constexpr int PER = NB / kSortThreads > 0 ? NB / kSortThreads : 1;  // digits per thread
+    __shared__ K skey[kSortTile];
+    __shared__ uint32_t sval[kSortTile];
+    __shared__ int lstart[NB];  // block-local exclusive start of each digit
+    __shared__ int lrun[NB];    // running local position per digit
     __shared__ int wcnt[kSortWaves][NB];
+    __shared__ int wsum[kSortWaves];
     const int tid = threadIdx.x;
     const int lane = tid & (LIG_WAVE - 1);
     const int wave = tid / LIG_WAVE;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (LIG_WAVE - lane));
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    const int count = (int)min((size_t)kSortTile, (size_t)n - base);
+
     for (int d = tid; d < NB; d += kSortThreads) {
-        run[d] = offsets[(size_t)d * nblocks + blockIdx.x];
+        lrun[d] = 0;
 #pragma unroll
         for (int w = 0; w < kSortWaves; ++w) wcnt[w][d] = 0;
     }
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kSortTile;
+    K key[kSortRounds];
+    uint32_t val[kSortRounds];
+#pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
-        const size_t i = base + (size_t)r * kSortThreads + tid;
-        if (base + (size_t)r * kSortThreads >= (size_t)n) break;  // uniform across the block
-        const bool valid = i < (size_t)n;
-        K key = 0;
-        uint32_t val = 0;
-        if (valid) {
-            key = keys_in[i];
-            val = vals_in ? vals_in[i] : (uint32_t)i;
+        const int j = r * kSortThreads + tid;
+        key[r] = 0;
+        val[r] = 0;
+        if (j < count) {
+            key[r] = keys_in[base + j];
+            val[r] = vals_in ? vals_in[base + j] : (uint32_t)(base + j);
+            atomicAdd(&lrun[digit_of<K, RB>(key[r], shift, flip)], 1);
         }
-        const unsigned d = valid ? digit_of<K, RB>(key, shift, flip) : 0u;
+    }
+    __syncthreads();
+    // block exclusive scan of the digit counts (PER consecutive digits per thread)
+    int c[PER], sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int d = tid * PER + q;
+        c[q] = d < NB ? lrun[d] : 0;
+        sum += c[q];
+    }
+    int total;
+    int pre = block_exclusive_scan(sum, wsum, &total);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int d = tid * PER + q;
+        if (d < NB) {
+            lstart[d] = pre;
+            lrun[d] = pre;
+        }
+        pre += c[q];
+    }
+    __syncthreads();
+    // stable ranking into LDS, round by round
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        if (r * kSortThreads >= count) break;  // uniform across the block
+        const bool valid = r * kSortThreads + tid < count;
+        const unsigned d = valid ? digit_of<K, RB>(key[r], shift, flip) : 0u;
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < RB; ++b) {
@@ -207,19 +248,28 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
         if (leader) wcnt[wave][d] = cnt;
         __syncthreads();
         if (valid) {
-            int pos = run[d] + rank;
+            int pos = lrun[d] + rank;
 #pragma unroll
             for (int w = 0; w < kSortWaves; ++w) pos += w < wave ? wcnt[w][d] : 0;
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            skey[pos] = key[r];
+            sval[pos] = val[r];
         }
         __syncthreads();
-        // leaders advance the digit's running offset and clear their counter
         if (leader) {
-            atomicAdd(&run[d], cnt);
+            atomicAdd(&lrun[d], cnt);
             wcnt[wave][d] = 0;
         }
         __syncthreads();
+    }
+    // global position of local slot j of digit d: offsets[d] + (j - lstart[d])
+    for (int d = tid; d < NB; d += kSortThreads)
+        lstart[d] = offsets[(size_t)d * nblocks + blockIdx.x] - lstart[d];
+    __syncthreads();
+    for (int j = tid; j < count; j += kSortThreads) {
+        const K k = skey[j];
+        const int pos = lstart[digit_of<K, RB>(k, shift, flip)] + j;
+        keys_out[pos] = k;
+        vals_out[pos] = sval[j];
     }
 }
 
