@@ -56,6 +56,24 @@ def raster_bytes(I, T, P, N, backward):
     return 36 * I + 8 * T + 16 * P + (40 * N if backward else 0)
 
 
+TRAFFIC_FILES = {"c3": os.path.join(ROOT, "profiles", "raster_c3_8192_traffic.json")}
+
+
+def measured_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` on the level-2 canvas, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE record of this workload
+    (each counter scaled by its reading for a copy of known size), or None."""
+    path = TRAFFIC_FILES.get(config)
+    if not path or not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        rec = json.load(f)
+    k = rec.get("kernels", {}).get(kernel)
+    if not k:
+        return None, None
+    return int(k["traffic_bytes"]), os.path.relpath(path, ROOT)
+
+
 def cpu_baseline(level, target, crop, oracle, torch):
     """Oracle (FP64 CPU port) fit step on a crop x crop window of `level`:
     project, bin, rasterise forward, loss gradient, rasterise backward,
@@ -140,7 +158,7 @@ def main():
     lvl_ms = {"l1": 0.0, "l2": 0.0}
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     fwd_t = KernelTimer("lig_rasterize_forward", dev)
-    bwd_t = KernelTimer("lig_rasterize_backward", dev)
+    bwd_t = KernelTimer("lig_rasterize_backward_records", dev)
     torch.cuda.synchronize()
     start = time.perf_counter()
     for i in range(args.steps):
@@ -184,9 +202,11 @@ def main():
     dom_name = max(kernels, key=lambda k: kernels[k]["ms"])
     dom = kernels[dom_name]
     achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    traffic, traffic_src = measured_traffic(args.config, dom_name)
     roofline = {"bound": "valu", "kernel": dom_name, "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                "traffic_source": traffic_src,
                 "kernel_ms": round(dom["ms"], 3), "algorithmic_flops": dom["flops"],
                 "algorithmic_bytes": dom["bytes"],
                 "hbm_gbs_algorithmic": round(dom["bytes"] / (dom["ms"] * 1e-3) / 1e9, 1)}

@@ -176,6 +176,25 @@ int lig_rasterize_backward(int n, int img_height, int img_width, int block_width
                            float *v_xy, float *v_xy_abs, float *v_conic, float *v_colors,
                            float *v_opacity, void *stream);
 
+/* The two halves of the deterministic lig_rasterize_backward, for callers that
+ * schedule (or time) them separately.  No reference counterpart.
+ * lig_rasterize_backward_records writes each (tile, Gaussian) partial
+ * gradient to its emission slot in the workspace;
+ * lig_reduce_gradient_records sums each Gaussian's slots in order and writes
+ * the gradients.  Same workspace, lig_rasterize_backward_workspace_size. */
+int lig_rasterize_backward_records(int n, int img_height, int img_width, int block_width,
+                                   const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
+                                   const void *packed, const int32_t *final_idx,
+                                   const float *v_output, const float *v_wsum,
+                                   const float *v_output_dx, const float *v_output_dy,
+                                   const float *v_output_dxy, int has_opacities,
+                                   const int32_t *emission_ids_sorted, int num_intersects,
+                                   void *workspace, size_t workspace_bytes, void *stream);
+int lig_reduce_gradient_records(int n, int num_intersects, const int32_t *cum_tiles_hit,
+                                int has_opacities, const void *workspace,
+                                size_t workspace_bytes, float *v_xy, float *v_xy_abs,
+                                float *v_conic, float *v_colors, float *v_opacity, void *stream);
+
 /* No reference counterpart: workload counters for roofline accounting.
  * counts[0] = E, pixel/Gaussian pairs whose pixel centre lies inside the
  * Gaussian's clipped SnugBox box |p - mean| <= extent (Gaussians with

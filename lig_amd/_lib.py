@@ -55,6 +55,11 @@ _SIGNATURES = {
     "lig_l2_clamped_loss_forward": (
         _c_int, [ctypes.c_longlong, _vp, _vp, _vp, _vp, _c_size, _vp]),
     "lig_l2_clamped_loss_backward": (_c_int, [ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp]),
+    "lig_rasterize_backward_records": (
+        _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                 _c_int, _vp, _c_int, _vp, _c_size, _vp]),
+    "lig_reduce_gradient_records": (
+        _c_int, [_c_int, _c_int, _vp, _c_int, _vp, _c_size, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_count_pairs": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_gradient_aware_upscale_forward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,

@@ -762,65 +762,31 @@ extern "C" size_t lig_rasterize_backward_workspace_size(int num_intersects) {
     return num_intersects > 0 ? (size_t)num_intersects * sizeof(GradRec) + 256 : 256;
 }
 
-extern "C" int lig_rasterize_backward(int n, int img_height, int img_width, int block_width,
-                                      const int32_t *gaussian_ids_sorted,
-                                      const int32_t *tile_bins, const void *packed,
-                                      const int32_t *final_idx, const float *v_output,
-                                      const float *v_wsum, const float *v_output_dx,
-                                      const float *v_output_dy, const float *v_output_dxy,
-                                      int has_opacities, const int32_t *emission_ids_sorted,
-                                      const int32_t *cum_tiles_hit, int num_intersects,
-                                      void *workspace, size_t workspace_bytes, float *v_xy,
-                                      float *v_xy_abs, float *v_conic, float *v_colors,
-                                      float *v_opacity, void *stream) {
-    if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
-        return LIG_ERR_INVALID_ARGUMENT;
+namespace lig {
+namespace {
+
+// Launches the tile kernel of the backward: records (det) or atomics.
+int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
+                          const int32_t *gids, const int32_t *tile_bins, const void *packed,
+                          const int32_t *final_idx, const float *v_output, const float *v_wsum,
+                          const float *v_output_dx, const float *v_output_dy,
+                          const float *v_output_dxy, int has_opacities, const int32_t *slots,
+                          GradRec *rec, float *v_xy, float *v_xy_abs, float *v_conic,
+                          float *v_colors, float *v_opacity, hipStream_t st) {
     const bool ga = v_output_dx != nullptr;
-    if (ga && (!v_output_dy || !v_output_dxy)) return LIG_ERR_INVALID_ARGUMENT;
-    if (n == 0) return LIG_OK;
-    if (!gaussian_ids_sorted || !tile_bins || !packed || !final_idx || !v_output || !v_xy ||
-        !v_xy_abs || !v_conic || !v_colors || (has_opacities && !v_opacity))
-        return LIG_ERR_INVALID_ARGUMENT;
-    const bool det = emission_ids_sorted != nullptr;
-    if (det) {
-        if (!cum_tiles_hit || num_intersects < 0) return LIG_ERR_INVALID_ARGUMENT;
-        if (!workspace || workspace_bytes < lig_rasterize_backward_workspace_size(num_intersects))
-            return LIG_ERR_WORKSPACE;
-    }
-    hipStream_t st = as_stream(stream);
-    const size_t nn = (size_t)n;
-    GradRec *rec = nullptr;
-    float *phantom = nullptr;
-    if (det) {
-        phantom = reinterpret_cast<float *>(workspace);
-        rec = reinterpret_cast<GradRec *>(reinterpret_cast<char *>(workspace) + 256);
-        hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, st, phantom, (size_t)16);
-    } else {
-        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(2 * nn, 256)), dim3(256), 0, st, v_xy,
-                           2 * nn);
-        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(2 * nn, 256)), dim3(256), 0, st,
-                           v_xy_abs, 2 * nn);
-        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(3 * nn, 256)), dim3(256), 0, st,
-                           v_conic, 3 * nn);
-        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(3 * nn, 256)), dim3(256), 0, st,
-                           v_colors, 3 * nn);
-        if (has_opacities)
-            hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st,
-                               v_opacity, nn);
-    }
+    const bool det = slots != nullptr;
     const int tiles_x = (img_width + block_width - 1) / block_width;
     const int tiles_y = (img_height + block_width - 1) / block_width;
     const dim3 grid((unsigned)(tiles_x * tiles_y));
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
     const GaussPack *pk = reinterpret_cast<const GaussPack *>(packed);
-    const int32_t *slots = emission_ids_sorted;
 #define LIG_BWD_ARGS_PLAIN                                                                    \
-    tiles_x, block_width, img_width, img_height, gaussian_ids_sorted, slots, bins, pk, final_idx, \
-        v_output, v_wsum, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity
+    tiles_x, block_width, img_width, img_height, gids, slots, bins, pk, final_idx, v_output,    \
+        v_wsum, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity
 #define LIG_BWD_ARGS_GA                                                                      \
-    tiles_x, block_width, img_width, img_height, gaussian_ids_sorted, slots, bins, pk, final_idx, \
-        v_output, v_wsum, v_output_dx, v_output_dy, v_output_dxy, has_opacities, rec, v_xy,    \
-        v_xy_abs, v_conic, v_colors, v_opacity
+    tiles_x, block_width, img_width, img_height, gids, slots, bins, pk, final_idx, v_output,    \
+        v_wsum, v_output_dx, v_output_dy, v_output_dxy, has_opacities, rec, v_xy, v_xy_abs,    \
+        v_conic, v_colors, v_opacity
     if (ga) {
         if (det)
             hipLaunchKernelGGL(raster_bwd_ga_kernel<true>, grid, dim3(kThreads), 0, st,
@@ -844,14 +810,117 @@ extern "C" int lig_rasterize_backward(int n, int img_height, int img_width, int 
     }
 #undef LIG_BWD_ARGS_PLAIN
 #undef LIG_BWD_ARGS_GA
-    if (det) {
-        hipLaunchKernelGGL(reduce_records_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, n,
-                           num_intersects, cum_tiles_hit, (const GradRec *)rec, has_opacities, v_xy, v_xy_abs,
-                           v_conic, v_colors, v_opacity, phantom);
-        hipLaunchKernelGGL(add_phantom_kernel, dim3(1), dim3(64), 0, st, (const float *)phantom,
-                           has_opacities, v_xy, v_xy_abs, v_conic, v_colors, v_opacity);
-    }
     return launch_status();
+}
+
+bool bwd_args_ok(int n, int img_height, int img_width, int block_width, const int32_t *gids,
+                 const int32_t *tile_bins, const void *packed, const int32_t *final_idx,
+                 const float *v_output, const float *v_output_dx, const float *v_output_dy,
+                 const float *v_output_dxy) {
+    if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
+        return false;
+    if (v_output_dx && (!v_output_dy || !v_output_dxy)) return false;
+    if (n > 0 && (!gids || !tile_bins || !packed || !final_idx || !v_output)) return false;
+    return true;
+}
+
+}  // namespace
+}  // namespace lig
+
+extern "C" int lig_rasterize_backward_records(int n, int img_height, int img_width,
+                                              int block_width, const int32_t *gaussian_ids_sorted,
+                                              const int32_t *tile_bins, const void *packed,
+                                              const int32_t *final_idx, const float *v_output,
+                                              const float *v_wsum, const float *v_output_dx,
+                                              const float *v_output_dy,
+                                              const float *v_output_dxy, int has_opacities,
+                                              const int32_t *emission_ids_sorted,
+                                              int num_intersects, void *workspace,
+                                              size_t workspace_bytes, void *stream) {
+    if (!bwd_args_ok(n, img_height, img_width, block_width, gaussian_ids_sorted, tile_bins,
+                     packed, final_idx, v_output, v_output_dx, v_output_dy, v_output_dxy) ||
+        num_intersects < 0 || (n > 0 && !emission_ids_sorted))
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (!workspace || workspace_bytes < lig_rasterize_backward_workspace_size(num_intersects))
+        return LIG_ERR_WORKSPACE;
+    if (n == 0) return LIG_OK;
+    GradRec *rec = reinterpret_cast<GradRec *>(reinterpret_cast<char *>(workspace) + 256);
+    return launch_backward_tiles(n, img_height, img_width, block_width, gaussian_ids_sorted,
+                                 tile_bins, packed, final_idx, v_output, v_wsum, v_output_dx,
+                                 v_output_dy, v_output_dxy, has_opacities, emission_ids_sorted,
+                                 rec, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                 as_stream(stream));
+}
+
+extern "C" int lig_reduce_gradient_records(int n, int num_intersects,
+                                           const int32_t *cum_tiles_hit, int has_opacities,
+                                           const void *workspace, size_t workspace_bytes,
+                                           float *v_xy, float *v_xy_abs, float *v_conic,
+                                           float *v_colors, float *v_opacity, void *stream) {
+    if (n < 0 || num_intersects < 0) return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    if (!cum_tiles_hit || !v_xy || !v_xy_abs || !v_conic || !v_colors ||
+        (has_opacities && !v_opacity))
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (!workspace || workspace_bytes < lig_rasterize_backward_workspace_size(num_intersects))
+        return LIG_ERR_WORKSPACE;
+    hipStream_t st = as_stream(stream);
+    // the first 256 bytes of the workspace hold the zero-extent-slot accumulator
+    float *phantom = reinterpret_cast<float *>(const_cast<void *>(workspace));
+    const GradRec *rec =
+        reinterpret_cast<const GradRec *>(reinterpret_cast<const char *>(workspace) + 256);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, st, phantom, (size_t)16);
+    hipLaunchKernelGGL(reduce_records_kernel, dim3(ceil_div((size_t)n, 256)), dim3(256), 0, st, n,
+                       num_intersects, cum_tiles_hit, rec, has_opacities, v_xy, v_xy_abs,
+                       v_conic, v_colors, v_opacity, phantom);
+    hipLaunchKernelGGL(add_phantom_kernel, dim3(1), dim3(64), 0, st, (const float *)phantom,
+                       has_opacities, v_xy, v_xy_abs, v_conic, v_colors, v_opacity);
+    return launch_status();
+}
+
+extern "C" int lig_rasterize_backward(int n, int img_height, int img_width, int block_width,
+                                      const int32_t *gaussian_ids_sorted,
+                                      const int32_t *tile_bins, const void *packed,
+                                      const int32_t *final_idx, const float *v_output,
+                                      const float *v_wsum, const float *v_output_dx,
+                                      const float *v_output_dy, const float *v_output_dxy,
+                                      int has_opacities, const int32_t *emission_ids_sorted,
+                                      const int32_t *cum_tiles_hit, int num_intersects,
+                                      void *workspace, size_t workspace_bytes, float *v_xy,
+                                      float *v_xy_abs, float *v_conic, float *v_colors,
+                                      float *v_opacity, void *stream) {
+    if (!bwd_args_ok(n, img_height, img_width, block_width, gaussian_ids_sorted, tile_bins,
+                     packed, final_idx, v_output, v_output_dx, v_output_dy, v_output_dxy))
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    if (!v_xy || !v_xy_abs || !v_conic || !v_colors || (has_opacities && !v_opacity))
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (emission_ids_sorted) {
+        int rc = lig_rasterize_backward_records(
+            n, img_height, img_width, block_width, gaussian_ids_sorted, tile_bins, packed,
+            final_idx, v_output, v_wsum, v_output_dx, v_output_dy, v_output_dxy, has_opacities,
+            emission_ids_sorted, num_intersects, workspace, workspace_bytes, stream);
+        if (rc != LIG_OK) return rc;
+        return lig_reduce_gradient_records(n, num_intersects, cum_tiles_hit, has_opacities,
+                                           workspace, workspace_bytes, v_xy, v_xy_abs, v_conic,
+                                           v_colors, v_opacity, stream);
+    }
+    hipStream_t st = as_stream(stream);
+    const size_t nn = (size_t)n;
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(2 * nn, 256)), dim3(256), 0, st, v_xy, 2 * nn);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(2 * nn, 256)), dim3(256), 0, st, v_xy_abs,
+                       2 * nn);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(3 * nn, 256)), dim3(256), 0, st, v_conic,
+                       3 * nn);
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(3 * nn, 256)), dim3(256), 0, st, v_colors,
+                       3 * nn);
+    if (has_opacities)
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, v_opacity,
+                           nn);
+    return launch_backward_tiles(n, img_height, img_width, block_width, gaussian_ids_sorted,
+                                 tile_bins, packed, final_idx, v_output, v_wsum, v_output_dx,
+                                 v_output_dy, v_output_dxy, has_opacities, nullptr, nullptr, v_xy,
+                                 v_xy_abs, v_conic, v_colors, v_opacity, st);
 }
 
 extern "C" int lig_count_pairs(int n, int img_height, int img_width, const float *xys,

@@ -70,31 +70,29 @@ def _edges(rng, H, W, dev, out):
 
 
 def _cells(rng, H, W, dev, out, cell=1024, count=5000):
-    tile = torch.zeros(cell, cell, 3, device=dev)
-    alpha = torch.zeros(cell, cell, 1, device=dev)
-    ys = torch.arange(cell, device=dev, dtype=torch.float32)[:, None] + 0.5
-    xs = torch.arange(cell, device=dev, dtype=torch.float32)[None, :] + 0.5
+    """Ellipse texture drawn once in a cell x cell numpy canvas (CPU), then
+    tiled over the image and blended 50 % on the device."""
+    tile = np.zeros((cell, cell, 3), np.float32)
+    alpha = np.zeros((cell, cell, 1), np.float32)
     params = rng.uniform(size=(count, 8))
     for k in range(count):
         cx, cy = params[k, 0] * cell, params[k, 1] * cell
         a, b = 4 + 20 * params[k, 2], 4 + 20 * params[k, 3]
         ang = params[k, 4] * math.pi
-        col = torch.tensor(params[k, 5:8], device=dev, dtype=torch.float32)
         r = int(max(a, b)) + 1
         x0, x1 = max(0, int(cx) - r), min(cell, int(cx) + r + 1)
         y0, y1 = max(0, int(cy) - r), min(cell, int(cy) + r + 1)
-        dx = xs[:, x0:x1] - cx
-        dy = ys[y0:y1, :] - cy
+        dx = (np.arange(x0, x1, dtype=np.float32) + 0.5 - cx)[None, :]
+        dy = (np.arange(y0, y1, dtype=np.float32) + 0.5 - cy)[:, None]
         c, s = math.cos(ang), math.sin(ang)
         u = (c * dx + s * dy) / a
         v = (-s * dx + c * dy) / b
-        inside = ((u * u + v * v) <= 1.0)[..., None]
-        tile[y0:y1, x0:x1] = torch.where(inside, col, tile[y0:y1, x0:x1])
-        alpha[y0:y1, x0:x1] = torch.where(inside, torch.ones_like(alpha[y0:y1, x0:x1]),
-                                          alpha[y0:y1, x0:x1])
+        inside = (u * u + v * v) <= 1.0
+        tile[y0:y1, x0:x1][inside] = params[k, 5:8].astype(np.float32)
+        alpha[y0:y1, x0:x1][inside] = 1.0
     reps_y, reps_x = -(-H // cell), -(-W // cell)
-    big = tile.repeat(reps_y, reps_x, 1)[:H, :W]
-    mask = alpha.repeat(reps_y, reps_x, 1)[:H, :W]
+    big = torch.tensor(tile, device=dev).repeat(reps_y, reps_x, 1)[:H, :W]
+    mask = torch.tensor(alpha, device=dev).repeat(reps_y, reps_x, 1)[:H, :W]
     out.mul_(1 - 0.5 * mask).add_(0.5 * mask * big)
 
 

@@ -324,17 +324,23 @@ def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_s
     v_conic = _empty((n, 3), like)
     v_colors = _empty((n, 3), like)
     v_opacity = _empty((n,), like) if has_opacities else None
-    m = int(emission_ids_sorted.numel()) if det else 0
-    ws = _workspace(L.size_query("lig_rasterize_backward_workspace_size", m), like) if det \
-        else None
+    hop = int(bool(has_opacities))
+    common = (n, int(img_height), int(img_width), int(block_width), L.ptr(gaussian_ids_sorted),
+              L.ptr(tile_bins), L.ptr(packed), L.ptr(final_idx), L.ptr(v_output),
+              L.ptr(v_render_wsum), L.ptr(v_output_dx), L.ptr(v_output_dy),
+              L.ptr(v_output_dxy), hop)
+    outs = (L.ptr(v_xy), L.ptr(v_xy_abs), L.ptr(v_conic), L.ptr(v_colors), L.ptr(v_opacity))
     with torch.cuda.device(like.device):
-        L.call("lig_rasterize_backward", n, int(img_height), int(img_width), int(block_width),
-               L.ptr(gaussian_ids_sorted), L.ptr(tile_bins), L.ptr(packed), L.ptr(final_idx),
-               L.ptr(v_output), L.ptr(v_render_wsum), L.ptr(v_output_dx), L.ptr(v_output_dy),
-               L.ptr(v_output_dxy), int(bool(has_opacities)), L.ptr(emission_ids_sorted),
-               L.ptr(cum_tiles_hit), m, L.ptr(ws), ws.numel() if det else 0, L.ptr(v_xy),
-               L.ptr(v_xy_abs), L.ptr(v_conic), L.ptr(v_colors), L.ptr(v_opacity),
-               L.stream_of(like))
+        if det:
+            m = int(emission_ids_sorted.numel())
+            ws = _workspace(L.size_query("lig_rasterize_backward_workspace_size", m), like)
+            L.call("lig_rasterize_backward_records", *common, L.ptr(emission_ids_sorted), m,
+                   L.ptr(ws), ws.numel(), L.stream_of(like))
+            L.call("lig_reduce_gradient_records", n, m, L.ptr(cum_tiles_hit), hop, L.ptr(ws),
+                   ws.numel(), *outs, L.stream_of(like))
+        else:
+            L.call("lig_rasterize_backward", *common, None, None, 0, None, 0, *outs,
+                   L.stream_of(like))
     return v_xy, v_xy_abs, v_conic, v_colors, v_opacity
 
 

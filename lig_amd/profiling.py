@@ -78,14 +78,52 @@ def summarize_rocpd(db_path: str, top: int = 25, by_grid: bool = True) -> str:
     return "\n".join(lines)
 
 
+def copy_probe(nbytes: int = 1 << 30) -> None:
+    """One device-to-device copy of `nbytes` (torch's copy kernel): the
+    reference reading for scaling FETCH_SIZE / WRITE_SIZE to bytes."""
+    n = nbytes // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {nbytes} bytes")
+
+
+def summarize_pmc(db_path: str, pattern: str = "") -> str:
+    """Per (kernel, launch size) mean of every collected counter per dispatch."""
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    cols = [r[1] for r in con.execute("pragma table_info(counters_collection)")]
+    name_col = "kernel_name" if "kernel_name" in cols else "name"
+    grid_col = "grid_size" if "grid_size" in cols else ("grid_x" if "grid_x" in cols else None)
+    q = (f"select {name_col}, {grid_col if grid_col else '0'}, counter_name, value "
+         f"from counters_collection")
+    groups = {}
+    for name, grid, cname, val in con.execute(q):
+        if pattern and pattern not in name:
+            continue
+        key = (_short(name), grid, cname)
+        groups.setdefault(key, []).append(float(val))
+    lines = [f"{'kernel':<48} {'threads':>10} {'counter':<12} {'dispatches':>10} {'mean':>14}"]
+    for (k, grid, cname), v in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"{k[:48]:<48} {grid:>10} {cname:<12} {len(v):>10} {sum(v) / len(v):>14.1f}")
+    return "\n".join(lines)
+
+
 if __name__ == "__main__":
     import argparse
     import glob
-    ap = argparse.ArgumentParser(description="summarise a rocprofv3 kernel-trace database")
-    ap.add_argument("path", help="rocprofv3 output directory or *_results.db file")
+    ap = argparse.ArgumentParser(description="rocprofv3 helpers")
+    ap.add_argument("cmd", choices=["summarize", "pmc", "copy-probe"])
+    ap.add_argument("path", nargs="?", help="rocprofv3 output directory or *_results.db file")
     ap.add_argument("--top", type=int, default=25)
+    ap.add_argument("--match", default="")
     a = ap.parse_args()
-    dbs = [a.path] if a.path.endswith(".db") else sorted(
-        glob.glob(f"{a.path}/**/*_results.db", recursive=True))
-    for db in dbs:
-        print(summarize_rocpd(db, a.top))
+    if a.cmd == "copy-probe":
+        copy_probe()
+    else:
+        dbs = [a.path] if a.path.endswith(".db") else sorted(
+            glob.glob(f"{a.path}/**/*_results.db", recursive=True))
+        for db in dbs:
+            print(summarize_rocpd(db, a.top) if a.cmd == "summarize" else summarize_pmc(db, a.match))
