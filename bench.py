@@ -56,6 +56,38 @@ def raster_bytes(I, T, P, N, backward):
     return 36 * I + 8 * T + 16 * P + (40 * N if backward else 0)
 
 
+def level_stages(N, I, T, P, E, C):
+    """Algorithmic (FLOP, byte) tallies of one fit step of one level, stage by
+    stage: Cholesky projection forward / backward, tile intersection (scan,
+    emit, one-pass-equivalent sort, bins), raster forward, clamp + L2 loss,
+    raster backward, Adam."""
+    return {
+        "project_fwd": (0, 52 * N),
+        "intersect": (0, 28 * N + 28 * I + 8 * T),
+        "raster_fwd": (raster_flops(E, C, False), raster_bytes(I, T, P, N, False)),
+        "loss": (0, 36 * P),
+        "raster_bwd": (raster_flops(E, C, True), raster_bytes(I, T, P, N, True)),
+        "project_bwd": (0, 72 * N),
+        "adam": (0, 224 * N),
+    }
+
+
+def stage_roof_ms(flops, nbytes):
+    return max(flops / (PEAK_FP32_TFLOPS * 1e12), nbytes / (PEAK_HBM_GBS * 1e9)) * 1e3
+
+
+def level_counts(level, C, torch):
+    """Exact (N, I, T, P, E, C) of a level's current parameters."""
+    with torch.no_grad():
+        xys, ext, con, nt = level.project()
+        E, Cc = C.count_pairs(xys, ext, con, None, nt, level.H, level.W)
+        I = int(nt.sum().item())
+        mean_tiles = float(nt.float().mean().item())
+    T = ((level.H + 15) // 16) * ((level.W + 15) // 16)
+    return dict(N=int(level.means.shape[0]), I=I, T=T, P=level.H * level.W, E=E, C=Cc,
+                mean_tiles_per_gaussian=round(mean_tiles, 2))
+
+
 TRAFFIC_FILES = {"c3": os.path.join(ROOT, "profiles", "raster_c3_8192_traffic.json")}
 
 
@@ -186,12 +218,9 @@ def main():
             l2.render()
         torch.cuda.synchronize()
         render_s = (time.perf_counter() - r0) / args.render_reps
-        xys, ext, con, nt = l2.project()
-        E, Cc = C.count_pairs(xys, ext, con, None, nt, l2.H, l2.W)
-        I = int(nt.sum().item())
-        mean_tiles = float(nt.float().mean().item())
-    T = (l2.H // 16) * (l2.W // 16)
-    P = l2.H * l2.W
+    counts = {"l1": level_counts(l1, C, torch), "l2": level_counts(l2, C, torch)}
+    k2 = counts["l2"]
+    I, T, P, E, Cc = k2["I"], k2["T"], k2["P"], k2["E"], k2["C"]
     fwd_ms, bwd_ms = fwd_t.mean_ms(), bwd_t.mean_ms()
     kernels = {
         "raster_fwd": dict(ms=fwd_ms, flops=raster_flops(E, Cc, False),
@@ -203,6 +232,19 @@ def main():
     dom = kernels[dom_name]
     achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
     traffic, traffic_src = measured_traffic(args.config, dom_name)
+
+    # step roofline: sum over both levels' stages of max(FLOP time, byte time)
+    t_roof = 0.0
+    stage_ms = {}
+    for lv, k in counts.items():
+        for st, (fl, by) in level_stages(k["N"], k["I"], k["T"], k["P"], k["E"],
+                                         k["C"]).items():
+            ms = stage_roof_ms(fl, by)
+            stage_ms[f"{lv}.{st}"] = round(ms, 4)
+            t_roof += ms
+    step_ms = 1e3 / steps_per_s
+    step_roofline = {"t_roof_ms": round(t_roof, 3), "t_step_ms": round(step_ms, 3),
+                     "frac": round(t_roof / step_ms, 4), "stage_roof_ms": stage_ms}
     roofline = {"bound": "valu", "kernel": dom_name, "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
@@ -252,11 +294,10 @@ def main():
         "level_ms": {k: round(v / args.steps, 3) for k, v in lvl_ms.items()},
         "render_gpix_s": round(P / render_s / 1e9, 4),
         "render_ms": round(render_s * 1e3, 3),
-        "workload_counts_l2": {"N": l2.means.shape[0], "I": I, "T": T, "P": P, "E": E, "C": Cc,
-                               "mean_tiles_per_gaussian": round(mean_tiles, 2),
-                               "tile_pairs": I * 256},
+        "workload_counts": counts,
         "kernel_ms_l2": {k: round(v["ms"], 3) for k, v in kernels.items()},
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -433,10 +433,94 @@ __global__ void add_phantom_kernel(const float *__restrict__ phantom, int has_op
 //   v_conic += (v_sigma dx^2 / 2, v_sigma dx dy, v_sigma dy^2 / 2),
 //   v_xy += v_sigma (a dx + b dy, b dx + c dy),  v_xy_abs += |that|,
 //   v_opacity += vis * v_alpha.
-// v_xy and v_xy_abs are accumulated in the scaled conic (u', v' = K u, K v,
-// shared with the alpha evaluation) and divided by K once at the end; v_conic
-// accumulates the moments v_sigma dx^2, v_sigma dx dy, v_sigma dy^2.
-template <bool HAS_OPAC, bool DET, bool BW16>
+// Moments: within a pixel row dy is fixed, so per pixel only q0 = sum v_sigma,
+// q1 = sum v_sigma dx and sxx = sum v_sigma dx^2 are accumulated; at the end of
+// the row q0 and q1 fold into sum v_sigma dy (my), v_sigma dy^2 (syy) and
+// v_sigma dx dy (sxy).  v_xy is formed once at the end as (a mx + b my,
+// b mx + c my); v_xy_abs accumulates |v_sigma| |u'| with the scaled conic's
+// u' = K (a dx + b dy) and is divided by K once at the end.
+struct PlainAcc {
+    float r, g, b, sxx, sxy, syy, mx, my, ax, ay, o;
+    float q0, q1;  // current row
+};
+
+__device__ __forceinline__ void plain_acc_zero(PlainAcc &A) {
+    A.r = A.g = A.b = A.sxx = A.sxy = A.syy = A.mx = A.my = A.ax = A.ay = A.o = 0.0f;
+    A.q0 = A.q1 = 0.0f;
+}
+
+__device__ __forceinline__ void plain_row_end(PlainAcc &A, float dy) {
+    A.mx += A.q1;
+    A.my = __builtin_fmaf(dy, A.q0, A.my);
+    A.sxy = __builtin_fmaf(dy, A.q1, A.sxy);
+    A.syy = __builtin_fmaf(dy * A.q0, dy, A.syy);
+    A.q0 = A.q1 = 0.0f;
+}
+
+// Wave-uniform value moved into a VGPR: an operand held in a VGPR issues at
+// the full VALU rate where an SGPR operand does not, and LDS addresses built
+// from it stay in one register with immediate offsets.
+__device__ __forceinline__ int in_vgpr(int x) {
+    int r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+__device__ __forceinline__ float in_vgpr(float x) {
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+
+// One pair of the plain backward.  Rejected pairs (lane test, sigma < 0,
+// alpha < 1/255) leave the accumulators untouched, as the reference's
+// `continue`: the accumulation runs under the lane mask of accepted pairs and
+// is skipped when no lane accepts.  amin = 1/255 and clampv = 0.99 come in
+// VGPRs.
+template <bool HAS_OPAC>
+__device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, float bp,
+                                           float bpdy, float cpdy, float op, float cr, float cg,
+                                           float cb, float dx, float dy, float amin,
+                                           float clampv, const float4 *__restrict__ wp) {
+    const float4 w = *wp;
+    float u, v;
+    const float s = pair_s2(ap, bp, bpdy, cpdy, dx, dy, u, v);
+    const float vis = fast_exp2(-s);
+    const float al = fminf(HAS_OPAC ? op * vis : vis, clampv);
+    if (lane_ok && !(s < 0.0f) && !(al < amin)) {
+        const float va =
+            __builtin_fmaf(cr, w.x, __builtin_fmaf(cg, w.y, __builtin_fmaf(cb, w.z, w.w)));
+        A.r = __builtin_fmaf(al, w.x, A.r);
+        A.g = __builtin_fmaf(al, w.y, A.g);
+        A.b = __builtin_fmaf(al, w.z, A.b);
+        if constexpr (HAS_OPAC) A.o = __builtin_fmaf(vis, va, A.o);
+        const float vsig = -al * va;
+        const float t1 = vsig * dx;
+        A.q0 += vsig;
+        A.q1 += t1;
+        A.sxx = __builtin_fmaf(t1, dx, A.sxx);
+        A.ax = __builtin_fmaf(fabsf(vsig), fabsf(u), A.ax);
+        A.ay = __builtin_fmaf(fabsf(vsig), fabsf(v), A.ay);
+    }
+}
+
+template <bool HAS_OPAC, bool DET>
+__device__ __forceinline__ void plain_emit(const PlainAcc &A, const GaussPack &gp, bool act,
+                                           int slot, int g, GradRec *__restrict__ rec,
+                                           float *__restrict__ g_xy, float *__restrict__ g_xy_abs,
+                                           float *__restrict__ g_conic, float *__restrict__ g_rgb,
+                                           float *__restrict__ g_opac) {
+    constexpr float kInvK = 1.0f / kHalfLog2e;
+    const float a = gp.p2.y, b = gp.p2.z, c = gp.p2.w;
+    const float gx = __builtin_fmaf(a, A.mx, b * A.my);
+    const float gy = __builtin_fmaf(b, A.mx, c * A.my);
+    if (act)
+        emit_grads(DET, rec, slot, g, A.r, A.g, A.b, 0.5f * A.sxx, A.sxy, 0.5f * A.syy, gx, gy,
+                   A.ax * kInvK, A.ay * kInvK, A.o, HAS_OPAC, g_xy, g_xy_abs, g_conic, g_rgb,
+                   g_opac);
+}
+
+// Generic tile width (bw <= 16): pixel loop over the bw x bw tile.
+template <bool HAS_OPAC, bool DET>
 __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
     int tiles_x, int bw, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
@@ -457,63 +541,107 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
     stage_tile_grads<false>(bw, x_org, y_org, img_w, img_h, final_idx, v_out, v_wsum, nullptr,
                             nullptr, nullptr, sv, sfin, nullptr, nullptr, nullptr);
 
-    constexpr float kClamp = 0.99f;
-    constexpr float kInvK = 1.0f / kHalfLog2e;
+    const float amin = in_vgpr(kAlphaMin), clampv = in_vgpr(0.99f);
+    for (int base = range.x; base < range.y; base += kThreads) {
+        const bool act = base + lane < range.y;
+        const int kidx = act ? base + lane : INT32_MAX;
+        const int g = act ? gids[base + lane] : 0;
+        const GaussPack gp = pack[g];
+        PlainAcc A;
+        plain_acc_zero(A);
+        for (int ly = 0; ly < bw; ++ly) {
+            const float dy = gp.p0.y - ((float)(y_org + ly) + 0.5f);
+            const float bpdy = gp.p0.w * dy, cpdy = gp.p1.x * dy;
+            float pxf = (float)x_org + 0.5f;  // exact: integer + 0.5, incremented by 1
+            for (int lx = 0; lx < bw; ++lx, pxf += 1.0f) {
+                const int p = ly * bw + lx;
+                const int fp = sfin[p];
+                if (fp < base) continue;  // wave-uniform: nothing in this batch reaches p
+                plain_pair<HAS_OPAC>(A, kidx <= fp, gp.p0.z, gp.p0.w, bpdy, cpdy, gp.p1.y,
+                                     gp.p1.z, gp.p1.w, gp.p2.x, gp.p0.x - pxf, dy, amin, clampv,
+                                     sv + p);
+            }
+            plain_row_end(A, dy);
+        }
+        plain_emit<HAS_OPAC, DET>(A, gp, act, DET && act ? slots[base + lane] : 0, g, rec, g_xy,
+                                  g_xy_abs, g_conic, g_rgb, g_opac);
+    }
+}
+
+// 16 x 16 tiles.  Per batch and pixel row, two ballots over the row's final
+// indices give wave-uniform bit masks: pixels some lane of the batch reaches
+// (fp >= base) and pixels every lane reaches (fp >= base + 63).  Only pixels
+// of the first kind are evaluated.  A row whose reached pixels are all of the
+// second kind runs without the per-lane test kidx <= fp; any other row makes
+// the test for every pixel, with fp read from the row's lane by v_readlane.
+template <bool HAS_OPAC, bool DET>
+__global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
+    int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
+    const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
+    const GaussPack *__restrict__ pack, const int32_t *__restrict__ final_idx,
+    const float *__restrict__ v_out, const float *__restrict__ v_wsum,
+    GradRec *__restrict__ rec, float *__restrict__ g_xy, float *__restrict__ g_xy_abs,
+    float *__restrict__ g_conic, float *__restrict__ g_rgb, float *__restrict__ g_opac) {
+    __shared__ float4 sv[kMaxPix];
+    __shared__ int sfin[kMaxPix];
+
+    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x;
+    const int2 range = bins[tile];
+    if (range.x >= range.y) return;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
+    const int x_org = tx * 16, y_org = ty * 16;
+    stage_tile_grads<false>(16, x_org, y_org, img_w, img_h, final_idx, v_out, v_wsum, nullptr,
+                            nullptr, nullptr, sv, sfin, nullptr, nullptr, nullptr);
+
     for (int base = range.x; base < range.y; base += kThreads) {
         const bool act = base + lane < range.y;
         const int kidx = act ? base + lane : INT32_MAX;
         const int g = act ? gids[base + lane] : 0;
         const GaussPack gp = pack[g];
         const float mx = gp.p0.x, my = gp.p0.y, ap = gp.p0.z, bp = gp.p0.w;
-        const float cp = gp.p1.x, op = gp.p1.y, cr = gp.p1.z, cg = gp.p1.w;
-        const float cb = gp.p2.x;
-
-        float a_r = 0.f, a_g = 0.f, a_b = 0.f;
-        float sxx = 0.f, sxy = 0.f, syy = 0.f;
-        float sgx = 0.f, sgy = 0.f, sax = 0.f, say = 0.f, a_o = 0.f;
-
-        const int tw = BW16 ? 16 : bw;
-        for (int ly = 0; ly < tw; ++ly) {
+        const float cp = gp.p1.x, op = gp.p1.y, cr = gp.p1.z, cg = gp.p1.w, cb = gp.p2.x;
+        // exact: x_org + lx + 0.5 is representable, one rounding in the difference
+        float dxs[16];
+#pragma unroll
+        for (int lx = 0; lx < 16; ++lx) dxs[lx] = mx - ((float)(x_org + lx) + 0.5f);
+        PlainAcc A;
+        plain_acc_zero(A);
+        const int full_from = base + kThreads - 1;
+        const float amin = in_vgpr(kAlphaMin), clampv = in_vgpr(0.99f);
+        for (int ly = 0; ly < 16; ++ly) {
+            const int fcol = sfin[ly * 16 + (lane & 15)];
+            const uint32_t any = (uint32_t)__ballot(fcol >= base) & 0xffffu;
+            if (any == 0) continue;
+            const uint32_t full = (uint32_t)__ballot(fcol >= full_from) & 0xffffu;
             const float dy = my - ((float)(y_org + ly) + 0.5f);
             const float bpdy = bp * dy, cpdy = cp * dy;
-            float pxf = (float)x_org + 0.5f;  // exact: integer + 0.5, incremented by 1
+            const float4 *row = sv + in_vgpr(ly * 16);
+            if (full == 0xffffu) {
 #pragma unroll
-            for (int lx = 0; lx < tw; ++lx, pxf += 1.0f) {
-                const int p = ly * tw + lx;
-                const int fp = sfin[p];
-                if (fp < base) continue;  // wave-uniform: nothing in this batch reaches p
-                const float dx = mx - pxf;
-                float u, v;
-                const float s = pair_s2(ap, bp, bpdy, cpdy, dx, dy, u, v);
-                const float vis = fast_exp2(-s);
-                const float al0 = fminf(HAS_OPAC ? op * vis : vis, kClamp);
-                const bool ok = kidx <= fp && !(s < 0.0f) && !(al0 < kAlphaMin);
-                const float al = ok ? al0 : 0.0f;
-                const float4 w = sv[p];
-                const float va =
-                    __builtin_fmaf(cr, w.x, __builtin_fmaf(cg, w.y, __builtin_fmaf(cb, w.z, w.w)));
-                a_r = __builtin_fmaf(al, w.x, a_r);
-                a_g = __builtin_fmaf(al, w.y, a_g);
-                a_b = __builtin_fmaf(al, w.z, a_b);
-                if constexpr (HAS_OPAC) a_o = __builtin_fmaf(ok ? vis : 0.0f, va, a_o);
-                const float vsig = -al * va;
-                const float t1 = vsig * dx;
-                const float t2 = vsig * dy;
-                sxx = __builtin_fmaf(t1, dx, sxx);
-                sxy = __builtin_fmaf(t1, dy, sxy);
-                syy = __builtin_fmaf(t2, dy, syy);
-                const float gx = vsig * u;
-                const float gy = vsig * v;
-                sgx += gx;
-                sgy += gy;
-                sax += fabsf(gx);
-                say += fabsf(gy);
+                for (int lx = 0; lx < 16; ++lx)
+                    plain_pair<HAS_OPAC>(A, true, ap, bp, bpdy, cpdy, op, cr, cg, cb, dxs[lx], dy,
+                                         amin, clampv, row + lx);
+            } else if (full == any) {
+#pragma unroll
+                for (int lx = 0; lx < 16; ++lx)
+                    if (any & (1u << lx))
+                        plain_pair<HAS_OPAC>(A, true, ap, bp, bpdy, cpdy, op, cr, cg, cb,
+                                             dxs[lx], dy, amin,
+                                             clampv, row + lx);
+            } else {
+#pragma unroll
+                for (int lx = 0; lx < 16; ++lx)
+                    if (any & (1u << lx))
+                        plain_pair<HAS_OPAC>(A, kidx <= __builtin_amdgcn_readlane(fcol, lx), ap,
+                                             bp, bpdy, cpdy, op, cr, cg, cb, dxs[lx], dy,
+                                             amin, clampv, row + lx);
             }
+            plain_row_end(A, dy);
         }
-        if (act)
-            emit_grads(DET, rec, DET ? slots[base + lane] : 0, g, a_r, a_g, a_b, 0.5f * sxx, sxy,
-                       0.5f * syy, sgx * kInvK, sgy * kInvK, sax * kInvK, say * kInvK, a_o,
-                       HAS_OPAC, g_xy, g_xy_abs, g_conic, g_rgb, g_opac);
+        plain_emit<HAS_OPAC, DET>(A, gp, act, DET && act ? slots[base + lane] : 0, g, rec, g_xy,
+                                  g_xy_abs, g_conic, g_rgb, g_opac);
     }
 }
 
@@ -783,6 +911,9 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
 #define LIG_BWD_ARGS_PLAIN                                                                    \
     tiles_x, block_width, img_width, img_height, gids, slots, bins, pk, final_idx, v_output,    \
         v_wsum, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity
+#define LIG_BWD_ARGS_16                                                                      \
+    tiles_x, img_width, img_height, gids, slots, bins, pk, final_idx, v_output, v_wsum, rec,    \
+        v_xy, v_xy_abs, v_conic, v_colors, v_opacity
 #define LIG_BWD_ARGS_GA                                                                      \
     tiles_x, block_width, img_width, img_height, gids, slots, bins, pk, final_idx, v_output,    \
         v_wsum, v_output_dx, v_output_dy, v_output_dxy, has_opacities, rec, v_xy, v_xy_abs,    \
@@ -798,17 +929,18 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
         // plain: (opacities, deterministic, 16x16) -> one of eight instantiations
         const int sel = (has_opacities ? 4 : 0) | (det ? 2 : 0) | (block_width == 16 ? 1 : 0);
         switch (sel) {
-            case 0: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 1: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, false, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 2: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 3: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, true, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 4: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 5: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, false, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 6: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            default: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, true, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            case 0: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            case 1: hipLaunchKernelGGL((raster_bwd16_kernel<false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 2: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            case 3: hipLaunchKernelGGL((raster_bwd16_kernel<false, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 4: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            case 5: hipLaunchKernelGGL((raster_bwd16_kernel<true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 6: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
+            default: hipLaunchKernelGGL((raster_bwd16_kernel<true, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
         }
     }
 #undef LIG_BWD_ARGS_PLAIN
+#undef LIG_BWD_ARGS_16
 #undef LIG_BWD_ARGS_GA
     return launch_status();
 }
