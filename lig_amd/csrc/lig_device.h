@@ -78,18 +78,44 @@ constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kHalfLog2e = 0.5f * kLog2e;
 constexpr float kAlphaMin = 1.0f / 255.0f;
 
-// sigma*log2(e) of a pair and the scaled conic products u', v', evaluated with
+// sigma*log2(e) of a pair from the scaled conic (a', b', c'), evaluated with
 // explicit FMAs so that every kernel computes bit-identical values for the
-// same pair:  u' = fma(a', dx, b'dy),  v' = fma(b', dx, c'dy),
-// s = fma(dx, u', dy * v').  bp_dy = b'*dy and cp_dy = c'*dy are passed in so
-// callers can hoist them out of loops over a row.
-__device__ __forceinline__ float pair_s2(float ap, float bp, float bp_dy, float cp_dy, float dx,
-                                         float dy, float &u, float &v) {
-    u = __builtin_fmaf(ap, dx, bp_dy);
-    v = __builtin_fmaf(bp, dx, cp_dy);
-    return __builtin_fmaf(dx, u, dy * v);
+// same pair (the forward's accept test and the backward's must agree):
+//   s = fma(dx, fma(a', dx, b2dy), c2dy),  b2dy = (b' * dy) * 2,  c2dy = (c' * dy) * dy.
+// The two row terms depend on dy only and are hoisted out of loops over a row
+// (row_terms).
+struct RowTerms {
+    float bdy, cdy;    // b' dy, c' dy
+    float b2dy, c2dy;  // 2 b' dy, c' dy^2
+};
+
+__device__ __forceinline__ RowTerms row_terms(float bp, float cp, float dy) {
+    RowTerms r;
+    r.bdy = bp * dy;
+    r.cdy = cp * dy;
+    r.b2dy = r.bdy * 2.0f;
+    r.c2dy = r.cdy * dy;
+    return r;
+}
+
+__device__ __forceinline__ float pair_s(float ap, const RowTerms &r, float dx) {
+    return __builtin_fmaf(dx, __builtin_fmaf(ap, dx, r.b2dy), r.c2dy);
 }
 
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// Wave-uniform value moved into a VGPR: an operand held in a VGPR issues at
+// the full VALU rate where an SGPR operand does not, and LDS addresses built
+// from it stay in one register with immediate offsets.
+__device__ __forceinline__ int in_vgpr(int x) {
+    int r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+__device__ __forceinline__ float in_vgpr(float x) {
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
 
 }  // namespace lig

@@ -14,7 +14,7 @@
 //   * tiles are assigned XCD-contiguously (see xcd_remap).
 //
 // Pair evaluation: sigma*log2(e) from the pre-scaled conic with explicit FMAs
-// (pair_s2) and alpha = min(clamp, o * 2^-(sigma*log2 e)); a pair contributes
+// (pair_s) and alpha = min(clamp, o * 2^-(sigma*log2 e)); a pair contributes
 // unless sigma < 0 or alpha < 1/255.  Clamps: 0.999 in both forwards and the
 // gradient-aware backward, 0.99 in the plain backward; no backward
 // differentiates the clamp.
@@ -123,8 +123,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
                 // so the compiler can share dx, a'dx and b'dx between them
                 const float dx = a0.x - px[k];
                 const float dy = a0.y - py[k];
-                float u_, v_;
-                const float s = pair_s2(a0.z, a0.w, a0.w * dy, a1.x * dy, dx, dy, u_, v_);
+                const float s = pair_s(a0.z, row_terms(a0.w, a1.x, dy), dx);
                 const float e = fast_exp2(-s);
                 float al = fminf(a1.y * e, 0.999f);
                 const bool ok = !(s < 0.0f) && !(al < kAlphaMin);
@@ -211,6 +210,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
     const float X0 = (float)(x_org + qx0) + 0.5f, X1 = X0 + 8.0f;
     const float Y0 = (float)(y_org + qy0) + 0.5f, Y1 = Y0 + 8.0f;
 
+    const float amin = in_vgpr(kAlphaMin), clampv = in_vgpr(0.999f);
     float ar[4], ag[4], ab[4], aw[4];
     int fin[4];
 #pragma unroll
@@ -248,20 +248,15 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
             const float4 a1 = s1[t];
             const int kidx = base + t;
             const float dx0 = a0.x - X0, dx1 = a0.x - X1;
-            const float dy0 = a0.y - Y0, dy1 = a0.y - Y1;
-            const float bpdy0 = a0.w * dy0, cpdy0 = a1.x * dy0;
-            const float bpdy1 = a0.w * dy1, cpdy1 = a1.x * dy1;
+            const RowTerms r0 = row_terms(a0.w, a1.x, a0.y - Y0);
+            const RowTerms r1 = row_terms(a0.w, a1.x, a0.y - Y1);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 if (!(mask & (1 << q))) continue;
-                const float dx = (q & 1) ? dx1 : dx0;
-                const float dy = (q & 2) ? dy1 : dy0;
-                float u_, v_;
-                const float s = pair_s2(a0.z, a0.w, (q & 2) ? bpdy1 : bpdy0,
-                                        (q & 2) ? cpdy1 : cpdy0, dx, dy, u_, v_);
+                const float s = pair_s(a0.z, (q & 2) ? r1 : r0, (q & 1) ? dx1 : dx0);
                 const float e = fast_exp2(-s);
-                float al = fminf(HAS_OPAC ? a1.y * e : e, 0.999f);
-                const bool ok = !(s < 0.0f) && !(al < kAlphaMin);
+                float al = fminf(HAS_OPAC ? a1.y * e : e, clampv);
+                const bool ok = !(s < 0.0f) && !(al < amin);
                 al = ok ? al : 0.0f;
                 ar[q] = __builtin_fmaf(al, a1.z, ar[q]);
                 ag[q] = __builtin_fmaf(al, a1.w, ag[q]);
@@ -457,20 +452,6 @@ __device__ __forceinline__ void plain_row_end(PlainAcc &A, float dy) {
     A.q0 = A.q1 = 0.0f;
 }
 
-// Wave-uniform value moved into a VGPR: an operand held in a VGPR issues at
-// the full VALU rate where an SGPR operand does not, and LDS addresses built
-// from it stay in one register with immediate offsets.
-__device__ __forceinline__ int in_vgpr(int x) {
-    int r;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
-    return r;
-}
-__device__ __forceinline__ float in_vgpr(float x) {
-    float r;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
-    return r;
-}
-
 // One pair of the plain backward.  Rejected pairs (lane test, sigma < 0,
 // alpha < 1/255) leave the accumulators untouched, as the reference's
 // `continue`: the accumulation runs under the lane mask of accepted pairs and
@@ -478,12 +459,11 @@ __device__ __forceinline__ float in_vgpr(float x) {
 // VGPRs.
 template <bool HAS_OPAC>
 __device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, float bp,
-                                           float bpdy, float cpdy, float op, float cr, float cg,
-                                           float cb, float dx, float dy, float amin,
-                                           float clampv, const float4 *__restrict__ wp) {
+                                           const RowTerms &rt, float op, float cr, float cg,
+                                           float cb, float dx, float amin, float clampv,
+                                           const float4 *__restrict__ wp) {
     const float4 w = *wp;
-    float u, v;
-    const float s = pair_s2(ap, bp, bpdy, cpdy, dx, dy, u, v);
+    const float s = pair_s(ap, rt, dx);
     const float vis = fast_exp2(-s);
     const float al = fminf(HAS_OPAC ? op * vis : vis, clampv);
     if (lane_ok && !(s < 0.0f) && !(al < amin)) {
@@ -498,6 +478,8 @@ __device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, 
         A.q0 += vsig;
         A.q1 += t1;
         A.sxx = __builtin_fmaf(t1, dx, A.sxx);
+        const float u = __builtin_fmaf(ap, dx, rt.bdy);
+        const float v = __builtin_fmaf(bp, dx, rt.cdy);
         A.ax = __builtin_fmaf(fabsf(vsig), fabsf(u), A.ax);
         A.ay = __builtin_fmaf(fabsf(vsig), fabsf(v), A.ay);
     }
@@ -551,15 +533,14 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
         plain_acc_zero(A);
         for (int ly = 0; ly < bw; ++ly) {
             const float dy = gp.p0.y - ((float)(y_org + ly) + 0.5f);
-            const float bpdy = gp.p0.w * dy, cpdy = gp.p1.x * dy;
+            const RowTerms rt = row_terms(gp.p0.w, gp.p1.x, dy);
             float pxf = (float)x_org + 0.5f;  // exact: integer + 0.5, incremented by 1
             for (int lx = 0; lx < bw; ++lx, pxf += 1.0f) {
                 const int p = ly * bw + lx;
                 const int fp = sfin[p];
                 if (fp < base) continue;  // wave-uniform: nothing in this batch reaches p
-                plain_pair<HAS_OPAC>(A, kidx <= fp, gp.p0.z, gp.p0.w, bpdy, cpdy, gp.p1.y,
-                                     gp.p1.z, gp.p1.w, gp.p2.x, gp.p0.x - pxf, dy, amin, clampv,
-                                     sv + p);
+                plain_pair<HAS_OPAC>(A, kidx <= fp, gp.p0.z, gp.p0.w, rt, gp.p1.y, gp.p1.z,
+                                     gp.p1.w, gp.p2.x, gp.p0.x - pxf, amin, clampv, sv + p);
             }
             plain_row_end(A, dy);
         }
@@ -616,27 +597,26 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
             if (any == 0) continue;
             const uint32_t full = (uint32_t)__ballot(fcol >= full_from) & 0xffffu;
             const float dy = my - ((float)(y_org + ly) + 0.5f);
-            const float bpdy = bp * dy, cpdy = cp * dy;
+            const RowTerms rt = row_terms(bp, cp, dy);
             const float4 *row = sv + in_vgpr(ly * 16);
             if (full == 0xffffu) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
-                    plain_pair<HAS_OPAC>(A, true, ap, bp, bpdy, cpdy, op, cr, cg, cb, dxs[lx], dy,
-                                         amin, clampv, row + lx);
+                    plain_pair<HAS_OPAC>(A, true, ap, bp, rt, op, cr, cg, cb, dxs[lx], amin,
+                                         clampv, row + lx);
             } else if (full == any) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
-                        plain_pair<HAS_OPAC>(A, true, ap, bp, bpdy, cpdy, op, cr, cg, cb,
-                                             dxs[lx], dy, amin,
-                                             clampv, row + lx);
+                        plain_pair<HAS_OPAC>(A, true, ap, bp, rt, op, cr, cg, cb, dxs[lx],
+                                             amin, clampv, row + lx);
             } else {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
                         plain_pair<HAS_OPAC>(A, kidx <= __builtin_amdgcn_readlane(fcol, lx), ap,
-                                             bp, bpdy, cpdy, op, cr, cg, cb, dxs[lx], dy,
-                                             amin, clampv, row + lx);
+                                             bp, rt, op, cr, cg, cb, dxs[lx], amin, clampv,
+                                             row + lx);
             }
             plain_row_end(A, dy);
         }
@@ -689,15 +669,14 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
 
         for (int ly = 0; ly < bw; ++ly) {
             const float dy = my - ((float)(y_org + ly) + 0.5f);
-            const float bpdy = bp * dy, cpdy = cp * dy;
+            const RowTerms rt = row_terms(bp, cp, dy);
             float pxf = (float)x_org + 0.5f;
             for (int lx = 0; lx < bw; ++lx, pxf += 1.0f) {
                 const int p = ly * bw + lx;
                 const int fp = sfin[p];
                 if (fp < base) continue;
                 const float dx = mx - pxf;
-                float u, v;
-                const float s = pair_s2(ap, bp, bpdy, cpdy, dx, dy, u, v);
+                const float s = pair_s(ap, rt, dx);
                 const float vis = fast_exp2(-s);
                 const float al0 = fminf(op * vis, kClamp);
                 const bool ok = kidx <= fp && !(s < 0.0f) && !(al0 < kAlphaMin);
