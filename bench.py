@@ -189,8 +189,8 @@ def main():
 
     lvl_ms = {"l1": 0.0, "l2": 0.0}
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    fwd_t = KernelTimer("lig_rasterize_forward", dev)
-    bwd_t = KernelTimer("lig_rasterize_backward_records", dev)
+    fwd_t = KernelTimer("lig_fit_render_l2", dev)
+    bwd_t = KernelTimer("lig_fit_backward_records_l2", dev)
     torch.cuda.synchronize()
     start = time.perf_counter()
     for i in range(args.steps):

@@ -30,6 +30,20 @@ namespace {
 constexpr int kThreads = LIG_WAVE;
 constexpr int kMaxPix = 256;  // 16 x 16
 
+// (clamp(x, 0, 1) - t)^2: the caller's loss term per channel (gaussianlig.py:95-103,
+// utils.py:21-22).
+__device__ __forceinline__ float clamped_sq_err(float x, float t) {
+    const float d = fminf(fmaxf(x, 0.0f), 1.0f) - t;
+    return d * d;
+}
+
+// dL/dI of mean((clamp(I, 0, 1) - t)^2) for upstream gradient g: zero outside
+// [0, 1] (the clamp's mask, inclusive at both bounds), else (norm * (I - t)) * g
+// with norm = 2 / numel.  Same expression as loss.hip's gradient kernel.
+__device__ __forceinline__ float l2_grad(float x, float t, float norm, float g) {
+    return (x >= 0.0f && x <= 1.0f) ? (norm * (x - t)) * g : 0.0f;
+}
+
 __global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restrict__ xys,
                                                    const float *__restrict__ conics,
                                                    const float *__restrict__ colors,
@@ -191,12 +205,17 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
 // quadrant once, in the staging lane: quadrants it cannot reach are skipped
 // by a wave-uniform branch.  Pair evaluation and accumulation order are those
 // of raster_fwd_kernel, so the outputs are identical.
-template <bool HAS_OPAC>
+//
+// TRAIN (fit step): no wsum / final index; instead the tile's clamp + L2 sum
+// against `target`, sum over its pixels and channels of (clamp(I, 0, 1) - t)^2,
+// is written to loss_partial[tile] (FP32, fixed lane order).
+template <bool HAS_OPAC, bool TRAIN>
 __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
     float *__restrict__ out_img, float *__restrict__ out_wsum,
-    int32_t *__restrict__ final_idx) {
+    int32_t *__restrict__ final_idx, const float *__restrict__ target,
+    float *__restrict__ loss_partial) {
     __shared__ float4 s0[kThreads];
     __shared__ float4 s1[kThreads];
     __shared__ float2 s2[kThreads];  // (blue, quadrant mask bits)
@@ -261,13 +280,16 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
                 ar[q] = __builtin_fmaf(al, a1.z, ar[q]);
                 ag[q] = __builtin_fmaf(al, a1.w, ag[q]);
                 ab[q] = __builtin_fmaf(al, a2.x, ab[q]);
-                aw[q] += al;
-                fin[q] = ok ? kidx : fin[q];
+                if constexpr (!TRAIN) {
+                    aw[q] += al;
+                    fin[q] = ok ? kidx : fin[q];
+                }
             }
         }
         __syncthreads();
     }
 
+    float lsum = 0.0f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int i = y_org + (q >> 1) * 8 + qy0;
@@ -277,9 +299,19 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
             out_img[3 * pid + 0] = ar[q];
             out_img[3 * pid + 1] = ag[q];
             out_img[3 * pid + 2] = ab[q];
-            out_wsum[pid] = aw[q];
-            final_idx[pid] = fin[q];
+            if constexpr (TRAIN) {
+                lsum += clamped_sq_err(ar[q], target[3 * pid + 0]) +
+                        clamped_sq_err(ag[q], target[3 * pid + 1]) +
+                        clamped_sq_err(ab[q], target[3 * pid + 2]);
+            } else {
+                out_wsum[pid] = aw[q];
+                final_idx[pid] = fin[q];
+            }
         }
+    }
+    if constexpr (TRAIN) {
+        for (int off = LIG_WAVE / 2; off > 0; off >>= 1) lsum += __shfl_down(lsum, off, LIG_WAVE);
+        if (lane == 0) loss_partial[tile] = lsum;
     }
 }
 
@@ -319,6 +351,28 @@ __device__ __forceinline__ void stage_tile_grads(
                 sc[p] = 0.0f;
             }
         }
+    }
+    __syncthreads();
+}
+
+// Stage the loss gradient of a 16 x 16 tile in LDS for the fit step:
+// sv = (dL/dI rgb, 0) with dL/dI = l2_grad(img, target) and upstream gradient
+// *grad_loss.  Pixels outside the image get zeros (they are never evaluated).
+__device__ __forceinline__ void stage_tile_loss_grads(int x_org, int y_org, int img_w, int img_h,
+                                                      const float *__restrict__ img,
+                                                      const float *__restrict__ target,
+                                                      float g, float norm, float4 *sv) {
+    for (int p = threadIdx.x; p < kMaxPix; p += kThreads) {
+        const int i = y_org + (p >> 4);
+        const int j = x_org + (p & 15);
+        float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i < img_h && j < img_w) {
+            const size_t k = 3 * ((size_t)i * img_w + j);
+            w.x = l2_grad(img[k + 0], target[k + 0], norm, g);
+            w.y = l2_grad(img[k + 1], target[k + 1], norm, g);
+            w.z = l2_grad(img[k + 2], target[k + 2], norm, g);
+        }
+        sv[p] = w;
     }
     __syncthreads();
 }
@@ -555,16 +609,25 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
 // of the first kind are evaluated.  A row whose reached pixels are all of the
 // second kind runs without the per-lane test kidx <= fp; any other row makes
 // the test for every pixel, with fp read from the row's lane by v_readlane.
-template <bool HAS_OPAC, bool DET>
+//
+// TRAIN (fit step): the upstream gradient is the clamp + L2 loss gradient,
+// formed while staging from (img, target, *grad_loss), and there is no final
+// index.  In sum mode the final index only skips pairs that the accept test
+// rejects anyway (the forward and this kernel evaluate the same test on the
+// same FP32 values), so every in-image pixel is evaluated and the records are
+// those of the gated path.
+template <bool HAS_OPAC, bool DET, bool TRAIN>
 __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
     const GaussPack *__restrict__ pack, const int32_t *__restrict__ final_idx,
     const float *__restrict__ v_out, const float *__restrict__ v_wsum,
-    GradRec *__restrict__ rec, float *__restrict__ g_xy, float *__restrict__ g_xy_abs,
-    float *__restrict__ g_conic, float *__restrict__ g_rgb, float *__restrict__ g_opac) {
+    const float *__restrict__ img, const float *__restrict__ target,
+    const float *__restrict__ grad_loss, float norm, GradRec *__restrict__ rec,
+    float *__restrict__ g_xy, float *__restrict__ g_xy_abs, float *__restrict__ g_conic,
+    float *__restrict__ g_rgb, float *__restrict__ g_opac) {
     __shared__ float4 sv[kMaxPix];
-    __shared__ int sfin[kMaxPix];
+    __shared__ int sfin[TRAIN ? 1 : kMaxPix];
 
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x;
@@ -573,8 +636,14 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     const int ty = tile / tiles_x;
     const int tx = tile - ty * tiles_x;
     const int x_org = tx * 16, y_org = ty * 16;
-    stage_tile_grads<false>(16, x_org, y_org, img_w, img_h, final_idx, v_out, v_wsum, nullptr,
-                            nullptr, nullptr, sv, sfin, nullptr, nullptr, nullptr);
+    if constexpr (TRAIN)
+        stage_tile_loss_grads(x_org, y_org, img_w, img_h, img, target, grad_loss[0], norm, sv);
+    else
+        stage_tile_grads<false>(16, x_org, y_org, img_w, img_h, final_idx, v_out, v_wsum,
+                                nullptr, nullptr, nullptr, sv, sfin, nullptr, nullptr, nullptr);
+    // in-image rows and columns of the tile (TRAIN)
+    const int nrows = TRAIN ? min(16, img_h - y_org) : 16;
+    const uint32_t cols = img_w - x_org >= 16 ? 0xffffu : ((1u << (img_w - x_org)) - 1u);
 
     for (int base = range.x; base < range.y; base += kThreads) {
         const bool act = base + lane < range.y;
@@ -591,11 +660,15 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
         plain_acc_zero(A);
         const int full_from = base + kThreads - 1;
         const float amin = in_vgpr(kAlphaMin), clampv = in_vgpr(0.99f);
-        for (int ly = 0; ly < 16; ++ly) {
-            const int fcol = sfin[ly * 16 + (lane & 15)];
-            const uint32_t any = (uint32_t)__ballot(fcol >= base) & 0xffffu;
-            if (any == 0) continue;
-            const uint32_t full = (uint32_t)__ballot(fcol >= full_from) & 0xffffu;
+        for (int ly = 0; ly < nrows; ++ly) {
+            int fcol = 0;
+            uint32_t any = cols, full = cols;
+            if constexpr (!TRAIN) {
+                fcol = sfin[ly * 16 + (lane & 15)];
+                any = (uint32_t)__ballot(fcol >= base) & 0xffffu;
+                if (any == 0) continue;
+                full = (uint32_t)__ballot(fcol >= full_from) & 0xffffu;
+            }
             const float dy = my - ((float)(y_org + ly) + 0.5f);
             const RowTerms rt = row_terms(bp, cp, dy);
             const float4 *row = sv + in_vgpr(ly * 16);
@@ -728,6 +801,22 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
     }
 }
 
+// out = inv_n * (FP64 sum of partial[0 .. count)), in a fixed order.
+__global__ __launch_bounds__(256) void sum_partials_kernel(int count,
+                                                           const float *__restrict__ partial,
+                                                           double inv_n, float *__restrict__ out) {
+    __shared__ double ws[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < count; i += 256) acc += partial[i];
+    ws[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) ws[threadIdx.x] += ws[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = (float)(ws[0] * inv_n);
+}
+
 __global__ void zero_f32_kernel(float *__restrict__ p, size_t count) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = 0.0f;
@@ -851,17 +940,58 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
                          gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, out_dx,
                          out_dy, out_dxy);
     else if (block_width == 16 && opacities)
-        hipLaunchKernelGGL(raster_fwd16_kernel<true>, grid, dim3(kThreads), 0, st, tiles_x,
-                           img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           out_wsum, final_idx);
+        hipLaunchKernelGGL((raster_fwd16_kernel<true, false>), grid, dim3(kThreads), 0, st,
+                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
+                           out_wsum, final_idx, nullptr, nullptr);
     else if (block_width == 16)
-        hipLaunchKernelGGL(raster_fwd16_kernel<false>, grid, dim3(kThreads), 0, st, tiles_x,
-                           img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           out_wsum, final_idx);
+        hipLaunchKernelGGL((raster_fwd16_kernel<false, false>), grid, dim3(kThreads), 0, st,
+                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
+                           out_wsum, final_idx, nullptr, nullptr);
     else
         launch_fwd<false>(px, grid, st, tiles_x, block_width, img_width, img_height,
                           gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, nullptr,
                           nullptr, nullptr);
+    return launch_status();
+}
+
+extern "C" size_t lig_fit_render_workspace_size(int img_height, int img_width) {
+    if (img_height <= 0 || img_width <= 0) return 256;
+    const size_t tiles = (size_t)((img_height + 15) / 16) * (size_t)((img_width + 15) / 16);
+    return tiles * sizeof(float) + 256;
+}
+
+extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
+                                 const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
+                                 const float *xys, const float *conics, const float *colors,
+                                 const float *opacities, void *packed, const float *target,
+                                 float *out_img, float *loss, void *workspace,
+                                 size_t workspace_bytes, void *stream) {
+    if (n <= 0 || img_height <= 0 || img_width <= 0 || !gaussian_ids_sorted || !tile_bins ||
+        !xys || !conics || !colors || !packed || !target || !out_img || !loss)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (!workspace || workspace_bytes < lig_fit_render_workspace_size(img_height, img_width))
+        return LIG_ERR_WORKSPACE;
+    hipStream_t st = as_stream(stream);
+    GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
+    hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
+                       colors, opacities, pk);
+    const int tiles_x = (img_width + 15) / 16;
+    const int tiles = tiles_x * ((img_height + 15) / 16);
+    const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
+    float *partial = reinterpret_cast<float *>(workspace);
+    if (opacities)
+        hipLaunchKernelGGL((raster_fwd16_kernel<true, true>), dim3((unsigned)tiles),
+                           dim3(kThreads), 0, st, tiles_x, img_width, img_height,
+                           gaussian_ids_sorted, bins, pk, out_img, nullptr, nullptr, target,
+                           partial);
+    else
+        hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), dim3((unsigned)tiles),
+                           dim3(kThreads), 0, st, tiles_x, img_width, img_height,
+                           gaussian_ids_sorted, bins, pk, out_img, nullptr, nullptr, target,
+                           partial);
+    const double numel = 3.0 * (double)img_height * (double)img_width;
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, tiles,
+                       (const float *)partial, 1.0 / numel, loss);
     return launch_status();
 }
 
@@ -891,8 +1021,8 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
     tiles_x, block_width, img_width, img_height, gids, slots, bins, pk, final_idx, v_output,    \
         v_wsum, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity
 #define LIG_BWD_ARGS_16                                                                      \
-    tiles_x, img_width, img_height, gids, slots, bins, pk, final_idx, v_output, v_wsum, rec,    \
-        v_xy, v_xy_abs, v_conic, v_colors, v_opacity
+    tiles_x, img_width, img_height, gids, slots, bins, pk, final_idx, v_output, v_wsum,         \
+        nullptr, nullptr, nullptr, 0.0f, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity
 #define LIG_BWD_ARGS_GA                                                                      \
     tiles_x, block_width, img_width, img_height, gids, slots, bins, pk, final_idx, v_output,    \
         v_wsum, v_output_dx, v_output_dy, v_output_dxy, has_opacities, rec, v_xy, v_xy_abs,    \
@@ -909,13 +1039,13 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
         const int sel = (has_opacities ? 4 : 0) | (det ? 2 : 0) | (block_width == 16 ? 1 : 0);
         switch (sel) {
             case 0: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 1: hipLaunchKernelGGL((raster_bwd16_kernel<false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 1: hipLaunchKernelGGL((raster_bwd16_kernel<false, false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
             case 2: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 3: hipLaunchKernelGGL((raster_bwd16_kernel<false, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 3: hipLaunchKernelGGL((raster_bwd16_kernel<false, true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
             case 4: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 5: hipLaunchKernelGGL((raster_bwd16_kernel<true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 5: hipLaunchKernelGGL((raster_bwd16_kernel<true, false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
             case 6: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            default: hipLaunchKernelGGL((raster_bwd16_kernel<true, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            default: hipLaunchKernelGGL((raster_bwd16_kernel<true, true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
         }
     }
 #undef LIG_BWD_ARGS_PLAIN
@@ -961,6 +1091,41 @@ extern "C" int lig_rasterize_backward_records(int n, int img_height, int img_wid
                                  v_output_dy, v_output_dxy, has_opacities, emission_ids_sorted,
                                  rec, nullptr, nullptr, nullptr, nullptr, nullptr,
                                  as_stream(stream));
+}
+
+extern "C" int lig_fit_backward_records_l2(int n, int img_height, int img_width,
+                                           const int32_t *gaussian_ids_sorted,
+                                           const int32_t *tile_bins, const void *packed,
+                                           const float *img, const float *target,
+                                           const float *grad_loss, int has_opacities,
+                                           const int32_t *emission_ids_sorted,
+                                           int num_intersects, void *workspace,
+                                           size_t workspace_bytes, void *stream) {
+    if (n <= 0 || img_height <= 0 || img_width <= 0 || num_intersects < 0 ||
+        !gaussian_ids_sorted || !tile_bins || !packed || !img || !target || !grad_loss ||
+        !emission_ids_sorted)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (!workspace || workspace_bytes < lig_rasterize_backward_workspace_size(num_intersects))
+        return LIG_ERR_WORKSPACE;
+    GradRec *rec = reinterpret_cast<GradRec *>(reinterpret_cast<char *>(workspace) + 256);
+    const int tiles_x = (img_width + 15) / 16;
+    const dim3 grid((unsigned)(tiles_x * ((img_height + 15) / 16)));
+    const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
+    const GaussPack *pk = reinterpret_cast<const GaussPack *>(packed);
+    const float norm = (float)(2.0 / (3.0 * (double)img_height * (double)img_width));
+    hipStream_t st = as_stream(stream);
+#define LIG_FIT_BWD_ARGS                                                                     \
+    tiles_x, img_width, img_height, gaussian_ids_sorted, emission_ids_sorted, bins, pk,        \
+        nullptr, nullptr, nullptr, img, target, grad_loss, norm, rec, nullptr, nullptr,        \
+        nullptr, nullptr, nullptr
+    if (has_opacities)
+        hipLaunchKernelGGL((raster_bwd16_kernel<true, true, true>), grid, dim3(kThreads), 0, st,
+                           LIG_FIT_BWD_ARGS);
+    else
+        hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), grid, dim3(kThreads), 0, st,
+                           LIG_FIT_BWD_ARGS);
+#undef LIG_FIT_BWD_ARGS
+    return launch_status();
 }
 
 extern "C" int lig_reduce_gradient_records(int n, int num_intersects,

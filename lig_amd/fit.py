@@ -19,6 +19,7 @@ import torch.nn.functional as F
 
 from . import gsplat2d as g
 from .data import init_gaussians
+from .gsplat2d import cuda as _C
 from .loss import l2_loss_clamped
 
 
@@ -29,18 +30,67 @@ def _adam(params, lr):
         return torch.optim.Adam(params, lr=lr, foreach=True)
 
 
+class _RenderL2(torch.autograd.Function):
+    """mean((clamp(rasterize(project_cholesky(...)), 0, 1) - target)^2) as one
+    autograd node, 16 x 16 tiles, sum mode, no opacities: the loss is fused
+    into the rasteriser (lig_fit_render_l2 / lig_fit_backward_records_l2).
+    Image and gradients carry the bits of the unfused chain
+    project_gaussians_cholesky -> rasterize_gaussians -> l2_loss_clamped;
+    the loss value is summed per tile instead of per 8192-float block."""
+
+    @staticmethod
+    def forward(ctx, means, cholesky, rgbs, target, H, W):
+        n = means.shape[0]
+        xys, ext, conics, nt = _C.project_gaussians_forward_cholesky(n, cholesky, means, None,
+                                                                     H, W, 16)
+        num_int, cum = g.compute_cumulative_intersects(nt)
+        ctx.n, ctx.hw, ctx.empty = n, (H, W), num_int < 1
+        if num_int < 1:
+            # rasterize.py:80-90: no intersection renders an image of ones,
+            # which depends on no parameter
+            ctx.shapes = (means.shape, cholesky.shape, rgbs.shape)
+            return ((1.0 - target) ** 2).mean()
+        gids, bins, eids = _C.bin_and_sort_fast(n, num_int, xys, ext, cum, H, W, 16,
+                                                with_emission_ids=True)
+        img, loss, packed = _C.fit_render_l2(H, W, gids, bins, xys, conics, rgbs, None, target)
+        ctx.save_for_backward(cholesky, ext, conics, gids, bins, eids, cum, img, target, packed)
+        return loss
+
+    @staticmethod
+    def backward(ctx, grad_loss):
+        if ctx.empty:
+            z = [torch.zeros(s, dtype=grad_loss.dtype, device=grad_loss.device)
+                 for s in ctx.shapes]
+            return z[0], z[1], z[2], None, None, None
+        cholesky, ext, conics, gids, bins, eids, cum, img, target, packed = ctx.saved_tensors
+        H, W = ctx.hw
+        v_xy, _, v_conic, v_rgb, _ = _C.fit_backward_l2(
+            H, W, gids, bins, packed, ctx.n, img, target, grad_loss.float(), False, eids, cum)
+        v_chol, v_mean = _C.project_gaussians_backward_cholesky(ctx.n, ext, cholesky, conics,
+                                                                v_xy, v_conic)
+        return v_mean, v_chol, v_rgb, None, None, None
+
+
+def render_l2_loss(means, cholesky, rgbs, target, H: int, W: int) -> torch.Tensor:
+    """Fused clamp + L2 fit loss of a Cholesky level (16 x 16 tiles)."""
+    return _RenderL2.apply(means, cholesky, rgbs, target.contiguous(), H, W)
+
+
 class Level(nn.Module):
     """One level: n Gaussians on an H x W canvas, Cholesky parameterisation.
-    step(): render -> fused clamp + L2 loss -> backward -> Adam."""
+    step(): render -> clamp + L2 loss -> backward -> Adam, by default through
+    the fused render + loss node (16 x 16 tiles); fused=False runs the
+    gsplat2d API chain (rasterize_gaussians + l2_loss_clamped) instead."""
 
     def __init__(self, n: int, H: int, W: int, seed: int, scale: float = 1.0, lr: float = 1e-3,
-                 block_width: int = 16, device="cuda"):
+                 block_width: int = 16, device="cuda", fused: bool = True):
         super().__init__()
         means, chol, rgbs = init_gaussians(n, H, W, seed, scale, device)
         self.means = nn.Parameter(means)
         self.cholesky = nn.Parameter(chol)
         self.rgbs = nn.Parameter(rgbs)
         self.H, self.W, self.block_width = H, W, block_width
+        self.fused = fused and block_width == 16
         self.optimizer = _adam([self.rgbs, self.means, self.cholesky], lr)
 
     def project(self):
@@ -54,10 +104,16 @@ class Level(nn.Module):
                                     self.block_width, compute_upscale_gradients=gradient_aware)
         return out if gradient_aware else out[0]
 
+    def loss(self, target_hwc: torch.Tensor) -> torch.Tensor:
+        if self.fused:
+            return render_l2_loss(self.means, self.cholesky, self.rgbs, target_hwc, self.H,
+                                  self.W)
+        return l2_loss_clamped(self.render(), target_hwc)
+
     def step(self, target_hwc: torch.Tensor) -> torch.Tensor:
         """One optimiser step against an [H, W, 3] target; returns the loss
         tensor (no host synchronisation beyond the binning's count)."""
-        loss = l2_loss_clamped(self.render(), target_hwc)
+        loss = self.loss(target_hwc)
         loss.backward()
         self.optimizer.step()
         self.optimizer.zero_grad(set_to_none=True)

@@ -40,3 +40,78 @@ def test_two_level_and_checkpoint(device, tmp_path):
     l2b = fit.Level(2400, 128, 128, seed=9, device=device)
     smin, smax = fit.load_checkpoint(path, [l1b, l2b], map_location=device)
     assert torch.equal(l2b.means, l2.means) and float(smax[0]) == pytest.approx(0.9)
+
+
+def _level_pair(n, H, W, seed, scale, device):
+    from lig_amd import fit
+    a = fit.Level(n, H, W, seed=seed, scale=scale, lr=1e-2, device=device, fused=True)
+    b = fit.Level(n, H, W, seed=seed, scale=scale, lr=1e-2, device=device, fused=False)
+    return a, b
+
+
+@pytest.mark.parametrize("n,H,W,scale,seed", [
+    (3000, 100, 130, 1.0, 3),     # edge tiles in x and y
+    (6000, 128, 160, 2.5, 4),     # several 64-Gaussian batches per tile
+    (800, 48, 40, 4.0, 5),        # Gaussians larger than the canvas
+])
+def test_fused_step_matches_api_chain(device, n, H, W, scale, seed):
+    """The fused render + loss node gives the gradients of the API chain
+    project_gaussians_cholesky -> rasterize_gaussians -> l2_loss_clamped
+    bit for bit (same pair test, same record order) and the loss to FP32
+    summation order; one Adam step later the parameters are still equal."""
+    from lig_amd import data
+    print(f"fused-vs-api n={n} H={H} W={W} scale={scale} seed={seed}", flush=True)
+    tgt = data.make_image(H, W, seed, device=device)
+    a, b = _level_pair(n, H, W, seed, scale, device)
+    with torch.no_grad():
+        b.rgbs.mul_(0.05)   # keep part of the image inside (0, 1) so the loss mask varies
+        a.rgbs.copy_(b.rgbs)
+    la, lb = a.loss(tgt), b.loss(tgt)
+    assert float(la) == pytest.approx(float(lb), rel=1e-6)
+    la.backward()
+    lb.backward()
+    for name in ("means", "cholesky", "rgbs"):
+        ga, gb = getattr(a, name).grad, getattr(b, name).grad
+        assert torch.count_nonzero(gb) > 0, name
+        assert torch.equal(ga, gb), (name, float((ga - gb).abs().max()))
+    a.optimizer.step()
+    b.optimizer.step()
+    for name in ("means", "cholesky", "rgbs"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+
+
+def test_fused_render_image_matches_api(device):
+    from lig_amd import data
+    from lig_amd.gsplat2d import cuda as C
+    import lig_amd.gsplat2d as g
+    H, W = 90, 150
+    a, _ = _level_pair(2500, H, W, 7, 1.5, device)
+    tgt = data.make_image(H, W, 7, device=device)
+    with torch.no_grad():
+        xys, ext, con, nt = a.project()
+        num, cum = g.compute_cumulative_intersects(nt)
+        gids, bins = C.bin_and_sort_fast(a.means.shape[0], num, xys, ext, cum, H, W, 16)
+        img, loss, _ = C.fit_render_l2(H, W, gids, bins, xys, con, a.rgbs, None, tgt)
+        ref = g.rasterize_gaussians(xys, ext, con, nt, a.rgbs, None, H, W, 16)[0]
+    assert torch.equal(img, ref)
+    want = ((ref.clamp(0, 1).double() - tgt.double()) ** 2).mean()
+    assert float(loss) == pytest.approx(float(want), rel=1e-6)
+
+
+def test_fused_no_intersections(device):
+    """I < 1 (every Gaussian off screen): the reference renders ones; the loss
+    is that of the ones image and every gradient is zero."""
+    from lig_amd import fit
+    H, W = 64, 64
+    a, b = _level_pair(50, H, W, 2, 1.0, device)
+    with torch.no_grad():
+        a.means.add_(1e4)
+        b.means.add_(1e4)
+    tgt = torch.rand(H, W, 3, device=device)
+    la, lb = a.loss(tgt), b.loss(tgt)
+    assert float(la) == pytest.approx(float(lb), rel=1e-6)
+    la.backward()
+    lb.backward()
+    for name in ("means", "cholesky", "rgbs"):
+        assert torch.count_nonzero(getattr(a, name).grad) == 0
+        assert torch.equal(getattr(a, name).grad, getattr(b, name).grad)
