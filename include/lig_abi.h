@@ -199,12 +199,15 @@ int lig_reduce_gradient_records(int n, int num_intersects, const int32_t *cum_ti
 
 /* No reference counterpart: the training step of the gsplat2d caller
  * (gaussianlig.py:89-103: rasterize, clamp(0, 1), mse; backward) with the
- * loss fused into the rasteriser.  Sum mode, block width 16, num_intersects
- * >= 1 (the caller takes the reference's I < 1 path otherwise).
+ * loss fused into the rasteriser.  Sum mode, block width 16, no opacities
+ * (so no zero-extent phantom intersections), num_intersects >= 1 (the caller
+ * takes the reference's I < 1 path otherwise).  Bins come from
+ * lig_bin_and_sort_gaussians without emission ids.
  *
- * lig_fit_render_l2: packs the Gaussians into `packed`, rasterises out_img
- * [H,W,3] exactly as lig_rasterize_forward (same pair test, same order, so
- * the same bits) but without out_wsum / final_idx, and writes
+ * lig_fit_render_l2: packs the Gaussians into `packed` (adding each one's
+ * record addressing from extents / cum_tiles_hit), rasterises out_img [H,W,3]
+ * exactly as lig_rasterize_forward (same pair test, same order, so the same
+ * bits) but without out_wsum / final_idx, and writes
  * loss[0] = mean((clamp(out_img,0,1) - target)^2) over H*W*3 values, summed
  * per tile in FP32 and over tiles in FP64 (deterministic).  Workspace:
  * lig_fit_render_workspace_size(H, W).
@@ -214,21 +217,21 @@ int lig_reduce_gradient_records(int n, int num_intersects, const int32_t *cum_ti
  * grad_loss[0] : 0 (the expression of lig_l2_clamped_loss_backward), formed
  * per tile from img and target; no v_wsum, no final index (in sum mode it only
  * skips pairs that the accept test rejects, so the records equal those of
- * lig_rasterize_backward_records fed the forward's final_idx).  Follow with
- * lig_reduce_gradient_records on the same workspace
+ * lig_rasterize_backward_records fed the forward's final_idx).  Each record
+ * lands at its emission slot, computed from `packed`.  Follow with
+ * lig_reduce_gradient_records(has_opacities = 0) on the same workspace
  * (lig_rasterize_backward_workspace_size). */
 size_t lig_fit_render_workspace_size(int img_height, int img_width);
 int lig_fit_render_l2(int n, int img_height, int img_width, const int32_t *gaussian_ids_sorted,
-                      const int32_t *tile_bins, const float *xys, const float *conics,
-                      const float *colors, const float *opacities, void *packed,
-                      const float *target, float *out_img, float *loss, void *workspace,
-                      size_t workspace_bytes, void *stream);
+                      const int32_t *tile_bins, const float *xys, const float *extents,
+                      const int32_t *cum_tiles_hit, const float *conics, const float *colors,
+                      void *packed, const float *target, float *out_img, float *loss,
+                      void *workspace, size_t workspace_bytes, void *stream);
 int lig_fit_backward_records_l2(int n, int img_height, int img_width,
                                 const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
                                 const void *packed, const float *img, const float *target,
-                                const float *grad_loss, int has_opacities,
-                                const int32_t *emission_ids_sorted, int num_intersects,
-                                void *workspace, size_t workspace_bytes, void *stream);
+                                const float *grad_loss, int num_intersects, void *workspace,
+                                size_t workspace_bytes, void *stream);
 
 /* No reference counterpart: workload counters for roofline accounting.
  * counts[0] = E, pixel/Gaussian pairs whose pixel centre lies inside the

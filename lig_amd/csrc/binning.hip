@@ -617,10 +617,22 @@ extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float
     int32_t *slot_gid = reinterpret_cast<int32_t *>(take());
     uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(take());
     uint32_t *slots_sorted = reinterpret_cast<uint32_t *>(take());
-    if (emission_ids_sorted) slots_sorted = reinterpret_cast<uint32_t *>(emission_ids_sorted);
     hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(ceil_div(n, kEmitThreads)), dim3(kEmitThreads),
                        0, st, n, num_intersects, xys, extents, cum_tiles_hit, tx, ty,
                        block_width, keys, slot_gid);
+    if (!emission_ids_sorted) {
+        // sort (tile, gid) pairs directly: the stable sort of the emission
+        // order gives the same gids, and no gather is needed afterwards
+        int rc = radix_sort<uint32_t>(keys, reinterpret_cast<const uint32_t *>(slot_gid),
+                                      keys_sorted,
+                                      reinterpret_cast<uint32_t *>(gaussian_ids_sorted), m,
+                                      bits_for(nt), 0u, p, st);
+        if (rc != LIG_OK) return rc;
+        hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(m, 256)), dim3(256), 0,
+                           st, num_intersects, (const uint32_t *)keys_sorted, nt, tile_bins);
+        return launch_status();
+    }
+    slots_sorted = reinterpret_cast<uint32_t *>(emission_ids_sorted);
     int rc = radix_sort<uint32_t>(keys, nullptr, keys_sorted, slots_sorted, m, bits_for(nt), 0u,
                                   p, st);
     if (rc != LIG_OK) return rc;

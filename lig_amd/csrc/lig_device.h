@@ -63,13 +63,16 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
 
 // Per-Gaussian record the rasteriser kernels read (64 bytes, 16-byte aligned):
 //   p0 = (mean.x, mean.y, a', b')   p1 = (c', opacity, red, green)
-//   p2 = (blue, a, b, c)            p3 = (cull_x, cull_y, 0, 0)
+//   p2 = (blue, a, b, c)            p3 = (cull_x, cull_y, slot base, slot width)
 // with (a, b, c) the conic and (a', b', c') = K * (a, b, c), K = log2(e)/2, so
 // that with d = mean - pixel, u' = a'dx + b'dy, v' = b'dx + c'dy:
 //   sigma * log2(e) = dx*u' + dy*v'   and   alpha = o * 2^-(dx*u' + dy*v').
 // (cull_x, cull_y) bound |pixel - mean| beyond which alpha < 1/255 is certain
 // (infinite when the conic is not positive definite, negative when the
 // Gaussian can never reach 1/255); kernels may skip pairs outside that box.
+// The slot fields (int bits; 0 and 1 unless packed for the fit step) give the
+// emission slot of the Gaussian's intersection with tile (tx, ty) as
+// base + ty * width + tx.
 struct __align__(16) GaussPack {
     float4 p0, p1, p2, p3;
 };

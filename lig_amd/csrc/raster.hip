@@ -44,11 +44,18 @@ __device__ __forceinline__ float l2_grad(float x, float t, float norm, float g) 
     return (x >= 0.0f && x <= 1.0f) ? (norm * (x - t)) * g : 0.0f;
 }
 
+// With `extents` (fit step: 16 x 16 tiles, no phantom intersections) p3.zw
+// also carry the Gaussian's record addressing: its intersection with tile
+// (tx, ty) has emission slot  base + ty * w + tx,  where w is the width of its
+// tile rectangle [x0, x1) x [y0, y1) (emission is row-major over it, from slot
+// cum[g-1]) and base = cum[g-1] - y0 * w - x0.
 __global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restrict__ xys,
                                                    const float *__restrict__ conics,
                                                    const float *__restrict__ colors,
                                                    const float *__restrict__ opacities,
-                                                   GaussPack *__restrict__ pack) {
+                                                   const float *__restrict__ extents,
+                                                   const int32_t *__restrict__ cum, int tiles_x,
+                                                   int tiles_y, GaussPack *__restrict__ pack) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float a = conics[3 * i], b = conics[3 * i + 1], c = conics[3 * i + 2];
@@ -74,7 +81,17 @@ __global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restric
             cy = __builtin_sqrtf(t * a / det) + 0.5f;
         }
     }
-    g.p3 = make_float4(cx, cy, 0.0f, 0.0f);
+    int base = 0, w = 1;
+    if (extents) {
+        const float ex = extents[2 * i], ey = extents[2 * i + 1];
+        if (ex > 0.0f && ey > 0.0f) {
+            int x0, y0, x1, y1;
+            tile_bbox(xys[2 * i], xys[2 * i + 1], ex, ey, tiles_x, tiles_y, 16, x0, y0, x1, y1);
+            w = max(x1 - x0, 1);
+            base = (i == 0 ? 0 : cum[i - 1]) - y0 * w - x0;
+        }
+    }
+    g.p3 = make_float4(cx, cy, __int_as_float(base), __int_as_float(w));
     pack[i] = g;
 }
 
@@ -611,8 +628,9 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
 // the test for every pixel, with fp read from the row's lane by v_readlane.
 //
 // TRAIN (fit step): the upstream gradient is the clamp + L2 loss gradient,
-// formed while staging from (img, target, *grad_loss), and there is no final
-// index.  In sum mode the final index only skips pairs that the accept test
+// formed while staging from (img, target, *grad_loss), there is no final
+// index, and each record's emission slot comes from the pack (base + ty * w +
+// tx) instead of an emission-id array.  In sum mode the final index only skips pairs that the accept test
 // rejects anyway (the forward and this kernel evaluate the same test on the
 // same FP32 values), so every in-image pixel is evaluated and the records are
 // those of the gated path.
@@ -693,8 +711,13 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
             }
             plain_row_end(A, dy);
         }
-        plain_emit<HAS_OPAC, DET>(A, gp, act, DET && act ? slots[base + lane] : 0, g, rec, g_xy,
-                                  g_xy_abs, g_conic, g_rgb, g_opac);
+        int slot = 0;
+        if (TRAIN)
+            slot = __float_as_int(gp.p3.z) + ty * __float_as_int(gp.p3.w) + tx;
+        else if (DET && act)
+            slot = slots[base + lane];
+        plain_emit<HAS_OPAC, DET>(A, gp, act, slot, g, rec, g_xy, g_xy_abs, g_conic, g_rgb,
+                                  g_opac);
     }
 }
 
@@ -801,20 +824,46 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
     }
 }
 
-// out = inv_n * (FP64 sum of partial[0 .. count)), in a fixed order.
-__global__ __launch_bounds__(256) void sum_partials_kernel(int count,
-                                                           const float *__restrict__ partial,
-                                                           double inv_n, float *__restrict__ out) {
-    __shared__ double ws[256];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < count; i += 256) acc += partial[i];
-    ws[threadIdx.x] = acc;
+// Deterministic FP64 sum of `count` FP32 partials in two launches:
+// sum_partials_kernel: block b sums partial[b*kSumChunk, (b+1)*kSumChunk) into
+// block_sum[b]; sum_blocks_kernel: out = inv_n * sum of the block sums.  Fixed
+// order throughout.
+constexpr int kSumThreads = 256;
+constexpr int kSumChunk = kSumThreads * 16;
+
+__device__ __forceinline__ double block_sum_f64(double v, double *ws) {
+    ws[threadIdx.x] = v;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = kSumThreads / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) ws[threadIdx.x] += ws[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = (float)(ws[0] * inv_n);
+    return ws[0];
+}
+
+__global__ __launch_bounds__(kSumThreads) void sum_partials_kernel(
+    int count, const float *__restrict__ partial, double *__restrict__ block_sum) {
+    __shared__ double ws[kSumThreads];
+    const int base = blockIdx.x * kSumChunk;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kSumChunk / kSumThreads; ++k) {
+        const int i = base + k * kSumThreads + (int)threadIdx.x;
+        if (i < count) acc += partial[i];
+    }
+    const double t = block_sum_f64(acc, ws);
+    if (threadIdx.x == 0) block_sum[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kSumThreads) void sum_blocks_kernel(int nblocks,
+                                                                const double *__restrict__ bsum,
+                                                                double inv_n,
+                                                                float *__restrict__ out) {
+    __shared__ double ws[kSumThreads];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += kSumThreads) acc += bsum[i];
+    const double t = block_sum_f64(acc, ws);
+    if (threadIdx.x == 0) out[0] = (float)(t * inv_n);
 }
 
 __global__ void zero_f32_kernel(float *__restrict__ p, size_t count) {
@@ -907,7 +956,8 @@ extern "C" int lig_pack_gaussians(int n, const float *xys, const float *conics,
         return LIG_ERR_INVALID_ARGUMENT;
     if (n == 0) return LIG_OK;
     hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), n,
-                       xys, conics, colors, opacities, reinterpret_cast<GaussPack *>(packed));
+                       xys, conics, colors, opacities, nullptr, nullptr, 0, 0,
+                       reinterpret_cast<GaussPack *>(packed));
     return launch_status();
 }
 
@@ -929,7 +979,7 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
     GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
     if (n > 0)
         hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
-                           colors, opacities, pk);
+                           colors, opacities, nullptr, nullptr, 0, 0, pk);
     const int tiles_x = (img_width + block_width - 1) / block_width;
     const int tiles_y = (img_height + block_width - 1) / block_width;
     const dim3 grid((unsigned)(tiles_x * tiles_y));
@@ -957,41 +1007,43 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
 extern "C" size_t lig_fit_render_workspace_size(int img_height, int img_width) {
     if (img_height <= 0 || img_width <= 0) return 256;
     const size_t tiles = (size_t)((img_height + 15) / 16) * (size_t)((img_width + 15) / 16);
-    return tiles * sizeof(float) + 256;
+    const size_t blocks = (tiles + kSumChunk - 1) / kSumChunk;
+    // [block sums, FP64][tile partials, FP32]
+    return blocks * sizeof(double) + tiles * sizeof(float) + 256;
 }
 
 extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
                                  const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
-                                 const float *xys, const float *conics, const float *colors,
-                                 const float *opacities, void *packed, const float *target,
+                                 const float *xys, const float *extents,
+                                 const int32_t *cum_tiles_hit, const float *conics,
+                                 const float *colors, void *packed, const float *target,
                                  float *out_img, float *loss, void *workspace,
                                  size_t workspace_bytes, void *stream) {
     if (n <= 0 || img_height <= 0 || img_width <= 0 || !gaussian_ids_sorted || !tile_bins ||
-        !xys || !conics || !colors || !packed || !target || !out_img || !loss)
+        !xys || !extents || !cum_tiles_hit || !conics || !colors || !packed || !target ||
+        !out_img || !loss)
         return LIG_ERR_INVALID_ARGUMENT;
     if (!workspace || workspace_bytes < lig_fit_render_workspace_size(img_height, img_width))
         return LIG_ERR_WORKSPACE;
     hipStream_t st = as_stream(stream);
     GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
-    hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
-                       colors, opacities, pk);
     const int tiles_x = (img_width + 15) / 16;
-    const int tiles = tiles_x * ((img_height + 15) / 16);
+    const int tiles_y = (img_height + 15) / 16;
+    const int tiles = tiles_x * tiles_y;
+    hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
+                       colors, nullptr, extents, cum_tiles_hit, tiles_x, tiles_y, pk);
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
-    float *partial = reinterpret_cast<float *>(workspace);
-    if (opacities)
-        hipLaunchKernelGGL((raster_fwd16_kernel<true, true>), dim3((unsigned)tiles),
-                           dim3(kThreads), 0, st, tiles_x, img_width, img_height,
-                           gaussian_ids_sorted, bins, pk, out_img, nullptr, nullptr, target,
-                           partial);
-    else
-        hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), dim3((unsigned)tiles),
-                           dim3(kThreads), 0, st, tiles_x, img_width, img_height,
-                           gaussian_ids_sorted, bins, pk, out_img, nullptr, nullptr, target,
-                           partial);
+    const int nsum = (tiles + kSumChunk - 1) / kSumChunk;
+    double *bsum = reinterpret_cast<double *>(workspace);
+    float *partial = reinterpret_cast<float *>(bsum + nsum);
+    hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), dim3((unsigned)tiles), dim3(kThreads),
+                       0, st, tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk,
+                       out_img, nullptr, nullptr, target, partial);
     const double numel = 3.0 * (double)img_height * (double)img_width;
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, tiles,
-                       (const float *)partial, 1.0 / numel, loss);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)nsum), dim3(kSumThreads), 0, st, tiles,
+                       (const float *)partial, bsum);
+    hipLaunchKernelGGL(sum_blocks_kernel, dim3(1), dim3(kSumThreads), 0, st, nsum,
+                       (const double *)bsum, 1.0 / numel, loss);
     return launch_status();
 }
 
@@ -1097,13 +1149,11 @@ extern "C" int lig_fit_backward_records_l2(int n, int img_height, int img_width,
                                            const int32_t *gaussian_ids_sorted,
                                            const int32_t *tile_bins, const void *packed,
                                            const float *img, const float *target,
-                                           const float *grad_loss, int has_opacities,
-                                           const int32_t *emission_ids_sorted,
-                                           int num_intersects, void *workspace,
-                                           size_t workspace_bytes, void *stream) {
+                                           const float *grad_loss, int num_intersects,
+                                           void *workspace, size_t workspace_bytes,
+                                           void *stream) {
     if (n <= 0 || img_height <= 0 || img_width <= 0 || num_intersects < 0 ||
-        !gaussian_ids_sorted || !tile_bins || !packed || !img || !target || !grad_loss ||
-        !emission_ids_sorted)
+        !gaussian_ids_sorted || !tile_bins || !packed || !img || !target || !grad_loss)
         return LIG_ERR_INVALID_ARGUMENT;
     if (!workspace || workspace_bytes < lig_rasterize_backward_workspace_size(num_intersects))
         return LIG_ERR_WORKSPACE;
@@ -1113,18 +1163,10 @@ extern "C" int lig_fit_backward_records_l2(int n, int img_height, int img_width,
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
     const GaussPack *pk = reinterpret_cast<const GaussPack *>(packed);
     const float norm = (float)(2.0 / (3.0 * (double)img_height * (double)img_width));
-    hipStream_t st = as_stream(stream);
-#define LIG_FIT_BWD_ARGS                                                                     \
-    tiles_x, img_width, img_height, gaussian_ids_sorted, emission_ids_sorted, bins, pk,        \
-        nullptr, nullptr, nullptr, img, target, grad_loss, norm, rec, nullptr, nullptr,        \
-        nullptr, nullptr, nullptr
-    if (has_opacities)
-        hipLaunchKernelGGL((raster_bwd16_kernel<true, true, true>), grid, dim3(kThreads), 0, st,
-                           LIG_FIT_BWD_ARGS);
-    else
-        hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), grid, dim3(kThreads), 0, st,
-                           LIG_FIT_BWD_ARGS);
-#undef LIG_FIT_BWD_ARGS
+    hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), grid, dim3(kThreads), 0,
+                       as_stream(stream), tiles_x, img_width, img_height, gaussian_ids_sorted,
+                       nullptr, bins, pk, nullptr, nullptr, nullptr, img, target, grad_loss, norm,
+                       rec, nullptr, nullptr, nullptr, nullptr, nullptr);
     return launch_status();
 }
 

@@ -50,10 +50,11 @@ class _RenderL2(torch.autograd.Function):
             # which depends on no parameter
             ctx.shapes = (means.shape, cholesky.shape, rgbs.shape)
             return ((1.0 - target) ** 2).mean()
-        gids, bins, eids = _C.bin_and_sort_fast(n, num_int, xys, ext, cum, H, W, 16,
-                                                with_emission_ids=True)
-        img, loss, packed = _C.fit_render_l2(H, W, gids, bins, xys, conics, rgbs, None, target)
-        ctx.save_for_backward(cholesky, ext, conics, gids, bins, eids, cum, img, target, packed)
+        gids, bins = _C.bin_and_sort_fast(n, num_int, xys, ext, cum, H, W, 16)
+        img, loss, packed = _C.fit_render_l2(H, W, gids, bins, xys, ext, cum, conics, rgbs,
+                                             target)
+        ctx.num_int = num_int
+        ctx.save_for_backward(cholesky, ext, conics, gids, bins, cum, img, target, packed)
         return loss
 
     @staticmethod
@@ -62,10 +63,11 @@ class _RenderL2(torch.autograd.Function):
             z = [torch.zeros(s, dtype=grad_loss.dtype, device=grad_loss.device)
                  for s in ctx.shapes]
             return z[0], z[1], z[2], None, None, None
-        cholesky, ext, conics, gids, bins, eids, cum, img, target, packed = ctx.saved_tensors
+        cholesky, ext, conics, gids, bins, cum, img, target, packed = ctx.saved_tensors
         H, W = ctx.hw
-        v_xy, _, v_conic, v_rgb, _ = _C.fit_backward_l2(
-            H, W, gids, bins, packed, ctx.n, img, target, grad_loss.float(), False, eids, cum)
+        v_xy, _, v_conic, v_rgb = _C.fit_backward_l2(H, W, gids, bins, packed, ctx.n,
+                                                     ctx.num_int, img, target, grad_loss.float(),
+                                                     cum)
         v_chol, v_mean = _C.project_gaussians_backward_cholesky(ctx.n, ext, cholesky, conics,
                                                                 v_xy, v_conic)
         return v_mean, v_chol, v_rgb, None, None, None

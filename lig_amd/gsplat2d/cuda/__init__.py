@@ -344,15 +344,17 @@ def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_s
     return v_xy, v_xy_abs, v_conic, v_colors, v_opacity
 
 
-def fit_render_l2(img_height, img_width, gaussian_ids_sorted, tile_bins, xys, conics, colors,
-                  opacities, target):
-    """Fused fit-step render (16 x 16 tiles): (out_img [H,W,3], loss [] tensor,
-    packed).  See lig_fit_render_l2 in include/lig_abi.h."""
+def fit_render_l2(img_height, img_width, gaussian_ids_sorted, tile_bins, xys, extents,
+                  cum_tiles_hit, conics, colors, target):
+    """Fused fit-step render (16 x 16 tiles, no opacities): (out_img [H,W,3],
+    loss [] tensor, packed).  Bins from bin_and_sort_fast without emission
+    ids.  See lig_fit_render_l2 in include/lig_abi.h."""
     colors, target = _contig(colors), _contig(target)
     for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),
                       ("tile_bins", tile_bins, _i32), ("xys", xys, _f32),
+                      ("extents", extents, _f32), ("cum_tiles_hit", cum_tiles_hit, _i32),
                       ("conics", conics, _f32), ("colors", colors, _f32),
-                      ("opacities", opacities, _f32), ("target", target, _f32)):
+                      ("target", target, _f32)):
         _check(n_, t, dt)
     H, W = int(img_height), int(img_width)
     if tuple(target.shape) != (H, W, 3):
@@ -364,42 +366,37 @@ def fit_render_l2(img_height, img_width, gaussian_ids_sorted, tile_bins, xys, co
     ws = _workspace(L.size_query("lig_fit_render_workspace_size", H, W), xys)
     with torch.cuda.device(xys.device):
         L.call("lig_fit_render_l2", n, H, W, L.ptr(gaussian_ids_sorted), L.ptr(tile_bins),
-               L.ptr(xys), L.ptr(conics), L.ptr(colors), L.ptr(opacities), L.ptr(packed),
-               L.ptr(target), L.ptr(out_img), L.ptr(loss), L.ptr(ws), ws.numel(),
+               L.ptr(xys), L.ptr(extents), L.ptr(cum_tiles_hit), L.ptr(conics), L.ptr(colors),
+               L.ptr(packed), L.ptr(target), L.ptr(out_img), L.ptr(loss), L.ptr(ws), ws.numel(),
                L.stream_of(xys))
     return out_img, loss, packed
 
 
 def fit_backward_l2(img_height, img_width, gaussian_ids_sorted, tile_bins, packed, num_points,
-                    img, target, grad_loss, has_opacities, emission_ids_sorted, cum_tiles_hit):
-    """Backward of fit_render_l2's loss: (v_xy, v_xy_abs, v_conic, v_colors,
-    v_opacity), deterministic (records + slot-order reduction)."""
+                    num_intersects, img, target, grad_loss, cum_tiles_hit):
+    """Backward of fit_render_l2's loss: (v_xy, v_xy_abs, v_conic, v_colors),
+    deterministic (records at emission slots + slot-order reduction)."""
     img, target, grad_loss = _contig(img), _contig(target), _contig(grad_loss)
     for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),
                       ("tile_bins", tile_bins, _i32), ("img", img, _f32),
                       ("target", target, _f32), ("grad_loss", grad_loss, _f32),
-                      ("emission_ids_sorted", emission_ids_sorted, _i32),
                       ("cum_tiles_hit", cum_tiles_hit, _i32)):
         _check(n_, t, dt)
-    n = int(num_points)
+    n, m = int(num_points), int(num_intersects)
     like = img
     v_xy = _empty((n, 2), like)
     v_xy_abs = _empty((n, 2), like)
     v_conic = _empty((n, 3), like)
     v_colors = _empty((n, 3), like)
-    v_opacity = _empty((n,), like) if has_opacities else None
-    hop = int(bool(has_opacities))
-    m = int(emission_ids_sorted.numel())
     ws = _workspace(L.size_query("lig_rasterize_backward_workspace_size", m), like)
     with torch.cuda.device(like.device):
         L.call("lig_fit_backward_records_l2", n, int(img_height), int(img_width),
                L.ptr(gaussian_ids_sorted), L.ptr(tile_bins), L.ptr(packed), L.ptr(img),
-               L.ptr(target), L.ptr(grad_loss), hop, L.ptr(emission_ids_sorted), m, L.ptr(ws),
-               ws.numel(), L.stream_of(like))
-        L.call("lig_reduce_gradient_records", n, m, L.ptr(cum_tiles_hit), hop, L.ptr(ws),
-               ws.numel(), L.ptr(v_xy), L.ptr(v_xy_abs), L.ptr(v_conic), L.ptr(v_colors),
-               L.ptr(v_opacity), L.stream_of(like))
-    return v_xy, v_xy_abs, v_conic, v_colors, v_opacity
+               L.ptr(target), L.ptr(grad_loss), m, L.ptr(ws), ws.numel(), L.stream_of(like))
+        L.call("lig_reduce_gradient_records", n, m, L.ptr(cum_tiles_hit), 0, L.ptr(ws),
+               ws.numel(), L.ptr(v_xy), L.ptr(v_xy_abs), L.ptr(v_conic), L.ptr(v_colors), None,
+               L.stream_of(like))
+    return v_xy, v_xy_abs, v_conic, v_colors
 
 
 def count_pairs(xys, extents, conics, opacities, num_tiles_hit, img_height, img_width):

@@ -67,7 +67,7 @@ def test_fused_step_matches_api_chain(device, n, H, W, scale, seed):
         b.rgbs.mul_(0.05)   # keep part of the image inside (0, 1) so the loss mask varies
         a.rgbs.copy_(b.rgbs)
     la, lb = a.loss(tgt), b.loss(tgt)
-    assert float(la) == pytest.approx(float(lb), rel=1e-6)
+    assert float(la.detach()) == pytest.approx(float(lb.detach()), rel=1e-6)
     la.backward()
     lb.backward()
     for name in ("means", "cholesky", "rgbs"):
@@ -91,7 +91,7 @@ def test_fused_render_image_matches_api(device):
         xys, ext, con, nt = a.project()
         num, cum = g.compute_cumulative_intersects(nt)
         gids, bins = C.bin_and_sort_fast(a.means.shape[0], num, xys, ext, cum, H, W, 16)
-        img, loss, _ = C.fit_render_l2(H, W, gids, bins, xys, con, a.rgbs, None, tgt)
+        img, loss, _ = C.fit_render_l2(H, W, gids, bins, xys, ext, cum, con, a.rgbs, tgt)
         ref = g.rasterize_gaussians(xys, ext, con, nt, a.rgbs, None, H, W, 16)[0]
     assert torch.equal(img, ref)
     want = ((ref.clamp(0, 1).double() - tgt.double()) ** 2).mean()
@@ -109,7 +109,7 @@ def test_fused_no_intersections(device):
         b.means.add_(1e4)
     tgt = torch.rand(H, W, 3, device=device)
     la, lb = a.loss(tgt), b.loss(tgt)
-    assert float(la) == pytest.approx(float(lb), rel=1e-6)
+    assert float(la.detach()) == pytest.approx(float(lb.detach()), rel=1e-6)
     la.backward()
     lb.backward()
     for name in ("means", "cholesky", "rgbs"):
