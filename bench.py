@@ -46,10 +46,12 @@ def log(msg):
 
 
 def raster_flops(E, C, backward):
-    """Algorithmic FLOPs of the sum rasteriser (FMA = 2; exp, min, compare = 1):
-    17 per pair inside the SnugBox box, plus 7 (forward) or 37 (backward) per
-    contributing pair."""
-    return 17 * E + (37 if backward else 7) * C
+    """Algorithmic FLOPs of the fit step's sum rasteriser (FMA = 2; exp, min,
+    compare = 1): 17 per pair inside the SnugBox box, plus 7 (forward) or 29
+    (backward) per contributing pair.  29 is the reference backward's 37
+    without the 8 of v_xy_abs (two products and two |.| accumulations), which
+    the fit step does not form: no parameter gradient depends on it."""
+    return 17 * E + (29 if backward else 7) * C
 
 
 def raster_bytes(I, T, P, N, backward):

@@ -216,11 +216,14 @@ int lig_reduce_gradient_records(int n, int num_intersects, const int32_t *cum_ti
  * for upstream gradient dL/dI = (0 <= img <= 1) ? (2/(3HW))(img - target) *
  * grad_loss[0] : 0 (the expression of lig_l2_clamped_loss_backward), formed
  * per tile from img and target; no v_wsum, no final index (in sum mode it only
- * skips pairs that the accept test rejects, so the records equal those of
- * lig_rasterize_backward_records fed the forward's final_idx).  Each record
- * lands at its emission slot, computed from `packed`.  Follow with
- * lig_reduce_gradient_records(has_opacities = 0) on the same workspace
- * (lig_rasterize_backward_workspace_size). */
+ * skips pairs that the accept test rejects, so the gradients equal those of
+ * lig_rasterize_backward_records fed the forward's final_idx) and no v_xy_abs
+ * (no parameter gradient depends on it).  Each 32-byte record lands at its
+ * emission slot, computed from `packed`.
+ *
+ * lig_fit_reduce_records: sums each Gaussian's records in slot order into
+ * v_xy [n,2], v_conic [n,3], v_colors [n,3].  Same workspace as the records
+ * call, lig_rasterize_backward_workspace_size(num_intersects). */
 size_t lig_fit_render_workspace_size(int img_height, int img_width);
 int lig_fit_render_l2(int n, int img_height, int img_width, const int32_t *gaussian_ids_sorted,
                       const int32_t *tile_bins, const float *xys, const float *extents,
@@ -232,6 +235,9 @@ int lig_fit_backward_records_l2(int n, int img_height, int img_width,
                                 const void *packed, const float *img, const float *target,
                                 const float *grad_loss, int num_intersects, void *workspace,
                                 size_t workspace_bytes, void *stream);
+int lig_fit_reduce_records(int n, int num_intersects, const int32_t *cum_tiles_hit,
+                           const void *workspace, size_t workspace_bytes, float *v_xy,
+                           float *v_conic, float *v_colors, void *stream);
 
 /* No reference counterpart: workload counters for roofline accounting.
  * counts[0] = E, pixel/Gaussian pairs whose pixel centre lies inside the

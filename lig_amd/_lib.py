@@ -67,6 +67,8 @@ _SIGNATURES = {
     "lig_fit_backward_records_l2": (
         _c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_size,
                  _vp]),
+    "lig_fit_reduce_records": (
+        _c_int, [_c_int, _c_int, _vp, _vp, _c_size, _vp, _vp, _vp, _vp]),
     "lig_count_pairs": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_gradient_aware_upscale_forward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,

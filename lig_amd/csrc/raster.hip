@@ -403,6 +403,12 @@ struct __align__(16) GradRec {
     float4 r0, r1, r2;
 };
 
+// Fit-step gradient record (32 bytes): r0 = (v_r, v_g, v_b, v_conic.a),
+// r1 = (v_conic.b, v_conic.c, v_x, v_y); no v_xy_abs, no opacity.
+struct __align__(16) FitRec {
+    float4 r0, r1;
+};
+
 __device__ __forceinline__ void emit_grads(bool det, GradRec *__restrict__ rec, int slot, int g,
                                            float vr, float vg, float vb, float c0, float c1,
                                            float c2, float gx, float gy, float ax, float ay,
@@ -476,6 +482,31 @@ __global__ __launch_bounds__(256) void reduce_records_kernel(
     if (has_opac) g_opac[g] = s2z;
 }
 
+// Fit-step reduction: sum each Gaussian's FitRec slots [cum[g-1], cum[g]) in
+// slot order (tile row-major).  No phantom slots exist without opacities.
+__global__ __launch_bounds__(256) void reduce_fit_records_kernel(
+    int n, int m, const int32_t *__restrict__ cum, const FitRec *__restrict__ rec,
+    float *__restrict__ g_xy, float *__restrict__ g_conic, float *__restrict__ g_rgb) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= n) return;
+    const int lo = max(0, g == 0 ? 0 : cum[g - 1]);
+    const int hi = min(cum[g], m);
+    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+    for (int e = lo; e < hi; ++e) {
+        const FitRec r = rec[e];
+        s0.x += r.r0.x; s0.y += r.r0.y; s0.z += r.r0.z; s0.w += r.r0.w;
+        s1.x += r.r1.x; s1.y += r.r1.y; s1.z += r.r1.z; s1.w += r.r1.w;
+    }
+    g_rgb[3 * g + 0] = s0.x;
+    g_rgb[3 * g + 1] = s0.y;
+    g_rgb[3 * g + 2] = s0.z;
+    g_conic[3 * g + 0] = s0.w;
+    g_conic[3 * g + 1] = s1.x;
+    g_conic[3 * g + 2] = s1.y;
+    g_xy[2 * g + 0] = s1.z;
+    g_xy[2 * g + 1] = s1.w;
+}
+
 __global__ void add_phantom_kernel(const float *__restrict__ phantom, int has_opac,
                                    float *__restrict__ g_xy, float *__restrict__ g_xy_abs,
                                    float *__restrict__ g_conic, float *__restrict__ g_rgb,
@@ -528,7 +559,7 @@ __device__ __forceinline__ void plain_row_end(PlainAcc &A, float dy) {
 // `continue`: the accumulation runs under the lane mask of accepted pairs and
 // is skipped when no lane accepts.  amin = 1/255 and clampv = 0.99 come in
 // VGPRs.
-template <bool HAS_OPAC>
+template <bool HAS_OPAC, bool ABS = true>
 __device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, float bp,
                                            const RowTerms &rt, float op, float cr, float cg,
                                            float cb, float dx, float amin, float clampv,
@@ -549,12 +580,15 @@ __device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, 
         A.q0 += vsig;
         A.q1 += t1;
         A.sxx = __builtin_fmaf(t1, dx, A.sxx);
-        const float u = __builtin_fmaf(ap, dx, rt.bdy);
-        const float v = __builtin_fmaf(bp, dx, rt.cdy);
-        A.ax = __builtin_fmaf(fabsf(vsig), fabsf(u), A.ax);
-        A.ay = __builtin_fmaf(fabsf(vsig), fabsf(v), A.ay);
+        if constexpr (ABS) {
+            const float u = __builtin_fmaf(ap, dx, rt.bdy);
+            const float v = __builtin_fmaf(bp, dx, rt.cdy);
+            A.ax = __builtin_fmaf(fabsf(vsig), fabsf(u), A.ax);
+            A.ay = __builtin_fmaf(fabsf(vsig), fabsf(v), A.ay);
+        }
     }
 }
+
 
 template <bool HAS_OPAC, bool DET>
 __device__ __forceinline__ void plain_emit(const PlainAcc &A, const GaussPack &gp, bool act,
@@ -693,14 +727,14 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
             if (full == 0xffffu) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
-                    plain_pair<HAS_OPAC>(A, true, ap, bp, rt, op, cr, cg, cb, dxs[lx], amin,
-                                         clampv, row + lx);
+                    plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb, dxs[lx],
+                                                 amin, clampv, row + lx);
             } else if (full == any) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
-                        plain_pair<HAS_OPAC>(A, true, ap, bp, rt, op, cr, cg, cb, dxs[lx],
-                                             amin, clampv, row + lx);
+                        plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb,
+                                                     dxs[lx], amin, clampv, row + lx);
             } else {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
@@ -711,13 +745,20 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
             }
             plain_row_end(A, dy);
         }
-        int slot = 0;
-        if (TRAIN)
-            slot = __float_as_int(gp.p3.z) + ty * __float_as_int(gp.p3.w) + tx;
-        else if (DET && act)
-            slot = slots[base + lane];
-        plain_emit<HAS_OPAC, DET>(A, gp, act, slot, g, rec, g_xy, g_xy_abs, g_conic, g_rgb,
-                                  g_opac);
+        if constexpr (TRAIN) {
+            if (act) {
+                const int slot = __float_as_int(gp.p3.z) + ty * __float_as_int(gp.p3.w) + tx;
+                const float a = gp.p2.y, b = gp.p2.z, c = gp.p2.w;
+                FitRec r;
+                r.r0 = make_float4(A.r, A.g, A.b, 0.5f * A.sxx);
+                r.r1 = make_float4(A.sxy, 0.5f * A.syy, __builtin_fmaf(a, A.mx, b * A.my),
+                                   __builtin_fmaf(b, A.mx, c * A.my));
+                reinterpret_cast<FitRec *>(rec)[slot] = r;
+            }
+        } else {
+            plain_emit<HAS_OPAC, DET>(A, gp, act, DET && act ? slots[base + lane] : 0, g, rec,
+                                      g_xy, g_xy_abs, g_conic, g_rgb, g_opac);
+        }
     }
 }
 
@@ -1167,6 +1208,22 @@ extern "C" int lig_fit_backward_records_l2(int n, int img_height, int img_width,
                        as_stream(stream), tiles_x, img_width, img_height, gaussian_ids_sorted,
                        nullptr, bins, pk, nullptr, nullptr, nullptr, img, target, grad_loss, norm,
                        rec, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return launch_status();
+}
+
+extern "C" int lig_fit_reduce_records(int n, int num_intersects, const int32_t *cum_tiles_hit,
+                                      const void *workspace, size_t workspace_bytes,
+                                      float *v_xy, float *v_conic, float *v_colors,
+                                      void *stream) {
+    if (n <= 0 || num_intersects < 0 || !cum_tiles_hit || !v_xy || !v_conic || !v_colors)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (!workspace || workspace_bytes < lig_rasterize_backward_workspace_size(num_intersects))
+        return LIG_ERR_WORKSPACE;
+    const FitRec *rec = reinterpret_cast<const FitRec *>(
+        reinterpret_cast<const char *>(workspace) + 256);
+    hipLaunchKernelGGL(reduce_fit_records_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
+                       as_stream(stream), n, num_intersects, cum_tiles_hit, rec, v_xy, v_conic,
+                       v_colors);
     return launch_status();
 }
 

@@ -65,7 +65,7 @@ class _RenderL2(torch.autograd.Function):
             return z[0], z[1], z[2], None, None, None
         cholesky, ext, conics, gids, bins, cum, img, target, packed = ctx.saved_tensors
         H, W = ctx.hw
-        v_xy, _, v_conic, v_rgb = _C.fit_backward_l2(H, W, gids, bins, packed, ctx.n,
+        v_xy, v_conic, v_rgb = _C.fit_backward_l2(H, W, gids, bins, packed, ctx.n,
                                                      ctx.num_int, img, target, grad_loss.float(),
                                                      cum)
         v_chol, v_mean = _C.project_gaussians_backward_cholesky(ctx.n, ext, cholesky, conics,

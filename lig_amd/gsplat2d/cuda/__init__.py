@@ -374,7 +374,7 @@ def fit_render_l2(img_height, img_width, gaussian_ids_sorted, tile_bins, xys, ex
 
 def fit_backward_l2(img_height, img_width, gaussian_ids_sorted, tile_bins, packed, num_points,
                     num_intersects, img, target, grad_loss, cum_tiles_hit):
-    """Backward of fit_render_l2's loss: (v_xy, v_xy_abs, v_conic, v_colors),
+    """Backward of fit_render_l2's loss: (v_xy, v_conic, v_colors),
     deterministic (records at emission slots + slot-order reduction)."""
     img, target, grad_loss = _contig(img), _contig(target), _contig(grad_loss)
     for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),
@@ -385,7 +385,6 @@ def fit_backward_l2(img_height, img_width, gaussian_ids_sorted, tile_bins, packe
     n, m = int(num_points), int(num_intersects)
     like = img
     v_xy = _empty((n, 2), like)
-    v_xy_abs = _empty((n, 2), like)
     v_conic = _empty((n, 3), like)
     v_colors = _empty((n, 3), like)
     ws = _workspace(L.size_query("lig_rasterize_backward_workspace_size", m), like)
@@ -393,10 +392,9 @@ def fit_backward_l2(img_height, img_width, gaussian_ids_sorted, tile_bins, packe
         L.call("lig_fit_backward_records_l2", n, int(img_height), int(img_width),
                L.ptr(gaussian_ids_sorted), L.ptr(tile_bins), L.ptr(packed), L.ptr(img),
                L.ptr(target), L.ptr(grad_loss), m, L.ptr(ws), ws.numel(), L.stream_of(like))
-        L.call("lig_reduce_gradient_records", n, m, L.ptr(cum_tiles_hit), 0, L.ptr(ws),
-               ws.numel(), L.ptr(v_xy), L.ptr(v_xy_abs), L.ptr(v_conic), L.ptr(v_colors), None,
-               L.stream_of(like))
-    return v_xy, v_xy_abs, v_conic, v_colors
+        L.call("lig_fit_reduce_records", n, m, L.ptr(cum_tiles_hit), L.ptr(ws), ws.numel(),
+               L.ptr(v_xy), L.ptr(v_conic), L.ptr(v_colors), L.stream_of(like))
+    return v_xy, v_conic, v_colors
 
 
 def count_pairs(xys, extents, conics, opacities, num_tiles_hit, img_height, img_width):
