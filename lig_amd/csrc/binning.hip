@@ -164,11 +164,15 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const K *__res
         hist[(size_t)d * nblocks + blockIdx.x] = h[d];
 }
 
-// Scatter of one pass.  The block's 4096 pairs are first ranked stably into
-// LDS in digit order (wave ballots give the rank among equal digits of a
-// wave, LDS counters the offset of earlier waves and rounds), then written
-// out in LDS order, so that each digit's run goes to consecutive global
-// addresses (coalesced stores instead of one scattered store per pair).
+// Scatter of one pass.  Each wave owns 1024 consecutive pairs of the block's
+// 4096 and ranks them round by round (64 at a time) against wave-private
+// digit counters in LDS: lanes with equal digits are matched by RB ballots,
+// the round's leader per digit advances the counter.  No block barrier is
+// needed until all rounds are ranked; then the per-wave counts are turned into
+// per-(wave, digit) starts, the pairs are placed in LDS in (digit, input
+// order) order, and written out so that each digit's run goes to consecutive
+// global addresses.  Order within a digit is wave, round, lane: input order,
+// so the pass is stable.
 template <typename K, int RB>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
@@ -176,11 +180,11 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const int32_t *__restrict__ offsets, int n, int shift, K flip, int nblocks) {
     constexpr int NB = 1 << RB;
     constexpr int PER = NB / kSortThreads > 0 ? NB / kSortThreads : 1;  // digits per thread
+    constexpr int WSPAN = kSortRounds * LIG_WAVE;                        // pairs per wave
     __shared__ K skey[kSortTile];
     __shared__ uint32_t sval[kSortTile];
-    __shared__ int lstart[NB];  // block-local exclusive start of each digit
-    __shared__ int lrun[NB];    // running local position per digit
-    __shared__ int wcnt[kSortWaves][NB];
+    __shared__ int wcnt[kSortWaves][NB];  // per-wave digit counts, then per-wave digit starts
+    __shared__ int lstart[NB];            // block-local exclusive start of each digit
     __shared__ int wsum[kSortWaves];
     const int tid = threadIdx.x;
     const int lane = tid & (LIG_WAVE - 1);
@@ -188,52 +192,33 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (LIG_WAVE - lane));
     const size_t base = (size_t)blockIdx.x * kSortTile;
     const int count = (int)min((size_t)kSortTile, (size_t)n - base);
+    const int wbase = wave * WSPAN;
 
     for (int d = tid; d < NB; d += kSortThreads) {
-        lrun[d] = 0;
 #pragma unroll
         for (int w = 0; w < kSortWaves; ++w) wcnt[w][d] = 0;
     }
-    __syncthreads();
     K key[kSortRounds];
     uint32_t val[kSortRounds];
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
-        const int j = r * kSortThreads + tid;
+        const int j = wbase + r * LIG_WAVE + lane;
         key[r] = 0;
         val[r] = 0;
         if (j < count) {
             key[r] = keys_in[base + j];
             val[r] = vals_in ? vals_in[base + j] : (uint32_t)(base + j);
-            atomicAdd(&lrun[digit_of<K, RB>(key[r], shift, flip)], 1);
         }
     }
     __syncthreads();
-    // block exclusive scan of the digit counts (PER consecutive digits per thread)
-    int c[PER], sum = 0;
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int d = tid * PER + q;
-        c[q] = d < NB ? lrun[d] : 0;
-        sum += c[q];
-    }
-    int total;
-    int pre = block_exclusive_scan(sum, wsum, &total);
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int d = tid * PER + q;
-        if (d < NB) {
-            lstart[d] = pre;
-            lrun[d] = pre;
-        }
-        pre += c[q];
-    }
-    __syncthreads();
-    // stable ranking into LDS, round by round
+    // wave-local stable ranking: lpos = position among the wave's pairs of the same digit
+    int lpos[kSortRounds];
+    int *cnt_w = wcnt[wave];
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
-        if (r * kSortThreads >= count) break;  // uniform across the block
-        const bool valid = r * kSortThreads + tid < count;
+        lpos[r] = 0;
+        if (wbase + r * LIG_WAVE >= count) continue;  // wave-uniform
+        const bool valid = wbase + r * LIG_WAVE + lane < count;
         const unsigned d = valid ? digit_of<K, RB>(key[r], shift, flip) : 0u;
         uint64_t m = __ballot(valid);
 #pragma unroll
@@ -243,23 +228,52 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             m &= bit ? bb : ~bb;
         }
         const int rank = __popcll(m & lt_mask);
-        const bool leader = valid && rank == 0;
-        const int cnt = __popcll(m);
-        if (leader) wcnt[wave][d] = cnt;
-        __syncthreads();
-        if (valid) {
-            int pos = lrun[d] + rank;
+        const int before = valid ? cnt_w[d] : 0;
+        lpos[r] = before + rank;
+        // one lane per digit advances the wave's counter; LDS operations of a
+        // wave complete in order, so the next round reads the new value
+        if (valid && rank == 0) cnt_w[d] = before + __popcll(m);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // block exclusive scan of the digit totals; per-(wave, digit) starts
+    int c[PER], sum = 0;
 #pragma unroll
-            for (int w = 0; w < kSortWaves; ++w) pos += w < wave ? wcnt[w][d] : 0;
+    for (int q = 0; q < PER; ++q) {
+        const int d = tid * PER + q;
+        int t = 0;
+        if (d < NB) {
+#pragma unroll
+            for (int w = 0; w < kSortWaves; ++w) t += wcnt[w][d];
+        }
+        c[q] = t;
+        sum += t;
+    }
+    int total;
+    int pre = block_exclusive_scan(sum, wsum, &total);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int d = tid * PER + q;
+        if (d < NB) {
+            lstart[d] = pre;
+            int run = pre;
+#pragma unroll
+            for (int w = 0; w < kSortWaves; ++w) {
+                const int t = wcnt[w][d];
+                wcnt[w][d] = run;
+                run += t;
+            }
+        }
+        pre += c[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        if (wbase + r * LIG_WAVE + lane < count) {
+            const int pos = cnt_w[digit_of<K, RB>(key[r], shift, flip)] + lpos[r];
             skey[pos] = key[r];
             sval[pos] = val[r];
         }
-        __syncthreads();
-        if (leader) {
-            atomicAdd(&lrun[d], cnt);
-            wcnt[wave][d] = 0;
-        }
-        __syncthreads();
     }
     // global position of local slot j of digit d: offsets[d] + (j - lstart[d])
     for (int d = tid; d < NB; d += kSortThreads)
