@@ -484,19 +484,47 @@ __global__ __launch_bounds__(256) void reduce_records_kernel(
 
 // Fit-step reduction: sum each Gaussian's FitRec slots [cum[g-1], cum[g]) in
 // slot order (tile row-major).  No phantom slots exist without opacities.
-__global__ __launch_bounds__(256) void reduce_fit_records_kernel(
+// One wave per 64 consecutive Gaussians: their records form one contiguous
+// range, streamed through LDS in chunks with coalesced 16-byte loads; each
+// lane then adds its own Gaussian's records of the chunk from LDS, in slot
+// order.
+constexpr int kRedChunk = 256;  // records per LDS chunk (8 KB)
+
+__global__ __launch_bounds__(LIG_WAVE) void reduce_fit_records_kernel(
     int n, int m, const int32_t *__restrict__ cum, const FitRec *__restrict__ rec,
     float *__restrict__ g_xy, float *__restrict__ g_conic, float *__restrict__ g_rgb) {
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= n) return;
-    const int lo = max(0, g == 0 ? 0 : cum[g - 1]);
-    const int hi = min(cum[g], m);
-    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
-    for (int e = lo; e < hi; ++e) {
-        const FitRec r = rec[e];
-        s0.x += r.r0.x; s0.y += r.r0.y; s0.z += r.r0.z; s0.w += r.r0.w;
-        s1.x += r.r1.x; s1.y += r.r1.y; s1.z += r.r1.z; s1.w += r.r1.w;
+    __shared__ float4 srec[2 * kRedChunk];
+    const int lane = threadIdx.x;
+    const int g0 = blockIdx.x * LIG_WAVE;
+    const int g = g0 + lane;
+    const int glast = min(g0 + LIG_WAVE, n) - 1;
+    int lo = 0, hi = 0;
+    if (g < n) {
+        lo = max(0, g == 0 ? 0 : cum[g - 1]);
+        hi = min(cum[g], m);
     }
+    const int wlo = max(0, g0 == 0 ? 0 : cum[g0 - 1]);
+    const int whi = min(cum[glast], m);
+    const float4 *src = reinterpret_cast<const float4 *>(rec);
+    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+    for (int c0 = wlo; c0 < whi; c0 += kRedChunk) {
+        const int cn = min(kRedChunk, whi - c0);
+#pragma unroll
+        for (int k = 0; k < 2 * kRedChunk / LIG_WAVE; ++k) {
+            const int q = k * LIG_WAVE + lane;
+            if (q < 2 * cn) srec[q] = src[2 * (size_t)c0 + q];
+        }
+        __syncthreads();
+        const int e0 = max(lo, c0), e1 = min(hi, c0 + cn);
+        for (int e = e0; e < e1; ++e) {
+            const float4 a = srec[2 * (e - c0)];
+            const float4 b = srec[2 * (e - c0) + 1];
+            s0.x += a.x; s0.y += a.y; s0.z += a.z; s0.w += a.w;
+            s1.x += b.x; s1.y += b.y; s1.z += b.z; s1.w += b.w;
+        }
+        __syncthreads();
+    }
+    if (g >= n) return;
     g_rgb[3 * g + 0] = s0.x;
     g_rgb[3 * g + 1] = s0.y;
     g_rgb[3 * g + 2] = s0.z;
@@ -1221,7 +1249,7 @@ extern "C" int lig_fit_reduce_records(int n, int num_intersects, const int32_t *
         return LIG_ERR_WORKSPACE;
     const FitRec *rec = reinterpret_cast<const FitRec *>(
         reinterpret_cast<const char *>(workspace) + 256);
-    hipLaunchKernelGGL(reduce_fit_records_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(reduce_fit_records_kernel, dim3(ceil_div(n, LIG_WAVE)), dim3(LIG_WAVE), 0,
                        as_stream(stream), n, num_intersects, cum_tiles_hit, rec, v_xy, v_conic,
                        v_colors);
     return launch_status();
