@@ -441,26 +441,53 @@ __device__ __forceinline__ void emit_grads(bool det, GradRec *__restrict__ rec, 
 // Sum each Gaussian's records in slot order (tile row-major) and write its
 // gradients.  Slots of a zero-extent Gaussian hold records of Gaussian 0 (the
 // reference bins them as (tile 0, gid 0)); their sums go to `phantom` and are
-// added to Gaussian 0 by add_phantom_kernel.
-__global__ __launch_bounds__(256) void reduce_records_kernel(
+// added to Gaussian 0 by add_phantom_kernel.  One wave per 64 consecutive
+// Gaussians streams their contiguous record range through LDS (coalesced
+// 16-byte loads); each lane adds its own Gaussian's records in slot order.
+constexpr int kRedChunk = 256;  // records per LDS chunk
+
+__global__ __launch_bounds__(LIG_WAVE) void reduce_records_kernel(
     int n, int m, const int32_t *__restrict__ cum, const GradRec *__restrict__ rec,
     int has_opac, float *__restrict__ g_xy, float *__restrict__ g_xy_abs,
     float *__restrict__ g_conic, float *__restrict__ g_rgb, float *__restrict__ g_opac,
     float *__restrict__ phantom) {
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= n) return;
-    const int lo = max(0, g == 0 ? 0 : cum[g - 1]);
-    const int hi = min(cum[g], m);
+    __shared__ float4 srec[3 * kRedChunk];
+    const int lane = threadIdx.x;
+    const int g0 = blockIdx.x * LIG_WAVE;
+    const int g = g0 + lane;
+    const int glast = min(g0 + LIG_WAVE, n) - 1;
+    int lo = 0, hi = 0;
+    if (g < n) {
+        lo = max(0, g == 0 ? 0 : cum[g - 1]);
+        hi = min(cum[g], m);
+    }
+    const int wlo = max(0, g0 == 0 ? 0 : cum[g0 - 1]);
+    const int whi = min(cum[glast], m);
+    const float4 *src = reinterpret_cast<const float4 *>(rec);
     float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
     float s2x = 0.f, s2y = 0.f, s2z = 0.f;
     bool foreign = false;
-    for (int e = lo; e < hi; ++e) {
-        const GradRec r = rec[e];
-        foreign = __float_as_int(r.r2.w) != g;
-        s0.x += r.r0.x; s0.y += r.r0.y; s0.z += r.r0.z; s0.w += r.r0.w;
-        s1.x += r.r1.x; s1.y += r.r1.y; s1.z += r.r1.z; s1.w += r.r1.w;
-        s2x += r.r2.x; s2y += r.r2.y; s2z += r.r2.z;
+    for (int c0 = wlo; c0 < whi; c0 += kRedChunk) {
+        const int cn = min(kRedChunk, whi - c0);
+#pragma unroll
+        for (int k = 0; k < 3 * kRedChunk / LIG_WAVE; ++k) {
+            const int q = k * LIG_WAVE + lane;
+            if (q < 3 * cn) srec[q] = src[3 * (size_t)c0 + q];
+        }
+        __syncthreads();
+        const int e0 = max(lo, c0), e1 = min(hi, c0 + cn);
+        for (int e = e0; e < e1; ++e) {
+            const float4 a = srec[3 * (e - c0)];
+            const float4 b = srec[3 * (e - c0) + 1];
+            const float4 c = srec[3 * (e - c0) + 2];
+            foreign = __float_as_int(c.w) != g;
+            s0.x += a.x; s0.y += a.y; s0.z += a.z; s0.w += a.w;
+            s1.x += b.x; s1.y += b.y; s1.z += b.z; s1.w += b.w;
+            s2x += c.x; s2y += c.y; s2z += c.z;
+        }
+        __syncthreads();
     }
+    if (g >= n) return;
     if (foreign) {
         const float v[11] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w, s2x, s2y, s2z};
 #pragma unroll
@@ -488,7 +515,6 @@ __global__ __launch_bounds__(256) void reduce_records_kernel(
 // range, streamed through LDS in chunks with coalesced 16-byte loads; each
 // lane then adds its own Gaussian's records of the chunk from LDS, in slot
 // order.
-constexpr int kRedChunk = 256;  // records per LDS chunk (8 KB)
 
 __global__ __launch_bounds__(LIG_WAVE) void reduce_fit_records_kernel(
     int n, int m, const int32_t *__restrict__ cum, const FitRec *__restrict__ rec,
@@ -558,28 +584,35 @@ __global__ void add_phantom_kernel(const float *__restrict__ phantom, int has_op
 //   v_conic += (v_sigma dx^2 / 2, v_sigma dx dy, v_sigma dy^2 / 2),
 //   v_xy += v_sigma (a dx + b dy, b dx + c dy),  v_xy_abs += |that|,
 //   v_opacity += vis * v_alpha.
-// Moments: within a pixel row dy is fixed, so per pixel only q0 = sum v_sigma,
-// q1 = sum v_sigma dx and sxx = sum v_sigma dx^2 are accumulated; at the end of
-// the row q0 and q1 fold into sum v_sigma dy (my), v_sigma dy^2 (syy) and
-// v_sigma dx dy (sxy).  v_xy is formed once at the end as (a mx + b my,
+// Moments: within a pixel row dy is fixed and dx = D - lx, so per pixel only
+// q_k = sum v_sigma lx^k (k = 0, 1, 2; lx the column in the tile, a constant of
+// the unrolled pixel loop) are accumulated; at the end of the row they fold
+// into sum v_sigma dx (mx), v_sigma dx^2 (sxx), v_sigma dy (my), v_sigma dy^2
+// (syy) and v_sigma dx dy (sxy).  v_xy is formed once at the end as (a mx + b my,
 // b mx + c my); v_xy_abs accumulates |v_sigma| |u'| with the scaled conic's
 // u' = K (a dx + b dy) and is divided by K once at the end.
 struct PlainAcc {
     float r, g, b, sxx, sxy, syy, mx, my, ax, ay, o;
-    float q0, q1;  // current row
+    float q0, q1, q2;  // current row: sum v_sigma * lx^k, k = 0, 1, 2
 };
 
 __device__ __forceinline__ void plain_acc_zero(PlainAcc &A) {
     A.r = A.g = A.b = A.sxx = A.sxy = A.syy = A.mx = A.my = A.ax = A.ay = A.o = 0.0f;
-    A.q0 = A.q1 = 0.0f;
+    A.q0 = A.q1 = A.q2 = 0.0f;
 }
 
-__device__ __forceinline__ void plain_row_end(PlainAcc &A, float dy) {
-    A.mx += A.q1;
+// End of a pixel row: with dx = D - lx (D = mean.x - (x_org + 0.5), lx the
+// pixel's column in the tile), sum v_sigma dx = D q0 - q1 and
+// sum v_sigma dx^2 = D (D q0 - 2 q1) + q2; the dy moments take dy once.
+__device__ __forceinline__ void plain_row_end(PlainAcc &A, float D, float dy) {
+    const float s1 = __builtin_fmaf(D, A.q0, -A.q1);
+    const float s2 = __builtin_fmaf(D, __builtin_fmaf(D, A.q0, -2.0f * A.q1), A.q2);
+    A.mx += s1;
+    A.sxx += s2;
     A.my = __builtin_fmaf(dy, A.q0, A.my);
-    A.sxy = __builtin_fmaf(dy, A.q1, A.sxy);
+    A.sxy = __builtin_fmaf(dy, s1, A.sxy);
     A.syy = __builtin_fmaf(dy * A.q0, dy, A.syy);
-    A.q0 = A.q1 = 0.0f;
+    A.q0 = A.q1 = A.q2 = 0.0f;
 }
 
 // One pair of the plain backward.  Rejected pairs (lane test, sigma < 0,
@@ -590,8 +623,8 @@ __device__ __forceinline__ void plain_row_end(PlainAcc &A, float dy) {
 template <bool HAS_OPAC, bool ABS = true>
 __device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, float bp,
                                            const RowTerms &rt, float op, float cr, float cg,
-                                           float cb, float dx, float amin, float clampv,
-                                           const float4 *__restrict__ wp) {
+                                           float cb, float dx, float lx, float amin,
+                                           float clampv, const float4 *__restrict__ wp) {
     const float4 w = *wp;
     const float s = pair_s(ap, rt, dx);
     const float vis = fast_exp2(-s);
@@ -604,10 +637,9 @@ __device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, 
         A.b = __builtin_fmaf(al, w.z, A.b);
         if constexpr (HAS_OPAC) A.o = __builtin_fmaf(vis, va, A.o);
         const float vsig = -al * va;
-        const float t1 = vsig * dx;
         A.q0 += vsig;
-        A.q1 += t1;
-        A.sxx = __builtin_fmaf(t1, dx, A.sxx);
+        A.q1 = __builtin_fmaf(vsig, lx, A.q1);
+        A.q2 = __builtin_fmaf(vsig, lx * lx, A.q2);
         if constexpr (ABS) {
             const float u = __builtin_fmaf(ap, dx, rt.bdy);
             const float v = __builtin_fmaf(bp, dx, rt.cdy);
@@ -664,6 +696,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
         const GaussPack gp = pack[g];
         PlainAcc A;
         plain_acc_zero(A);
+        const float D = gp.p0.x - ((float)x_org + 0.5f);
         for (int ly = 0; ly < bw; ++ly) {
             const float dy = gp.p0.y - ((float)(y_org + ly) + 0.5f);
             const RowTerms rt = row_terms(gp.p0.w, gp.p1.x, dy);
@@ -673,9 +706,10 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
                 const int fp = sfin[p];
                 if (fp < base) continue;  // wave-uniform: nothing in this batch reaches p
                 plain_pair<HAS_OPAC>(A, kidx <= fp, gp.p0.z, gp.p0.w, rt, gp.p1.y, gp.p1.z,
-                                     gp.p1.w, gp.p2.x, gp.p0.x - pxf, amin, clampv, sv + p);
+                                     gp.p1.w, gp.p2.x, gp.p0.x - pxf, (float)lx, amin, clampv,
+                                     sv + p);
             }
-            plain_row_end(A, dy);
+            plain_row_end(A, D, dy);
         }
         plain_emit<HAS_OPAC, DET>(A, gp, act, DET && act ? slots[base + lane] : 0, g, rec, g_xy,
                                   g_xy_abs, g_conic, g_rgb, g_opac);
@@ -756,22 +790,22 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb, dxs[lx],
-                                                 amin, clampv, row + lx);
+                                                 (float)lx, amin, clampv, row + lx);
             } else if (full == any) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
                         plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb,
-                                                     dxs[lx], amin, clampv, row + lx);
+                                                     dxs[lx], (float)lx, amin, clampv, row + lx);
             } else {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
                         plain_pair<HAS_OPAC>(A, kidx <= __builtin_amdgcn_readlane(fcol, lx), ap,
-                                             bp, rt, op, cr, cg, cb, dxs[lx], amin, clampv,
-                                             row + lx);
+                                             bp, rt, op, cr, cg, cb, dxs[lx], (float)lx, amin,
+                                             clampv, row + lx);
             }
-            plain_row_end(A, dy);
+            plain_row_end(A, dxs[0], dy);
         }
         if constexpr (TRAIN) {
             if (act) {
@@ -1273,9 +1307,9 @@ extern "C" int lig_reduce_gradient_records(int n, int num_intersects,
     const GradRec *rec =
         reinterpret_cast<const GradRec *>(reinterpret_cast<const char *>(workspace) + 256);
     hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, st, phantom, (size_t)16);
-    hipLaunchKernelGGL(reduce_records_kernel, dim3(ceil_div((size_t)n, 256)), dim3(256), 0, st, n,
-                       num_intersects, cum_tiles_hit, rec, has_opacities, v_xy, v_xy_abs,
-                       v_conic, v_colors, v_opacity, phantom);
+    hipLaunchKernelGGL(reduce_records_kernel, dim3(ceil_div((size_t)n, LIG_WAVE)), dim3(LIG_WAVE),
+                       0, st, n, num_intersects, cum_tiles_hit, rec, has_opacities, v_xy,
+                       v_xy_abs, v_conic, v_colors, v_opacity, phantom);
     hipLaunchKernelGGL(add_phantom_kernel, dim3(1), dim3(64), 0, st, (const float *)phantom,
                        has_opacities, v_xy, v_xy_abs, v_conic, v_colors, v_opacity);
     return launch_status();
