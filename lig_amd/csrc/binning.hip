@@ -491,22 +491,32 @@ __global__ void fill_i32_kernel(int32_t *__restrict__ p, size_t count, int32_t v
 }
 
 // Bin edges from tile ids sorted ascending (forward.cu:148-170 semantics on
-// a zero-filled bins array).
-template <typename K, int SHIFT>
-__global__ void bin_edges_kernel(int m, const K *__restrict__ sorted, int num_tiles,
-                                 int32_t *__restrict__ bins) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const int cur = (int)(sorted[i] >> SHIFT);
+// a zero-filled bins array).  Each thread handles 4 consecutive entries.
+__device__ __forceinline__ void bin_edge_at(int i, int m, int prev, int cur, int num_tiles,
+                                            int32_t *__restrict__ bins) {
     if (cur < 0 || cur >= num_tiles) return;
     if (i == 0) bins[2 * cur] = 0;
     if (i == m - 1) bins[2 * cur + 1] = m;
     if (i == 0) return;
-    const int prev = (int)(sorted[i - 1] >> SHIFT);
     if (prev != cur) {
         if (prev >= 0 && prev < num_tiles) bins[2 * prev + 1] = i;
         bins[2 * cur] = i;
     }
+}
+
+template <typename K, int SHIFT>
+__global__ void bin_edges_kernel(int m, const K *__restrict__ sorted, int num_tiles,
+                                 int32_t *__restrict__ bins) {
+    const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i0 >= (size_t)m) return;
+    int t[5];
+    t[0] = i0 > 0 ? (int)(sorted[i0 - 1] >> SHIFT) : -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        t[k + 1] = i0 + k < (size_t)m ? (int)(sorted[i0 + k] >> SHIFT) : -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (i0 + k < (size_t)m) bin_edge_at((int)(i0 + k), m, t[k], t[k + 1], num_tiles, bins);
 }
 
 int bits_for(int num_tiles) {
@@ -586,7 +596,7 @@ extern "C" int lig_get_tile_bin_edges(int num_intersects, const int64_t *isect_i
                        2 * nt, 0);
     if (num_intersects > 0) {
         if (!isect_ids_sorted) return LIG_ERR_INVALID_ARGUMENT;
-        hipLaunchKernelGGL((bin_edges_kernel<int64_t, 32>), dim3(ceil_div(num_intersects, 256)),
+        hipLaunchKernelGGL((bin_edges_kernel<int64_t, 32>), dim3(ceil_div(num_intersects, 1024)),
                            dim3(256), 0, st, num_intersects, isect_ids_sorted, (int)nt,
                            tile_bins);
     }
@@ -642,7 +652,7 @@ extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float
                                       reinterpret_cast<uint32_t *>(gaussian_ids_sorted), m,
                                       bits_for(nt), 0u, p, st);
         if (rc != LIG_OK) return rc;
-        hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(m, 256)), dim3(256), 0,
+        hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(m, 1024)), dim3(256), 0,
                            st, num_intersects, (const uint32_t *)keys_sorted, nt, tile_bins);
         return launch_status();
     }

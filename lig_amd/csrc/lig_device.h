@@ -62,7 +62,7 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
 }
 
 // Per-Gaussian record the rasteriser kernels read (64 bytes, 16-byte aligned):
-//   p0 = (mean.x, mean.y, a', b')   p1 = (c', opacity, red, green)
+//   p0 = (mean.x, mean.y, a', 2b')  p1 = (c', opacity, red, green)
 //   p2 = (blue, a, b, c)            p3 = (cull_x, cull_y, slot base, slot width)
 // with (a, b, c) the conic and (a', b', c') = K * (a, b, c), K = log2(e)/2, so
 // that with d = mean - pixel, u' = a'dx + b'dy, v' = b'dx + c'dy:
@@ -84,19 +84,20 @@ constexpr float kAlphaMin = 1.0f / 255.0f;
 // sigma*log2(e) of a pair from the scaled conic (a', b', c'), evaluated with
 // explicit FMAs so that every kernel computes bit-identical values for the
 // same pair (the forward's accept test and the backward's must agree):
-//   s = fma(dx, fma(a', dx, b2dy), c2dy),  b2dy = (b' * dy) * 2,  c2dy = (c' * dy) * dy.
-// The two row terms depend on dy only and are hoisted out of loops over a row
-// (row_terms).
+//   s = fma(dx, fma(a', dx, b2dy), c2dy),  b2dy = (2b') * dy,  c2dy = (c' * dy) * dy.
+// The row terms depend on dy only and are hoisted out of loops over a row
+// (row_terms takes 2b' as packed); bdy = b' dy and cdy = c' dy serve the
+// backward's u' = a'dx + b'dy and v' = b'dx + c'dy.
 struct RowTerms {
     float bdy, cdy;    // b' dy, c' dy
     float b2dy, c2dy;  // 2 b' dy, c' dy^2
 };
 
-__device__ __forceinline__ RowTerms row_terms(float bp, float cp, float dy) {
+__device__ __forceinline__ RowTerms row_terms(float bp2, float cp, float dy) {
     RowTerms r;
-    r.bdy = bp * dy;
+    r.b2dy = bp2 * dy;
+    r.bdy = r.b2dy * 0.5f;  // exact
     r.cdy = cp * dy;
-    r.b2dy = r.bdy * 2.0f;
     r.c2dy = r.cdy * dy;
     return r;
 }

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restric
     const float a = conics[3 * i], b = conics[3 * i + 1], c = conics[3 * i + 2];
     const float o = opacities ? opacities[i] : 1.0f;
     GaussPack g;
-    g.p0 = make_float4(xys[2 * i], xys[2 * i + 1], a * kHalfLog2e, b * kHalfLog2e);
+    g.p0 = make_float4(xys[2 * i], xys[2 * i + 1], a * kHalfLog2e, (b * kHalfLog2e) * 2.0f);
     g.p1 = make_float4(c * kHalfLog2e, o, colors[3 * i], colors[3 * i + 1]);
     g.p2 = make_float4(colors[3 * i + 2], a, b, c);
     // Cull box: for a positive definite conic, alpha >= 1/255 needs
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
                 const int p = ly * bw + lx;
                 const int fp = sfin[p];
                 if (fp < base) continue;  // wave-uniform: nothing in this batch reaches p
-                plain_pair<HAS_OPAC>(A, kidx <= fp, gp.p0.z, gp.p0.w, rt, gp.p1.y, gp.p1.z,
+                plain_pair<HAS_OPAC>(A, kidx <= fp, gp.p0.z, 0.5f * gp.p0.w, rt, gp.p1.y, gp.p1.z,
                                      gp.p1.w, gp.p2.x, gp.p0.x - pxf, (float)lx, amin, clampv,
                                      sv + p);
             }
@@ -764,7 +764,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
         const int kidx = act ? base + lane : INT32_MAX;
         const int g = act ? gids[base + lane] : 0;
         const GaussPack gp = pack[g];
-        const float mx = gp.p0.x, my = gp.p0.y, ap = gp.p0.z, bp = gp.p0.w;
+        const float mx = gp.p0.x, my = gp.p0.y, ap = gp.p0.z, bp2 = gp.p0.w, bp = 0.5f * bp2;
         const float cp = gp.p1.x, op = gp.p1.y, cr = gp.p1.z, cg = gp.p1.w, cb = gp.p2.x;
         // exact: x_org + lx + 0.5 is representable, one rounding in the difference
         float dxs[16];
@@ -784,7 +784,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
                 full = (uint32_t)__ballot(fcol >= full_from) & 0xffffu;
             }
             const float dy = my - ((float)(y_org + ly) + 0.5f);
-            const RowTerms rt = row_terms(bp, cp, dy);
+            const RowTerms rt = row_terms(bp2, cp, dy);
             const float4 *row = sv + in_vgpr(ly * 16);
             if (full == 0xffffu) {
 #pragma unroll
@@ -858,7 +858,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
         const int kidx = act ? base + lane : INT32_MAX;
         const int g = act ? gids[base + lane] : 0;
         const GaussPack gp = pack[g];
-        const float mx = gp.p0.x, my = gp.p0.y, ap = gp.p0.z, bp = gp.p0.w;
+        const float mx = gp.p0.x, my = gp.p0.y, ap = gp.p0.z, bp2 = gp.p0.w;
         const float cp = gp.p1.x, op = gp.p1.y, cr = gp.p1.z, cg = gp.p1.w;
         const float cb = gp.p2.x, ka = gp.p2.y, kb = gp.p2.z, kc = gp.p2.w;
 
@@ -868,7 +868,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
 
         for (int ly = 0; ly < bw; ++ly) {
             const float dy = my - ((float)(y_org + ly) + 0.5f);
-            const RowTerms rt = row_terms(bp, cp, dy);
+            const RowTerms rt = row_terms(bp2, cp, dy);
             float pxf = (float)x_org + 0.5f;
             for (int lx = 0; lx < bw; ++lx, pxf += 1.0f) {
                 const int p = ly * bw + lx;
