@@ -24,6 +24,30 @@ constexpr int kScanTile = kScanThreads * kScanItems;
 
 // Exclusive scan of one int per thread across a 256-thread block.  `wsum` is
 // 4 ints of LDS.  Returns the exclusive prefix; *total gets the block sum.
+template <int THREADS>
+__device__ __forceinline__ int block_exclusive_scan_n(int v, int *wsum, int *total) {
+    const int lane = threadIdx.x & (LIG_WAVE - 1);
+    const int wave = threadIdx.x / LIG_WAVE;
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < LIG_WAVE; off <<= 1) {
+        const int y = __shfl_up(x, off, LIG_WAVE);
+        if (lane >= off) x += y;
+    }
+    if (lane == LIG_WAVE - 1) wsum[wave] = x;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < THREADS / LIG_WAVE; ++w) {
+        const int s = wsum[w];
+        before += w < wave ? s : 0;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
 __device__ __forceinline__ int block_exclusive_scan(int v, int *wsum, int *total) {
     const int lane = threadIdx.x & (LIG_WAVE - 1);
     const int wave = threadIdx.x / LIG_WAVE;
@@ -128,14 +152,16 @@ void scan_i32(const int32_t *in, int32_t *out, size_t n, bool inclusive, int32_t
 
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort of (key, uint32 value) pairs, RB-bit digits (RB <= 10).
-// Each 256-thread block owns 4096 consecutive pairs: a histogram kernel
+// Each 512-thread block owns 4096 consecutive pairs: a histogram kernel
 // counts digits per block (digit-major table), an exclusive scan turns the
 // table into output offsets, and a scatter kernel ranks each key stably
 // (wave ballots for lane order, LDS counters for wave and round order).
 // A NULL value array stands for the identity (value = input index).
 // ---------------------------------------------------------------------------
-constexpr int kSortThreads = 256;
-constexpr int kSortRounds = 16;
+// 8 waves per block, 512 pairs per wave: measured faster than 4 x 1024 and
+// than 8192-pair blocks at both 79 M and 39 M intersections.
+constexpr int kSortThreads = 512;
+constexpr int kSortRounds = 8;
 constexpr int kSortTile = kSortThreads * kSortRounds;
 constexpr int kSortWaves = kSortThreads / LIG_WAVE;
 
@@ -164,7 +190,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const K *__res
         hist[(size_t)d * nblocks + blockIdx.x] = h[d];
 }
 
-// Scatter of one pass.  Each wave owns 1024 consecutive pairs of the block's
+// Scatter of one pass.  Each wave owns 512 consecutive pairs of the block's
 // 4096 and ranks them round by round (64 at a time) against wave-private
 // digit counters in LDS: lanes with equal digits are matched by RB ballots,
 // the round's leader per digit advances the counter.  No block barrier is
@@ -250,7 +276,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
         sum += t;
     }
     int total;
-    int pre = block_exclusive_scan(sum, wsum, &total);
+    int pre = block_exclusive_scan_n<kSortThreads>(sum, wsum, &total);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int d = tid * PER + q;
