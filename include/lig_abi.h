@@ -143,11 +143,16 @@ int lig_pack_gaussians(int n, const float *xys, const float *conics, const float
  * final_idx [H,W] int32; when out_dx != NULL (compute_upscale_gradients) also
  * out_dx/out_dy/out_dxy [H,W,3].  `packed` is a caller buffer of
  * lig_packed_gaussians_bytes(n) bytes that the call fills with the per-Gaussian
- * records the kernels read; pass the same buffer, unchanged, to the backward. */
+ * records the kernels read; pass the same buffer, unchanged, to the backward.
+ * extents [n,2] and cum_tiles_hit [n] may be NULL; when given (block_width 16,
+ * opacities NULL, bins from lig_bin_and_sort_gaussians without emission ids)
+ * `packed` also carries each Gaussian's record addressing, and the backward
+ * (lig_rasterize_backward_records) then needs no emission ids. */
 int lig_rasterize_forward(int n, int img_height, int img_width, int block_width,
                           const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
                           const float *xys, const float *conics, const float *colors,
-                          const float *opacities, void *packed, float *out_img,
+                          const float *opacities, const float *extents,
+                          const int32_t *cum_tiles_hit, void *packed, float *out_img,
                           float *out_wsum, int32_t *final_idx, float *out_dx, float *out_dy,
                           float *out_dxy, void *stream);
 
@@ -181,7 +186,10 @@ int lig_rasterize_backward(int n, int img_height, int img_width, int block_width
  * lig_rasterize_backward_records writes each (tile, Gaussian) partial
  * gradient to its emission slot in the workspace;
  * lig_reduce_gradient_records sums each Gaussian's slots in order and writes
- * the gradients.  Same workspace, lig_rasterize_backward_workspace_size. */
+ * the gradients.  Same workspace, lig_rasterize_backward_workspace_size.
+ * emission_ids_sorted may be NULL when `packed` came from a forward given
+ * extents and cum_tiles_hit (16 x 16 tiles, no opacities): each record's slot
+ * is then computed from the pack. */
 int lig_rasterize_backward_records(int n, int img_height, int img_width, int block_width,
                                    const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
                                    const void *packed, const int32_t *final_idx,

@@ -46,7 +46,7 @@ _SIGNATURES = {
     "lig_pack_gaussians": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_rasterize_forward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                 _vp, _vp, _vp, _vp, _vp]),
+                 _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_rasterize_backward_workspace_size": (_c_size, [_c_int]),
     "lig_rasterize_backward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -409,6 +409,14 @@ struct __align__(16) FitRec {
     float4 r0, r1;
 };
 
+// Emission slot of the intersection at sorted position k: from the sorted
+// emission ids when given, else from the pack's record addressing (16 x 16
+// tiles, no opacities): base + ty * width + tx.
+__device__ __forceinline__ int record_slot(const int32_t *__restrict__ slots, int k,
+                                           const GaussPack &gp, int tx, int ty) {
+    return slots ? slots[k] : __float_as_int(gp.p3.z) + ty * __float_as_int(gp.p3.w) + tx;
+}
+
 __device__ __forceinline__ void emit_grads(bool det, GradRec *__restrict__ rec, int slot, int g,
                                            float vr, float vg, float vb, float c0, float c1,
                                            float c2, float gx, float gy, float ax, float ay,
@@ -818,8 +826,9 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
                 reinterpret_cast<FitRec *>(rec)[slot] = r;
             }
         } else {
-            plain_emit<HAS_OPAC, DET>(A, gp, act, DET && act ? slots[base + lane] : 0, g, rec,
-                                      g_xy, g_xy_abs, g_conic, g_rgb, g_opac);
+            const int slot = DET && act ? record_slot(slots, base + lane, gp, tx, ty) : 0;
+            plain_emit<HAS_OPAC, DET>(A, gp, act, slot, g, rec, g_xy, g_xy_abs, g_conic, g_rgb,
+                                      g_opac);
         }
     }
 }
@@ -921,9 +930,9 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
             }
         }
         if (act)
-            emit_grads(DET, rec, DET ? slots[base + lane] : 0, g, a_r, a_g, a_b, a_c0, a_c1,
-                       a_c2, a_x, a_y, a_ax, a_ay, a_o, has_opac != 0, g_xy, g_xy_abs, g_conic,
-                       g_rgb, g_opac);
+            emit_grads(DET, rec, DET ? record_slot(slots, base + lane, gp, tx, ty) : 0, g, a_r,
+                       a_g, a_b, a_c0, a_c1, a_c2, a_x, a_y, a_ax, a_ay, a_o, has_opac != 0,
+                       g_xy, g_xy_abs, g_conic, g_rgb, g_opac);
     }
 }
 
@@ -1068,7 +1077,8 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
                                      const int32_t *gaussian_ids_sorted,
                                      const int32_t *tile_bins, const float *xys,
                                      const float *conics, const float *colors,
-                                     const float *opacities, void *packed, float *out_img,
+                                     const float *opacities, const float *extents,
+                                     const int32_t *cum_tiles_hit, void *packed, float *out_img,
                                      float *out_wsum, int32_t *final_idx, float *out_dx,
                                      float *out_dy, float *out_dxy, void *stream) {
     if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
@@ -1078,13 +1088,18 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
     if (ga && (!out_dy || !out_dxy)) return LIG_ERR_INVALID_ARGUMENT;
     if (n > 0 && (!gaussian_ids_sorted || !xys || !conics || !colors || !packed))
         return LIG_ERR_INVALID_ARGUMENT;
+    // record addressing for the backward without emission ids: 16 x 16 tiles,
+    // no opacities (so no zero-extent phantom slots)
+    const bool slots_in_pack = extents != nullptr;
+    if (slots_in_pack && (!cum_tiles_hit || block_width != 16 || opacities))
+        return LIG_ERR_INVALID_ARGUMENT;
     hipStream_t st = as_stream(stream);
     GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
-    if (n > 0)
-        hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
-                           colors, opacities, nullptr, nullptr, 0, 0, pk);
     const int tiles_x = (img_width + block_width - 1) / block_width;
     const int tiles_y = (img_height + block_width - 1) / block_width;
+    if (n > 0)
+        hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
+                           colors, opacities, extents, cum_tiles_hit, tiles_x, tiles_y, pk);
     const dim3 grid((unsigned)(tiles_x * tiles_y));
     const int px = (block_width * block_width + LIG_WAVE - 1) / LIG_WAVE;
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
@@ -1166,7 +1181,7 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
                           GradRec *rec, float *v_xy, float *v_xy_abs, float *v_conic,
                           float *v_colors, float *v_opacity, hipStream_t st) {
     const bool ga = v_output_dx != nullptr;
-    const bool det = slots != nullptr;
+    const bool det = rec != nullptr;
     const int tiles_x = (img_width + block_width - 1) / block_width;
     const int tiles_y = (img_height + block_width - 1) / block_width;
     const dim3 grid((unsigned)(tiles_x * tiles_y));
@@ -1235,7 +1250,8 @@ extern "C" int lig_rasterize_backward_records(int n, int img_height, int img_wid
                                               size_t workspace_bytes, void *stream) {
     if (!bwd_args_ok(n, img_height, img_width, block_width, gaussian_ids_sorted, tile_bins,
                      packed, final_idx, v_output, v_output_dx, v_output_dy, v_output_dxy) ||
-        num_intersects < 0 || (n > 0 && !emission_ids_sorted))
+        num_intersects < 0 ||
+        (n > 0 && !emission_ids_sorted && (block_width != 16 || has_opacities)))
         return LIG_ERR_INVALID_ARGUMENT;
     if (!workspace || workspace_bytes < lig_rasterize_backward_workspace_size(num_intersects))
         return LIG_ERR_WORKSPACE;

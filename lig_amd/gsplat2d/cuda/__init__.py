@@ -267,9 +267,12 @@ def pack_gaussians(xys, conics, colors, opacities):
 
 
 def rasterize_forward_packed(img_height, img_width, block_width, gaussian_ids_sorted, tile_bins,
-                             xys, conics, colors, opacities, compute_upscale_gradients):
+                             xys, conics, colors, opacities, compute_upscale_gradients,
+                             extents=None, cum_tiles_hit=None):
     """Forward that also returns the packed records for the backward:
-    (out_img, out_wsum, out_dx, out_dy, out_dxy, final_idx, packed)."""
+    (out_img, out_wsum, out_dx, out_dy, out_dxy, final_idx, packed).  With
+    extents and cum_tiles_hit (16 x 16 tiles, no opacities) the pack also
+    addresses the backward's records, which then need no emission ids."""
     for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),
                       ("tile_bins", tile_bins, _i32), ("xys", xys, _f32),
                       ("conics", conics, _f32), ("colors", colors, _f32),
@@ -289,18 +292,21 @@ def rasterize_forward_packed(img_height, img_width, block_width, gaussian_ids_so
     with torch.cuda.device(xys.device):
         L.call("lig_rasterize_forward", n, H, W, int(block_width), L.ptr(gaussian_ids_sorted),
                L.ptr(tile_bins), L.ptr(xys), L.ptr(conics), L.ptr(colors), L.ptr(opacities),
-               L.ptr(packed), L.ptr(out_img), L.ptr(out_wsum), L.ptr(final_idx), L.ptr(dx),
-               L.ptr(dy), L.ptr(dxy), L.stream_of(xys))
+               L.ptr(extents), L.ptr(cum_tiles_hit), L.ptr(packed), L.ptr(out_img),
+               L.ptr(out_wsum), L.ptr(final_idx), L.ptr(dx), L.ptr(dy), L.ptr(dxy),
+               L.stream_of(xys))
     return out_img, out_wsum, dx, dy, dxy, final_idx, packed
 
 
 def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_sorted, tile_bins,
                               packed, num_points, final_idx, v_output, v_render_wsum,
                               has_opacities, v_output_dx=None, v_output_dy=None,
-                              v_output_dxy=None, emission_ids_sorted=None, cum_tiles_hit=None):
+                              v_output_dxy=None, emission_ids_sorted=None, cum_tiles_hit=None,
+                              slots_in_pack=False):
     """Backward over packed records; v_render_wsum may be None (zeros).  With
-    emission_ids_sorted and cum_tiles_hit (from bin_and_sort_fast) the
-    per-Gaussian sums are reduced deterministically; without, by atomics."""
+    cum_tiles_hit and either emission_ids_sorted (from bin_and_sort_fast) or
+    slots_in_pack (packed by a forward given extents and cum_tiles_hit) the
+    per-Gaussian sums are reduced deterministically; otherwise by atomics."""
     v_output, v_render_wsum, v_output_dx, v_output_dy, v_output_dxy = map(
         _contig, (v_output, v_render_wsum, v_output_dx, v_output_dy, v_output_dxy))
     for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),
@@ -314,9 +320,9 @@ def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_s
     ga = v_output_dx is not None
     if ga and (v_output_dy is None or v_output_dxy is None):
         raise RuntimeError("v_output_dx, v_output_dy and v_output_dxy go together")
-    det = emission_ids_sorted is not None
+    det = emission_ids_sorted is not None or slots_in_pack
     if det and cum_tiles_hit is None:
-        raise RuntimeError("emission_ids_sorted needs cum_tiles_hit")
+        raise RuntimeError("the deterministic backward needs cum_tiles_hit")
     n = int(num_points)
     like = v_output
     v_xy = _empty((n, 2), like)
@@ -332,7 +338,7 @@ def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_s
     outs = (L.ptr(v_xy), L.ptr(v_xy_abs), L.ptr(v_conic), L.ptr(v_colors), L.ptr(v_opacity))
     with torch.cuda.device(like.device):
         if det:
-            m = int(emission_ids_sorted.numel())
+            m = int(gaussian_ids_sorted.numel())
             ws = _workspace(L.size_query("lig_rasterize_backward_workspace_size", m), like)
             L.call("lig_rasterize_backward_records", *common, L.ptr(emission_ids_sorted), m,
                    L.ptr(ws), ws.numel(), L.stream_of(like))

@@ -74,13 +74,24 @@ class _RasterizeGaussians(Function):
             tile_bins = torch.zeros(0, 2, device=dev)
             final_idx = torch.zeros(img_height, img_width, device=dev)
         else:
-            gaussian_ids_sorted, tile_bins, emission_ids = _C.bin_and_sort_fast(
-                num_points, num_intersects, xys, extents, cum_tiles_hit, img_height, img_width,
-                block_width, with_emission_ids=True)
+            # 16 x 16 tiles without opacities (no zero-extent phantom slots): the
+            # sort carries gids and the pack addresses the backward's records;
+            # otherwise the sort carries emission slots
+            slots_in_pack = block_width == 16 and opacities is None
+            if slots_in_pack:
+                gaussian_ids_sorted, tile_bins = _C.bin_and_sort_fast(
+                    num_points, num_intersects, xys, extents, cum_tiles_hit, img_height,
+                    img_width, block_width)
+            else:
+                gaussian_ids_sorted, tile_bins, emission_ids = _C.bin_and_sort_fast(
+                    num_points, num_intersects, xys, extents, cum_tiles_hit, img_height,
+                    img_width, block_width, with_emission_ids=True)
             (out_img, out_wsum, out_dx, out_dy, out_dxy, final_idx,
-             packed) = _C.rasterize_forward_packed(img_height, img_width, block_width,
-                                                   gaussian_ids_sorted, tile_bins, xys, conics,
-                                                   colors, opacities, compute_upscale_gradients)
+             packed) = _C.rasterize_forward_packed(
+                img_height, img_width, block_width, gaussian_ids_sorted, tile_bins, xys, conics,
+                colors, opacities, compute_upscale_gradients,
+                extents=extents if slots_in_pack else None,
+                cum_tiles_hit=cum_tiles_hit if slots_in_pack else None)
 
         ctx.img_width = img_width
         ctx.img_height = img_height
@@ -89,6 +100,7 @@ class _RasterizeGaussians(Function):
         ctx.compute_upscale_gradients = compute_upscale_gradients
         ctx.packed = packed
         ctx.emission_ids = emission_ids
+        ctx.slots_in_pack = packed is not None and emission_ids is None
         ctx.cum_tiles_hit = cum_tiles_hit
         ctx.save_for_backward(gaussian_ids_sorted, tile_bins, xys, conics, colors, opacities,
                               final_idx)
@@ -123,7 +135,8 @@ class _RasterizeGaussians(Function):
                 H, W, ctx.block_width, gaussian_ids_sorted, tile_bins, packed, xys.size(0),
                 final_idx, v_out_img, v_out_wsum, opacities is not None,
                 v_out_dx if ga else None, v_out_dy if ga else None, v_out_dxy if ga else None,
-                emission_ids_sorted=ctx.emission_ids, cum_tiles_hit=ctx.cum_tiles_hit)
+                emission_ids_sorted=ctx.emission_ids, cum_tiles_hit=ctx.cum_tiles_hit,
+                slots_in_pack=ctx.slots_in_pack and packed is ctx.packed)
         ctx.packed = ctx.emission_ids = ctx.cum_tiles_hit = None
         xys.absgrad = v_xy_abs
         return (v_xy, None, v_conic, None, v_colors, v_opacity, None, None, None, None)

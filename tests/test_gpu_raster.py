@@ -142,6 +142,42 @@ def test_raster_api_end_to_end(device, oracle_lib):
     assert torch.all(far.grad == 0)
 
 
+@pytest.mark.parametrize("ga", [False, True])
+def test_record_addressing_modes_agree(device, oracle_lib, ga):
+    """16 x 16 tiles without opacities: the deterministic backward addressing
+    its records from the pack (bins sorted with gids) gives the same bits as
+    the one addressing them through sorted emission ids, and the API path
+    (which takes the pack mode) agrees with both."""
+    import lig_amd.gsplat2d as g
+    from lig_amd.gsplat2d import cuda as C
+    n, H, W = 900, 100, 130
+    s = random_scene(n, H, W, 21, opacities=False, scale=(0.5, 6.0), margin=10.0)
+    xys, ext, con, nt = (_t(a, device) if i < 3 else _t(a, device, torch.int32) for i, a in
+                         enumerate(oracle_lib.project_gaussians_forward(
+                             s["par"], s["means"], H, W, 16, None, cholesky=True)))
+    col = _t(s["colors"], device)
+    num, cum = g.compute_cumulative_intersects(nt)
+    g1, b1, eids = C.bin_and_sort_fast(n, num, xys, ext, cum, H, W, 16, with_emission_ids=True)
+    g2, b2 = C.bin_and_sort_fast(n, num, xys, ext, cum, H, W, 16)
+    assert torch.equal(g1, g2) and torch.equal(b1, b2)
+    f1 = C.rasterize_forward_packed(H, W, 16, g1, b1, xys, con, col, None, ga)
+    f2 = C.rasterize_forward_packed(H, W, 16, g2, b2, xys, con, col, None, ga,
+                                    extents=ext, cum_tiles_hit=cum)
+    for a, b in zip(f1[:6], f2[:6]):
+        assert (a is None and b is None) or torch.equal(a, b)
+    rng = np.random.default_rng(3)
+    v_out = _t(rng.normal(size=(H, W, 3)), device)
+    v_w = _t(rng.normal(size=(H, W)), device)
+    extra = [_t(rng.normal(size=(H, W, 3)), device) for _ in range(3)] if ga else [None] * 3
+    r1 = C.rasterize_backward_packed(H, W, 16, g1, b1, f1[6], n, f1[5], v_out, v_w, False,
+                                     *extra, emission_ids_sorted=eids, cum_tiles_hit=cum)
+    r2 = C.rasterize_backward_packed(H, W, 16, g2, b2, f2[6], n, f2[5], v_out, v_w, False,
+                                     *extra, cum_tiles_hit=cum, slots_in_pack=True)
+    for name, a, b in zip(("v_xy", "v_xy_abs", "v_conic", "v_colors"), r1[:4], r2[:4]):
+        assert torch.equal(a, b), name
+    assert torch.count_nonzero(r1[3]) > 0
+
+
 UPSCALE_CASES = [
     (16, 16, 64, 64, None),
     (20, 30, 80, 120, None),
