@@ -271,6 +271,25 @@ int lig_l2_clamped_loss_forward(long long n, const float *img, const float *targ
 int lig_l2_clamped_loss_backward(long long n, const float *img, const float *target,
                                  const float *grad_loss, float *grad_img, void *stream);
 
+/* ---- fused optimiser steps (one FP32 tensor per call) ----------------------- */
+
+/* No reference counterpart in the CUDA extension: the optimiser step that
+ * follows the backward (gaussianlig.py:64-65 uses torch Adam;
+ * minimal_trainer_upscale.py:205-210 uses the reference's Adan).
+ * lig_adam_step: torch.optim.Adam's update (decoupled from nothing: weight
+ * decay is added to the gradient), step = 1-based count after this update.
+ * lig_adan_step: optimizer.py:238-293 (_single_tensor_adan) per element, with
+ * neg_pre_grad the state that holds -grad of the previous step (initialised
+ * by the call itself when step == 1) and clip_global_grad_norm the factor the
+ * caller computed from max_grad_norm (1 when unused).  grad is not modified. */
+int lig_adam_step(long long n, float *param, const float *grad, float *exp_avg,
+                  float *exp_avg_sq, float lr, float beta1, float beta2, float eps,
+                  float weight_decay, int step, void *stream);
+int lig_adan_step(long long n, float *param, const float *grad, float *exp_avg,
+                  float *exp_avg_sq, float *exp_avg_diff, float *neg_pre_grad, float lr,
+                  float beta1, float beta2, float beta3, float eps, float weight_decay,
+                  int no_prox, float clip_global_grad_norm, int step, void *stream);
+
 /* ---- gradient-aware upscaler ---------------------------------------------- */
 
 /* Replaces gradient_aware_upscale_forward_tensor (csrc/bindings.cu:508-559,

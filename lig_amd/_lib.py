@@ -69,6 +69,12 @@ _SIGNATURES = {
                  _vp]),
     "lig_fit_reduce_records": (
         _c_int, [_c_int, _c_int, _vp, _vp, _c_size, _vp, _vp, _vp, _vp]),
+    "lig_adam_step": (
+        _c_int, [ctypes.c_longlong, _vp, _vp, _vp, _vp, _c_float, _c_float, _c_float, _c_float,
+                 _c_float, _c_int, _vp]),
+    "lig_adan_step": (
+        _c_int, [ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp, _vp, _c_float, _c_float, _c_float,
+                 _c_float, _c_float, _c_float, _c_int, _c_float, _c_int, _vp]),
     "lig_count_pairs": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_gradient_aware_upscale_forward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,

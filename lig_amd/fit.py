@@ -21,13 +21,12 @@ from . import gsplat2d as g
 from .data import init_gaussians
 from .gsplat2d import cuda as _C
 from .loss import l2_loss_clamped
+from .optim import FusedAdam
 
 
 def _adam(params, lr):
-    try:
-        return torch.optim.Adam(params, lr=lr, fused=True)
-    except (RuntimeError, TypeError):
-        return torch.optim.Adam(params, lr=lr, foreach=True)
+    """Adam with the fused HIP step (lig_adam_step), torch.optim.Adam's update."""
+    return FusedAdam(params, lr=lr)
 
 
 class _RenderL2(torch.autograd.Function):
