@@ -62,6 +62,21 @@ int lig_project_gaussians_forward_cholesky(int n, const float *cholesky, const f
                                            float *extents, float *conics,
                                            int32_t *num_tiles_hit, void *stream);
 
+/* No reference counterpart: crop-native projection for rendering a window.
+ * Each Gaussian is first mapped into the view with origin (origin_x,
+ * origin_y) on the source canvas and (scale_x, scale_y) output pixels per
+ * source pixel: mean' = (mean - origin) * scale and, for cholesky != 0,
+ * (l11, l21, l22)' = (l11 sx, l21 sy, l22 sy) (the reference's scale_to,
+ * minimal_trainer_upscale.py:139-159, with an origin), else
+ * cov' = (a sx sx, b sx sy, c sy sy); FP32 without contraction.  Then exactly
+ * lig_project_gaussians_forward(_cholesky) on an img_height x img_width
+ * canvas.  scale_x, scale_y > 0. */
+int lig_project_gaussians_view(int n, const float *par, const float *means2d,
+                               const float *opacities, int cholesky, float origin_x,
+                               float origin_y, float scale_x, float scale_y, int img_height,
+                               int img_width, int block_width, float *xys, float *extents,
+                               float *conics, int32_t *num_tiles_hit, void *stream);
+
 /* Replaces project_gaussians_backward_tensor (csrc/bindings.cu:119-151, kernel
  * csrc/backward.cu:185-204).  -> v_cov2d [n,3], v_mean2d [n,2]. */
 int lig_project_gaussians_backward(int n, const float *extents, const float *conics,
@@ -147,7 +162,9 @@ int lig_pack_gaussians(int n, const float *xys, const float *conics, const float
  * extents [n,2] and cum_tiles_hit [n] may be NULL; when given (block_width 16,
  * opacities NULL, bins from lig_bin_and_sort_gaussians without emission ids)
  * `packed` also carries each Gaussian's record addressing, and the backward
- * (lig_rasterize_backward_records) then needs no emission ids. */
+ * (lig_rasterize_backward_records) then needs no emission ids.
+ * out_wsum and final_idx may both be NULL (block_width 16, no out_dx): an
+ * image-only render, e.g. for serving. */
 int lig_rasterize_forward(int n, int img_height, int img_width, int block_width,
                           const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
                           const float *xys, const float *conics, const float *colors,

@@ -31,6 +31,9 @@ _SIGNATURES = {
     "lig_project_gaussians_backward": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_project_gaussians_backward_cholesky": (
         _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lig_project_gaussians_view": (
+        _c_int, [_c_int, _vp, _vp, _vp, _c_int, _c_float, _c_float, _c_float, _c_float, _c_int,
+                 _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     "lig_scan_workspace_size": (_c_size, [_c_int]),
     "lig_inclusive_scan_i32": (_c_int, [_c_int, _vp, _vp, _vp, _c_size, _vp]),
     "lig_map_gaussian_to_intersects": (

@@ -40,10 +40,9 @@ __device__ __forceinline__ bool cov_bounds(float ca, float cb, float cc, float o
 
 // One Gaussian of the forward projection; writes every output slot (zeros
 // where the reference leaves its zero fill).
-__device__ __forceinline__ void project_one(int i, float ca, float cb, float cc,
-                                            const float *__restrict__ means2d,
-                                            const float *__restrict__ opacities, int tiles_x,
-                                            int tiles_y, int block_width,
+__device__ __forceinline__ void project_one(int i, float ca, float cb, float cc, float mx,
+                                            float my, const float *__restrict__ opacities,
+                                            int tiles_x, int tiles_y, int block_width,
                                             float *__restrict__ xys,
                                             float *__restrict__ extents,
                                             float *__restrict__ conics,
@@ -52,7 +51,6 @@ __device__ __forceinline__ void project_one(int i, float ca, float cb, float cc,
     const float o = opacities ? opacities[i] : 1.0f;
     const bool ok = cov_bounds(ca, cb, cc, o, ka, kb, kc, ex, ey);
     int area = 0;
-    const float mx = means2d[2 * i], my = means2d[2 * i + 1];
     if (ok) {
         int x0, y0, x1, y1;
         tile_bbox(mx, my, ex, ey, tiles_x, tiles_y, block_width, x0, y0, x1, y1);
@@ -76,8 +74,9 @@ __global__ __launch_bounds__(kThreads) void project_cov_kernel(
     int32_t *__restrict__ num_tiles_hit) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    project_one(i, cov2d[3 * i], cov2d[3 * i + 1], cov2d[3 * i + 2], means2d, opacities,
-                tiles_x, tiles_y, block_width, xys, extents, conics, num_tiles_hit);
+    project_one(i, cov2d[3 * i], cov2d[3 * i + 1], cov2d[3 * i + 2], means2d[2 * i],
+                means2d[2 * i + 1], opacities, tiles_x, tiles_y, block_width, xys, extents,
+                conics, num_tiles_hit);
 }
 
 __global__ __launch_bounds__(kThreads) void project_chol_kernel(
@@ -92,8 +91,39 @@ __global__ __launch_bounds__(kThreads) void project_chol_kernel(
     const float ca = l11 * l11;
     const float cb = l11 * l21;
     const float cc = l21 * l21 + l22 * l22;
-    project_one(i, ca, cb, cc, means2d, opacities, tiles_x, tiles_y, block_width, xys,
-                extents, conics, num_tiles_hit);
+    project_one(i, ca, cb, cc, means2d[2 * i], means2d[2 * i + 1], opacities, tiles_x, tiles_y,
+                block_width, xys, extents, conics, num_tiles_hit);
+}
+
+// Projection into a view: the Gaussian seen in a window with origin (ox, oy)
+// of the source canvas, sampled at (sx, sy) output pixels per source pixel:
+//   mean' = (mean - o) * s,  L' = diag(s) L  (Cholesky: l11 sx, l21 sy, l22 sy),
+//   cov' = S cov S  (a sx sx, b sx sy, c sy sy),
+// the transform of the reference's scale_to (minimal_trainer_upscale.py:139-159)
+// with an origin, in FP32 without contraction; then the projection above.
+__global__ __launch_bounds__(kThreads) void project_view_kernel(
+    int n, int cholesky, const float *__restrict__ par, const float *__restrict__ means2d,
+    const float *__restrict__ opacities, float ox, float oy, float sx, float sy, int tiles_x,
+    int tiles_y, int block_width, float *__restrict__ xys, float *__restrict__ extents,
+    float *__restrict__ conics, int32_t *__restrict__ num_tiles_hit) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const float mx = (means2d[2 * i] - ox) * sx;
+    const float my = (means2d[2 * i + 1] - oy) * sy;
+    float ca, cb, cc;
+    if (cholesky) {
+        const float l11 = par[3 * i] * sx, l21 = par[3 * i + 1] * sy, l22 = par[3 * i + 2] * sy;
+        ca = l11 * l11;
+        cb = l11 * l21;
+        cc = l21 * l21 + l22 * l22;
+    } else {
+        ca = (par[3 * i] * sx) * sx;
+        cb = (par[3 * i + 1] * sx) * sy;
+        cc = (par[3 * i + 2] * sy) * sy;
+    }
+    project_one(i, ca, cb, cc, mx, my, opacities, tiles_x, tiles_y, block_width, xys, extents,
+                conics, num_tiles_hit);
 }
 
 __global__ __launch_bounds__(kThreads) void cov2d_bounds_kernel(
@@ -224,6 +254,27 @@ extern "C" int lig_project_gaussians_forward_cholesky(int n, const float *choles
                                                       void *stream) {
     return project_fwd(true, n, cholesky, means2d, opacities, img_height, img_width,
                        block_width, xys, extents, conics, num_tiles_hit, stream);
+}
+
+extern "C" int lig_project_gaussians_view(int n, const float *par, const float *means2d,
+                                          const float *opacities, int cholesky, float origin_x,
+                                          float origin_y, float scale_x, float scale_y,
+                                          int img_height, int img_width, int block_width,
+                                          float *xys, float *extents, float *conics,
+                                          int32_t *num_tiles_hit, void *stream) {
+    if (n < 0 || !dims_ok(img_height, img_width, block_width) || !(scale_x > 0.0f) ||
+        !(scale_y > 0.0f))
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    if (!par || !means2d || !xys || !extents || !conics || !num_tiles_hit)
+        return LIG_ERR_INVALID_ARGUMENT;
+    const int tx = (img_width + block_width - 1) / block_width;
+    const int ty = (img_height + block_width - 1) / block_width;
+    hipLaunchKernelGGL(project_view_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0,
+                       as_stream(stream), n, cholesky, par, means2d, opacities, origin_x,
+                       origin_y, scale_x, scale_y, tx, ty, block_width, xys, extents, conics,
+                       num_tiles_hit);
+    return launch_status();
 }
 
 static int project_bwd(int n, const float *extents, const float *chol, const float *conics,

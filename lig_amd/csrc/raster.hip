@@ -223,8 +223,8 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
 // by a wave-uniform branch.  Pair evaluation and accumulation order are those
 // of raster_fwd_kernel, so the outputs are identical.
 //
-// TRAIN (fit step): no wsum / final index; instead the tile's clamp + L2 sum
-// against `target`, sum over its pixels and channels of (clamp(I, 0, 1) - t)^2,
+// TRAIN (fit step and render-only): no wsum / final index; with a target the
+// tile's clamp + L2 sum, over its pixels and channels of (clamp(I, 0, 1) - t)^2,
 // is written to loss_partial[tile] (FP32, fixed lane order).
 template <bool HAS_OPAC, bool TRAIN>
 __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
@@ -317,9 +317,10 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
             out_img[3 * pid + 1] = ag[q];
             out_img[3 * pid + 2] = ab[q];
             if constexpr (TRAIN) {
-                lsum += clamped_sq_err(ar[q], target[3 * pid + 0]) +
-                        clamped_sq_err(ag[q], target[3 * pid + 1]) +
-                        clamped_sq_err(ab[q], target[3 * pid + 2]);
+                if (target)
+                    lsum += clamped_sq_err(ar[q], target[3 * pid + 0]) +
+                            clamped_sq_err(ag[q], target[3 * pid + 1]) +
+                            clamped_sq_err(ab[q], target[3 * pid + 2]);
             } else {
                 out_wsum[pid] = aw[q];
                 final_idx[pid] = fin[q];
@@ -327,8 +328,11 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
         }
     }
     if constexpr (TRAIN) {
-        for (int off = LIG_WAVE / 2; off > 0; off >>= 1) lsum += __shfl_down(lsum, off, LIG_WAVE);
-        if (lane == 0) loss_partial[tile] = lsum;
+        if (loss_partial) {
+            for (int off = LIG_WAVE / 2; off > 0; off >>= 1)
+                lsum += __shfl_down(lsum, off, LIG_WAVE);
+            if (lane == 0) loss_partial[tile] = lsum;
+        }
     }
 }
 
@@ -1083,8 +1087,12 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
                                      float *out_dy, float *out_dxy, void *stream) {
     if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
         return LIG_ERR_INVALID_ARGUMENT;
-    if (!tile_bins || !out_img || !out_wsum || !final_idx) return LIG_ERR_INVALID_ARGUMENT;
     const bool ga = out_dx != nullptr;
+    // image only (no wsum / final_idx): 16 x 16 tiles, plain forward
+    const bool image_only = !out_wsum && !final_idx;
+    if (!tile_bins || !out_img || (!image_only && (!out_wsum || !final_idx)) ||
+        (image_only && (ga || block_width != 16)))
+        return LIG_ERR_INVALID_ARGUMENT;
     if (ga && (!out_dy || !out_dxy)) return LIG_ERR_INVALID_ARGUMENT;
     if (n > 0 && (!gaussian_ids_sorted || !xys || !conics || !colors || !packed))
         return LIG_ERR_INVALID_ARGUMENT;
@@ -1107,6 +1115,14 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
         launch_fwd<true>(px, grid, st, tiles_x, block_width, img_width, img_height,
                          gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, out_dx,
                          out_dy, out_dxy);
+    else if (image_only && opacities)
+        hipLaunchKernelGGL((raster_fwd16_kernel<true, true>), grid, dim3(kThreads), 0, st,
+                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
+                           nullptr, nullptr, nullptr, nullptr);
+    else if (image_only)
+        hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), grid, dim3(kThreads), 0, st,
+                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
+                           nullptr, nullptr, nullptr, nullptr);
     else if (block_width == 16 && opacities)
         hipLaunchKernelGGL((raster_fwd16_kernel<true, false>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,

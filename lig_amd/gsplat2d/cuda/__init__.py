@@ -78,6 +78,48 @@ def project_gaussians_forward_cholesky(num_points, cholesky, means2d, opacities,
                             means2d, opacities, img_height, img_width, block_width)
 
 
+def project_gaussians_view(num_points, par, means2d, opacities, cholesky, origin, scale,
+                           img_height, img_width, block_width):
+    """Projection into a window (see lig_project_gaussians_view): origin =
+    (x, y) on the source canvas, scale = (sx, sy) output pixels per source
+    pixel.  -> (xys, extents, conics, num_tiles_hit) in view coordinates."""
+    _check("par", par)
+    _check("means2d", means2d)
+    _check("opacities", opacities)
+    xys = _empty((num_points, 2), par)
+    extents = _empty((num_points, 2), par)
+    conics = _empty((num_points, 3), par)
+    ntiles = _empty((num_points,), par, _i32)
+    with torch.cuda.device(par.device):
+        L.call("lig_project_gaussians_view", int(num_points), L.ptr(par), L.ptr(means2d),
+               L.ptr(opacities), int(bool(cholesky)), float(origin[0]), float(origin[1]),
+               float(scale[0]), float(scale[1]), int(img_height), int(img_width),
+               int(block_width), L.ptr(xys), L.ptr(extents), L.ptr(conics), L.ptr(ntiles),
+               L.stream_of(par))
+    return xys, extents, conics, ntiles
+
+
+def rasterize_image(img_height, img_width, gaussian_ids_sorted, tile_bins, xys, conics, colors,
+                    opacities=None):
+    """Image-only forward (16 x 16 tiles, no wsum / final_idx): out_img [H,W,3]."""
+    for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),
+                      ("tile_bins", tile_bins, _i32), ("xys", xys, _f32),
+                      ("conics", conics, _f32), ("colors", colors, _f32),
+                      ("opacities", opacities, _f32)):
+        _check(n_, t, dt)
+    colors = _contig(colors)
+    n = int(xys.size(0))
+    H, W = int(img_height), int(img_width)
+    out_img = _empty((H, W, 3), xys)
+    packed = _workspace(L.size_query("lig_packed_gaussians_bytes", n), xys)
+    with torch.cuda.device(xys.device):
+        L.call("lig_rasterize_forward", n, H, W, 16, L.ptr(gaussian_ids_sorted),
+               L.ptr(tile_bins), L.ptr(xys), L.ptr(conics), L.ptr(colors), L.ptr(opacities),
+               None, None, L.ptr(packed), L.ptr(out_img), None, None, None, None, None,
+               L.stream_of(xys))
+    return out_img
+
+
 def _contig(t):
     return t.contiguous() if t is not None else None
 
