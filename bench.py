@@ -141,6 +141,32 @@ def cpu_baseline(level, target, crop, oracle, torch):
     return time.perf_counter() - t0, int(idx.numel()), m
 
 
+def serve_run(args, dev, torch):
+    """Crop serving (C5): resident two-level representations, Poisson crop
+    requests, continuous batching over 4 stream slots, HBM admission."""
+    from lig_amd import serve
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    reps = serve.make_representations(args.serve_reps, seed=2000, device=dev)
+    reqs = serve.make_requests(reps, args.serve_requests, seed=3000)
+    # one request per representation first (untimed): first-use allocations
+    warm = serve.CropServer(reps, slots=4, device=dev)
+    warm.run_poisson([serve.Request(i, 0.0, 0.0, 256, 256, 1.0) for i in range(len(reps))],
+                     qps=1e4)
+    torch.cuda.synchronize()
+    log(f"serve setup {time.perf_counter() - t0:.1f}s ({len(reps)} representations)")
+    srv = serve.CropServer(reps, slots=4, hbm_budget=32 << 30, device=dev)
+    st = srv.run_poisson(reqs, qps=args.serve_qps, seed=3000)
+    res = st.summary()
+    res.update({"offered_qps": args.serve_qps, "slots": 4, "hbm_budget_gb": 32,
+                "representations": len(reps),
+                "workload": "32 two-level reps, 4096^2-8192^2 canvases, 0.5M-3M level-2 "
+                            "Gaussians (+ half at level 1); crops 256^2-4096^2 log-uniform, "
+                            "scale 1 or 0.5, Poisson arrivals"})
+    log(f"serve {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +175,10 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--render-reps", type=int, default=5)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0 to skip the CPU oracle leg")
+    ap.add_argument("--serve-requests", type=int, default=600,
+                    help="crop requests in the serving run (0 skips it)")
+    ap.add_argument("--serve-qps", type=float, default=150.0)
+    ap.add_argument("--serve-reps", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -277,6 +307,11 @@ def main():
                          "on a central crop of each level, scaled by image area / crop area; "
                          + "; ".join(parts)}
 
+    serve = None
+    if args.serve_requests > 0:
+        del l1, l2, gt, t1, t2
+        serve = serve_run(args, dev, torch)
+
     out = {
         "metric": "fit steps/s + render Gpix/s @8192², 2M Gaussians; p50/p99 crop latency",
         "value": round(steps_per_s, 4),
@@ -301,6 +336,7 @@ def main():
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
+        "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

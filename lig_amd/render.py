@@ -21,6 +21,7 @@ from dataclasses import dataclass
 
 import torch
 
+from . import _lib as L
 from . import gsplat2d as g
 from .fit import compose
 from .gsplat2d import cuda as _C
@@ -41,31 +42,73 @@ class LevelParams:
                    level.W)
 
 
-def _project(p: LevelParams, x0: float, y0: float, scale: float, h: int, w: int):
+@dataclass
+class Projected:
+    """A level projected into a window, with its intersection count."""
+    xys: torch.Tensor
+    extents: torch.Tensor
+    conics: torch.Tensor
+    num_tiles_hit: torch.Tensor
+    cum: torch.Tensor
+    num_intersects: int
+    h: int
+    w: int
+
+
+def project_window(p: LevelParams, x0: float, y0: float, w: int, h: int,
+                   scale: float = 1.0) -> Projected:
+    """View projection and the inclusive scan of tiles hit (one device-to-host
+    read of the intersection count, on the current stream)."""
     n = p.means.shape[0]
-    return _C.project_gaussians_view(n, p.cholesky, p.means, None, True, (x0, y0),
-                                     (scale, scale), h, w, 16)
+    xys, ext, con, nt = _C.project_gaussians_view(n, p.cholesky, p.means, None, True, (x0, y0),
+                                                  (scale, scale), h, w, 16)
+    num, cum = g.compute_cumulative_intersects(nt)
+    return Projected(xys, ext, con, nt, cum, int(num), h, w)
+
+
+def raster_window(p: LevelParams, q: Projected) -> torch.Tensor:
+    """Image-only render of a projected window."""
+    if q.num_intersects < 1:
+        return torch.ones(q.h, q.w, 3, device=p.means.device)   # rasterize.py:80-90
+    gids, bins = _C.bin_and_sort_fast(p.means.shape[0], q.num_intersects, q.xys, q.extents,
+                                      q.cum, q.h, q.w, 16)
+    return _C.rasterize_image(q.h, q.w, gids, bins, q.xys, q.conics, p.rgbs)
+
+
+def raster_window_ga(p: LevelParams, q: Projected):
+    """(img, dx, dy, dxy) of a projected window (gradient-aware forward)."""
+    dev = p.means.device
+    if q.num_intersects < 1:
+        z = torch.zeros(q.h, q.w, 3, device=dev)
+        return torch.ones(q.h, q.w, 3, device=dev), z, z.clone(), z.clone()
+    gids, bins = _C.bin_and_sort_fast(p.means.shape[0], q.num_intersects, q.xys, q.extents,
+                                      q.cum, q.h, q.w, 16)
+    img, _, dx, dy, dxy, _, _ = _C.rasterize_forward_packed(q.h, q.w, 16, gids, bins, q.xys,
+                                                            q.conics, p.rgbs, None, True)
+    return img, dx, dy, dxy
+
+
+def window_bytes(p: LevelParams, q: Projected, ga: bool) -> int:
+    """HBM the rest of a window's render allocates after projection: binning
+    workspace and outputs, packed records, images."""
+    tiles = ((q.h + 15) // 16) * ((q.w + 15) // 16)
+    ws = L.size_query("lig_bin_workspace_size", max(q.num_intersects, 0), tiles)
+    pix = q.h * q.w
+    per_pix = (12 + 4 + 36 + 4) if ga else 12
+    return int(ws + 4 * q.num_intersects + 8 * tiles + 64 * p.means.shape[0] + per_pix * pix)
 
 
 @torch.no_grad()
 def render_view(p: LevelParams, x0: float, y0: float, w: int, h: int,
                 scale: float = 1.0) -> torch.Tensor:
     """h x w x 3 raw sum image of the window (x0, y0) at `scale`."""
-    xys, ext, con, nt = _project(p, x0, y0, scale, h, w)
-    num, cum = g.compute_cumulative_intersects(nt)
-    if num < 1:
-        return torch.ones(h, w, 3, device=p.means.device)   # rasterize.py:80-90
-    gids, bins = _C.bin_and_sort_fast(p.means.shape[0], num, xys, ext, cum, h, w, 16)
-    return _C.rasterize_image(h, w, gids, bins, xys, con, p.rgbs)
+    return raster_window(p, project_window(p, x0, y0, w, h, scale))
 
 
 @torch.no_grad()
 def render_view_ga(p: LevelParams, x0: float, y0: float, w: int, h: int):
     """(img, dx, dy, dxy) of the window at scale 1 (gradient-aware forward)."""
-    xys, ext, con, nt = _project(p, x0, y0, 1.0, h, w)
-    out = g.rasterize_gaussians(xys, ext, con, nt, p.rgbs, None, h, w, 16,
-                                compute_upscale_gradients=True)
-    return out[0], out[2], out[3], out[4]
+    return raster_window_ga(p, project_window(p, x0, y0, w, h, 1.0))
 
 
 @dataclass
@@ -107,14 +150,27 @@ class TwoLevel:
         roi = (sx0 - rx0, sy0 - ry0, sx0 - rx0 + span_x, sy0 - ry0 + span_y)
         return (rx0, ry0, rx1, ry1), roi
 
+    def crop_plan(self, x0: float, y0: float, w: int, h: int, scale: float = 1.0):
+        """Projections of a crop's two windows (level-1 source region, level-2
+        window), the upscaler ROI and the HBM the rest of the render needs."""
+        (rx0, ry0, rx1, ry1), roi = self.l1_region(x0, y0, w, h, scale)
+        q1 = project_window(self.l1, float(rx0), float(ry0), rx1 - rx0, ry1 - ry0, 1.0)
+        q2 = project_window(self.l2, x0, y0, w, h, scale)
+        need = (window_bytes(self.l1, q1, True) + window_bytes(self.l2, q2, False) +
+                3 * 12 * w * h)
+        return q1, q2, roi, need
+
+    @torch.no_grad()
+    def finish_crop(self, plan, w: int, h: int) -> torch.Tensor:
+        q1, q2, roi, _ = plan
+        img1, dx, dy, dxy = raster_window_ga(self.l1, q1)
+        up = torch.clamp(g.gradient_aware_upscale(img1, dx, dy, dxy, h, w, roi), 0, 1)
+        img2 = torch.clamp(raster_window(self.l2, q2), 0, 1)
+        return compose(img2, up, self.lo, self.hi)
+
     @torch.no_grad()
     def render_crop(self, x0: float, y0: float, w: int, h: int,
                     scale: float = 1.0) -> torch.Tensor:
         """h x w x 3 window of the composed image at `scale` output pixels per
         full-resolution pixel, in [0, 1]."""
-        (rx0, ry0, rx1, ry1), roi = self.l1_region(x0, y0, w, h, scale)
-        img1, dx, dy, dxy = render_view_ga(self.l1, float(rx0), float(ry0), rx1 - rx0,
-                                           ry1 - ry0)
-        up = torch.clamp(g.gradient_aware_upscale(img1, dx, dy, dxy, h, w, roi), 0, 1)
-        img2 = torch.clamp(render_view(self.l2, x0, y0, w, h, scale), 0, 1)
-        return compose(img2, up, self.lo, self.hi)
+        return self.finish_crop(self.crop_plan(x0, y0, w, h, scale), w, h)
