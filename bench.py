@@ -141,6 +141,49 @@ def cpu_baseline(level, target, crop, oracle, torch):
     return time.perf_counter() - t0, int(idx.numel()), m
 
 
+def batched_run(args, dev, torch):
+    """Batched fitting (C4): 64 seeded images of uneven size at 0.05 Gaussians
+    per pixel; the plain form steps each image's Level in turn (one image per
+    launch sequence), the batched form steps all images with one launch per
+    stage.  Both start from the same draws, so after the same number of steps
+    their per-image PSNRs are equal (the batched form is bit-identical per
+    image)."""
+    import numpy as np
+    from lig_amd import batch, fit
+    sizes, counts, seeds = batch.batch_workload(64, seed=1000, gauss_per_px=0.05)
+    targets = batch.make_targets(sizes, seeds, dev)
+    scale = 2.0
+    plain = [fit.Level(c, h, w, s, scale, device=dev) for (h, w), c, s in
+             zip(sizes, counts, seeds)]
+    bf = batch.BatchedFit(targets, counts, seeds, scale=scale)
+    res = {"images": len(sizes), "pixels": int(sum(h * w for h, w in sizes)),
+           "gaussians": int(sum(counts)), "steps": args.batch_steps,
+           "workload": "64 seeded images, height log-uniform 256-2048, width log-uniform "
+                       "384-1536, 0.05 Gaussians/px, chol diag U(0.5,3)*2 px, L2, Adam"}
+    for name, stepper in (("plain", lambda: [lv.step(t) for lv, t in zip(plain, targets)]),
+                          ("batched", bf.step)):
+        for _ in range(2):
+            stepper()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            stepper()
+        torch.cuda.synchronize()
+        res[f"{name}_steps_s"] = round(args.batch_steps / (time.perf_counter() - t0), 3)
+    res["speedup"] = round(res["batched_steps_s"] / res["plain_steps_s"], 3)
+    with torch.no_grad():
+        p_plain = np.array([fit.psnr(torch.clamp(lv.render(), 0, 1), t)
+                            for lv, t in zip(plain, targets)])
+        bf.loss()
+        p_batch = bf.psnr()
+    res["max_abs_dpsnr_db"] = float(np.abs(p_plain - p_batch).max())
+    res["mean_psnr_db"] = round(float(p_batch.mean()), 3)
+    log(f"batched {res}")
+    del plain, bf, targets
+    torch.cuda.empty_cache()
+    return res
+
+
 def serve_run(args, dev, torch):
     """Crop serving (C5): resident two-level representations, Poisson crop
     requests, continuous batching over 4 stream slots, HBM admission."""
@@ -179,6 +222,8 @@ def main():
                     help="crop requests in the serving run (0 skips it)")
     ap.add_argument("--serve-qps", type=float, default=150.0)
     ap.add_argument("--serve-reps", type=int, default=32)
+    ap.add_argument("--batch-steps", type=int, default=5,
+                    help="timed steps of the batched-fitting comparison (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -307,10 +352,9 @@ def main():
                          "on a central crop of each level, scaled by image area / crop area; "
                          + "; ".join(parts)}
 
-    serve = None
-    if args.serve_requests > 0:
-        del l1, l2, gt, t1, t2
-        serve = serve_run(args, dev, torch)
+    del l1, l2, gt, t1, t2
+    batched = batched_run(args, dev, torch) if args.batch_steps > 0 else None
+    serve = serve_run(args, dev, torch) if args.serve_requests > 0 else None
 
     out = {
         "metric": "fit steps/s + render Gpix/s @8192², 2M Gaussians; p50/p99 crop latency",
@@ -336,6 +380,7 @@ def main():
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
         "serve": serve,
     }
     print(json.dumps(out), flush=True)

@@ -264,6 +264,46 @@ int lig_fit_reduce_records(int n, int num_intersects, const int32_t *cum_tiles_h
                            const void *workspace, size_t workspace_bytes, float *v_xy,
                            float *v_conic, float *v_colors, void *stream);
 
+/* ---- batched fit (many images, one launch per stage) ----------------------- */
+
+/* No reference counterpart.  B images fitted together: `batch_table` is a
+ * device array of B + 1 records {int tile_off, tiles_x, tiles_y, w, h,
+ * gauss_off; int64 pix_off} (32 bytes each, 8-byte aligned; the last record a
+ * sentinel whose tile_off / gauss_off are the totals).  Image b owns global
+ * tiles [tile_off, +tiles_x*tiles_y) (16 x 16 tiles), pixels [pix_off,
+ * +w*h) of the concatenated HWC buffers and Gaussians [gauss_off, next
+ * gauss_off); coordinates inside an image are its own, so each image computes
+ * exactly what lig_fit_render_l2 / lig_fit_backward_records_l2 compute for it
+ * alone.  Cholesky parameters, no opacities.
+ * lig_project_batched_cholesky: lig_project_gaussians_forward_cholesky per
+ * image.  lig_bin_batched: keys are global tile ids; tile_bins [num_tiles,2].
+ * lig_fit_render_l2_batched: loss_per_image[b] = image b's clamp + L2 mean,
+ * loss[0] = their sum; workspace lig_fit_batched_workspace_size(num_tiles).
+ * lig_fit_backward_records_l2_batched: records for upstream gradient
+ * grad_loss[0] of loss[0]; follow with lig_fit_reduce_records. */
+int lig_project_batched_cholesky(int n, const float *cholesky, const float *means2d,
+                                 const void *batch_table, int num_images, float *xys,
+                                 float *extents, float *conics, int32_t *num_tiles_hit,
+                                 void *stream);
+int lig_bin_batched(int n, int num_intersects, const float *xys, const float *extents,
+                    const int32_t *cum_tiles_hit, const void *batch_table, int num_images,
+                    int num_tiles, int32_t *gaussian_ids_sorted, int32_t *tile_bins,
+                    void *workspace, size_t workspace_bytes, void *stream);
+size_t lig_fit_batched_workspace_size(int num_tiles);
+int lig_fit_render_l2_batched(int n, const int32_t *gaussian_ids_sorted,
+                              const int32_t *tile_bins, const float *xys, const float *extents,
+                              const int32_t *cum_tiles_hit, const float *conics,
+                              const float *colors, void *packed, const float *target,
+                              const void *batch_table, int num_images, int num_tiles,
+                              float *out_img, float *loss_per_image, float *loss,
+                              void *workspace, size_t workspace_bytes, void *stream);
+int lig_fit_backward_records_l2_batched(int n, const int32_t *gaussian_ids_sorted,
+                                        const int32_t *tile_bins, const void *packed,
+                                        const float *img, const float *target,
+                                        const float *grad_loss, const void *batch_table,
+                                        int num_images, int num_tiles, int num_intersects,
+                                        void *workspace, size_t workspace_bytes, void *stream);
+
 /* No reference counterpart: workload counters for roofline accounting.
  * counts[0] = E, pixel/Gaussian pairs whose pixel centre lies inside the
  * Gaussian's clipped SnugBox box |p - mean| <= extent (Gaussians with

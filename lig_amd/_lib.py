@@ -78,6 +78,18 @@ _SIGNATURES = {
     "lig_adan_step": (
         _c_int, [ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp, _vp, _c_float, _c_float, _c_float,
                  _c_float, _c_float, _c_float, _c_int, _c_float, _c_int, _vp]),
+    "lig_project_batched_cholesky": (
+        _c_int, [_c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    "lig_bin_batched": (
+        _c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _c_size,
+                 _vp]),
+    "lig_fit_batched_workspace_size": (_c_size, [_c_int]),
+    "lig_fit_render_l2_batched": (
+        _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp,
+                 _vp, _vp, _vp, _c_size, _vp]),
+    "lig_fit_backward_records_l2_batched": (
+        _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp,
+                 _c_size, _vp]),
     "lig_count_pairs": (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_gradient_aware_upscale_forward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,

@@ -447,24 +447,36 @@ __global__ __launch_bounds__(kEmitThreads) void emit_isect_i64_kernel(
 // Gaussian's contiguous slot range.  Each wave computes the tile rectangles
 // of its 64 Gaussians, then writes their slots one Gaussian at a time with
 // all 64 lanes, so the stores are coalesced.
+//
+// Batched (tab != NULL, 16 x 16 tiles): each Gaussian's rectangle is taken in
+// its own image's tile grid and its keys are offset by the image's first
+// global tile.
 __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
     int n, int m, const float *__restrict__ xys, const float *__restrict__ extents,
     const int32_t *__restrict__ cum, int tiles_x, int tiles_y, int block_width,
-    uint32_t *__restrict__ keys, int32_t *__restrict__ slot_gid) {
+    const BatchImage *__restrict__ tab, int nimg, uint32_t *__restrict__ keys,
+    int32_t *__restrict__ slot_gid) {
     const int lane = threadIdx.x & (LIG_WAVE - 1);
     const int g0 = blockIdx.x * kEmitThreads + (threadIdx.x - lane);
     const int g = g0 + lane;
-    int start = 0, end = 0, x0 = 0, y0 = 0, w = 1, zero = 0;
+    int start = 0, end = 0, x0 = 0, y0 = 0, w = 1, zero = 0, row_len = tiles_x, toff = 0;
     if (g < n) {
         start = g == 0 ? 0 : cum[g - 1];
         end = min(cum[g], m);
+        int txs = tiles_x, tys = tiles_y;
+        if (tab) {
+            const BatchImage &im = tab[batch_find<1>(tab, nimg, g)];
+            txs = im.tiles_x;
+            tys = im.tiles_y;
+            toff = im.tile_off;
+            row_len = txs;
+        }
         const float ex = extents[2 * g], ey = extents[2 * g + 1];
         if (ex <= 0.0f || ey <= 0.0f) {
             zero = 1;
         } else {
             int x1, y1;
-            tile_bbox(xys[2 * g], xys[2 * g + 1], ex, ey, tiles_x, tiles_y, block_width, x0, y0,
-                      x1, y1);
+            tile_bbox(xys[2 * g], xys[2 * g + 1], ex, ey, txs, tys, block_width, x0, y0, x1, y1);
             w = max(x1 - x0, 1);
         }
     }
@@ -477,12 +489,14 @@ __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
         const int ry0 = __builtin_amdgcn_readlane(y0, j);
         const int rw = __builtin_amdgcn_readlane(w, j);
         const int rz = __builtin_amdgcn_readlane(zero, j);
+        const uint32_t rlen = (uint32_t)__builtin_amdgcn_readlane(row_len, j);
+        const uint32_t roff = (uint32_t)__builtin_amdgcn_readlane(toff, j);
         for (int k = lane; k < cnt; k += LIG_WAVE) {
             uint32_t key = 0;
             int gid = 0;
             if (!rz) {
                 const int row = k / rw;
-                key = (uint32_t)(ry0 + row) * (uint32_t)tiles_x + (uint32_t)(rx0 + k - row * rw);
+                key = roff + (uint32_t)(ry0 + row) * rlen + (uint32_t)(rx0 + k - row * rw);
                 gid = g0 + j;
             }
             keys[s0 + k] = key;
@@ -669,7 +683,7 @@ extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float
     uint32_t *slots_sorted = reinterpret_cast<uint32_t *>(take());
     hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(ceil_div(n, kEmitThreads)), dim3(kEmitThreads),
                        0, st, n, num_intersects, xys, extents, cum_tiles_hit, tx, ty,
-                       block_width, keys, slot_gid);
+                       block_width, (const BatchImage *)nullptr, 0, keys, slot_gid);
     if (!emission_ids_sorted) {
         // sort (tile, gid) pairs directly: the stable sort of the emission
         // order gives the same gids, and no gather is needed afterwards
@@ -690,5 +704,45 @@ extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float
                        num_intersects, (const uint32_t *)keys_sorted,
                        (const uint32_t *)slots_sorted, (const int32_t *)slot_gid, nt, tile_bins,
                        gaussian_ids_sorted);
+    return launch_status();
+}
+
+extern "C" int lig_bin_batched(int n, int num_intersects, const float *xys, const float *extents,
+                               const int32_t *cum_tiles_hit, const void *batch_table,
+                               int num_images, int num_tiles, int32_t *gaussian_ids_sorted,
+                               int32_t *tile_bins, void *workspace, size_t workspace_bytes,
+                               void *stream) {
+    if (n < 0 || num_intersects < 0 || num_images < 1 || num_tiles < 1 || !batch_table ||
+        !tile_bins)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (workspace_bytes < lig_bin_workspace_size(num_intersects, num_tiles))
+        return LIG_ERR_WORKSPACE;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * (size_t)num_tiles, 256)), dim3(256), 0,
+                       st, tile_bins, 2 * (size_t)num_tiles, 0);
+    if (num_intersects == 0 || n == 0) return launch_status();
+    if (!xys || !extents || !cum_tiles_hit || !gaussian_ids_sorted || !workspace)
+        return LIG_ERR_INVALID_ARGUMENT;
+    const size_t m = (size_t)num_intersects;
+    char *p = reinterpret_cast<char *>(workspace);
+    auto take = [&p, m]() {
+        char *q = p;
+        p += align_up(m * 4);
+        return q;
+    };
+    uint32_t *keys = reinterpret_cast<uint32_t *>(take());
+    int32_t *slot_gid = reinterpret_cast<int32_t *>(take());
+    uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(take());
+    take();
+    hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(ceil_div(n, kEmitThreads)), dim3(kEmitThreads),
+                       0, st, n, num_intersects, xys, extents, cum_tiles_hit, 0, 0, 16,
+                       reinterpret_cast<const BatchImage *>(batch_table), num_images, keys,
+                       slot_gid);
+    int rc = radix_sort<uint32_t>(keys, reinterpret_cast<const uint32_t *>(slot_gid), keys_sorted,
+                                  reinterpret_cast<uint32_t *>(gaussian_ids_sorted), m,
+                                  bits_for(num_tiles), 0u, p, st);
+    if (rc != LIG_OK) return rc;
+    hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(m, 1024)), dim3(256), 0, st,
+                       num_intersects, (const uint32_t *)keys_sorted, num_tiles, tile_bins);
     return launch_status();
 }

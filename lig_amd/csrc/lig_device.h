@@ -53,6 +53,34 @@ __device__ __forceinline__ void tile_bbox(float cx, float cy, float ex, float ey
     y1 = min(max(0, f2i_sat(tcy + tey + 1.0f)), tiles_y);
 }
 
+// One image of a batched fit: its tiles are [tile_off, tile_off + tiles_x *
+// tiles_y) of the batch's global tile space, its pixels [pix_off, pix_off +
+// w * h) of the concatenated HWC buffers, its Gaussians [gauss_off, next
+// image's gauss_off).  Coordinates inside an image are the image's own, so a
+// batched image computes exactly what the same image fitted alone does.  The
+// table has one entry per image plus a sentinel whose tile_off / gauss_off are
+// the totals.  Tiles are 16 x 16.
+struct BatchImage {
+    int tile_off, tiles_x, tiles_y, w, h, gauss_off;
+    long long pix_off;
+};
+
+// Index of the entry of `tab` (nimg real entries, sorted offsets) whose
+// [off, next off) range holds `key`, for the field selected by OFF.
+template <int OFF>
+__device__ __forceinline__ int batch_find(const BatchImage *__restrict__ tab, int nimg, int key) {
+    int lo = 0, hi = nimg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int v = OFF == 0 ? tab[mid].tile_off : tab[mid].gauss_off;
+        if (v <= key)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // Map a linear workgroup id to a work item so that each XCD (workgroups are
 // dispatched round-robin over the 8 XCDs) receives one contiguous range of
 // items: neighbouring tiles then share the XCD's L2.

@@ -95,6 +95,25 @@ __global__ __launch_bounds__(kThreads) void project_chol_kernel(
                 block_width, xys, extents, conics, num_tiles_hit);
 }
 
+// Batched fit: Gaussian i belongs to the image of `tab` whose Gaussian range
+// holds it and is projected onto that image's canvas (16 x 16 tiles).
+__global__ __launch_bounds__(kThreads) void project_batched_chol_kernel(
+    int n, const float *__restrict__ chol, const float *__restrict__ means2d,
+    const BatchImage *__restrict__ tab, int nimg, float *__restrict__ xys,
+    float *__restrict__ extents, float *__restrict__ conics,
+    int32_t *__restrict__ num_tiles_hit) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const BatchImage &im = tab[batch_find<1>(tab, nimg, i)];
+    const float l11 = chol[3 * i], l21 = chol[3 * i + 1], l22 = chol[3 * i + 2];
+    const float ca = l11 * l11;
+    const float cb = l11 * l21;
+    const float cc = l21 * l21 + l22 * l22;
+    project_one(i, ca, cb, cc, means2d[2 * i], means2d[2 * i + 1], nullptr, im.tiles_x,
+                im.tiles_y, 16, xys, extents, conics, num_tiles_hit);
+}
+
 // Projection into a view: the Gaussian seen in a window with origin (ox, oy)
 // of the source canvas, sampled at (sx, sy) output pixels per source pixel:
 //   mean' = (mean - o) * s,  L' = diag(s) L  (Cholesky: l11 sx, l21 sy, l22 sy),
@@ -274,6 +293,21 @@ extern "C" int lig_project_gaussians_view(int n, const float *par, const float *
                        as_stream(stream), n, cholesky, par, means2d, opacities, origin_x,
                        origin_y, scale_x, scale_y, tx, ty, block_width, xys, extents, conics,
                        num_tiles_hit);
+    return launch_status();
+}
+
+extern "C" int lig_project_batched_cholesky(int n, const float *cholesky, const float *means2d,
+                                            const void *batch_table, int num_images, float *xys,
+                                            float *extents, float *conics,
+                                            int32_t *num_tiles_hit, void *stream) {
+    if (n < 0 || num_images < 1 || !batch_table) return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    if (!cholesky || !means2d || !xys || !extents || !conics || !num_tiles_hit)
+        return LIG_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(project_batched_chol_kernel, dim3(ceil_div(n, kThreads)), dim3(kThreads),
+                       0, as_stream(stream), n, cholesky, means2d,
+                       reinterpret_cast<const BatchImage *>(batch_table), num_images, xys,
+                       extents, conics, num_tiles_hit);
     return launch_status();
 }
 

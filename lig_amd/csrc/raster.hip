@@ -55,7 +55,8 @@ __global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restric
                                                    const float *__restrict__ opacities,
                                                    const float *__restrict__ extents,
                                                    const int32_t *__restrict__ cum, int tiles_x,
-                                                   int tiles_y, GaussPack *__restrict__ pack) {
+                                                   int tiles_y, const BatchImage *__restrict__ tab,
+                                                   int nimg, GaussPack *__restrict__ pack) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float a = conics[3 * i], b = conics[3 * i + 1], c = conics[3 * i + 2];
@@ -85,8 +86,14 @@ __global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restric
     if (extents) {
         const float ex = extents[2 * i], ey = extents[2 * i + 1];
         if (ex > 0.0f && ey > 0.0f) {
+            int txs = tiles_x, tys = tiles_y;
+            if (tab) {  // batched: the Gaussian's own image grid, local tile coordinates
+                const BatchImage &im = tab[batch_find<1>(tab, nimg, i)];
+                txs = im.tiles_x;
+                tys = im.tiles_y;
+            }
             int x0, y0, x1, y1;
-            tile_bbox(xys[2 * i], xys[2 * i + 1], ex, ey, tiles_x, tiles_y, 16, x0, y0, x1, y1);
+            tile_bbox(xys[2 * i], xys[2 * i + 1], ex, ey, txs, tys, 16, x0, y0, x1, y1);
             w = max(x1 - x0, 1);
             base = (i == 0 ? 0 : cum[i - 1]) - y0 * w - x0;
         }
@@ -232,12 +239,22 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
     float *__restrict__ out_img, float *__restrict__ out_wsum,
     int32_t *__restrict__ final_idx, const float *__restrict__ target,
-    float *__restrict__ loss_partial) {
+    float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg) {
     __shared__ float4 s0[kThreads];
     __shared__ float4 s1[kThreads];
     __shared__ float2 s2[kThreads];  // (blue, quadrant mask bits)
 
-    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int gtile = (int)xcd_remap(blockIdx.x, gridDim.x);  // bins / partials index
+    int tile = gtile;                                            // tile of its own image
+    if (tab) {  // batched fit: the tile's image, its canvas and pixel buffers
+        const BatchImage &im = tab[batch_find<0>(tab, nimg, gtile)];
+        tile = gtile - im.tile_off;
+        tiles_x = im.tiles_x;
+        img_w = im.w;
+        img_h = im.h;
+        out_img += 3 * im.pix_off;
+        if (target) target += 3 * im.pix_off;
+    }
     const int lane = threadIdx.x;
     const int ty = tile / tiles_x;
     const int tx = tile - ty * tiles_x;
@@ -255,7 +272,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
         fin[q] = 0;
     }
 
-    const int2 range = bins[tile];
+    const int2 range = bins[gtile];
     for (int base = range.x; base < range.y; base += kThreads) {
         const int cnt = min(kThreads, range.y - base);
         if (lane < cnt) {
@@ -331,7 +348,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
         if (loss_partial) {
             for (int off = LIG_WAVE / 2; off > 0; off >>= 1)
                 lsum += __shfl_down(lsum, off, LIG_WAVE);
-            if (lane == 0) loss_partial[tile] = lsum;
+            if (lane == 0) loss_partial[gtile] = lsum;
         }
     }
 }
@@ -751,14 +768,26 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     const float *__restrict__ img, const float *__restrict__ target,
     const float *__restrict__ grad_loss, float norm, GradRec *__restrict__ rec,
     float *__restrict__ g_xy, float *__restrict__ g_xy_abs, float *__restrict__ g_conic,
-    float *__restrict__ g_rgb, float *__restrict__ g_opac) {
+    float *__restrict__ g_rgb, float *__restrict__ g_opac, const BatchImage *__restrict__ tab,
+    int nimg) {
     __shared__ float4 sv[kMaxPix];
     __shared__ int sfin[TRAIN ? 1 : kMaxPix];
 
-    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int gtile = (int)xcd_remap(blockIdx.x, gridDim.x);  // bins index
     const int lane = threadIdx.x;
-    const int2 range = bins[tile];
+    const int2 range = bins[gtile];
     if (range.x >= range.y) return;
+    int tile = gtile;  // tile of its own image
+    if (TRAIN && tab) {  // batched fit: the tile's image, canvas, buffers and loss scale
+        const BatchImage &im = tab[batch_find<0>(tab, nimg, gtile)];
+        tile = gtile - im.tile_off;
+        tiles_x = im.tiles_x;
+        img_w = im.w;
+        img_h = im.h;
+        img += 3 * im.pix_off;
+        target += 3 * im.pix_off;
+        norm = (float)(2.0 / (3.0 * (double)im.h * (double)im.w));
+    }
     const int ty = tile / tiles_x;
     const int tx = tile - ty * tiles_x;
     const int x_org = tx * 16, y_org = ty * 16;
@@ -982,6 +1011,30 @@ __global__ __launch_bounds__(kSumThreads) void sum_blocks_kernel(int nblocks,
     if (threadIdx.x == 0) out[0] = (float)(t * inv_n);
 }
 
+// Batched fit losses: image b's mean over its 3 w h values of the per-tile
+// clamp + L2 sums of its tile range (FP64, fixed order), then their total.
+__global__ __launch_bounds__(kSumThreads) void image_loss_kernel(
+    const BatchImage *__restrict__ tab, const float *__restrict__ partial,
+    float *__restrict__ loss_img) {
+    __shared__ double ws[kSumThreads];
+    const BatchImage im = tab[blockIdx.x];
+    const int t1 = tab[blockIdx.x + 1].tile_off;
+    double acc = 0.0;
+    for (int t = im.tile_off + (int)threadIdx.x; t < t1; t += kSumThreads) acc += partial[t];
+    const double sum = block_sum_f64(acc, ws);
+    if (threadIdx.x == 0) loss_img[blockIdx.x] = (float)(sum / (3.0 * (double)im.h * im.w));
+}
+
+__global__ __launch_bounds__(kSumThreads) void total_loss_kernel(int nimg,
+                                                                const float *__restrict__ loss_img,
+                                                                float *__restrict__ total) {
+    __shared__ double ws[kSumThreads];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nimg; b += kSumThreads) acc += loss_img[b];
+    const double sum = block_sum_f64(acc, ws);
+    if (threadIdx.x == 0) total[0] = (float)sum;
+}
+
 __global__ void zero_f32_kernel(float *__restrict__ p, size_t count) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = 0.0f;
@@ -1072,7 +1125,7 @@ extern "C" int lig_pack_gaussians(int n, const float *xys, const float *conics,
         return LIG_ERR_INVALID_ARGUMENT;
     if (n == 0) return LIG_OK;
     hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), n,
-                       xys, conics, colors, opacities, nullptr, nullptr, 0, 0,
+                       xys, conics, colors, opacities, nullptr, nullptr, 0, 0, nullptr, 0,
                        reinterpret_cast<GaussPack *>(packed));
     return launch_status();
 }
@@ -1107,7 +1160,8 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
     const int tiles_y = (img_height + block_width - 1) / block_width;
     if (n > 0)
         hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
-                           colors, opacities, extents, cum_tiles_hit, tiles_x, tiles_y, pk);
+                           colors, opacities, extents, cum_tiles_hit, tiles_x, tiles_y, nullptr,
+                           0, pk);
     const dim3 grid((unsigned)(tiles_x * tiles_y));
     const int px = (block_width * block_width + LIG_WAVE - 1) / LIG_WAVE;
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
@@ -1118,19 +1172,19 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
     else if (image_only && opacities)
         hipLaunchKernelGGL((raster_fwd16_kernel<true, true>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           nullptr, nullptr, nullptr, nullptr);
+                           nullptr, nullptr, nullptr, nullptr, nullptr, 0);
     else if (image_only)
         hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           nullptr, nullptr, nullptr, nullptr);
+                           nullptr, nullptr, nullptr, nullptr, nullptr, 0);
     else if (block_width == 16 && opacities)
         hipLaunchKernelGGL((raster_fwd16_kernel<true, false>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           out_wsum, final_idx, nullptr, nullptr);
+                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0);
     else if (block_width == 16)
         hipLaunchKernelGGL((raster_fwd16_kernel<false, false>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           out_wsum, final_idx, nullptr, nullptr);
+                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0);
     else
         launch_fwd<false>(px, grid, st, tiles_x, block_width, img_width, img_height,
                           gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, nullptr,
@@ -1165,19 +1219,54 @@ extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
     const int tiles_y = (img_height + 15) / 16;
     const int tiles = tiles_x * tiles_y;
     hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
-                       colors, nullptr, extents, cum_tiles_hit, tiles_x, tiles_y, pk);
+                       colors, nullptr, extents, cum_tiles_hit, tiles_x, tiles_y, nullptr, 0, pk);
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
     const int nsum = (tiles + kSumChunk - 1) / kSumChunk;
     double *bsum = reinterpret_cast<double *>(workspace);
     float *partial = reinterpret_cast<float *>(bsum + nsum);
     hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), dim3((unsigned)tiles), dim3(kThreads),
                        0, st, tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk,
-                       out_img, nullptr, nullptr, target, partial);
+                       out_img, nullptr, nullptr, target, partial, nullptr, 0);
     const double numel = 3.0 * (double)img_height * (double)img_width;
     hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)nsum), dim3(kSumThreads), 0, st, tiles,
                        (const float *)partial, bsum);
     hipLaunchKernelGGL(sum_blocks_kernel, dim3(1), dim3(kSumThreads), 0, st, nsum,
                        (const double *)bsum, 1.0 / numel, loss);
+    return launch_status();
+}
+
+extern "C" size_t lig_fit_batched_workspace_size(int num_tiles) {
+    return num_tiles > 0 ? (size_t)num_tiles * sizeof(float) + 256 : 256;
+}
+
+extern "C" int lig_fit_render_l2_batched(int n, const int32_t *gaussian_ids_sorted,
+                                         const int32_t *tile_bins, const float *xys,
+                                         const float *extents, const int32_t *cum_tiles_hit,
+                                         const float *conics, const float *colors, void *packed,
+                                         const float *target, const void *batch_table,
+                                         int num_images, int num_tiles, float *out_img,
+                                         float *loss_per_image, float *loss, void *workspace,
+                                         size_t workspace_bytes, void *stream) {
+    if (n <= 0 || num_images < 1 || num_tiles < 1 || !gaussian_ids_sorted || !tile_bins ||
+        !xys || !extents || !cum_tiles_hit || !conics || !colors || !packed || !target ||
+        !batch_table || !out_img || !loss_per_image || !loss)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (!workspace || workspace_bytes < lig_fit_batched_workspace_size(num_tiles))
+        return LIG_ERR_WORKSPACE;
+    hipStream_t st = as_stream(stream);
+    const BatchImage *tab = reinterpret_cast<const BatchImage *>(batch_table);
+    GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
+    hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
+                       colors, nullptr, extents, cum_tiles_hit, 0, 0, tab, num_images, pk);
+    float *partial = reinterpret_cast<float *>(workspace);
+    hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), dim3((unsigned)num_tiles),
+                       dim3(kThreads), 0, st, 1, 1, 1, gaussian_ids_sorted,
+                       reinterpret_cast<const int2 *>(tile_bins), pk, out_img, nullptr, nullptr,
+                       target, partial, tab, num_images);
+    hipLaunchKernelGGL(image_loss_kernel, dim3((unsigned)num_images), dim3(kSumThreads), 0, st,
+                       tab, (const float *)partial, loss_per_image);
+    hipLaunchKernelGGL(total_loss_kernel, dim3(1), dim3(kSumThreads), 0, st, num_images,
+                       (const float *)loss_per_image, loss);
     return launch_status();
 }
 
@@ -1208,7 +1297,8 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
         v_wsum, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity
 #define LIG_BWD_ARGS_16                                                                      \
     tiles_x, img_width, img_height, gids, slots, bins, pk, final_idx, v_output, v_wsum,         \
-        nullptr, nullptr, nullptr, 0.0f, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity
+        nullptr, nullptr, nullptr, 0.0f, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity,      \
+        nullptr, 0
 #define LIG_BWD_ARGS_GA                                                                      \
     tiles_x, block_width, img_width, img_height, gids, slots, bins, pk, final_idx, v_output,    \
         v_wsum, v_output_dx, v_output_dy, v_output_dxy, has_opacities, rec, v_xy, v_xy_abs,    \
@@ -1301,7 +1391,28 @@ extern "C" int lig_fit_backward_records_l2(int n, int img_height, int img_width,
     hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), grid, dim3(kThreads), 0,
                        as_stream(stream), tiles_x, img_width, img_height, gaussian_ids_sorted,
                        nullptr, bins, pk, nullptr, nullptr, nullptr, img, target, grad_loss, norm,
-                       rec, nullptr, nullptr, nullptr, nullptr, nullptr);
+                       rec, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+    return launch_status();
+}
+
+extern "C" int lig_fit_backward_records_l2_batched(
+    int n, const int32_t *gaussian_ids_sorted, const int32_t *tile_bins, const void *packed,
+    const float *img, const float *target, const float *grad_loss, const void *batch_table,
+    int num_images, int num_tiles, int num_intersects, void *workspace, size_t workspace_bytes,
+    void *stream) {
+    if (n <= 0 || num_images < 1 || num_tiles < 1 || num_intersects < 0 ||
+        !gaussian_ids_sorted || !tile_bins || !packed || !img || !target || !grad_loss ||
+        !batch_table)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (!workspace || workspace_bytes < lig_rasterize_backward_workspace_size(num_intersects))
+        return LIG_ERR_WORKSPACE;
+    GradRec *rec = reinterpret_cast<GradRec *>(reinterpret_cast<char *>(workspace) + 256);
+    hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), dim3((unsigned)num_tiles),
+                       dim3(kThreads), 0, as_stream(stream), 1, 1, 1, gaussian_ids_sorted,
+                       nullptr, reinterpret_cast<const int2 *>(tile_bins),
+                       reinterpret_cast<const GaussPack *>(packed), nullptr, nullptr, nullptr,
+                       img, target, grad_loss, 0.0f, rec, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, reinterpret_cast<const BatchImage *>(batch_table), num_images);
     return launch_status();
 }
 
