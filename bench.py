@@ -156,7 +156,13 @@ def batched_run(args, dev, torch):
     plain = [fit.Level(c, h, w, s, scale, device=dev) for (h, w), c, s in
              zip(sizes, counts, seeds)]
     bf = batch.BatchedFit(targets, counts, seeds, scale=scale)
-    res = {"images": len(sizes), "pixels": int(sum(h * w for h, w in sizes)),
+    from lig_amd import autotune
+    tuned = autotune.tune_batched(bf, use_cache=False)
+    for lv, t in zip(plain, targets):
+        autotune.tune_level(lv, t, reps=1, use_cache=False)
+    res = {"batch_depth": tuned["choice"], "batch_depth_ms": tuned["ms"],
+           "plain_batch_depths": sorted({lv.batch_depth for lv in plain}),
+           "images": len(sizes), "pixels": int(sum(h * w for h, w in sizes)),
            "gaussians": int(sum(counts)), "steps": args.batch_steps,
            "workload": "64 seeded images, height log-uniform 256-2048, width log-uniform "
                        "384-1536, 0.05 Gaussians/px, chol diag U(0.5,3)*2 px, L2, Adam"}
@@ -257,6 +263,10 @@ def main():
     del up
     torch.cuda.synchronize()
     log(f"setup {time.perf_counter() - t0:.1f}s")
+    from lig_amd import autotune
+    tuned = {"l1": autotune.tune_level(l1, t1, use_cache=False),
+             "l2": autotune.tune_level(l2, t2, use_cache=False)}
+    log(f"autotune {tuned}")
 
     for i in range(args.warmup):
         l1.step(t1)
@@ -380,6 +390,7 @@ def main():
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
+        "autotune": {k: {"batch_depth": v["choice"], "ms": v["ms"]} for k, v in tuned.items()},
         "batched": batched,
         "serve": serve,
     }

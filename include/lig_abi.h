@@ -234,7 +234,9 @@ int lig_reduce_gradient_records(int n, int num_intersects, const int32_t *cum_ti
  * exactly as lig_rasterize_forward (same pair test, same order, so the same
  * bits) but without out_wsum / final_idx, and writes
  * loss[0] = mean((clamp(out_img,0,1) - target)^2) over H*W*3 values, summed
- * per tile in FP32 and over tiles in FP64 (deterministic).  Workspace:
+ * per tile in FP32 and over tiles in FP64 (deterministic).  batch_depth
+ * (64, 128, 256; 0 = 64) is the number of Gaussians staged in LDS per batch;
+ * it changes speed only, not results.  Workspace:
  * lig_fit_render_workspace_size(H, W).
  *
  * lig_fit_backward_records_l2: the records half of the deterministic backward
@@ -254,7 +256,7 @@ int lig_fit_render_l2(int n, int img_height, int img_width, const int32_t *gauss
                       const int32_t *tile_bins, const float *xys, const float *extents,
                       const int32_t *cum_tiles_hit, const float *conics, const float *colors,
                       void *packed, const float *target, float *out_img, float *loss,
-                      void *workspace, size_t workspace_bytes, void *stream);
+                      int batch_depth, void *workspace, size_t workspace_bytes, void *stream);
 int lig_fit_backward_records_l2(int n, int img_height, int img_width,
                                 const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
                                 const void *packed, const float *img, const float *target,
@@ -296,7 +298,8 @@ int lig_fit_render_l2_batched(int n, const int32_t *gaussian_ids_sorted,
                               const float *colors, void *packed, const float *target,
                               const void *batch_table, int num_images, int num_tiles,
                               float *out_img, float *loss_per_image, float *loss,
-                              void *workspace, size_t workspace_bytes, void *stream);
+                              int batch_depth, void *workspace, size_t workspace_bytes,
+                              void *stream);
 int lig_fit_backward_records_l2_batched(int n, const int32_t *gaussian_ids_sorted,
                                         const int32_t *tile_bins, const void *packed,
                                         const float *img, const float *target,

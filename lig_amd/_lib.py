@@ -66,7 +66,7 @@ _SIGNATURES = {
     "lig_fit_render_workspace_size": (_c_size, [_c_int, _c_int]),
     "lig_fit_render_l2": (
         _c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                 _vp, _c_size, _vp]),
+                 _c_int, _vp, _c_size, _vp]),
     "lig_fit_backward_records_l2": (
         _c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_size,
                  _vp]),
@@ -86,7 +86,7 @@ _SIGNATURES = {
     "lig_fit_batched_workspace_size": (_c_size, [_c_int]),
     "lig_fit_render_l2_batched": (
         _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp,
-                 _vp, _vp, _vp, _c_size, _vp]),
+                 _vp, _vp, _c_int, _vp, _c_size, _vp]),
     "lig_fit_backward_records_l2_batched": (
         _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp,
                  _c_size, _vp]),

@@ -1,0 +1,104 @@
+"""Autotuning of the fit step's launch configuration.
+
+The tuned knob is the forward's LDS batch depth: how many Gaussians a tile's
+wave stages in LDS per batch (64, 128 or 256; results do not depend on it).
+Each candidate is timed on the caller's own workload with HIP events around
+the fused render call (lig_amd.profiling.KernelTimer), a few repetitions
+after one warm-up, and the fastest is kept.  Choices are cached per workload
+signature in memory and, when writable, in a JSON file
+($LIG_AUTOTUNE_CACHE, default ~/.cache/lig_amd/autotune.json).
+
+Tile size is not a knob here: the fused fit kernels are written for 16 x 16
+tiles (the reference API caps block_width at 16, rasterize.py:26); other tile
+widths run through the gsplat2d API path.
+"""
+from __future__ import annotations
+
+import json
+import os
+from typing import Callable, Dict, Optional, Sequence
+
+import torch
+
+from .profiling import KernelTimer
+
+DEPTHS = (64, 128, 256)
+_cache: Dict[str, int] = {}
+
+
+def _cache_path() -> str:
+    return os.environ.get("LIG_AUTOTUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "lig_amd",
+                                       "autotune.json"))
+
+
+def _load():
+    try:
+        with open(_cache_path()) as f:
+            _cache.update({k: int(v) for k, v in json.load(f).items()})
+    except (OSError, ValueError):
+        pass
+
+
+def _store():
+    try:
+        path = _cache_path()
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as f:
+            json.dump(_cache, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
+def tune(key: str, run: Callable[[int], None], entry: str, device,
+         candidates: Sequence[int] = DEPTHS, reps: int = 3, use_cache: bool = True) -> dict:
+    """Time `run(depth)` for each depth (the C-ABI call `entry` timed by HIP
+    events); returns {"choice": depth, "ms": {depth: mean ms}}."""
+    if use_cache:
+        if not _cache:
+            _load()
+        if key in _cache:
+            return {"choice": _cache[key], "ms": None, "cached": True}
+    ms = {}
+    for d in candidates:
+        run(d)   # warm-up
+        t = KernelTimer(entry, device)
+        with t:
+            for _ in range(reps):
+                run(d)
+        torch.cuda.synchronize(device)
+        ms[d] = t.mean_ms()
+    choice = min(ms, key=ms.get)
+    _cache[key] = choice
+    _store()
+    return {"choice": choice, "ms": {k: round(v, 4) for k, v in ms.items()}, "cached": False}
+
+
+def tune_level(level, target: torch.Tensor, reps: int = 3, use_cache: bool = True) -> dict:
+    """Pick lig_amd.fit.Level.batch_depth for `level` fitting `target`."""
+    key = f"level:{level.H}x{level.W}:n{level.means.shape[0]}"
+
+    def run(d):
+        with torch.no_grad():
+            level.batch_depth = d
+            level.loss(target)
+
+    res = tune(key, run, "lig_fit_render_l2", level.means.device, reps=reps,
+               use_cache=use_cache)
+    level.batch_depth = res["choice"]
+    return res
+
+
+def tune_batched(bf, reps: int = 3, use_cache: bool = True) -> dict:
+    """Pick lig_amd.batch.BatchedFit.batch_depth."""
+    key = f"batch:{len(bf.sizes)}:{bf.num_tiles}:n{bf.means.shape[0]}"
+
+    def run(d):
+        with torch.no_grad():
+            bf.batch_depth = d
+            bf.loss()
+
+    res = tune(key, run, "lig_fit_render_l2_batched", bf.means.device, reps=reps,
+               use_cache=use_cache)
+    bf.batch_depth = res["choice"]
+    return res

@@ -46,7 +46,7 @@ def batch_table(sizes: Sequence[tuple], counts: Sequence[int]):
 
 class _BatchedRenderL2(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, means, cholesky, rgbs, target, table, num_images, num_tiles):
+    def forward(ctx, means, cholesky, rgbs, target, table, num_images, num_tiles, depth):
         n = means.shape[0]
         dev = means.device
         xys = torch.empty(n, 2, device=dev)
@@ -81,7 +81,7 @@ class _BatchedRenderL2(torch.autograd.Function):
             L.call("lig_fit_render_l2_batched", n, L.ptr(gids), L.ptr(bins), L.ptr(xys),
                    L.ptr(ext), L.ptr(cum), L.ptr(con), L.ptr(rgbs), L.ptr(packed),
                    L.ptr(target), L.ptr(table), num_images, num_tiles, L.ptr(img),
-                   L.ptr(loss_img), L.ptr(loss), L.ptr(rws), rws.numel(), s)
+                   L.ptr(loss_img), L.ptr(loss), int(depth), L.ptr(rws), rws.numel(), s)
         ctx.save_for_backward(cholesky, ext, con, gids, bins, cum, img, target, packed, table)
         ctx.mark_non_differentiable(loss_img)
         return loss, loss_img
@@ -106,7 +106,7 @@ class _BatchedRenderL2(torch.autograd.Function):
                    L.ptr(v_xy), L.ptr(v_conic), L.ptr(v_rgb), s)
         v_chol, v_mean = _C.project_gaussians_backward_cholesky(n, ext, cholesky, con, v_xy,
                                                                 v_conic)
-        return v_mean, v_chol, v_rgb, None, None, None, None
+        return v_mean, v_chol, v_rgb, None, None, None, None, None
 
 
 class BatchedFit(nn.Module):
@@ -130,11 +130,12 @@ class BatchedFit(nn.Module):
         self.target = torch.cat([t.reshape(-1, 3) for t in targets]).contiguous()
         self.optimizer = FusedAdam([self.rgbs, self.means, self.cholesky], lr=lr)
         self.last_losses = None
+        self.batch_depth = 0   # LDS batch depth of the forward (0 = default; see autotune)
 
     def loss(self):
         loss, per_image = _BatchedRenderL2.apply(self.means, self.cholesky, self.rgbs,
                                                  self.target, self.table, len(self.sizes),
-                                                 self.num_tiles)
+                                                 self.num_tiles, self.batch_depth)
         self.last_losses = per_image
         return loss
 

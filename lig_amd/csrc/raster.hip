@@ -233,16 +233,20 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
 // TRAIN (fit step and render-only): no wsum / final index; with a target the
 // tile's clamp + L2 sum, over its pixels and channels of (clamp(I, 0, 1) - t)^2,
 // is written to loss_partial[tile] (FP32, fixed lane order).
-template <bool HAS_OPAC, bool TRAIN>
+//
+// DEPTH (64, 128, 256): Gaussians staged in LDS per batch, DEPTH / 64 per lane;
+// deeper batches keep more gathers in flight per wave and halve or quarter the
+// barriers (chosen per workload by lig_amd.autotune).
+template <bool HAS_OPAC, bool TRAIN, int DEPTH = kThreads>
 __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
     float *__restrict__ out_img, float *__restrict__ out_wsum,
     int32_t *__restrict__ final_idx, const float *__restrict__ target,
     float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg) {
-    __shared__ float4 s0[kThreads];
-    __shared__ float4 s1[kThreads];
-    __shared__ float2 s2[kThreads];  // (blue, quadrant mask bits)
+    __shared__ float4 s0[DEPTH];
+    __shared__ float4 s1[DEPTH];
+    __shared__ float2 s2[DEPTH];  // (blue, quadrant mask bits)
 
     const int gtile = (int)xcd_remap(blockIdx.x, gridDim.x);  // bins / partials index
     int tile = gtile;                                            // tile of its own image
@@ -273,12 +277,21 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
     }
 
     const int2 range = bins[gtile];
-    for (int base = range.x; base < range.y; base += kThreads) {
-        const int cnt = min(kThreads, range.y - base);
-        if (lane < cnt) {
-            const GaussPack gp = pack[gids[base + lane]];
-            s0[lane] = gp.p0;
-            s1[lane] = gp.p1;
+    for (int base = range.x; base < range.y; base += DEPTH) {
+        const int cnt = min(DEPTH, range.y - base);
+        int gq[DEPTH / kThreads];
+#pragma unroll
+        for (int k = 0; k < DEPTH / kThreads; ++k) {
+            const int j = k * kThreads + lane;
+            gq[k] = j < cnt ? gids[base + j] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < DEPTH / kThreads; ++k) {
+            const int j = k * kThreads + lane;
+            if (gq[k] < 0) continue;
+            const GaussPack gp = pack[gq[k]];
+            s0[j] = gp.p0;
+            s1[j] = gp.p1;
             // quadrant q spans pixel centres [org + 8h + 0.5, org + 8h + 7.5]
             const float xl = gp.p0.x - gp.p3.x, xh = gp.p0.x + gp.p3.x;
             const float yl = gp.p0.y - gp.p3.y, yh = gp.p0.y + gp.p3.y;
@@ -290,7 +303,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
             const bool cy1 = !(yl > by0 + 15.0f) && !(yh < by0 + 8.0f);
             const int m = (cx0 && cy0 ? 1 : 0) | (cx1 && cy0 ? 2 : 0) | (cx0 && cy1 ? 4 : 0) |
                           (cx1 && cy1 ? 8 : 0);
-            s2[lane] = make_float2(gp.p2.x, __int_as_float(m));
+            s2[j] = make_float2(gp.p2.x, __int_as_float(m));
         }
         __syncthreads();
         for (int t = 0; t < cnt; ++t) {
@@ -1192,6 +1205,31 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
     return launch_status();
 }
 
+namespace lig {
+namespace {
+// LDS batch depths of the fit forward; 0 selects the default, 64.
+bool depth_ok(int d) { return d == 0 || d == 64 || d == 128 || d == 256; }
+
+// Fit-mode (image + loss partials) forward at LDS batch depth `depth`.
+void launch_fit_forward(int depth, dim3 grid, hipStream_t st, int tiles_x, int img_w, int img_h,
+                        const int32_t *gids, const int2 *bins, const GaussPack *pk,
+                        float *out_img, const float *target, float *partial,
+                        const BatchImage *tab, int nimg) {
+#define LIG_FIT_FWD(D)                                                                        \
+    hipLaunchKernelGGL((raster_fwd16_kernel<false, true, D>), grid, dim3(kThreads), 0, st,     \
+                       tiles_x, img_w, img_h, gids, bins, pk, out_img, nullptr, nullptr, target, \
+                       partial, tab, nimg)
+    if (depth == 256)
+        LIG_FIT_FWD(256);
+    else if (depth == 128)
+        LIG_FIT_FWD(128);
+    else
+        LIG_FIT_FWD(64);
+#undef LIG_FIT_FWD
+}
+}  // namespace
+}  // namespace lig
+
 extern "C" size_t lig_fit_render_workspace_size(int img_height, int img_width) {
     if (img_height <= 0 || img_width <= 0) return 256;
     const size_t tiles = (size_t)((img_height + 15) / 16) * (size_t)((img_width + 15) / 16);
@@ -1205,11 +1243,11 @@ extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
                                  const float *xys, const float *extents,
                                  const int32_t *cum_tiles_hit, const float *conics,
                                  const float *colors, void *packed, const float *target,
-                                 float *out_img, float *loss, void *workspace,
+                                 float *out_img, float *loss, int batch_depth, void *workspace,
                                  size_t workspace_bytes, void *stream) {
     if (n <= 0 || img_height <= 0 || img_width <= 0 || !gaussian_ids_sorted || !tile_bins ||
         !xys || !extents || !cum_tiles_hit || !conics || !colors || !packed || !target ||
-        !out_img || !loss)
+        !out_img || !loss || !depth_ok(batch_depth))
         return LIG_ERR_INVALID_ARGUMENT;
     if (!workspace || workspace_bytes < lig_fit_render_workspace_size(img_height, img_width))
         return LIG_ERR_WORKSPACE;
@@ -1224,9 +1262,8 @@ extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
     const int nsum = (tiles + kSumChunk - 1) / kSumChunk;
     double *bsum = reinterpret_cast<double *>(workspace);
     float *partial = reinterpret_cast<float *>(bsum + nsum);
-    hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), dim3((unsigned)tiles), dim3(kThreads),
-                       0, st, tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk,
-                       out_img, nullptr, nullptr, target, partial, nullptr, 0);
+    launch_fit_forward(batch_depth, dim3((unsigned)tiles), st, tiles_x, img_width, img_height,
+                       gaussian_ids_sorted, bins, pk, out_img, target, partial, nullptr, 0);
     const double numel = 3.0 * (double)img_height * (double)img_width;
     hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)nsum), dim3(kSumThreads), 0, st, tiles,
                        (const float *)partial, bsum);
@@ -1245,11 +1282,12 @@ extern "C" int lig_fit_render_l2_batched(int n, const int32_t *gaussian_ids_sort
                                          const float *conics, const float *colors, void *packed,
                                          const float *target, const void *batch_table,
                                          int num_images, int num_tiles, float *out_img,
-                                         float *loss_per_image, float *loss, void *workspace,
-                                         size_t workspace_bytes, void *stream) {
+                                         float *loss_per_image, float *loss, int batch_depth,
+                                         void *workspace, size_t workspace_bytes,
+                                         void *stream) {
     if (n <= 0 || num_images < 1 || num_tiles < 1 || !gaussian_ids_sorted || !tile_bins ||
         !xys || !extents || !cum_tiles_hit || !conics || !colors || !packed || !target ||
-        !batch_table || !out_img || !loss_per_image || !loss)
+        !batch_table || !out_img || !loss_per_image || !loss || !depth_ok(batch_depth))
         return LIG_ERR_INVALID_ARGUMENT;
     if (!workspace || workspace_bytes < lig_fit_batched_workspace_size(num_tiles))
         return LIG_ERR_WORKSPACE;
@@ -1259,10 +1297,9 @@ extern "C" int lig_fit_render_l2_batched(int n, const int32_t *gaussian_ids_sort
     hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
                        colors, nullptr, extents, cum_tiles_hit, 0, 0, tab, num_images, pk);
     float *partial = reinterpret_cast<float *>(workspace);
-    hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), dim3((unsigned)num_tiles),
-                       dim3(kThreads), 0, st, 1, 1, 1, gaussian_ids_sorted,
-                       reinterpret_cast<const int2 *>(tile_bins), pk, out_img, nullptr, nullptr,
-                       target, partial, tab, num_images);
+    launch_fit_forward(batch_depth, dim3((unsigned)num_tiles), st, 1, 1, 1, gaussian_ids_sorted,
+                       reinterpret_cast<const int2 *>(tile_bins), pk, out_img, target, partial,
+                       tab, num_images);
     hipLaunchKernelGGL(image_loss_kernel, dim3((unsigned)num_images), dim3(kSumThreads), 0, st,
                        tab, (const float *)partial, loss_per_image);
     hipLaunchKernelGGL(total_loss_kernel, dim3(1), dim3(kSumThreads), 0, st, num_images,

@@ -38,7 +38,7 @@ class _RenderL2(torch.autograd.Function):
     the loss value is summed per tile instead of per 8192-float block."""
 
     @staticmethod
-    def forward(ctx, means, cholesky, rgbs, target, H, W):
+    def forward(ctx, means, cholesky, rgbs, target, H, W, depth=0):
         n = means.shape[0]
         xys, ext, conics, nt = _C.project_gaussians_forward_cholesky(n, cholesky, means, None,
                                                                      H, W, 16)
@@ -51,7 +51,7 @@ class _RenderL2(torch.autograd.Function):
             return ((1.0 - target) ** 2).mean()
         gids, bins = _C.bin_and_sort_fast(n, num_int, xys, ext, cum, H, W, 16)
         img, loss, packed = _C.fit_render_l2(H, W, gids, bins, xys, ext, cum, conics, rgbs,
-                                             target)
+                                             target, depth)
         ctx.num_int = num_int
         ctx.save_for_backward(cholesky, ext, conics, gids, bins, cum, img, target, packed)
         return loss
@@ -61,7 +61,7 @@ class _RenderL2(torch.autograd.Function):
         if ctx.empty:
             z = [torch.zeros(s, dtype=grad_loss.dtype, device=grad_loss.device)
                  for s in ctx.shapes]
-            return z[0], z[1], z[2], None, None, None
+            return z[0], z[1], z[2], None, None, None, None
         cholesky, ext, conics, gids, bins, cum, img, target, packed = ctx.saved_tensors
         H, W = ctx.hw
         v_xy, v_conic, v_rgb = _C.fit_backward_l2(H, W, gids, bins, packed, ctx.n,
@@ -69,12 +69,14 @@ class _RenderL2(torch.autograd.Function):
                                                      cum)
         v_chol, v_mean = _C.project_gaussians_backward_cholesky(ctx.n, ext, cholesky, conics,
                                                                 v_xy, v_conic)
-        return v_mean, v_chol, v_rgb, None, None, None
+        return v_mean, v_chol, v_rgb, None, None, None, None
 
 
-def render_l2_loss(means, cholesky, rgbs, target, H: int, W: int) -> torch.Tensor:
-    """Fused clamp + L2 fit loss of a Cholesky level (16 x 16 tiles)."""
-    return _RenderL2.apply(means, cholesky, rgbs, target.contiguous(), H, W)
+def render_l2_loss(means, cholesky, rgbs, target, H: int, W: int,
+                   batch_depth: int = 0) -> torch.Tensor:
+    """Fused clamp + L2 fit loss of a Cholesky level (16 x 16 tiles); batch_depth
+    is the forward's LDS batch depth (64, 128, 256; 0 = 64)."""
+    return _RenderL2.apply(means, cholesky, rgbs, target.contiguous(), H, W, batch_depth)
 
 
 class Level(nn.Module):
@@ -92,6 +94,7 @@ class Level(nn.Module):
         self.rgbs = nn.Parameter(rgbs)
         self.H, self.W, self.block_width = H, W, block_width
         self.fused = fused and block_width == 16
+        self.batch_depth = 0   # forward LDS batch depth (0 = default; see lig_amd.autotune)
         self.optimizer = _adam([self.rgbs, self.means, self.cholesky], lr)
 
     def project(self):
@@ -108,7 +111,7 @@ class Level(nn.Module):
     def loss(self, target_hwc: torch.Tensor) -> torch.Tensor:
         if self.fused:
             return render_l2_loss(self.means, self.cholesky, self.rgbs, target_hwc, self.H,
-                                  self.W)
+                                  self.W, self.batch_depth)
         return l2_loss_clamped(self.render(), target_hwc)
 
     def step(self, target_hwc: torch.Tensor) -> torch.Tensor:

@@ -393,7 +393,7 @@ def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_s
 
 
 def fit_render_l2(img_height, img_width, gaussian_ids_sorted, tile_bins, xys, extents,
-                  cum_tiles_hit, conics, colors, target):
+                  cum_tiles_hit, conics, colors, target, batch_depth=0):
     """Fused fit-step render (16 x 16 tiles, no opacities): (out_img [H,W,3],
     loss [] tensor, packed).  Bins from bin_and_sort_fast without emission
     ids.  See lig_fit_render_l2 in include/lig_abi.h."""
@@ -415,8 +415,8 @@ def fit_render_l2(img_height, img_width, gaussian_ids_sorted, tile_bins, xys, ex
     with torch.cuda.device(xys.device):
         L.call("lig_fit_render_l2", n, H, W, L.ptr(gaussian_ids_sorted), L.ptr(tile_bins),
                L.ptr(xys), L.ptr(extents), L.ptr(cum_tiles_hit), L.ptr(conics), L.ptr(colors),
-               L.ptr(packed), L.ptr(target), L.ptr(out_img), L.ptr(loss), L.ptr(ws), ws.numel(),
-               L.stream_of(xys))
+               L.ptr(packed), L.ptr(target), L.ptr(out_img), L.ptr(loss), int(batch_depth),
+               L.ptr(ws), ws.numel(), L.stream_of(xys))
     return out_img, loss, packed
 
 

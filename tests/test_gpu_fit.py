@@ -115,3 +115,28 @@ def test_fused_no_intersections(device):
     for name in ("means", "cholesky", "rgbs"):
         assert torch.count_nonzero(getattr(a, name).grad) == 0
         assert torch.equal(getattr(a, name).grad, getattr(b, name).grad)
+
+
+def test_batch_depth_changes_speed_not_bits(device, tmp_path, monkeypatch):
+    """The forward's LDS batch depth (64 / 128 / 256) gives the same image,
+    loss and gradients; the autotuner picks one of them."""
+    from lig_amd import autotune, data, fit
+    monkeypatch.setenv("LIG_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    H, W = 120, 150
+    tgt = data.make_image(H, W, 9, device=device)
+    outs = []
+    for d in (64, 128, 256):
+        lv = fit.Level(9000, H, W, 5, 3.0, device=device)   # deep tiles: several batches
+        lv.batch_depth = d
+        loss = lv.loss(tgt)
+        loss.backward()
+        outs.append((float(loss.detach()), lv.means.grad.clone(), lv.cholesky.grad.clone(),
+                     lv.rgbs.grad.clone()))
+    for o in outs[1:]:
+        assert o[0] == outs[0][0]
+        for a, b in zip(o[1:], outs[0][1:]):
+            assert torch.equal(a, b)
+    lv = fit.Level(9000, H, W, 5, 3.0, device=device)
+    res = autotune.tune_level(lv, tgt, reps=2, use_cache=False)
+    assert res["choice"] in (64, 128, 256) and lv.batch_depth == res["choice"]
+    assert set(res["ms"]) == {64, 128, 256}
