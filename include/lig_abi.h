@@ -373,6 +373,27 @@ int lig_gradient_aware_upscale_backward(int src_h, int src_w, int dst_h, int dst
                                         float *grad_dxy, void *workspace,
                                         size_t workspace_bytes, void *stream);
 
+/* ---- image-quality metrics --------------------------------------------------- */
+
+/* SSIM / MS-SSIM of two image batches, replacing the third-party
+ * pytorch_msssim.ms_ssim / ssim the reference's evaluation and loss menu call
+ * (train.py:200, utils.py:26-34).  x, y: [n, c, h, w] FP32 addressed through
+ * element strides {n, c, h, w} (so NCHW and NHWC views both work, and x and y
+ * may differ in layout).  win: win_size taps (odd, 3..11) applied separably
+ * with no padding; C1 = (k1 data_range)^2, C2 = (k2 data_range)^2.  levels
+ * scales (1..8) with 2 x 2 zero-padded average pools between them; every
+ * scale must be at least win_size on both sides.  Outputs (any may be NULL,
+ * not all): out_value[n*c] = prod_l relu(cs_l)^weights[l] with the last scale's
+ * relu(ssim) in place of its cs; out_ssim / out_cs [levels][n*c] the per-scale
+ * plane means.  win and weights are host arrays.  Deterministic.  Workspace:
+ * lig_ssim_workspace_size(...), 0 when the sizes are invalid. */
+size_t lig_ssim_workspace_size(int n, int c, int h, int w, int win_size, int levels);
+int lig_ssim_multiscale(int n, int c, int h, int w, const float *x, const long long *x_strides,
+                        const float *y, const long long *y_strides, float data_range,
+                        int win_size, const float *win, float k1, float k2, int levels,
+                        const float *weights, float *out_value, float *out_ssim, float *out_cs,
+                        void *workspace, size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

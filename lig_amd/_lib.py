@@ -98,6 +98,10 @@ _SIGNATURES = {
     "lig_gradient_aware_upscale_backward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,
                  _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "lig_ssim_workspace_size": (_c_size, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
+    "lig_ssim_multiscale": (
+        _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_float, _c_int, _vp,
+                 _c_float, _c_float, _c_int, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

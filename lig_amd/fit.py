@@ -129,6 +129,15 @@ def psnr(img: torch.Tensor, target: torch.Tensor) -> float:
     return 10.0 * math.log10(1.0 / max(mse, 1e-20))
 
 
+def evaluate(out_hwc: torch.Tensor, gt_hwc: torch.Tensor):
+    """(PSNR, MS-SSIM) of a composed H x W x 3 image against the ground truth,
+    as the reference's test() scores a fit (train.py:197-200, data range 1)."""
+    from .metrics import ms_ssim
+    x = out_hwc.float().permute(2, 0, 1)[None]
+    y = gt_hwc.float().permute(2, 0, 1)[None]
+    return psnr(out_hwc, gt_hwc), float(ms_ssim(x, y, data_range=1, size_average=True))
+
+
 def downsample(img_hwc: torch.Tensor, factor: int, mode: str = "bicubic") -> torch.Tensor:
     x = img_hwc.permute(2, 0, 1)[None]
     H, W = img_hwc.shape[0] // factor, img_hwc.shape[1] // factor
