@@ -184,7 +184,20 @@ def batched_run(args, dev, torch):
         p_batch = bf.psnr()
     res["max_abs_dpsnr_db"] = float(np.abs(p_plain - p_batch).max())
     res["mean_psnr_db"] = round(float(p_batch.mean()), 3)
+    # the same step recorded once as a hipGraph (no host read of the
+    # intersection count; bit-identical to the eager step)
+    gs = batch.BatchedStepGraph(bf)
+    gs.run(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gs.run(args.batch_steps)
+    torch.cuda.synchronize()
+    res["graph_steps_s"] = round(args.batch_steps / (time.perf_counter() - t0), 3)
+    res["graph_speedup"] = round(res["graph_steps_s"] / res["plain_steps_s"], 3)
+    res["graph_capacity"] = gs.capacity
+    res["graph_captures"] = gs.captures
     log(f"batched {res}")
+    del gs
     del plain, bf, targets
     torch.cuda.empty_cache()
     return res

@@ -307,6 +307,43 @@ int lig_fit_backward_records_l2_batched(int n, const int32_t *gaussian_ids_sorte
                                         int num_images, int num_tiles, int num_intersects,
                                         void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- captured fit step (no host read of the intersection count) ------------ */
+
+/* No reference counterpart (the reference reads the count on the host every
+ * step, utils.py:65).  For a fit step recorded once as a hipGraph and
+ * replayed: buffers are sized for `capacity` intersections and the count is
+ * read on the device.
+ * lig_bin_batched_capacity: lig_bin_batched with the count taken from
+ * cum_tiles_hit[n-1] on the device; workspace
+ * lig_bin_workspace_size(capacity, num_tiles), gaussian_ids_sorted holds
+ * capacity entries.  When the count exceeds the capacity nothing is emitted
+ * and every bin is empty.  lig_fit_backward_records_l2_batched and
+ * lig_fit_reduce_records take the capacity as num_intersects.
+ * `control` is 4 device int32 {step, skipped, max_count, ok}.
+ * lig_fit_step_begin: ok = (*count <= capacity && step < table_len); then
+ * step += 1 when ok, else skipped += 1; max_count = max(max_count, *count).
+ * Launch it before the optimiser steps of the same fit step.
+ * lig_adam_step_device: lig_adam_step with the step number read from
+ * control (after lig_fit_step_begin) and the bias corrections from
+ * bias_table[2*(step-1) .. +1] (device floats written by
+ * lig_adam_bias_corrections, so they equal lig_adam_step's bit for bit);
+ * does nothing when ok == 0.  A step that was not admitted changes no
+ * parameter or optimiser state.
+ * lig_adam_bias_corrections: host function; out[2k], out[2k+1] = the
+ * (1 - beta1^t, sqrt(1 - beta2^t)) lig_adam_step uses for t = first_step + k. */
+int lig_bin_batched_capacity(int n, int capacity, const float *xys, const float *extents,
+                             const int32_t *cum_tiles_hit, const void *batch_table,
+                             int num_images, int num_tiles, int32_t *gaussian_ids_sorted,
+                             int32_t *tile_bins, void *workspace, size_t workspace_bytes,
+                             void *stream);
+int lig_fit_step_begin(const int32_t *count, int capacity, int table_len, void *control,
+                       void *stream);
+int lig_adam_step_device(long long n, float *param, const float *grad, float *exp_avg,
+                         float *exp_avg_sq, float lr, float beta1, float beta2, float eps,
+                         float weight_decay, const void *control, const float *bias_table,
+                         void *stream);
+int lig_adam_bias_corrections(float beta1, float beta2, int first_step, int count, float *out);
+
 /* No reference counterpart: workload counters for roofline accounting.
  * counts[0] = E, pixel/Gaussian pairs whose pixel centre lies inside the
  * Gaussian's clipped SnugBox box |p - mean| <= extent (Gaussians with

@@ -98,6 +98,14 @@ _SIGNATURES = {
     "lig_gradient_aware_upscale_backward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,
                  _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "lig_bin_batched_capacity": (
+        _c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _c_size,
+                 _vp]),
+    "lig_fit_step_begin": (_c_int, [_vp, _c_int, _c_int, _vp, _vp]),
+    "lig_adam_step_device": (
+        _c_int, [ctypes.c_longlong, _vp, _vp, _vp, _vp, _c_float, _c_float, _c_float, _c_float,
+                 _c_float, _vp, _vp, _vp]),
+    "lig_adam_bias_corrections": (_c_int, [_c_float, _c_float, _c_int, _c_int, _vp]),
     "lig_ssim_workspace_size": (_c_size, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
     "lig_ssim_multiscale": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _c_float, _c_int, _vp,

@@ -11,6 +11,7 @@ mean clamp + L2 losses, so each image's gradient is that of its own loss.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 from typing import List, Sequence
 
@@ -171,3 +172,191 @@ def batch_workload(count: int = 64, seed: int = 1000, gauss_per_px: float = 0.05
 
 def make_targets(sizes, seeds, device):
     return [make_image(h, w, s, device=device) for (h, w), s in zip(sizes, seeds)]
+
+
+class BatchedStepGraph:
+    """BatchedFit's fit step recorded once as a hipGraph and replayed.
+
+    Every buffer of the step is allocated up front: intersection buffers for
+    `capacity` intersections (the count measured at set-up times `headroom`,
+    unless given), and no stage reads the intersection count on the host.  The
+    graph holds projection, scan, lig_fit_step_begin, binning with the count
+    read on the device, the fused render + loss, the backward records and their
+    reduction, the projection backward and one device-step Adam update per
+    parameter tensor.  A step whose count exceeds the capacity changes nothing
+    (empty bins, no update) and is counted as skipped; `run` then grows the
+    capacity to the largest count seen times `headroom`, records the graph
+    again and takes the remaining steps, so the parameters after `run(k)` are
+    bit-identical to k calls of BatchedFit.step.
+
+    The optimiser's lr, betas, eps and weight decay are recorded in the graph:
+    build a new BatchedStepGraph after changing them.  Eager BatchedFit.step
+    calls may be interleaved with `run` (the step counter is kept in sync).
+    """
+
+    def __init__(self, bf: BatchedFit, capacity: int = None, headroom: float = 1.25,
+                 table_steps: int = 4096):
+        self.bf = bf
+        self.headroom = float(headroom)
+        self.table_steps = int(table_steps)
+        self.dev = bf.means.device
+        opt = bf.optimizer
+        if len(opt.param_groups) != 1:
+            raise ValueError("one parameter group expected")
+        grp = opt.param_groups[0]
+        self.lr, (self.b1, self.b2) = grp["lr"], grp["betas"]
+        self.eps, self.wd = grp["eps"], grp["weight_decay"]
+        self.params = [bf.rgbs, bf.means, bf.cholesky]
+        for p in self.params:
+            st = opt.state[p]
+            if not st:   # the state FusedAdam creates at its first step
+                st["step"] = 0
+                st["exp_avg"] = torch.zeros_like(p, memory_format=torch.contiguous_format)
+                st["exp_avg_sq"] = torch.zeros_like(p, memory_format=torch.contiguous_format)
+        if len({int(opt.state[p]["step"]) for p in self.params}) != 1:
+            raise ValueError("the parameters' optimiser steps differ")
+        n, dev = bf.means.shape[0], self.dev
+        self.n = n
+        f32 = dict(device=dev, dtype=torch.float32)
+        self.xys = torch.empty(n, 2, **f32)
+        self.ext = torch.empty(n, 2, **f32)
+        self.con = torch.empty(n, 3, **f32)
+        self.nt = torch.empty(n, dtype=torch.int32, device=dev)
+        self.cum = torch.empty(n, dtype=torch.int32, device=dev)
+        self.scan_ws = torch.empty(max(L.size_query("lig_scan_workspace_size", n), 1),
+                                   dtype=torch.uint8, device=dev)
+        self.bins = torch.empty(bf.num_tiles, 2, dtype=torch.int32, device=dev)
+        self.img = torch.empty_like(bf.target)
+        self.packed = torch.empty(L.size_query("lig_packed_gaussians_bytes", n),
+                                  dtype=torch.uint8, device=dev)
+        self.rws = torch.empty(L.size_query("lig_fit_batched_workspace_size", bf.num_tiles),
+                               dtype=torch.uint8, device=dev)
+        self.loss_img = torch.empty(len(bf.sizes), **f32)
+        self.loss = torch.empty((), **f32)
+        self.grad_loss = torch.ones((), **f32)
+        self.v_xy = torch.empty(n, 2, **f32)
+        self.v_conic = torch.empty(n, 3, **f32)
+        self.v_rgb = torch.empty(n, 3, **f32)
+        self.v_chol = torch.empty(n, 3, **f32)
+        self.v_mean = torch.empty(n, 2, **f32)
+        self.grads = [self.v_rgb, self.v_mean, self.v_chol]
+        self.ctl = torch.zeros(4, dtype=torch.int32, device=dev)
+        self.captures = 0
+        step = self._host_step()
+        self._make_table(step + self.table_steps)
+        if capacity is None:
+            with torch.cuda.device(dev):
+                self._project_and_scan(L.stream_of(bf.means))
+            capacity = max(1, math.ceil(int(self.cum[-1].item()) * self.headroom))
+        self._alloc(int(capacity))
+        self._capture()
+
+    # -- set-up --------------------------------------------------------------
+    def _host_step(self) -> int:
+        return int(self.bf.optimizer.state[self.params[0]]["step"])
+
+    def _make_table(self, steps: int):
+        host = np.empty(2 * steps, np.float32)
+        L.call("lig_adam_bias_corrections", float(self.b1), float(self.b2), 1, steps,
+               host.ctypes.data_as(ctypes.c_void_p))
+        self.table = torch.from_numpy(host).to(self.dev)
+        self.table_len = steps
+
+    def _alloc(self, capacity: int):
+        dev, ntiles = self.dev, self.bf.num_tiles
+        self.capacity = capacity
+        self.gids = torch.empty(capacity, dtype=torch.int32, device=dev)
+        self.bin_ws = torch.empty(L.size_query("lig_bin_workspace_size", capacity, ntiles),
+                                  dtype=torch.uint8, device=dev)
+        self.rec_ws = torch.empty(L.size_query("lig_rasterize_backward_workspace_size", capacity),
+                                  dtype=torch.uint8, device=dev)
+
+    def _project_and_scan(self, s):
+        bf, n = self.bf, self.n
+        L.call("lig_project_batched_cholesky", n, L.ptr(bf.cholesky), L.ptr(bf.means),
+               L.ptr(bf.table), len(bf.sizes), L.ptr(self.xys), L.ptr(self.ext),
+               L.ptr(self.con), L.ptr(self.nt), s)
+        L.call("lig_inclusive_scan_i32", n, L.ptr(self.nt), L.ptr(self.cum),
+               L.ptr(self.scan_ws), self.scan_ws.numel(), s)
+
+    def _launch(self):
+        bf, n, cap = self.bf, self.n, self.capacity
+        nimg, ntiles = len(bf.sizes), bf.num_tiles
+        s = L.stream_of(bf.means)
+        self._project_and_scan(s)
+        count = ctypes.c_void_p(self.cum.data_ptr() + 4 * (n - 1))
+        L.call("lig_fit_step_begin", count, cap, self.table_len, L.ptr(self.ctl), s)
+        L.call("lig_bin_batched_capacity", n, cap, L.ptr(self.xys), L.ptr(self.ext),
+               L.ptr(self.cum), L.ptr(bf.table), nimg, ntiles, L.ptr(self.gids),
+               L.ptr(self.bins), L.ptr(self.bin_ws), self.bin_ws.numel(), s)
+        L.call("lig_fit_render_l2_batched", n, L.ptr(self.gids), L.ptr(self.bins),
+               L.ptr(self.xys), L.ptr(self.ext), L.ptr(self.cum), L.ptr(self.con),
+               L.ptr(bf.rgbs), L.ptr(self.packed), L.ptr(bf.target), L.ptr(bf.table), nimg,
+               ntiles, L.ptr(self.img), L.ptr(self.loss_img), L.ptr(self.loss),
+               int(bf.batch_depth), L.ptr(self.rws), self.rws.numel(), s)
+        L.call("lig_fit_backward_records_l2_batched", n, L.ptr(self.gids), L.ptr(self.bins),
+               L.ptr(self.packed), L.ptr(self.img), L.ptr(bf.target), L.ptr(self.grad_loss),
+               L.ptr(bf.table), nimg, ntiles, cap, L.ptr(self.rec_ws), self.rec_ws.numel(), s)
+        L.call("lig_fit_reduce_records", n, cap, L.ptr(self.cum), L.ptr(self.rec_ws),
+               self.rec_ws.numel(), L.ptr(self.v_xy), L.ptr(self.v_conic), L.ptr(self.v_rgb), s)
+        L.call("lig_project_gaussians_backward_cholesky", n, L.ptr(self.ext),
+               L.ptr(bf.cholesky), L.ptr(self.con), L.ptr(self.v_xy), L.ptr(self.v_conic),
+               L.ptr(self.v_chol), L.ptr(self.v_mean), s)
+        opt = bf.optimizer
+        for p, g in zip(self.params, self.grads):
+            st = opt.state[p]
+            L.call("lig_adam_step_device", p.numel(), L.ptr(p), L.ptr(g), L.ptr(st["exp_avg"]),
+                   L.ptr(st["exp_avg_sq"]), self.lr, self.b1, self.b2, self.eps, self.wd,
+                   L.ptr(self.ctl), L.ptr(self.table), s)
+
+    def _capture(self):
+        for name, t in (("means", self.bf.means), ("cholesky", self.bf.cholesky),
+                        ("rgbs", self.bf.rgbs)):
+            L.check_device(name, t, torch.float32)
+        torch.cuda.synchronize(self.dev)
+        self.graph = torch.cuda.CUDAGraph()
+        stream = torch.cuda.Stream(device=self.dev)   # default priority
+        with torch.no_grad(), torch.cuda.device(self.dev):
+            with torch.cuda.graph(self.graph, stream=stream):
+                self._launch()
+        self.captures += 1
+
+    # -- stepping ------------------------------------------------------------
+    def _read_ctl(self):
+        return [int(v) for v in self.ctl.cpu()]
+
+    def run(self, steps: int) -> torch.Tensor:
+        """Takes `steps` fit steps by replaying the graph (one host
+        synchronisation at the end, more only when a step was skipped);
+        returns the last step's loss."""
+        if steps < 1:
+            raise ValueError("steps must be >= 1")
+        ctl = self._read_ctl()
+        host_step = self._host_step()
+        if ctl[0] != host_step:          # eager steps were taken in between
+            self.ctl[0] = host_step
+        remaining = steps
+        while remaining > 0:
+            before = self._host_step()
+            for _ in range(remaining):
+                self.graph.replay()
+            ctl = self._read_ctl()
+            taken = ctl[0] - before
+            for p in self.params:
+                self.bf.optimizer.state[p]["step"] = ctl[0]
+            remaining -= taken
+            if remaining == 0:
+                break
+            if ctl[2] <= self.capacity and ctl[0] < self.table_len:
+                raise RuntimeError(f"captured fit step skipped for no known reason: {ctl}")
+            if ctl[2] > self.capacity:
+                self._alloc(max(ctl[2], math.ceil(ctl[2] * self.headroom)))
+            if ctl[0] >= self.table_len:
+                self._make_table(ctl[0] + self.table_steps)
+            self._capture()
+        self.bf.last_losses = self.loss_img.clone()
+        return self.loss.clone()
+
+    @property
+    def skipped(self) -> int:
+        return self._read_ctl()[1]

@@ -22,6 +22,15 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
+// Items a kernel processes: `cap` when `dev` is NULL, else the count stored
+// at *dev, or -1 when that count exceeds `cap` (the buffers were sized for
+// `cap` items: the kernel then does nothing).
+__device__ __forceinline__ int live_count(int cap, const int32_t *dev) {
+    if (!dev) return cap;
+    const int v = *dev;
+    return v <= cap ? v : -1;
+}
+
 // Exclusive scan of one int per thread across a 256-thread block.  `wsum` is
 // 4 ints of LDS.  Returns the exclusive prefix; *total gets the block sum.
 template <int THREADS>
@@ -172,10 +181,12 @@ __device__ __forceinline__ unsigned digit_of(K key, int shift, K flip) {
 
 template <typename K, int RB>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const K *__restrict__ keys,
-                                                                  int n, int shift, K flip,
-                                                                  int nblocks,
+                                                                  int cap, const int32_t *n_dev,
+                                                                  int shift, K flip, int nblocks,
                                                                   int32_t *__restrict__ hist) {
     constexpr int NB = 1 << RB;
+    const int n = live_count(cap, n_dev);
+    if (n < 0) return;
     __shared__ int h[NB];
     for (int d = threadIdx.x; d < NB; d += kSortThreads) h[d] = 0;
     __syncthreads();
@@ -203,8 +214,11 @@ template <typename K, int RB>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const K *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
     K *__restrict__ keys_out, uint32_t *__restrict__ vals_out,
-    const int32_t *__restrict__ offsets, int n, int shift, K flip, int nblocks) {
+    const int32_t *__restrict__ offsets, int cap, const int32_t *n_dev, int shift, K flip,
+    int nblocks) {
     constexpr int NB = 1 << RB;
+    const int n = live_count(cap, n_dev);
+    if (n < 0) return;
     constexpr int PER = NB / kSortThreads > 0 ? NB / kSortThreads : 1;  // digits per thread
     constexpr int WSPAN = kSortRounds * LIG_WAVE;                        // pairs per wave
     __shared__ K skey[kSortTile];
@@ -217,7 +231,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const int wave = tid / LIG_WAVE;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (LIG_WAVE - lane));
     const size_t base = (size_t)blockIdx.x * kSortTile;
-    const int count = (int)min((size_t)kSortTile, (size_t)n - base);
+    // blocks past a device-side count get no pairs
+    const int count = (size_t)n > base ? (int)min((size_t)kSortTile, (size_t)n - base) : 0;
     const int wbase = wave * WSPAN;
 
     for (int d = tid; d < NB; d += kSortThreads) {
@@ -315,7 +330,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
 
 template <typename K>
 __global__ void copy_pairs_kernel(const K *__restrict__ ki, const uint32_t *__restrict__ vi,
-                                  K *__restrict__ ko, uint32_t *__restrict__ vo, int n) {
+                                  K *__restrict__ ko, uint32_t *__restrict__ vo, int cap,
+                                  const int32_t *n_dev) {
+    const int n = live_count(cap, n_dev);
+    if (n < 0) return;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < (size_t)n) {
         ko[i] = ki[i];
@@ -349,28 +367,32 @@ size_t sort_ws_bytes(size_t m, int key_bits) {
 }
 
 template <typename K, int RB>
-void sort_pass(const K *ksrc, const uint32_t *vsrc, K *kdst, uint32_t *vdst, size_t m, int shift,
-               K flip, int nblocks, int32_t *hist, int32_t *scan_ws, hipStream_t st) {
+void sort_pass(const K *ksrc, const uint32_t *vsrc, K *kdst, uint32_t *vdst, size_t m,
+               const int32_t *m_dev, int shift, K flip, int nblocks, int32_t *hist,
+               int32_t *scan_ws, hipStream_t st) {
     hipLaunchKernelGGL((radix_hist_kernel<K, RB>), dim3(nblocks), dim3(kSortThreads), 0, st, ksrc,
-                       (int)m, shift, flip, nblocks, hist);
+                       (int)m, m_dev, shift, flip, nblocks, hist);
     scan_i32(hist, hist, ((size_t)1 << RB) * nblocks, false, scan_ws, st);
     hipLaunchKernelGGL((radix_scatter_kernel<K, RB>), dim3(nblocks), dim3(kSortThreads), 0, st,
-                       ksrc, vsrc, kdst, vdst, (const int32_t *)hist, (int)m, shift, flip,
+                       ksrc, vsrc, kdst, vdst, (const int32_t *)hist, (int)m, m_dev, shift, flip,
                        nblocks);
 }
 
 // Sorts m pairs by the low `key_bits` bits of the key (XOR `flip` first: the
 // sign bit for signed 64-bit keys).  Input is read only; vals_in may be NULL
-// (identity values); the result lands in keys_out / vals_out.
+// (identity values); the result lands in keys_out / vals_out.  With m_dev,
+// m is the capacity and the pairs sorted are the first *m_dev (none when
+// *m_dev > m).
 template <typename K>
 int radix_sort(const K *keys_in, const uint32_t *vals_in, K *keys_out, uint32_t *vals_out,
-               size_t m, int key_bits, K flip, char *ws, hipStream_t st) {
+               size_t m, int key_bits, K flip, char *ws, hipStream_t st,
+               const int32_t *m_dev = nullptr) {
     if (m == 0) return LIG_OK;
     int passes, rb;
     digit_plan(key_bits, &passes, &rb);
     if (passes == 0) {
         hipLaunchKernelGGL(copy_pairs_kernel<K>, dim3(ceil_div(m, 256)), dim3(256), 0, st,
-                           keys_in, vals_in, keys_out, vals_out, (int)m);
+                           keys_in, vals_in, keys_out, vals_out, (int)m, m_dev);
         return launch_status();
     }
     const int nblocks = (int)((m + kSortTile - 1) / kSortTile);
@@ -390,13 +412,13 @@ int radix_sort(const K *keys_in, const uint32_t *vals_in, K *keys_out, uint32_t 
         uint32_t *vdst = to_out ? vals_out : vtmp;
         const int shift = rb * pass;
         switch (rb) {
-            case 4: sort_pass<K, 4>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
-            case 5: sort_pass<K, 5>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
-            case 6: sort_pass<K, 6>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
-            case 7: sort_pass<K, 7>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
-            case 8: sort_pass<K, 8>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
-            case 9: sort_pass<K, 9>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
-            default: sort_pass<K, 10>(ksrc, vsrc, kdst, vdst, m, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 4: sort_pass<K, 4>(ksrc, vsrc, kdst, vdst, m, m_dev, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 5: sort_pass<K, 5>(ksrc, vsrc, kdst, vdst, m, m_dev, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 6: sort_pass<K, 6>(ksrc, vsrc, kdst, vdst, m, m_dev, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 7: sort_pass<K, 7>(ksrc, vsrc, kdst, vdst, m, m_dev, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 8: sort_pass<K, 8>(ksrc, vsrc, kdst, vdst, m, m_dev, shift, flip, nblocks, hist, scan_ws, st); break;
+            case 9: sort_pass<K, 9>(ksrc, vsrc, kdst, vdst, m, m_dev, shift, flip, nblocks, hist, scan_ws, st); break;
+            default: sort_pass<K, 10>(ksrc, vsrc, kdst, vdst, m, m_dev, shift, flip, nblocks, hist, scan_ws, st); break;
         }
         ksrc = kdst;
         vsrc = vdst;
@@ -452,10 +474,12 @@ __global__ __launch_bounds__(kEmitThreads) void emit_isect_i64_kernel(
 // its own image's tile grid and its keys are offset by the image's first
 // global tile.
 __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
-    int n, int m, const float *__restrict__ xys, const float *__restrict__ extents,
-    const int32_t *__restrict__ cum, int tiles_x, int tiles_y, int block_width,
-    const BatchImage *__restrict__ tab, int nimg, uint32_t *__restrict__ keys,
+    int n, int cap, const int32_t *m_dev, const float *__restrict__ xys,
+    const float *__restrict__ extents, const int32_t *__restrict__ cum, int tiles_x, int tiles_y,
+    int block_width, const BatchImage *__restrict__ tab, int nimg, uint32_t *__restrict__ keys,
     int32_t *__restrict__ slot_gid) {
+    const int m = live_count(cap, m_dev);
+    if (m < 0) return;
     const int lane = threadIdx.x & (LIG_WAVE - 1);
     const int g0 = blockIdx.x * kEmitThreads + (threadIdx.x - lane);
     const int g = g0 + lane;
@@ -545,8 +569,10 @@ __device__ __forceinline__ void bin_edge_at(int i, int m, int prev, int cur, int
 }
 
 template <typename K, int SHIFT>
-__global__ void bin_edges_kernel(int m, const K *__restrict__ sorted, int num_tiles,
-                                 int32_t *__restrict__ bins) {
+__global__ void bin_edges_kernel(int cap, const int32_t *m_dev, const K *__restrict__ sorted,
+                                 int num_tiles, int32_t *__restrict__ bins) {
+    const int m = live_count(cap, m_dev);
+    if (m < 0) return;
     const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (i0 >= (size_t)m) return;
     int t[5];
@@ -637,8 +663,8 @@ extern "C" int lig_get_tile_bin_edges(int num_intersects, const int64_t *isect_i
     if (num_intersects > 0) {
         if (!isect_ids_sorted) return LIG_ERR_INVALID_ARGUMENT;
         hipLaunchKernelGGL((bin_edges_kernel<int64_t, 32>), dim3(ceil_div(num_intersects, 1024)),
-                           dim3(256), 0, st, num_intersects, isect_ids_sorted, (int)nt,
-                           tile_bins);
+                           dim3(256), 0, st, num_intersects, (const int32_t *)nullptr,
+                           isect_ids_sorted, (int)nt, tile_bins);
     }
     return launch_status();
 }
@@ -682,8 +708,9 @@ extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float
     uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(take());
     uint32_t *slots_sorted = reinterpret_cast<uint32_t *>(take());
     hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(ceil_div(n, kEmitThreads)), dim3(kEmitThreads),
-                       0, st, n, num_intersects, xys, extents, cum_tiles_hit, tx, ty,
-                       block_width, (const BatchImage *)nullptr, 0, keys, slot_gid);
+                       0, st, n, num_intersects, (const int32_t *)nullptr, xys, extents,
+                       cum_tiles_hit, tx, ty, block_width, (const BatchImage *)nullptr, 0, keys,
+                       slot_gid);
     if (!emission_ids_sorted) {
         // sort (tile, gid) pairs directly: the stable sort of the emission
         // order gives the same gids, and no gather is needed afterwards
@@ -693,7 +720,8 @@ extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float
                                       bits_for(nt), 0u, p, st);
         if (rc != LIG_OK) return rc;
         hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(m, 1024)), dim3(256), 0,
-                           st, num_intersects, (const uint32_t *)keys_sorted, nt, tile_bins);
+                           st, num_intersects, (const int32_t *)nullptr,
+                           (const uint32_t *)keys_sorted, nt, tile_bins);
         return launch_status();
     }
     slots_sorted = reinterpret_cast<uint32_t *>(emission_ids_sorted);
@@ -707,27 +735,30 @@ extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float
     return launch_status();
 }
 
-extern "C" int lig_bin_batched(int n, int num_intersects, const float *xys, const float *extents,
-                               const int32_t *cum_tiles_hit, const void *batch_table,
-                               int num_images, int num_tiles, int32_t *gaussian_ids_sorted,
-                               int32_t *tile_bins, void *workspace, size_t workspace_bytes,
-                               void *stream) {
-    if (n < 0 || num_intersects < 0 || num_images < 1 || num_tiles < 1 || !batch_table ||
-        !tile_bins)
+namespace lig {
+namespace {
+
+// Batched binning into the global tile space.  m_dev NULL: m is the exact
+// intersection count.  m_dev set: m is the capacity the buffers hold and the
+// count is read on the device from *m_dev; a count above the capacity leaves
+// every bin empty.
+int bin_batched(int n, int m, const int32_t *m_dev, const float *xys, const float *extents,
+                const int32_t *cum_tiles_hit, const void *batch_table, int num_images,
+                int num_tiles, int32_t *gaussian_ids_sorted, int32_t *tile_bins, void *workspace,
+                size_t workspace_bytes, hipStream_t st) {
+    if (n < 0 || m < 0 || num_images < 1 || num_tiles < 1 || !batch_table || !tile_bins)
         return LIG_ERR_INVALID_ARGUMENT;
-    if (workspace_bytes < lig_bin_workspace_size(num_intersects, num_tiles))
-        return LIG_ERR_WORKSPACE;
-    hipStream_t st = as_stream(stream);
+    if (workspace_bytes < lig_bin_workspace_size(m, num_tiles)) return LIG_ERR_WORKSPACE;
     hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * (size_t)num_tiles, 256)), dim3(256), 0,
                        st, tile_bins, 2 * (size_t)num_tiles, 0);
-    if (num_intersects == 0 || n == 0) return launch_status();
+    if (m == 0 || n == 0) return launch_status();
     if (!xys || !extents || !cum_tiles_hit || !gaussian_ids_sorted || !workspace)
         return LIG_ERR_INVALID_ARGUMENT;
-    const size_t m = (size_t)num_intersects;
+    const size_t mm = (size_t)m;
     char *p = reinterpret_cast<char *>(workspace);
-    auto take = [&p, m]() {
+    auto take = [&p, mm]() {
         char *q = p;
-        p += align_up(m * 4);
+        p += align_up(mm * 4);
         return q;
     };
     uint32_t *keys = reinterpret_cast<uint32_t *>(take());
@@ -735,14 +766,38 @@ extern "C" int lig_bin_batched(int n, int num_intersects, const float *xys, cons
     uint32_t *keys_sorted = reinterpret_cast<uint32_t *>(take());
     take();
     hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(ceil_div(n, kEmitThreads)), dim3(kEmitThreads),
-                       0, st, n, num_intersects, xys, extents, cum_tiles_hit, 0, 0, 16,
+                       0, st, n, m, m_dev, xys, extents, cum_tiles_hit, 0, 0, 16,
                        reinterpret_cast<const BatchImage *>(batch_table), num_images, keys,
                        slot_gid);
     int rc = radix_sort<uint32_t>(keys, reinterpret_cast<const uint32_t *>(slot_gid), keys_sorted,
-                                  reinterpret_cast<uint32_t *>(gaussian_ids_sorted), m,
-                                  bits_for(num_tiles), 0u, p, st);
+                                  reinterpret_cast<uint32_t *>(gaussian_ids_sorted), mm,
+                                  bits_for(num_tiles), 0u, p, st, m_dev);
     if (rc != LIG_OK) return rc;
-    hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(m, 1024)), dim3(256), 0, st,
-                       num_intersects, (const uint32_t *)keys_sorted, num_tiles, tile_bins);
+    hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(mm, 1024)), dim3(256), 0,
+                       st, m, m_dev, (const uint32_t *)keys_sorted, num_tiles, tile_bins);
     return launch_status();
+}
+
+}  // namespace
+}  // namespace lig
+
+extern "C" int lig_bin_batched(int n, int num_intersects, const float *xys, const float *extents,
+                               const int32_t *cum_tiles_hit, const void *batch_table,
+                               int num_images, int num_tiles, int32_t *gaussian_ids_sorted,
+                               int32_t *tile_bins, void *workspace, size_t workspace_bytes,
+                               void *stream) {
+    return bin_batched(n, num_intersects, nullptr, xys, extents, cum_tiles_hit, batch_table,
+                       num_images, num_tiles, gaussian_ids_sorted, tile_bins, workspace,
+                       workspace_bytes, as_stream(stream));
+}
+
+extern "C" int lig_bin_batched_capacity(int n, int capacity, const float *xys,
+                                        const float *extents, const int32_t *cum_tiles_hit,
+                                        const void *batch_table, int num_images, int num_tiles,
+                                        int32_t *gaussian_ids_sorted, int32_t *tile_bins,
+                                        void *workspace, size_t workspace_bytes, void *stream) {
+    if (n < 1 || capacity < 1 || !cum_tiles_hit) return LIG_ERR_INVALID_ARGUMENT;
+    return bin_batched(n, capacity, cum_tiles_hit + (n - 1), xys, extents, cum_tiles_hit,
+                       batch_table, num_images, num_tiles, gaussian_ids_sorted, tile_bins,
+                       workspace, workspace_bytes, as_stream(stream));
 }

@@ -26,11 +26,9 @@ __device__ __forceinline__ void adam_elem(float &p, float g, float &m, float &v,
     p -= (a.lr / a.bc1) * m / denom;
 }
 
-__global__ __launch_bounds__(kOptThreads) void adam_kernel(long long n, float *__restrict__ p,
-                                                           const float *__restrict__ g,
-                                                           float *__restrict__ m,
-                                                           float *__restrict__ v, AdamArgs a,
-                                                           int vec4) {
+__device__ __forceinline__ void adam_thread(long long n, float *__restrict__ p,
+                                            const float *__restrict__ g, float *__restrict__ m,
+                                            float *__restrict__ v, const AdamArgs &a, int vec4) {
     const long long t = (long long)blockIdx.x * kOptThreads + threadIdx.x;
     if (vec4) {
         const long long i = 4 * t;
@@ -52,6 +50,56 @@ __global__ __launch_bounds__(kOptThreads) void adam_kernel(long long n, float *_
     } else if (t < n) {
         adam_elem(p[t], g[t], m[t], v[t], a);
     }
+}
+
+__global__ __launch_bounds__(kOptThreads) void adam_kernel(long long n, float *__restrict__ p,
+                                                           const float *__restrict__ g,
+                                                           float *__restrict__ m,
+                                                           float *__restrict__ v, AdamArgs a,
+                                                           int vec4) {
+    adam_thread(n, p, g, m, v, a, vec4);
+}
+
+// Step control of a captured fit step: {step, skipped, max count, ok}.
+struct StepControl {
+    int32_t step, skipped, max_count, ok;
+};
+
+// Adam with the step number and its bias corrections taken from device
+// memory (so one captured launch serves every step); does nothing when the
+// step was not admitted.
+__global__ __launch_bounds__(kOptThreads) void adam_device_kernel(
+    long long n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+    float *__restrict__ v, AdamArgs a, int vec4, const StepControl *__restrict__ ctl,
+    const float2 *__restrict__ bias) {
+    if (!ctl->ok) return;
+    const float2 b = bias[ctl->step - 1];
+    a.bc1 = b.x;
+    a.bc2_sqrt = b.y;
+    adam_thread(n, p, g, m, v, a, vec4);
+}
+
+// Admits a step when its intersection count fits the capacity and the bias
+// table still has an entry for it.  One thread.
+__global__ void step_begin_kernel(const int32_t *__restrict__ count, int capacity, int table_len,
+                                  StepControl *__restrict__ ctl) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int c = *count;
+    StepControl s = *ctl;
+    s.ok = (c <= capacity && s.step < table_len) ? 1 : 0;
+    if (s.ok)
+        s.step += 1;
+    else
+        s.skipped += 1;
+    s.max_count = max(s.max_count, c);
+    *ctl = s;
+}
+
+// The bias corrections lig_adam_step uses for a step: 1 - beta1^t and
+// sqrt(1 - beta2^t), formed in FP64 from the FP32 betas and rounded once.
+void adam_bias(float beta1, float beta2, int step, float *bc1, float *bc2_sqrt) {
+    *bc1 = (float)(1.0 - std::pow((double)beta1, (double)step));
+    *bc2_sqrt = (float)std::sqrt(1.0 - std::pow((double)beta2, (double)step));
 }
 
 struct AdanArgs {
@@ -114,8 +162,7 @@ extern "C" int lig_adam_step(long long n, float *param, const float *grad, float
     a.b2 = beta2;
     a.eps = eps;
     a.wd = weight_decay;
-    a.bc1 = (float)(1.0 - std::pow((double)beta1, (double)step));
-    a.bc2_sqrt = (float)std::sqrt(1.0 - std::pow((double)beta2, (double)step));
+    adam_bias(beta1, beta2, step, &a.bc1, &a.bc2_sqrt);
     const int vec4 = aligned16(param) && aligned16(grad) && aligned16(exp_avg) &&
                      aligned16(exp_avg_sq);
     const long long threads = vec4 ? (n + 3) / 4 : n;
@@ -157,5 +204,49 @@ extern "C" int lig_adan_step(long long n, float *param, const float *grad, float
     hipLaunchKernelGGL(adan_kernel, dim3((unsigned)blocks), dim3(kOptThreads), 0,
                        as_stream(stream), n, param, grad, exp_avg, exp_avg_sq, exp_avg_diff,
                        neg_pre_grad, a);
+    return launch_status();
+}
+
+extern "C" int lig_adam_bias_corrections(float beta1, float beta2, int first_step, int count,
+                                         float *out) {
+    if (first_step < 1 || count < 0 || (count > 0 && !out)) return LIG_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < count; ++k) adam_bias(beta1, beta2, first_step + k, &out[2 * k],
+                                              &out[2 * k + 1]);
+    return LIG_OK;
+}
+
+extern "C" int lig_fit_step_begin(const int32_t *count, int capacity, int table_len,
+                                  void *control, void *stream) {
+    if (!count || !control || capacity < 0 || table_len < 0) return LIG_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(64), 0, as_stream(stream), count,
+                       capacity, table_len, reinterpret_cast<StepControl *>(control));
+    return launch_status();
+}
+
+extern "C" int lig_adam_step_device(long long n, float *param, const float *grad, float *exp_avg,
+                                    float *exp_avg_sq, float lr, float beta1, float beta2,
+                                    float eps, float weight_decay, const void *control,
+                                    const float *bias_table, void *stream) {
+    if (n < 0 || !control || !bias_table ||
+        (n > 0 && (!param || !grad || !exp_avg || !exp_avg_sq)))
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LIG_OK;
+    AdamArgs a;
+    a.lr = lr;
+    a.b1 = beta1;
+    a.b2 = beta2;
+    a.eps = eps;
+    a.wd = weight_decay;
+    a.bc1 = 1.0f;
+    a.bc2_sqrt = 1.0f;
+    const int vec4 = aligned16(param) && aligned16(grad) && aligned16(exp_avg) &&
+                     aligned16(exp_avg_sq);
+    const long long threads = vec4 ? (n + 3) / 4 : n;
+    const long long blocks = (threads + kOptThreads - 1) / kOptThreads;
+    if (blocks > INT32_MAX) return LIG_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(adam_device_kernel, dim3((unsigned)blocks), dim3(kOptThreads), 0,
+                       as_stream(stream), n, param, grad, exp_avg, exp_avg_sq, a, vec4,
+                       reinterpret_cast<const StepControl *>(control),
+                       reinterpret_cast<const float2 *>(bias_table));
     return launch_status();
 }

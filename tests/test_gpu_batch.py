@@ -31,3 +31,66 @@ def test_batched_matches_plain(device):
         o += c
     psnr = bf.psnr()
     assert psnr.shape == (len(SIZES),) and (psnr > 0).all()
+
+
+def _state(bf):
+    st = bf.optimizer.state
+    return [t.detach().clone() for p in (bf.rgbs, bf.means, bf.cholesky)
+            for t in (p, st[p]["exp_avg"], st[p]["exp_avg_sq"])]
+
+
+def _assert_same(a, b, where):
+    for k, (x, y) in enumerate(zip(_state(a), _state(b))):
+        assert torch.equal(x, y), (where, k)
+    for p, q in zip((a.rgbs, a.means, a.cholesky), (b.rgbs, b.means, b.cholesky)):
+        assert a.optimizer.state[p]["step"] == b.optimizer.state[q]["step"], where
+
+
+@pytest.mark.parametrize("capacity", [None, 64])
+def test_graph_step_matches_eager(device, capacity):
+    """The captured step replayed k times gives the bits of k eager steps,
+    also when the first capacity is far too small (the skipped steps change
+    nothing, the capacity grows and the graph is recorded again).  Replays are
+    checked one by one, with eager work and synchronisation between them."""
+    from lig_amd.batch import BatchedFit, BatchedStepGraph, make_targets
+    targets = make_targets(SIZES, SEEDS, device)
+    eager = BatchedFit(targets, COUNTS, SEEDS, scale=1.5, lr=1e-2)
+    graphed = BatchedFit(targets, COUNTS, SEEDS, scale=1.5, lr=1e-2)
+    gs = BatchedStepGraph(graphed, capacity=capacity)
+    print(f"capacity {capacity} -> {gs.capacity}", flush=True)
+    for step in range(3):
+        le = eager.step()
+        lg = gs.run(1)
+        torch.cuda.synchronize()
+        assert torch.equal(le.reshape(()), lg), step
+        assert torch.equal(eager.last_losses, graphed.last_losses), step
+        _assert_same(eager, graphed, step)
+    if capacity is not None:
+        assert gs.captures >= 2 and gs.skipped >= 1 and gs.capacity > capacity
+    # several replays per run, then an eager step on the graphed model, then more replays
+    for _ in range(3):
+        eager.step()
+    gs.run(3)
+    _assert_same(eager, graphed, "run(3)")
+    eager.step()
+    graphed.step()
+    _assert_same(eager, graphed, "interleaved eager")
+    eager.step()
+    eager.step()
+    gs.run(2)
+    _assert_same(eager, graphed, "after interleave")
+
+
+def test_graph_step_bias_table_extension(device):
+    """A bias table shorter than the run: the step past its end is skipped,
+    the table is extended and the result still equals the eager steps."""
+    from lig_amd.batch import BatchedFit, BatchedStepGraph, make_targets
+    targets = make_targets(SIZES[:3], SEEDS[:3], device)
+    eager = BatchedFit(targets, COUNTS[:3], SEEDS[:3], scale=1.5, lr=1e-2)
+    graphed = BatchedFit(targets, COUNTS[:3], SEEDS[:3], scale=1.5, lr=1e-2)
+    gs = BatchedStepGraph(graphed, table_steps=2)
+    for _ in range(5):
+        eager.step()
+    gs.run(5)
+    _assert_same(eager, graphed, "table")
+    assert gs.table_len >= 5 and gs.captures >= 2
