@@ -167,10 +167,13 @@ void scan_i32(const int32_t *in, int32_t *out, size_t n, bool inclusive, int32_t
 // (wave ballots for lane order, LDS counters for wave and round order).
 // A NULL value array stands for the identity (value = input index).
 // ---------------------------------------------------------------------------
-// 8 waves per block, 512 pairs per wave: measured faster than 4 x 1024 and
-// than 8192-pair blocks at both 79 M and 39 M intersections.
+// 8 waves per block, 1024 pairs per wave.  With digits of at most 7 bits
+// (digit_plan) a block's run of one digit averages 64 or more pairs, so the
+// scatter writes whole cache lines; at 79 M (18-bit keys) and 39 M (14-bit)
+// intersections this beat 4096-pair blocks with 9- and 10-bit digits by 19%
+// and 11%, and 10240-pair blocks (one block per CU) by 20% and 35%.
 constexpr int kSortThreads = 512;
-constexpr int kSortRounds = 8;
+constexpr int kSortRounds = 16;
 constexpr int kSortTile = kSortThreads * kSortRounds;
 constexpr int kSortWaves = kSortThreads / LIG_WAVE;
 
@@ -343,15 +346,17 @@ __global__ void copy_pairs_kernel(const K *__restrict__ ki, const uint32_t *__re
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Digit plan for `key_bits` bits: the fewest passes of at most 10 bits, bits
-// split evenly (18 -> 2 x 9, 14 -> 2 x 7, 10 -> 1 x 10); 64-bit keys use 8 x 8.
+// Digit plan for `key_bits` bits: up to 10 bits in one pass (small tile
+// grids, where launches dominate), else the fewest passes of at most 7 bits
+// with the bits split evenly (18 -> 3 x 6, 14 -> 2 x 7, 11 -> 2 x 6);
+// 64-bit keys use 8 x 8.
 void digit_plan(int key_bits, int *passes, int *rb) {
     if (key_bits > 32) {
         *passes = (key_bits + 7) / 8;
         *rb = 8;
         return;
     }
-    *passes = (key_bits + 9) / 10;
+    *passes = key_bits <= 10 ? (key_bits > 0 ? 1 : 0) : (key_bits + 6) / 7;
     *rb = *passes ? (key_bits + *passes - 1) / *passes : 0;
     if (*rb < 4 && *passes) *rb = 4;
 }
@@ -467,8 +472,11 @@ __global__ __launch_bounds__(kEmitThreads) void emit_isect_i64_kernel(
 // reference's zero fill leaves them).  The sort's values are the slots
 // themselves (implicit identity), so the sorted order maps back to each
 // Gaussian's contiguous slot range.  Each wave computes the tile rectangles
-// of its 64 Gaussians, then writes their slots one Gaussian at a time with
-// all 64 lanes, so the stores are coalesced.
+// of its 64 Gaussians, then covers their joint slot range 64 slots at a time:
+// a lane finds the Gaussian that owns its slot by a binary search over the
+// lanes' range starts (non-decreasing, as cum is), decodes the tile from the
+// slot's offset in that Gaussian's row-major rectangle, and every store is
+// coalesced.
 //
 // Batched (tab != NULL, 16 x 16 tiles): each Gaussian's rectangle is taken in
 // its own image's tile grid and its keys are offset by the image's first
@@ -482,11 +490,13 @@ __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
     if (m < 0) return;
     const int lane = threadIdx.x & (LIG_WAVE - 1);
     const int g0 = blockIdx.x * kEmitThreads + (threadIdx.x - lane);
+    if (g0 >= n) return;  // wave-uniform
     const int g = g0 + lane;
-    int start = 0, end = 0, x0 = 0, y0 = 0, w = 1, zero = 0, row_len = tiles_x, toff = 0;
+    // slots of the wave's Gaussians: [wstart, wend)
+    const int wend = min(cum[min(g0 + LIG_WAVE, n) - 1], m);
+    int start = wend, x0 = 0, y0 = 0, w = 0, row_len = tiles_x, toff = 0;
     if (g < n) {
-        start = g == 0 ? 0 : cum[g - 1];
-        end = min(cum[g], m);
+        start = min(g == 0 ? 0 : cum[g - 1], m);
         int txs = tiles_x, tys = tiles_y;
         if (tab) {
             const BatchImage &im = tab[batch_find<1>(tab, nimg, g)];
@@ -496,35 +506,39 @@ __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
             row_len = txs;
         }
         const float ex = extents[2 * g], ey = extents[2 * g + 1];
-        if (ex <= 0.0f || ey <= 0.0f) {
-            zero = 1;
-        } else {
+        if (ex > 0.0f && ey > 0.0f) {  // else w = 0: slots get key 0 and gid 0
             int x1, y1;
             tile_bbox(xys[2 * g], xys[2 * g + 1], ex, ey, txs, tys, block_width, x0, y0, x1, y1);
             w = max(x1 - x0, 1);
         }
     }
-    for (int j = 0; j < LIG_WAVE; ++j) {
-        if (g0 + j >= n) break;  // wave-uniform
-        const int s0 = __builtin_amdgcn_readlane(start, j);
-        const int cnt = __builtin_amdgcn_readlane(end, j) - s0;
-        if (cnt <= 0) continue;
-        const int rx0 = __builtin_amdgcn_readlane(x0, j);
-        const int ry0 = __builtin_amdgcn_readlane(y0, j);
-        const int rw = __builtin_amdgcn_readlane(w, j);
-        const int rz = __builtin_amdgcn_readlane(zero, j);
-        const uint32_t rlen = (uint32_t)__builtin_amdgcn_readlane(row_len, j);
-        const uint32_t roff = (uint32_t)__builtin_amdgcn_readlane(toff, j);
-        for (int k = lane; k < cnt; k += LIG_WAVE) {
+    const int wstart = __shfl(start, 0, LIG_WAVE);
+    for (int base = wstart; base < wend; base += LIG_WAVE) {
+        const int pos = base + lane;
+        // owner: the last lane whose range starts at or before pos; its range
+        // ends where the next lane's starts, past pos
+        int j = 0;
+#pragma unroll
+        for (int step = LIG_WAVE / 2; step > 0; step >>= 1) {
+            if (__shfl(start, j + step, LIG_WAVE) <= pos) j += step;
+        }
+        const int s0 = __shfl(start, j, LIG_WAVE);
+        const int rx0 = __shfl(x0, j, LIG_WAVE);
+        const int ry0 = __shfl(y0, j, LIG_WAVE);
+        const int rw = __shfl(w, j, LIG_WAVE);
+        const uint32_t rlen = (uint32_t)__shfl(row_len, j, LIG_WAVE);
+        const uint32_t roff = (uint32_t)__shfl(toff, j, LIG_WAVE);
+        if (pos < wend) {
             uint32_t key = 0;
             int gid = 0;
-            if (!rz) {
+            if (rw > 0) {
+                const int k = pos - s0;
                 const int row = k / rw;
                 key = roff + (uint32_t)(ry0 + row) * rlen + (uint32_t)(rx0 + k - row * rw);
                 gid = g0 + j;
             }
-            keys[s0 + k] = key;
-            slot_gid[s0 + k] = gid;
+            keys[pos] = key;
+            slot_gid[pos] = gid;
         }
     }
 }
