@@ -160,22 +160,27 @@ def batched_run(args, dev, torch):
     tuned = autotune.tune_batched(bf, use_cache=False)
     for lv, t in zip(plain, targets):
         autotune.tune_level(lv, t, reps=1, use_cache=False)
-    res = {"batch_depth": tuned["choice"], "batch_depth_ms": tuned["ms"],
+    res = {"batch_depth": tuned["choice"], "persistent": tuned["persistent"],
+           "tune_ms": tuned["ms"],
            "plain_batch_depths": sorted({lv.batch_depth for lv in plain}),
            "images": len(sizes), "pixels": int(sum(h * w for h, w in sizes)),
            "gaussians": int(sum(counts)), "steps": args.batch_steps,
            "workload": "64 seeded images, height log-uniform 256-2048, width log-uniform "
                        "384-1536, 0.05 Gaussians/px, chol diag U(0.5,3)*2 px, L2, Adam"}
-    for name, stepper in (("plain", lambda: [lv.step(t) for lv, t in zip(plain, targets)]),
-                          ("batched", bf.step)):
+
+    def timed(stepper, steps):
         for _ in range(2):
             stepper()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.batch_steps):
+        for _ in range(steps):
             stepper()
         torch.cuda.synchronize()
-        res[f"{name}_steps_s"] = round(args.batch_steps / (time.perf_counter() - t0), 3)
+        return round(steps / (time.perf_counter() - t0), 3)
+
+    res["plain_steps_s"] = timed(lambda: [lv.step(t) for lv, t in zip(plain, targets)],
+                                 args.batch_steps)
+    res["batched_steps_s"] = timed(bf.step, args.batch_steps)   # the tuned launch form
     res["speedup"] = round(res["batched_steps_s"] / res["plain_steps_s"], 3)
     with torch.no_grad():
         p_plain = np.array([fit.psnr(torch.clamp(lv.render(), 0, 1), t)
@@ -184,21 +189,39 @@ def batched_run(args, dev, torch):
         p_batch = bf.psnr()
     res["max_abs_dpsnr_db"] = float(np.abs(p_plain - p_batch).max())
     res["mean_psnr_db"] = round(float(p_batch.mean()), 3)
+    # both launch forms of the raster stages (bit-identical): one workgroup per
+    # tile, and the persistent grid fed by per-XCD tile queues
+    chosen = bf.persistent
+    for form, flag in (("grid", False), ("persistent", True)):
+        bf.persistent = flag
+        res[f"{form}_steps_s"] = timed(bf.step, args.batch_steps)
+    bf.persistent = chosen
     # the same step recorded once as a hipGraph (no host read of the
     # intersection count; bit-identical to the eager step)
     gs = batch.BatchedStepGraph(bf)
-    gs.run(2)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    gs.run(args.batch_steps)
-    torch.cuda.synchronize()
-    res["graph_steps_s"] = round(args.batch_steps / (time.perf_counter() - t0), 3)
+    res["graph_steps_s"] = timed(lambda: gs.run(1), args.batch_steps)
     res["graph_speedup"] = round(res["graph_steps_s"] / res["plain_steps_s"], 3)
     res["graph_capacity"] = gs.capacity
     res["graph_captures"] = gs.captures
+    del gs, bf
+    torch.cuda.empty_cache()
+    # skewed batch: the same images with 8 of them at 20x the Gaussian density,
+    # where whole XCD shares of the tile list cost far more than others
+    skew = list(counts)
+    for i in range(0, 16, 2):
+        skew[i] *= 20
+    bs = batch.BatchedFit(targets, skew, seeds, scale=scale)
+    tuned_s = autotune.tune_batched(bs, use_cache=False)
+    sk = {"gaussians": int(sum(skew)), "tuned_persistent": tuned_s["persistent"],
+          "tuned_depth": tuned_s["choice"]}
+    for form, flag in (("grid", False), ("persistent", True)):
+        bs.persistent = flag
+        sk[f"{form}_steps_s"] = timed(bs.step, 3)
+    sk["persistent_speedup"] = round(sk["persistent_steps_s"] / sk["grid_steps_s"], 3)
+    res["skewed"] = sk
+    del bs
     log(f"batched {res}")
-    del gs
-    del plain, bf, targets
+    del plain, targets
     torch.cuda.empty_cache()
     return res
 

@@ -307,6 +307,30 @@ int lig_fit_backward_records_l2_batched(int n, const int32_t *gaussian_ids_sorte
                                         int num_images, int num_tiles, int num_intersects,
                                         void *workspace, size_t workspace_bytes, void *stream);
 
+/* Persistent forms of the two batched raster stages: the same arguments,
+ * workspaces and results, bit for bit.  Instead of one workgroup per tile the
+ * grid is the waves that fit on the device at once, and each wave takes tiles
+ * from a work queue: one counter per XCD over that XCD's contiguous share of
+ * the batch's tile list (images in table order), then tiles left in the other
+ * XCDs' shares, so uneven images and uneven tiles balance across the chip.
+ * The counters live in the workspace (2 KB, reset by the call): after the tile
+ * partials in the render's, after the fit records in the backward's (when
+ * num_intersects >= 144 leaves room; else the backward runs one workgroup
+ * per tile). */
+int lig_fit_render_l2_batched_persistent(int n, const int32_t *gaussian_ids_sorted,
+                                         const int32_t *tile_bins, const float *xys,
+                                         const float *extents, const int32_t *cum_tiles_hit,
+                                         const float *conics, const float *colors, void *packed,
+                                         const float *target, const void *batch_table,
+                                         int num_images, int num_tiles, float *out_img,
+                                         float *loss_per_image, float *loss, int batch_depth,
+                                         void *workspace, size_t workspace_bytes, void *stream);
+int lig_fit_backward_records_l2_batched_persistent(
+    int n, const int32_t *gaussian_ids_sorted, const int32_t *tile_bins, const void *packed,
+    const float *img, const float *target, const float *grad_loss, const void *batch_table,
+    int num_images, int num_tiles, int num_intersects, void *workspace, size_t workspace_bytes,
+    void *stream);
+
 /* ---- captured fit step (no host read of the intersection count) ------------ */
 
 /* No reference counterpart (the reference reads the count on the host every

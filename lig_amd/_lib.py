@@ -98,6 +98,12 @@ _SIGNATURES = {
     "lig_gradient_aware_upscale_backward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_float, _c_float, _vp,
                  _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "lig_fit_render_l2_batched_persistent": (
+        _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp,
+                 _vp, _vp, _c_int, _vp, _c_size, _vp]),
+    "lig_fit_backward_records_l2_batched_persistent": (
+        _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp,
+                 _c_size, _vp]),
     "lig_bin_batched_capacity": (
         _c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _c_size,
                  _vp]),

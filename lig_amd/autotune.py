@@ -8,6 +8,11 @@ after one warm-up, and the fastest is kept.  Choices are cached per workload
 signature in memory and, when writable, in a JSON file
 ($LIG_AUTOTUNE_CACHE, default ~/.cache/lig_amd/autotune.json).
 
+For batched fitting the launch form is tuned too: one workgroup per tile
+(the hardware deals workgroups to the XCDs round-robin) or a persistent grid
+fed by per-XCD tile queues with stealing (lig_fit_render_l2_batched_persistent),
+which wins when some images' tiles cost much more than others'.
+
 Tile size is not a knob here: the fused fit kernels are written for 16 x 16
 tiles (the reference API caps block_width at 16, rasterize.py:26); other tile
 widths run through the gsplat2d API path.
@@ -35,7 +40,7 @@ def _cache_path() -> str:
 def _load():
     try:
         with open(_cache_path()) as f:
-            _cache.update({k: int(v) for k, v in json.load(f).items()})
+            _cache.update({k: v for k, v in json.load(f).items()})
     except (OSError, ValueError):
         pass
 
@@ -90,15 +95,36 @@ def tune_level(level, target: torch.Tensor, reps: int = 3, use_cache: bool = Tru
 
 
 def tune_batched(bf, reps: int = 3, use_cache: bool = True) -> dict:
-    """Pick lig_amd.batch.BatchedFit.batch_depth."""
-    key = f"batch:{len(bf.sizes)}:{bf.num_tiles}:n{bf.means.shape[0]}"
+    """Pick lig_amd.batch.BatchedFit.batch_depth and .persistent: each depth
+    is timed in both launch forms (HIP events around the render call) and the
+    fastest pair is kept.  Returns {"choice": depth, "persistent": bool, "ms":
+    {form: {depth: ms}}}."""
+    key = f"batch:{len(bf.sizes)}:{bf.num_tiles}:n{bf.means.shape[0]}:{sum(bf.counts)}"
+    if use_cache:
+        if not _cache:
+            _load()
+        hit = _cache.get(key)
+        if isinstance(hit, list) and len(hit) == 2:
+            bf.batch_depth, bf.persistent = int(hit[0]), bool(hit[1])
+            return {"choice": bf.batch_depth, "persistent": bf.persistent, "ms": None,
+                    "cached": True}
+    ms = {}
+    for persistent in (False, True):
+        bf.persistent = persistent
 
-    def run(d):
-        with torch.no_grad():
-            bf.batch_depth = d
-            bf.loss()
+        def run(d):
+            with torch.no_grad():
+                bf.batch_depth = d
+                bf.loss()
 
-    res = tune(key, run, "lig_fit_render_l2_batched", bf.means.device, reps=reps,
-               use_cache=use_cache)
-    bf.batch_depth = res["choice"]
-    return res
+        entry = "lig_fit_render_l2_batched" + ("_persistent" if persistent else "")
+        r = tune(key + f":{int(persistent)}", run, entry, bf.means.device, reps=reps,
+                 use_cache=False)
+        ms["persistent" if persistent else "grid"] = r["ms"]
+    best = min(((f, d, t) for f, row in ms.items() for d, t in row.items()),
+               key=lambda x: x[2])
+    bf.persistent = best[0] == "persistent"
+    bf.batch_depth = best[1]
+    _cache[key] = [bf.batch_depth, bf.persistent]
+    _store()
+    return {"choice": bf.batch_depth, "persistent": bf.persistent, "ms": ms, "cached": False}

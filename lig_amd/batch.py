@@ -47,7 +47,8 @@ def batch_table(sizes: Sequence[tuple], counts: Sequence[int]):
 
 class _BatchedRenderL2(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, means, cholesky, rgbs, target, table, num_images, num_tiles, depth):
+    def forward(ctx, means, cholesky, rgbs, target, table, num_images, num_tiles, depth,
+                persistent=False):
         n = means.shape[0]
         dev = means.device
         xys = torch.empty(n, 2, device=dev)
@@ -61,6 +62,8 @@ class _BatchedRenderL2(torch.autograd.Function):
         loss_img = torch.empty(num_images, device=dev)
         loss = torch.empty((), device=dev)
         ctx.n, ctx.num_int, ctx.nimg, ctx.ntiles = n, int(num_int), num_images, num_tiles
+        ctx.persistent = bool(persistent)
+        sfx = "_persistent" if persistent else ""
         if num_int < 1:
             raise RuntimeError("batched fit: no Gaussian intersects any tile")
         gids = torch.empty(num_int, dtype=torch.int32, device=dev)
@@ -79,7 +82,7 @@ class _BatchedRenderL2(torch.autograd.Function):
                    L.ptr(table), num_images, num_tiles, L.ptr(gids), L.ptr(bins), L.ptr(ws),
                    ws.numel(), s)
             del ws
-            L.call("lig_fit_render_l2_batched", n, L.ptr(gids), L.ptr(bins), L.ptr(xys),
+            L.call("lig_fit_render_l2_batched" + sfx, n, L.ptr(gids), L.ptr(bins), L.ptr(xys),
                    L.ptr(ext), L.ptr(cum), L.ptr(con), L.ptr(rgbs), L.ptr(packed),
                    L.ptr(target), L.ptr(table), num_images, num_tiles, L.ptr(img),
                    L.ptr(loss_img), L.ptr(loss), int(depth), L.ptr(rws), rws.numel(), s)
@@ -100,14 +103,15 @@ class _BatchedRenderL2(torch.autograd.Function):
         v_rgb = torch.empty(n, 3, device=dev)
         with torch.cuda.device(dev):
             s = L.stream_of(img)
-            L.call("lig_fit_backward_records_l2_batched", n, L.ptr(gids), L.ptr(bins),
+            sfx = "_persistent" if ctx.persistent else ""
+            L.call("lig_fit_backward_records_l2_batched" + sfx, n, L.ptr(gids), L.ptr(bins),
                    L.ptr(packed), L.ptr(img), L.ptr(target), L.ptr(grad_loss), L.ptr(table),
                    ctx.nimg, ctx.ntiles, m, L.ptr(ws), ws.numel(), s)
             L.call("lig_fit_reduce_records", n, m, L.ptr(cum), L.ptr(ws), ws.numel(),
                    L.ptr(v_xy), L.ptr(v_conic), L.ptr(v_rgb), s)
         v_chol, v_mean = _C.project_gaussians_backward_cholesky(n, ext, cholesky, con, v_xy,
                                                                 v_conic)
-        return v_mean, v_chol, v_rgb, None, None, None, None, None
+        return v_mean, v_chol, v_rgb, None, None, None, None, None, None
 
 
 class BatchedFit(nn.Module):
@@ -132,11 +136,15 @@ class BatchedFit(nn.Module):
         self.optimizer = FusedAdam([self.rgbs, self.means, self.cholesky], lr=lr)
         self.last_losses = None
         self.batch_depth = 0   # LDS batch depth of the forward (0 = default; see autotune)
+        # raster stages as persistent grids fed by the per-XCD tile queue
+        # (lig_fit_render_l2_batched_persistent); same bits either way
+        self.persistent = False
 
     def loss(self):
         loss, per_image = _BatchedRenderL2.apply(self.means, self.cholesky, self.rgbs,
                                                  self.target, self.table, len(self.sizes),
-                                                 self.num_tiles, self.batch_depth)
+                                                 self.num_tiles, self.batch_depth,
+                                                 self.persistent)
         self.last_losses = per_image
         return loss
 
@@ -289,13 +297,15 @@ class BatchedStepGraph:
         L.call("lig_bin_batched_capacity", n, cap, L.ptr(self.xys), L.ptr(self.ext),
                L.ptr(self.cum), L.ptr(bf.table), nimg, ntiles, L.ptr(self.gids),
                L.ptr(self.bins), L.ptr(self.bin_ws), self.bin_ws.numel(), s)
-        L.call("lig_fit_render_l2_batched", n, L.ptr(self.gids), L.ptr(self.bins),
+        sfx = "_persistent" if bf.persistent else ""
+        L.call("lig_fit_render_l2_batched" + sfx, n, L.ptr(self.gids), L.ptr(self.bins),
                L.ptr(self.xys), L.ptr(self.ext), L.ptr(self.cum), L.ptr(self.con),
                L.ptr(bf.rgbs), L.ptr(self.packed), L.ptr(bf.target), L.ptr(bf.table), nimg,
                ntiles, L.ptr(self.img), L.ptr(self.loss_img), L.ptr(self.loss),
                int(bf.batch_depth), L.ptr(self.rws), self.rws.numel(), s)
-        L.call("lig_fit_backward_records_l2_batched", n, L.ptr(self.gids), L.ptr(self.bins),
-               L.ptr(self.packed), L.ptr(self.img), L.ptr(bf.target), L.ptr(self.grad_loss),
+        L.call("lig_fit_backward_records_l2_batched" + sfx, n, L.ptr(self.gids),
+               L.ptr(self.bins), L.ptr(self.packed), L.ptr(self.img), L.ptr(bf.target),
+               L.ptr(self.grad_loss),
                L.ptr(bf.table), nimg, ntiles, cap, L.ptr(self.rec_ws), self.rec_ws.numel(), s)
         L.call("lig_fit_reduce_records", n, cap, L.ptr(self.cum), L.ptr(self.rec_ws),
                self.rec_ws.numel(), L.ptr(self.v_xy), L.ptr(self.v_conic), L.ptr(self.v_rgb), s)

@@ -190,18 +190,39 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const K *__res
     constexpr int NB = 1 << RB;
     const int n = live_count(cap, n_dev);
     if (n < 0) return;
-    __shared__ int h[NB];
-    for (int d = threadIdx.x; d < NB; d += kSortThreads) h[d] = 0;
+    // one histogram per wave (fewer same-address LDS atomics); 32-bit keys
+    // of full blocks are read 4 at a time
+    __shared__ int h[kSortWaves][NB];
+    for (int d = threadIdx.x; d < kSortWaves * NB; d += kSortThreads) (&h[0][0])[d] = 0;
     __syncthreads();
+    int *hw = h[threadIdx.x / LIG_WAVE];
     const size_t base = (size_t)blockIdx.x * kSortTile;
+    if (sizeof(K) == 4 && base + kSortTile <= (size_t)n) {
+        const uint4 *k4 = reinterpret_cast<const uint4 *>(keys + base);
+        uint4 v[kSortRounds / 4];
+#pragma unroll
+        for (int r = 0; r < kSortRounds / 4; ++r) v[r] = k4[r * kSortThreads + threadIdx.x];
+#pragma unroll
+        for (int r = 0; r < kSortRounds / 4; ++r) {
+            atomicAdd(&hw[digit_of<K, RB>((K)v[r].x, shift, flip)], 1);
+            atomicAdd(&hw[digit_of<K, RB>((K)v[r].y, shift, flip)], 1);
+            atomicAdd(&hw[digit_of<K, RB>((K)v[r].z, shift, flip)], 1);
+            atomicAdd(&hw[digit_of<K, RB>((K)v[r].w, shift, flip)], 1);
+        }
+    } else {
 #pragma unroll 4
-    for (int r = 0; r < kSortRounds; ++r) {
-        const size_t i = base + (size_t)r * kSortThreads + threadIdx.x;
-        if (i < (size_t)n) atomicAdd(&h[digit_of<K, RB>(keys[i], shift, flip)], 1);
+        for (int r = 0; r < kSortRounds; ++r) {
+            const size_t i = base + (size_t)r * kSortThreads + threadIdx.x;
+            if (i < (size_t)n) atomicAdd(&hw[digit_of<K, RB>(keys[i], shift, flip)], 1);
+        }
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < NB; d += kSortThreads)
-        hist[(size_t)d * nblocks + blockIdx.x] = h[d];
+    for (int d = threadIdx.x; d < NB; d += kSortThreads) {
+        int s = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) s += h[w][d];
+        hist[(size_t)d * nblocks + blockIdx.x] = s;
+    }
 }
 
 // Scatter of one pass.  Each wave owns 512 consecutive pairs of the block's

@@ -89,6 +89,70 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// Tile work queue of a persistent launch: one counter per XCD (zeroed before
+// the launch, kQueueStride ints apart so that each sits in its own 256-byte
+// line) over that XCD's contiguous share of the tiles, the split of
+// xcd_remap.  A wave (blocks are dispatched to XCDs round-robin) takes
+// kQueueGrab tiles at a time from its own XCD's share while any is left,
+// then from the other shares in turn, so an XCD whose share runs long is
+// helped by the others.  The next grab is issued before the current tiles
+// are processed.
+constexpr int kQueueStride = 64;
+constexpr int kQueueGrab = 2;
+constexpr int kQueueInts = 8 * kQueueStride;
+
+struct TileQueue {
+    int *ctr;  // kQueueInts ints
+    int ntiles;
+};
+
+__device__ __forceinline__ void xcd_share(int x, int nt, int &start, int &size) {
+    const int q = nt >> 3, r = nt & 7;
+    size = q + (x < r ? 1 : 0);
+    start = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+}
+
+// Lane 0 takes the next grab of its own XCD's share (the raw counter value;
+// -1 on the other lanes).
+__device__ __forceinline__ int queue_issue(const TileQueue &q) {
+    int i = -1;
+    if ((threadIdx.x & (LIG_WAVE - 1)) == 0)
+        i = atomicAdd(&q.ctr[(blockIdx.x & 7) * kQueueStride], kQueueGrab);
+    return i;
+}
+
+// First tile and tile count of the grab `own` (from queue_issue), or of a
+// grab taken from another XCD's share when the own share is done; -1 when
+// every share is taken.  Wave-uniform results.
+__device__ __forceinline__ int queue_resolve(const TileQueue &q, int own, int &count) {
+    int t = -1, c = 0;
+    if ((threadIdx.x & (LIG_WAVE - 1)) == 0) {
+        const int x0 = blockIdx.x & 7;
+        int start, size;
+        xcd_share(x0, q.ntiles, start, size);
+        if (own < size) {
+            t = start + own;
+            c = min(kQueueGrab, size - own);
+        } else {
+            for (int k = 1; k < 8; ++k) {
+                const int x = (x0 + k) & 7;
+                xcd_share(x, q.ntiles, start, size);
+                int *ctr = &q.ctr[x * kQueueStride];
+                if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= size)
+                    continue;  // share already taken
+                const int i = atomicAdd(ctr, kQueueGrab);
+                if (i < size) {
+                    t = start + i;
+                    c = min(kQueueGrab, size - i);
+                    break;
+                }
+            }
+        }
+    }
+    count = __builtin_amdgcn_readfirstlane(c);
+    return __builtin_amdgcn_readfirstlane(t);
+}
+
 // Per-Gaussian record the rasteriser kernels read (64 bytes, 16-byte aligned):
 //   p0 = (mean.x, mean.y, a', 2b')  p1 = (c', opacity, red, green)
 //   p2 = (blue, a, b, c)            p3 = (cull_x, cull_y, slot base, slot width)

@@ -237,19 +237,16 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
 // DEPTH (64, 128, 256): Gaussians staged in LDS per batch, DEPTH / 64 per lane;
 // deeper batches keep more gathers in flight per wave and halve or quarter the
 // barriers (chosen per workload by lig_amd.autotune).
-template <bool HAS_OPAC, bool TRAIN, int DEPTH = kThreads>
-__global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
-    int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
+template <bool HAS_OPAC, bool TRAIN, int DEPTH>
+__device__ __forceinline__ void fwd16_tile(
+    int gtile, int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
     float *__restrict__ out_img, float *__restrict__ out_wsum,
     int32_t *__restrict__ final_idx, const float *__restrict__ target,
-    float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg) {
-    __shared__ float4 s0[DEPTH];
-    __shared__ float4 s1[DEPTH];
-    __shared__ float2 s2[DEPTH];  // (blue, quadrant mask bits)
-
-    const int gtile = (int)xcd_remap(blockIdx.x, gridDim.x);  // bins / partials index
-    int tile = gtile;                                            // tile of its own image
+    float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg, float4 *s0,
+    float4 *s1, float2 *s2) {
+    // gtile: bins / partials index
+    int tile = gtile;  // tile of its own image
     if (tab) {  // batched fit: the tile's image, its canvas and pixel buffers
         const BatchImage &im = tab[batch_find<0>(tab, nimg, gtile)];
         tile = gtile - im.tile_off;
@@ -363,6 +360,47 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
                 lsum += __shfl_down(lsum, off, LIG_WAVE);
             if (lane == 0) loss_partial[gtile] = lsum;
         }
+    }
+}
+
+// One tile per block: the tile xcd_remap(blockIdx.x).
+template <bool HAS_OPAC, bool TRAIN, int DEPTH = kThreads>
+__global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
+    int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
+    const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
+    float *__restrict__ out_img, float *__restrict__ out_wsum,
+    int32_t *__restrict__ final_idx, const float *__restrict__ target,
+    float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg) {
+    __shared__ float4 s0[DEPTH];
+    __shared__ float4 s1[DEPTH];
+    __shared__ float2 s2[DEPTH];  // (blue, quadrant mask bits)
+    fwd16_tile<HAS_OPAC, TRAIN, DEPTH>((int)xcd_remap(blockIdx.x, gridDim.x), tiles_x, img_w,
+                                       img_h, gids, bins, pack, out_img, out_wsum, final_idx,
+                                       target, loss_partial, tab, nimg, s0, s1, s2);
+}
+
+// Persistent form: each wave takes tiles from the per-XCD queue until none is
+// left.
+template <bool HAS_OPAC, bool TRAIN, int DEPTH = kThreads>
+__global__ __launch_bounds__(kThreads) void raster_fwd16_queue_kernel(
+    int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
+    const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
+    float *__restrict__ out_img, float *__restrict__ out_wsum,
+    int32_t *__restrict__ final_idx, const float *__restrict__ target,
+    float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg,
+    TileQueue queue) {
+    __shared__ float4 s0[DEPTH];
+    __shared__ float4 s1[DEPTH];
+    __shared__ float2 s2[DEPTH];
+    int cnt;
+    int t = queue_resolve(queue, queue_issue(queue), cnt);
+    while (t >= 0) {
+        const int pending = queue_issue(queue);
+        for (int k = 0; k < cnt; ++k)
+            fwd16_tile<HAS_OPAC, TRAIN, DEPTH>(t + k, tiles_x, img_w, img_h, gids, bins, pack,
+                                               out_img, out_wsum, final_idx, target,
+                                               loss_partial, tab, nimg, s0, s1, s2);
+        t = queue_resolve(queue, pending, cnt);
     }
 }
 
@@ -773,8 +811,8 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
 // same FP32 values), so every in-image pixel is evaluated and the records are
 // those of the gated path.
 template <bool HAS_OPAC, bool DET, bool TRAIN>
-__global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
-    int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
+__device__ __forceinline__ void bwd16_tile(
+    int gtile, int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
     const GaussPack *__restrict__ pack, const int32_t *__restrict__ final_idx,
     const float *__restrict__ v_out, const float *__restrict__ v_wsum,
@@ -782,11 +820,8 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     const float *__restrict__ grad_loss, float norm, GradRec *__restrict__ rec,
     float *__restrict__ g_xy, float *__restrict__ g_xy_abs, float *__restrict__ g_conic,
     float *__restrict__ g_rgb, float *__restrict__ g_opac, const BatchImage *__restrict__ tab,
-    int nimg) {
-    __shared__ float4 sv[kMaxPix];
-    __shared__ int sfin[TRAIN ? 1 : kMaxPix];
-
-    const int gtile = (int)xcd_remap(blockIdx.x, gridDim.x);  // bins index
+    int nimg, float4 *sv, int *sfin) {
+    // gtile: bins index
     const int lane = threadIdx.x;
     const int2 range = bins[gtile];
     if (range.x >= range.y) return;
@@ -876,6 +911,50 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
             plain_emit<HAS_OPAC, DET>(A, gp, act, slot, g, rec, g_xy, g_xy_abs, g_conic, g_rgb,
                                       g_opac);
         }
+    }
+}
+
+// One tile per block: the tile xcd_remap(blockIdx.x).
+template <bool HAS_OPAC, bool DET, bool TRAIN>
+__global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
+    int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
+    const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
+    const GaussPack *__restrict__ pack, const int32_t *__restrict__ final_idx,
+    const float *__restrict__ v_out, const float *__restrict__ v_wsum,
+    const float *__restrict__ img, const float *__restrict__ target,
+    const float *__restrict__ grad_loss, float norm, GradRec *__restrict__ rec,
+    float *__restrict__ g_xy, float *__restrict__ g_xy_abs, float *__restrict__ g_conic,
+    float *__restrict__ g_rgb, float *__restrict__ g_opac, const BatchImage *__restrict__ tab,
+    int nimg) {
+    __shared__ float4 sv[kMaxPix];
+    __shared__ int sfin[TRAIN ? 1 : kMaxPix];
+    bwd16_tile<HAS_OPAC, DET, TRAIN>((int)xcd_remap(blockIdx.x, gridDim.x), tiles_x, img_w,
+                                     img_h, gids, slots, bins, pack, final_idx, v_out, v_wsum,
+                                     img, target, grad_loss, norm, rec, g_xy, g_xy_abs, g_conic,
+                                     g_rgb, g_opac, tab, nimg, sv, sfin);
+}
+
+// Persistent form of the fit-step backward (batched): each wave takes tiles
+// from the per-XCD queue until none is left.
+__global__ __launch_bounds__(kThreads) void raster_bwd16_fit_queue_kernel(
+    const int32_t *__restrict__ gids, const int2 *__restrict__ bins,
+    const GaussPack *__restrict__ pack, const float *__restrict__ img,
+    const float *__restrict__ target, const float *__restrict__ grad_loss,
+    GradRec *__restrict__ rec, const BatchImage *__restrict__ tab, int nimg, TileQueue queue) {
+    __shared__ float4 sv[kMaxPix];
+    __shared__ int sfin[1];
+    int cnt;
+    int t = queue_resolve(queue, queue_issue(queue), cnt);
+    while (t >= 0) {
+        const int pending = queue_issue(queue);
+        for (int k = 0; k < cnt; ++k) {
+            bwd16_tile<false, true, true>(t + k, 1, 1, 1, gids, nullptr, bins, pack, nullptr,
+                                          nullptr, nullptr, img, target, grad_loss, 0.0f, rec,
+                                          nullptr, nullptr, nullptr, nullptr, nullptr, tab, nimg,
+                                          sv, sfin);
+            __syncthreads();  // the next tile restages sv
+        }
+        t = queue_resolve(queue, pending, cnt);
     }
 }
 
@@ -1210,21 +1289,45 @@ namespace {
 // LDS batch depths of the fit forward; 0 selects the default, 64.
 bool depth_ok(int d) { return d == 0 || d == 64 || d == 128 || d == 256; }
 
-// Fit-mode (image + loss partials) forward at LDS batch depth `depth`.
+// Grid of a persistent launch of `kernel` over `ntiles` tiles: the waves
+// that fit on the device at once (a multiple of 8, so every XCD gets the same
+// number), at most one per tile.
+template <typename KernelT>
+unsigned persistent_grid(KernelT kernel, int ntiles) {
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kThreads, 0);
+    long long g = (long long)(cus > 0 ? cus : 1) * (per > 0 ? per : 1);
+    if (g >= 8) g &= ~7ll;
+    return (unsigned)(g < ntiles ? g : ntiles);
+}
+
+// Fit-mode (image + loss partials) forward at LDS batch depth `depth`; with
+// queue.ctr set, a persistent grid over queue.ntiles tiles (the counters
+// must be zero).
 void launch_fit_forward(int depth, dim3 grid, hipStream_t st, int tiles_x, int img_w, int img_h,
                         const int32_t *gids, const int2 *bins, const GaussPack *pk,
                         float *out_img, const float *target, float *partial,
-                        const BatchImage *tab, int nimg) {
+                        const BatchImage *tab, int nimg, TileQueue queue = TileQueue{nullptr, 0}) {
 #define LIG_FIT_FWD(D)                                                                        \
-    hipLaunchKernelGGL((raster_fwd16_kernel<false, true, D>), grid, dim3(kThreads), 0, st,     \
-                       tiles_x, img_w, img_h, gids, bins, pk, out_img, nullptr, nullptr, target, \
-                       partial, tab, nimg)
-    if (depth == 256)
-        LIG_FIT_FWD(256);
-    else if (depth == 128)
-        LIG_FIT_FWD(128);
-    else
-        LIG_FIT_FWD(64);
+    if (queue.ctr) {                                                                          \
+        auto kern = raster_fwd16_queue_kernel<false, true, D>;                                \
+        hipLaunchKernelGGL(kern, dim3(persistent_grid(kern, queue.ntiles)), dim3(kThreads), 0, \
+                           st, tiles_x, img_w, img_h, gids, bins, pk, out_img, nullptr,        \
+                           nullptr, target, partial, tab, nimg, queue);                       \
+    } else {                                                                                  \
+        hipLaunchKernelGGL((raster_fwd16_kernel<false, true, D>), grid, dim3(kThreads), 0, st, \
+                           tiles_x, img_w, img_h, gids, bins, pk, out_img, nullptr, nullptr,   \
+                           target, partial, tab, nimg);                                        \
+    }
+    if (depth == 256) {
+        LIG_FIT_FWD(256)
+    } else if (depth == 128) {
+        LIG_FIT_FWD(128)
+    } else {
+        LIG_FIT_FWD(64)
+    }
 #undef LIG_FIT_FWD
 }
 }  // namespace
@@ -1273,18 +1376,21 @@ extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
 }
 
 extern "C" size_t lig_fit_batched_workspace_size(int num_tiles) {
-    return num_tiles > 0 ? (size_t)num_tiles * sizeof(float) + 256 : 256;
+    // [tile partials, FP32][tile-queue counters at a 256-byte boundary]
+    const size_t t = num_tiles > 0 ? (size_t)num_tiles : 0;
+    return ((t * sizeof(float) + 255) & ~size_t(255)) + kQueueInts * sizeof(int);
 }
 
-extern "C" int lig_fit_render_l2_batched(int n, const int32_t *gaussian_ids_sorted,
-                                         const int32_t *tile_bins, const float *xys,
-                                         const float *extents, const int32_t *cum_tiles_hit,
-                                         const float *conics, const float *colors, void *packed,
-                                         const float *target, const void *batch_table,
-                                         int num_images, int num_tiles, float *out_img,
-                                         float *loss_per_image, float *loss, int batch_depth,
-                                         void *workspace, size_t workspace_bytes,
-                                         void *stream) {
+namespace lig {
+namespace {
+
+int fit_render_batched(int n, const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
+                       const float *xys, const float *extents, const int32_t *cum_tiles_hit,
+                       const float *conics, const float *colors, void *packed,
+                       const float *target, const void *batch_table, int num_images,
+                       int num_tiles, float *out_img, float *loss_per_image, float *loss,
+                       int batch_depth, void *workspace, size_t workspace_bytes, void *stream,
+                       bool persistent) {
     if (n <= 0 || num_images < 1 || num_tiles < 1 || !gaussian_ids_sorted || !tile_bins ||
         !xys || !extents || !cum_tiles_hit || !conics || !colors || !packed || !target ||
         !batch_table || !out_img || !loss_per_image || !loss || !depth_ok(batch_depth))
@@ -1297,14 +1403,51 @@ extern "C" int lig_fit_render_l2_batched(int n, const int32_t *gaussian_ids_sort
     hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
                        colors, nullptr, extents, cum_tiles_hit, 0, 0, tab, num_images, pk);
     float *partial = reinterpret_cast<float *>(workspace);
+    TileQueue q{nullptr, num_tiles};
+    if (persistent) {
+        q.ctr = reinterpret_cast<int *>(reinterpret_cast<char *>(workspace) +
+                                        (((size_t)num_tiles * sizeof(float) + 255) & ~size_t(255)));
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(kQueueInts, 256)), dim3(256), 0, st,
+                           reinterpret_cast<float *>(q.ctr), (size_t)kQueueInts);
+    }
     launch_fit_forward(batch_depth, dim3((unsigned)num_tiles), st, 1, 1, 1, gaussian_ids_sorted,
                        reinterpret_cast<const int2 *>(tile_bins), pk, out_img, target, partial,
-                       tab, num_images);
+                       tab, num_images, q);
     hipLaunchKernelGGL(image_loss_kernel, dim3((unsigned)num_images), dim3(kSumThreads), 0, st,
                        tab, (const float *)partial, loss_per_image);
     hipLaunchKernelGGL(total_loss_kernel, dim3(1), dim3(kSumThreads), 0, st, num_images,
                        (const float *)loss_per_image, loss);
     return launch_status();
+}
+
+}  // namespace
+}  // namespace lig
+
+extern "C" int lig_fit_render_l2_batched(int n, const int32_t *gaussian_ids_sorted,
+                                         const int32_t *tile_bins, const float *xys,
+                                         const float *extents, const int32_t *cum_tiles_hit,
+                                         const float *conics, const float *colors, void *packed,
+                                         const float *target, const void *batch_table,
+                                         int num_images, int num_tiles, float *out_img,
+                                         float *loss_per_image, float *loss, int batch_depth,
+                                         void *workspace, size_t workspace_bytes,
+                                         void *stream) {
+    return fit_render_batched(n, gaussian_ids_sorted, tile_bins, xys, extents, cum_tiles_hit,
+                              conics, colors, packed, target, batch_table, num_images, num_tiles,
+                              out_img, loss_per_image, loss, batch_depth, workspace,
+                              workspace_bytes, stream, false);
+}
+
+extern "C" int lig_fit_render_l2_batched_persistent(
+    int n, const int32_t *gaussian_ids_sorted, const int32_t *tile_bins, const float *xys,
+    const float *extents, const int32_t *cum_tiles_hit, const float *conics, const float *colors,
+    void *packed, const float *target, const void *batch_table, int num_images, int num_tiles,
+    float *out_img, float *loss_per_image, float *loss, int batch_depth, void *workspace,
+    size_t workspace_bytes, void *stream) {
+    return fit_render_batched(n, gaussian_ids_sorted, tile_bins, xys, extents, cum_tiles_hit,
+                              conics, colors, packed, target, batch_table, num_images, num_tiles,
+                              out_img, loss_per_image, loss, batch_depth, workspace,
+                              workspace_bytes, stream, true);
 }
 
 extern "C" size_t lig_rasterize_backward_workspace_size(int num_intersects) {
@@ -1432,25 +1575,68 @@ extern "C" int lig_fit_backward_records_l2(int n, int img_height, int img_width,
     return launch_status();
 }
 
-extern "C" int lig_fit_backward_records_l2_batched(
-    int n, const int32_t *gaussian_ids_sorted, const int32_t *tile_bins, const void *packed,
-    const float *img, const float *target, const float *grad_loss, const void *batch_table,
-    int num_images, int num_tiles, int num_intersects, void *workspace, size_t workspace_bytes,
-    void *stream) {
+namespace lig {
+namespace {
+
+int fit_backward_batched(int n, const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
+                         const void *packed, const float *img, const float *target,
+                         const float *grad_loss, const void *batch_table, int num_images,
+                         int num_tiles, int num_intersects, void *workspace,
+                         size_t workspace_bytes, void *stream, bool persistent) {
     if (n <= 0 || num_images < 1 || num_tiles < 1 || num_intersects < 0 ||
         !gaussian_ids_sorted || !tile_bins || !packed || !img || !target || !grad_loss ||
         !batch_table)
         return LIG_ERR_INVALID_ARGUMENT;
     if (!workspace || workspace_bytes < lig_rasterize_backward_workspace_size(num_intersects))
         return LIG_ERR_WORKSPACE;
+    hipStream_t st = as_stream(stream);
     GradRec *rec = reinterpret_cast<GradRec *>(reinterpret_cast<char *>(workspace) + 256);
-    hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), dim3((unsigned)num_tiles),
-                       dim3(kThreads), 0, as_stream(stream), 1, 1, 1, gaussian_ids_sorted,
-                       nullptr, reinterpret_cast<const int2 *>(tile_bins),
-                       reinterpret_cast<const GaussPack *>(packed), nullptr, nullptr, nullptr,
-                       img, target, grad_loss, 0.0f, rec, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, reinterpret_cast<const BatchImage *>(batch_table), num_images);
+    const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
+    const GaussPack *pk = reinterpret_cast<const GaussPack *>(packed);
+    const BatchImage *tab = reinterpret_cast<const BatchImage *>(batch_table);
+    // fit records (32 bytes) start 256 bytes in; the workspace is sized for
+    // 48-byte records, and the tile-queue counters go after the fit records
+    // when that slack holds them (else the one-tile-per-block launch runs)
+    const size_t qoff = (256 + (size_t)num_intersects * sizeof(FitRec) + 255) & ~size_t(255);
+    if (persistent && qoff + kQueueInts * sizeof(int) <= workspace_bytes) {
+        TileQueue q{reinterpret_cast<int *>(reinterpret_cast<char *>(workspace) + qoff),
+                    num_tiles};
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(kQueueInts, 256)), dim3(256), 0, st,
+                           reinterpret_cast<float *>(q.ctr), (size_t)kQueueInts);
+        hipLaunchKernelGGL(raster_bwd16_fit_queue_kernel,
+                           dim3(persistent_grid(raster_bwd16_fit_queue_kernel, num_tiles)),
+                           dim3(kThreads), 0, st, gaussian_ids_sorted, bins, pk, img, target,
+                           grad_loss, rec, tab, num_images, q);
+    } else {
+        hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), dim3((unsigned)num_tiles),
+                           dim3(kThreads), 0, st, 1, 1, 1, gaussian_ids_sorted, nullptr, bins, pk,
+                           nullptr, nullptr, nullptr, img, target, grad_loss, 0.0f, rec, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, tab, num_images);
+    }
     return launch_status();
+}
+
+}  // namespace
+}  // namespace lig
+
+extern "C" int lig_fit_backward_records_l2_batched(
+    int n, const int32_t *gaussian_ids_sorted, const int32_t *tile_bins, const void *packed,
+    const float *img, const float *target, const float *grad_loss, const void *batch_table,
+    int num_images, int num_tiles, int num_intersects, void *workspace, size_t workspace_bytes,
+    void *stream) {
+    return fit_backward_batched(n, gaussian_ids_sorted, tile_bins, packed, img, target,
+                                grad_loss, batch_table, num_images, num_tiles, num_intersects,
+                                workspace, workspace_bytes, stream, false);
+}
+
+extern "C" int lig_fit_backward_records_l2_batched_persistent(
+    int n, const int32_t *gaussian_ids_sorted, const int32_t *tile_bins, const void *packed,
+    const float *img, const float *target, const float *grad_loss, const void *batch_table,
+    int num_images, int num_tiles, int num_intersects, void *workspace, size_t workspace_bytes,
+    void *stream) {
+    return fit_backward_batched(n, gaussian_ids_sorted, tile_bins, packed, img, target,
+                                grad_loss, batch_table, num_images, num_tiles, num_intersects,
+                                workspace, workspace_bytes, stream, true);
 }
 
 extern "C" int lig_fit_reduce_records(int n, int num_intersects, const int32_t *cum_tiles_hit,

@@ -94,3 +94,52 @@ def test_graph_step_bias_table_extension(device):
     gs.run(5)
     _assert_same(eager, graphed, "table")
     assert gs.table_len >= 5 and gs.captures >= 2
+
+
+@pytest.mark.parametrize("skewed", [False, True])
+def test_persistent_queue_matches_grid(device, skewed):
+    """The persistent raster stages (per-XCD tile queue with stealing) give
+    the bits of the one-tile-per-workgroup launches, eagerly and captured; the
+    skewed batch puts 20x the Gaussians per pixel into two images so that the
+    XCD shares are uneven."""
+    from lig_amd.batch import BatchedFit, BatchedStepGraph, make_targets
+    counts = list(COUNTS)
+    if skewed:
+        counts[0] *= 20
+        counts[2] *= 20
+    targets = make_targets(SIZES, SEEDS, device)
+    grid = BatchedFit(targets, counts, SEEDS, scale=1.5, lr=1e-2)
+    queue = BatchedFit(targets, counts, SEEDS, scale=1.5, lr=1e-2)
+    queue.persistent = True
+    for step in range(3):
+        a = grid.step()
+        b = queue.step()
+        assert torch.equal(a, b), step
+        assert torch.equal(grid.last_losses, queue.last_losses), step
+    _assert_same(grid, queue, "eager")
+    gs = BatchedStepGraph(queue)
+    for _ in range(2):
+        grid.step()
+    gs.run(2)
+    _assert_same(grid, queue, "graph")
+
+
+def test_autotune_batched_picks_form_and_depth(device, tmp_path, monkeypatch):
+    """tune_batched times every (launch form, depth) pair, sets the fastest
+    on the model and caches it; a second call reads the cache."""
+    from lig_amd import autotune
+    from lig_amd.batch import BatchedFit, make_targets
+    monkeypatch.setenv("LIG_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_cache", {})
+    targets = make_targets(SIZES, SEEDS, device)
+    bf = BatchedFit(targets, COUNTS, SEEDS, scale=1.5, lr=1e-2)
+    res = autotune.tune_batched(bf, reps=1)
+    assert set(res["ms"]) == {"grid", "persistent"}
+    assert all(set(row) == set(autotune.DEPTHS) for row in res["ms"].values())
+    best = min(t for row in res["ms"].values() for t in row.values())
+    form = "persistent" if bf.persistent else "grid"
+    assert res["ms"][form][bf.batch_depth] == best
+    again = autotune.tune_batched(bf, reps=1)
+    assert again["cached"] and again["choice"] == res["choice"]
+    assert again["persistent"] == res["persistent"]
+    bf.step()   # the tuned configuration runs
