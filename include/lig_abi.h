@@ -185,7 +185,9 @@ int lig_rasterize_forward(int n, int img_height, int img_width, int block_width,
  *   emission_ids_sorted != NULL (from lig_bin_and_sort_gaussians, with its
  *     cum_tiles_hit and num_intersects): per-tile sums are written to the
  *     workspace (lig_rasterize_backward_workspace_size(num_intersects) bytes)
- *     and reduced per Gaussian in slot order: bit-reproducible, no atomics. */
+ *     and reduced per Gaussian in a fixed order that depends only on its own
+ *     records (chunks of 16 slots from its first, each summed by a fixed
+ *     tree, chunk sums in order): bit-reproducible, no atomics. */
 size_t lig_rasterize_backward_workspace_size(int num_intersects);
 int lig_rasterize_backward(int n, int img_height, int img_width, int block_width,
                            const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
@@ -248,8 +250,9 @@ int lig_reduce_gradient_records(int n, int num_intersects, const int32_t *cum_ti
  * (no parameter gradient depends on it).  Each 32-byte record lands at its
  * emission slot, computed from `packed`.
  *
- * lig_fit_reduce_records: sums each Gaussian's records in slot order into
- * v_xy [n,2], v_conic [n,3], v_colors [n,3].  Same workspace as the records
+ * lig_fit_reduce_records: sums each Gaussian's records (in the order of the
+ * API reduction, so both paths give the same bits) into v_xy [n,2],
+ * v_conic [n,3], v_colors [n,3].  Same workspace as the records
  * call, lig_rasterize_backward_workspace_size(num_intersects). */
 size_t lig_fit_render_workspace_size(int img_height, int img_width);
 int lig_fit_render_l2(int n, int img_height, int img_width, const int32_t *gaussian_ids_sorted,

@@ -518,127 +518,129 @@ __device__ __forceinline__ void emit_grads(bool det, GradRec *__restrict__ rec, 
     }
 }
 
-// Sum each Gaussian's records in slot order (tile row-major) and write its
-// gradients.  Slots of a zero-extent Gaussian hold records of Gaussian 0 (the
-// reference bins them as (tile 0, gid 0)); their sums go to `phantom` and are
-// added to Gaussian 0 by add_phantom_kernel.  One wave per 64 consecutive
-// Gaussians streams their contiguous record range through LDS (coalesced
-// 16-byte loads); each lane adds its own Gaussian's records in slot order.
-constexpr int kRedChunk = 256;  // records per LDS chunk
+// Slots of a zero-extent Gaussian hold records of Gaussian 0 (the reference
+// bins them as (tile 0, gid 0)); their sums go to `phantom` and are added to
+// Gaussian 0 by add_phantom_kernel.
+// Record reductions.  Each Gaussian's records are its contiguous slot range
+// [cum[g-1], cum[g]); a 16-lane group sums them in chunks of 16 counted from
+// the range's own start: the group loads a chunk (coalesced), adds it by a
+// fixed butterfly (xor 8, 4, 2, 1) whose result is taken from the group's
+// first lane, and adds the chunk sums in chunk order.  The order depends only
+// on the Gaussian's own records, so the API and fit paths, and a batched and
+// a single-image fit, give the same bits.  The chunk loop runs to the wave's
+// longest range (all lanes active for the shuffles; missing records read 0).
+constexpr int kSegLanes = 16;
+constexpr int kSegThreads = 256;
 
-__global__ __launch_bounds__(LIG_WAVE) void reduce_records_kernel(
+// Sum over a 16-lane DPP row, valid in the row's first lane: quad swaps
+// (xor 1, xor 2), then the half-row and row mirrors; a fixed pairing.
+__device__ __forceinline__ float dpp_f(float v, int ctrl_sel) {
+    const int x = __float_as_int(v);
+    int y;
+    switch (ctrl_sel) {
+        case 0: y = __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false); break;
+        case 1: y = __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false); break;
+        case 2: y = __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false); break;
+        default: y = __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false); break;
+    }
+    return __int_as_float(y);
+}
+__device__ __forceinline__ float row16_sum(float v) {
+    v += dpp_f(v, 0);
+    v += dpp_f(v, 1);
+    v += dpp_f(v, 2);
+    v += dpp_f(v, 3);
+    return v;
+}
+
+template <int NF>
+__device__ __forceinline__ void seg_sum(const float4 *__restrict__ src, int lo, int hi,
+                                        float (&s)[NF]) {
+    const int sub = threadIdx.x & (kSegLanes - 1);
+    int nchunk = hi > lo ? (hi - lo + kSegLanes - 1) / kSegLanes : 0;
+#pragma unroll
+    for (int off = LIG_WAVE / 2; off > 0; off >>= 1)
+        nchunk = max(nchunk, __shfl_xor(nchunk, off, LIG_WAVE));
+#pragma unroll
+    for (int k = 0; k < NF; ++k) s[k] = 0.0f;
+    for (int c = 0; c < nchunk; ++c) {
+        const int r = lo + c * kSegLanes + sub;
+        float v[NF];
+        if (r < hi) {
+#pragma unroll
+            for (int q = 0; q < NF / 4; ++q) {
+                const float4 x = src[(size_t)r * (NF / 4) + q];
+                v[4 * q] = x.x;
+                v[4 * q + 1] = x.y;
+                v[4 * q + 2] = x.z;
+                v[4 * q + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NF; ++k) v[k] = 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < NF; ++k) v[k] = row16_sum(v[k]);
+#pragma unroll
+        for (int k = 0; k < NF; ++k) s[k] += v[k];
+    }
+}
+
+__global__ __launch_bounds__(kSegThreads) void reduce_records_kernel(
     int n, int m, const int32_t *__restrict__ cum, const GradRec *__restrict__ rec,
     int has_opac, float *__restrict__ g_xy, float *__restrict__ g_xy_abs,
     float *__restrict__ g_conic, float *__restrict__ g_rgb, float *__restrict__ g_opac,
     float *__restrict__ phantom) {
-    __shared__ float4 srec[3 * kRedChunk];
-    const int lane = threadIdx.x;
-    const int g0 = blockIdx.x * LIG_WAVE;
-    const int g = g0 + lane;
-    const int glast = min(g0 + LIG_WAVE, n) - 1;
+    const int g = (int)(((size_t)blockIdx.x * kSegThreads + threadIdx.x) / kSegLanes);
     int lo = 0, hi = 0;
     if (g < n) {
         lo = max(0, g == 0 ? 0 : cum[g - 1]);
         hi = min(cum[g], m);
     }
-    const int wlo = max(0, g0 == 0 ? 0 : cum[g0 - 1]);
-    const int whi = min(cum[glast], m);
-    const float4 *src = reinterpret_cast<const float4 *>(rec);
-    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
-    float s2x = 0.f, s2y = 0.f, s2z = 0.f;
-    bool foreign = false;
-    for (int c0 = wlo; c0 < whi; c0 += kRedChunk) {
-        const int cn = min(kRedChunk, whi - c0);
+    float s[12];
+    seg_sum<12>(reinterpret_cast<const float4 *>(rec), lo, hi, s);
+    if (g >= n || (threadIdx.x & (kSegLanes - 1)) != 0) return;
+    // records of zero-extent slots carry gid 0: their sums go to Gaussian 0
+    if (hi > lo && __float_as_int(rec[lo].r2.w) != g) {
 #pragma unroll
-        for (int k = 0; k < 3 * kRedChunk / LIG_WAVE; ++k) {
-            const int q = k * LIG_WAVE + lane;
-            if (q < 3 * cn) srec[q] = src[3 * (size_t)c0 + q];
-        }
-        __syncthreads();
-        const int e0 = max(lo, c0), e1 = min(hi, c0 + cn);
-        for (int e = e0; e < e1; ++e) {
-            const float4 a = srec[3 * (e - c0)];
-            const float4 b = srec[3 * (e - c0) + 1];
-            const float4 c = srec[3 * (e - c0) + 2];
-            foreign = __float_as_int(c.w) != g;
-            s0.x += a.x; s0.y += a.y; s0.z += a.z; s0.w += a.w;
-            s1.x += b.x; s1.y += b.y; s1.z += b.z; s1.w += b.w;
-            s2x += c.x; s2y += c.y; s2z += c.z;
-        }
-        __syncthreads();
-    }
-    if (g >= n) return;
-    if (foreign) {
-        const float v[11] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w, s2x, s2y, s2z};
+        for (int q = 0; q < 11; ++q) unsafeAtomicAdd(&phantom[q], s[q]);
 #pragma unroll
-        for (int q = 0; q < 11; ++q) unsafeAtomicAdd(&phantom[q], v[q]);
-        s0 = make_float4(0.f, 0.f, 0.f, 0.f);
-        s1 = s0;
-        s2x = s2y = s2z = 0.f;
+        for (int q = 0; q < 11; ++q) s[q] = 0.0f;
     }
-    g_rgb[3 * g + 0] = s0.x;
-    g_rgb[3 * g + 1] = s0.y;
-    g_rgb[3 * g + 2] = s0.z;
-    g_conic[3 * g + 0] = s0.w;
-    g_conic[3 * g + 1] = s1.x;
-    g_conic[3 * g + 2] = s1.y;
-    g_xy[2 * g + 0] = s1.z;
-    g_xy[2 * g + 1] = s1.w;
-    g_xy_abs[2 * g + 0] = s2x;
-    g_xy_abs[2 * g + 1] = s2y;
-    if (has_opac) g_opac[g] = s2z;
+    g_rgb[3 * g + 0] = s[0];
+    g_rgb[3 * g + 1] = s[1];
+    g_rgb[3 * g + 2] = s[2];
+    g_conic[3 * g + 0] = s[3];
+    g_conic[3 * g + 1] = s[4];
+    g_conic[3 * g + 2] = s[5];
+    g_xy[2 * g + 0] = s[6];
+    g_xy[2 * g + 1] = s[7];
+    g_xy_abs[2 * g + 0] = s[8];
+    g_xy_abs[2 * g + 1] = s[9];
+    if (has_opac) g_opac[g] = s[10];
 }
 
-// Fit-step reduction: sum each Gaussian's FitRec slots [cum[g-1], cum[g]) in
-// slot order (tile row-major).  No phantom slots exist without opacities.
-// One wave per 64 consecutive Gaussians: their records form one contiguous
-// range, streamed through LDS in chunks with coalesced 16-byte loads; each
-// lane then adds its own Gaussian's records of the chunk from LDS, in slot
-// order.
-
-__global__ __launch_bounds__(LIG_WAVE) void reduce_fit_records_kernel(
+// Fit-step reduction (FitRec: no phantom slots exist without opacities).
+__global__ __launch_bounds__(kSegThreads) void reduce_fit_records_kernel(
     int n, int m, const int32_t *__restrict__ cum, const FitRec *__restrict__ rec,
     float *__restrict__ g_xy, float *__restrict__ g_conic, float *__restrict__ g_rgb) {
-    __shared__ float4 srec[2 * kRedChunk];
-    const int lane = threadIdx.x;
-    const int g0 = blockIdx.x * LIG_WAVE;
-    const int g = g0 + lane;
-    const int glast = min(g0 + LIG_WAVE, n) - 1;
+    const int g = (int)(((size_t)blockIdx.x * kSegThreads + threadIdx.x) / kSegLanes);
     int lo = 0, hi = 0;
     if (g < n) {
         lo = max(0, g == 0 ? 0 : cum[g - 1]);
         hi = min(cum[g], m);
     }
-    const int wlo = max(0, g0 == 0 ? 0 : cum[g0 - 1]);
-    const int whi = min(cum[glast], m);
-    const float4 *src = reinterpret_cast<const float4 *>(rec);
-    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
-    for (int c0 = wlo; c0 < whi; c0 += kRedChunk) {
-        const int cn = min(kRedChunk, whi - c0);
-#pragma unroll
-        for (int k = 0; k < 2 * kRedChunk / LIG_WAVE; ++k) {
-            const int q = k * LIG_WAVE + lane;
-            if (q < 2 * cn) srec[q] = src[2 * (size_t)c0 + q];
-        }
-        __syncthreads();
-        const int e0 = max(lo, c0), e1 = min(hi, c0 + cn);
-        for (int e = e0; e < e1; ++e) {
-            const float4 a = srec[2 * (e - c0)];
-            const float4 b = srec[2 * (e - c0) + 1];
-            s0.x += a.x; s0.y += a.y; s0.z += a.z; s0.w += a.w;
-            s1.x += b.x; s1.y += b.y; s1.z += b.z; s1.w += b.w;
-        }
-        __syncthreads();
-    }
-    if (g >= n) return;
-    g_rgb[3 * g + 0] = s0.x;
-    g_rgb[3 * g + 1] = s0.y;
-    g_rgb[3 * g + 2] = s0.z;
-    g_conic[3 * g + 0] = s0.w;
-    g_conic[3 * g + 1] = s1.x;
-    g_conic[3 * g + 2] = s1.y;
-    g_xy[2 * g + 0] = s1.z;
-    g_xy[2 * g + 1] = s1.w;
+    float s[8];
+    seg_sum<8>(reinterpret_cast<const float4 *>(rec), lo, hi, s);
+    if (g >= n || (threadIdx.x & (kSegLanes - 1)) != 0) return;
+    g_rgb[3 * g + 0] = s[0];
+    g_rgb[3 * g + 1] = s[1];
+    g_rgb[3 * g + 2] = s[2];
+    g_conic[3 * g + 0] = s[3];
+    g_conic[3 * g + 1] = s[4];
+    g_conic[3 * g + 2] = s[5];
+    g_xy[2 * g + 0] = s[6];
+    g_xy[2 * g + 1] = s[7];
 }
 
 __global__ void add_phantom_kernel(const float *__restrict__ phantom, int has_opac,
@@ -1649,7 +1651,8 @@ extern "C" int lig_fit_reduce_records(int n, int num_intersects, const int32_t *
         return LIG_ERR_WORKSPACE;
     const FitRec *rec = reinterpret_cast<const FitRec *>(
         reinterpret_cast<const char *>(workspace) + 256);
-    hipLaunchKernelGGL(reduce_fit_records_kernel, dim3(ceil_div(n, LIG_WAVE)), dim3(LIG_WAVE), 0,
+    hipLaunchKernelGGL(reduce_fit_records_kernel,
+                       dim3(ceil_div((size_t)n * kSegLanes, kSegThreads)), dim3(kSegThreads), 0,
                        as_stream(stream), n, num_intersects, cum_tiles_hit, rec, v_xy, v_conic,
                        v_colors);
     return launch_status();
@@ -1673,8 +1676,8 @@ extern "C" int lig_reduce_gradient_records(int n, int num_intersects,
     const GradRec *rec =
         reinterpret_cast<const GradRec *>(reinterpret_cast<const char *>(workspace) + 256);
     hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, st, phantom, (size_t)16);
-    hipLaunchKernelGGL(reduce_records_kernel, dim3(ceil_div((size_t)n, LIG_WAVE)), dim3(LIG_WAVE),
-                       0, st, n, num_intersects, cum_tiles_hit, rec, has_opacities, v_xy,
+    hipLaunchKernelGGL(reduce_records_kernel, dim3(ceil_div((size_t)n * kSegLanes, kSegThreads)),
+                       dim3(kSegThreads), 0, st, n, num_intersects, cum_tiles_hit, rec, has_opacities, v_xy,
                        v_xy_abs, v_conic, v_colors, v_opacity, phantom);
     hipLaunchKernelGGL(add_phantom_kernel, dim3(1), dim3(64), 0, st, (const float *)phantom,
                        has_opacities, v_xy, v_xy_abs, v_conic, v_colors, v_opacity);
