@@ -111,17 +111,93 @@ def summarize_pmc(db_path: str, pattern: str = "") -> str:
     return "\n".join(lines)
 
 
+def _pmc_means(path: str) -> dict:
+    """{(kernel short name, launch threads, counter): (mean per dispatch,
+    dispatches)} over every *_results.db under `path`."""
+    import glob
+    import sqlite3
+    out = {}
+    dbs = [path] if path.endswith(".db") else sorted(
+        glob.glob(f"{path}/**/*_results.db", recursive=True))
+    for db in dbs:
+        con = sqlite3.connect(db)
+        cols = [r[1] for r in con.execute("pragma table_info(counters_collection)")]
+        name_col = "kernel_name" if "kernel_name" in cols else "name"
+        grid_col = "grid_size" if "grid_size" in cols else "grid_x"
+        groups = {}
+        for name, grid, cname, val in con.execute(
+                f"select {name_col}, {grid_col}, counter_name, value from counters_collection"):
+            groups.setdefault((_short(name), int(grid), cname), []).append(float(val))
+        for k, v in groups.items():
+            out[k] = (sum(v) / len(v), len(v))
+    return out
+
+
+# kernels whose HBM traffic bench.py reports: (name prefix, launch threads)
+TRAFFIC_KERNELS = {
+    "raster_bwd": ("raster_bwd16_kernel<false, true, true>", 16777216,
+                   "fit-step backward, 8192^2"),
+    "raster_fwd": ("raster_fwd16_kernel<false, true", 16777216, "fit-step forward, 8192^2"),
+}
+
+
+def traffic(fetch: str, write: str, copy_fetch: str, copy_write: str, copy_bytes: int,
+            workload: str, commands: list) -> dict:
+    """HBM bytes per launch of TRAFFIC_KERNELS from FETCH_SIZE and WRITE_SIZE
+    runs (KiB per dispatch), each counter scaled by what it reads for a
+    device copy of `copy_bytes` (copy-probe)."""
+    cf = max(v[0] for (k, g, c), v in _pmc_means(copy_fetch).items() if c == "FETCH_SIZE")
+    cw = max(v[0] for (k, g, c), v in _pmc_means(copy_write).items() if c == "WRITE_SIZE")
+    fs, ws = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
+    fm, wm = _pmc_means(fetch), _pmc_means(write)
+    rec = {"workload": workload,
+           "calibration": {"copy_bytes": copy_bytes, "FETCH_SIZE_KiB": round(cf, 1),
+                           "WRITE_SIZE_KiB": round(cw, 1), "fetch_scale": round(fs, 5),
+                           "write_scale": round(ws, 5)},
+           "commands": commands, "kernels": {}}
+    for key, (prefix, threads, what) in TRAFFIC_KERNELS.items():
+        # the instance launched most often: the fit steps' (the autotuner's
+        # trial depths run once or twice each)
+        f = sorted(((v[1], k, v[0]) for (k, g, c), v in fm.items()
+                    if k.startswith(prefix) and g == threads and c == "FETCH_SIZE"),
+                   reverse=True)
+        if not f:
+            continue
+        name = f[0][1]
+        w = [v[0] for (k, g, c), v in wm.items()
+             if k == name and g == threads and c == "WRITE_SIZE"]
+        if not w:
+            continue
+        rec["kernels"][key] = {
+            "kernel": f"{name} ({what})", "launch_threads": threads, "dispatches": f[0][0],
+            "FETCH_SIZE_KiB": round(f[0][2], 1), "WRITE_SIZE_KiB": round(w[0], 1),
+            "traffic_bytes": int(round(f[0][2] * 1024 * fs + w[0] * 1024 * ws))}
+    return rec
+
+
 if __name__ == "__main__":
     import argparse
     import glob
     ap = argparse.ArgumentParser(description="rocprofv3 helpers")
-    ap.add_argument("cmd", choices=["summarize", "pmc", "copy-probe"])
+    ap.add_argument("cmd", choices=["summarize", "pmc", "copy-probe", "traffic"])
     ap.add_argument("path", nargs="?", help="rocprofv3 output directory or *_results.db file")
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--match", default="")
+    ap.add_argument("--fetch", help="traffic: FETCH_SIZE run directory")
+    ap.add_argument("--write", help="traffic: WRITE_SIZE run directory")
+    ap.add_argument("--copy-fetch", help="traffic: copy-probe FETCH_SIZE run directory")
+    ap.add_argument("--copy-write", help="traffic: copy-probe WRITE_SIZE run directory")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--command", action="append", default=[],
+                    help="traffic: a command that produced the runs (repeatable)")
     a = ap.parse_args()
     if a.cmd == "copy-probe":
         copy_probe()
+    elif a.cmd == "traffic":
+        import json
+        print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes,
+                                 a.workload, a.command), indent=1))
     else:
         dbs = [a.path] if a.path.endswith(".db") else sorted(
             glob.glob(f"{a.path}/**/*_results.db", recursive=True))
