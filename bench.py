@@ -10,7 +10,8 @@ render at 8192^2 with 2M Gaussians.  Cholesky diagonal ~U(0.5, 3) * 7.2 px
 loss, Adam lr 1e-3.  One step = one optimiser step of level 1 plus one of
 level 2; value = steps per second.
 
-Also reported: render Gpix/s (level-2 forward: project + bin + rasterise),
+Also reported: render Gpix/s of the level-2 representation (image-only
+renderer: project + bin + rasterise; the reference API's forward beside it),
 the dominant kernel's roofline (HIP events around its C-ABI call on the
 stream it runs on; algorithmic FLOPs from exact pair counts), and the CPU
 oracle timed on a bounded crop of the same workload.
@@ -331,16 +332,26 @@ def main():
     steps_per_s = args.steps / elapsed
     log(f"timed {args.steps} steps in {elapsed:.3f}s")
 
-    # render throughput: level-2 forward only
-    with torch.no_grad():
-        for _ in range(2):
-            l2.render()
-        torch.cuda.synchronize()
-        r0 = time.perf_counter()
-        for _ in range(args.render_reps):
-            l2.render()
-        torch.cuda.synchronize()
-        render_s = (time.perf_counter() - r0) / args.render_reps
+    # render throughput of the level-2 representation: the image-only renderer
+    # (lig_amd.render.render_view: view projection, binning, image-only raster)
+    # and, beside it, the reference API's forward (rasterize_gaussians, which
+    # also writes the weight sum and final index a backward would need)
+    from lig_amd import render as R
+
+    def render_time(fn):
+        with torch.no_grad():
+            for _ in range(2):
+                fn()
+            torch.cuda.synchronize()
+            r0 = time.perf_counter()
+            for _ in range(args.render_reps):
+                fn()
+            torch.cuda.synchronize()
+        return (time.perf_counter() - r0) / args.render_reps
+
+    lp2 = R.LevelParams.of(l2)
+    render_s = render_time(lambda: R.render_view(lp2, 0.0, 0.0, l2.W, l2.H))
+    render_api_s = render_time(l2.render)
     counts = {"l1": level_counts(l1, C, torch), "l2": level_counts(l2, C, torch)}
     k2 = counts["l2"]
     I, T, P, E, Cc = k2["I"], k2["T"], k2["P"], k2["E"], k2["C"]
@@ -421,6 +432,7 @@ def main():
         "level_ms": {k: round(v / args.steps, 3) for k, v in lvl_ms.items()},
         "render_gpix_s": round(P / render_s / 1e9, 4),
         "render_ms": round(render_s * 1e3, 3),
+        "render_api_gpix_s": round(P / render_api_s / 1e9, 4),
         "workload_counts": counts,
         "kernel_ms_l2": {k: round(v["ms"], 3) for k, v in kernels.items()},
         "roofline": roofline,

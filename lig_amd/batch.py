@@ -182,6 +182,29 @@ def make_targets(sizes, seeds, device):
     return [make_image(h, w, s, device=device) for (h, w), s in zip(sizes, seeds)]
 
 
+class LevelBatch:
+    """A single-image fit (lig_amd.fit.Level and its target) seen as a batch of
+    one image: it shares the Level's parameter tensors and optimiser, so a
+    BatchedStepGraph over it steps the Level itself.  A batch of one computes
+    exactly what the Level's own fused step computes."""
+
+    def __init__(self, level, target_hwc: torch.Tensor):
+        dev = level.means.device
+        if tuple(target_hwc.shape) != (level.H, level.W, 3):
+            raise ValueError(f"target {tuple(target_hwc.shape)} does not match the level "
+                             f"({level.H}, {level.W}, 3)")
+        self.means, self.cholesky, self.rgbs = level.means, level.cholesky, level.rgbs
+        self.optimizer = level.optimizer
+        self.sizes = [(level.H, level.W)]
+        self.counts = [int(level.means.shape[0])]
+        tab, self.num_tiles, self.num_pixels = batch_table(self.sizes, self.counts)
+        self.table = torch.from_numpy(tab.copy()).to(dev)
+        self.target = target_hwc.reshape(-1, 3).contiguous()
+        self.batch_depth = level.batch_depth
+        self.persistent = False
+        self.last_losses = None
+
+
 class BatchedStepGraph:
     """BatchedFit's fit step recorded once as a hipGraph and replayed.
 

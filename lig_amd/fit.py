@@ -114,6 +114,16 @@ class Level(nn.Module):
                                   self.W, self.batch_depth)
         return l2_loss_clamped(self.render(), target_hwc)
 
+    def graph(self, target_hwc: torch.Tensor, headroom: float = 1.05):
+        """The fused step against `target_hwc` recorded once as a hipGraph
+        (lig_amd.batch.BatchedStepGraph over this level as a batch of one):
+        graph.run(k) takes k steps with the bits of k step() calls and no host
+        read of the intersection count."""
+        from .batch import BatchedStepGraph, LevelBatch
+        if not self.fused:
+            raise ValueError("the captured step is the fused (16 x 16 tile) step")
+        return BatchedStepGraph(LevelBatch(self, target_hwc), headroom=headroom)
+
     def step(self, target_hwc: torch.Tensor) -> torch.Tensor:
         """One optimiser step against an [H, W, 3] target; returns the loss
         tensor (no host synchronisation beyond the binning's count)."""

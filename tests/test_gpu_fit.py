@@ -140,3 +140,29 @@ def test_batch_depth_changes_speed_not_bits(device, tmp_path, monkeypatch):
     res = autotune.tune_level(lv, tgt, reps=2, use_cache=False)
     assert res["choice"] in (64, 128, 256) and lv.batch_depth == res["choice"]
     assert set(res["ms"]) == {64, 128, 256}
+
+
+def test_level_graph_matches_eager_steps(device):
+    """Level.graph(): the fused step replayed as a hipGraph (a batch of one)
+    gives the parameters and optimiser state of the same number of eager
+    Level.step calls, bit for bit, with eager steps interleaved."""
+    from lig_amd import data, fit
+    H, W = 96, 136
+    tgt = data.make_image(H, W, 5, device=device)
+    a = fit.Level(900, H, W, 7, 1.5, lr=1e-2, device=device)
+    b = fit.Level(900, H, W, 7, 1.5, lr=1e-2, device=device)
+    gb = b.graph(tgt)
+    for _ in range(3):
+        a.step(tgt)
+    gb.run(3)
+    a.step(tgt)
+    b.step(tgt)
+    for _ in range(2):
+        a.step(tgt)
+    gb.run(2)
+    for name in ("means", "cholesky", "rgbs"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+        pa, pb = getattr(a, name), getattr(b, name)
+        for key in ("exp_avg", "exp_avg_sq"):
+            assert torch.equal(a.optimizer.state[pa][key], b.optimizer.state[pb][key])
+        assert a.optimizer.state[pa]["step"] == b.optimizer.state[pb]["step"] == 6
