@@ -393,3 +393,44 @@ class BatchedStepGraph:
     @property
     def skipped(self) -> int:
         return self._read_ctl()[1]
+
+
+def _main(argv=None):
+    """Steps the 64-image batched workload (batch_workload) in one launch form,
+    for timing and profiling: python -m lig_amd.batch [--skew-factor F]
+    [--persistent 0|1] [--steps K]."""
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--images", type=int, default=64)
+    ap.add_argument("--skew-factor", type=int, default=1,
+                    help="Gaussian density multiplier of every second one of the first "
+                         "--skew-images images")
+    ap.add_argument("--skew-images", type=int, default=16)
+    ap.add_argument("--persistent", type=int, default=0)
+    ap.add_argument("--depth", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    a = ap.parse_args(argv)
+    dev = torch.device("cuda:0")
+    sizes, counts, seeds = batch_workload(a.images, seed=1000, gauss_per_px=0.05)
+    counts = [c * a.skew_factor if i < a.skew_images and i % 2 == 0 else c
+              for i, c in enumerate(counts)]
+    bf = BatchedFit(make_targets(sizes, seeds, dev), counts, seeds, scale=2.0)
+    bf.batch_depth, bf.persistent = a.depth, bool(a.persistent)
+    for _ in range(a.warmup):
+        bf.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        bf.step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / a.steps
+    print(json.dumps({"images": a.images, "gaussians": int(sum(counts)),
+                      "persistent": bf.persistent, "depth": bf.batch_depth,
+                      "ms_per_step": round(dt * 1e3, 3), "steps_s": round(1 / dt, 3)}))
+
+
+if __name__ == "__main__":
+    _main()
