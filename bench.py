@@ -249,6 +249,21 @@ def serve_run(args, dev, torch):
                 "workload": "32 two-level reps, 4096^2-8192^2 canvases, 0.5M-3M level-2 "
                             "Gaussians (+ half at level 1); crops 256^2-4096^2 log-uniform, "
                             "scale 1 or 0.5, Poisson arrivals"})
+    # load curve: the same request stream offered faster; and one slot (a
+    # plain FIFO queue, no overlap of requests) at the headline rate
+    curve = []
+    for q in [float(x) for x in args.serve_load_qps.split(",") if x]:
+        st = serve.CropServer(reps, slots=4, hbm_budget=32 << 30,
+                              device=dev).run_poisson(reqs, qps=q, seed=3000)
+        curve.append(dict(offered_qps=q, **{k: v for k, v in st.summary().items()
+                                           if k in ("p50_ms", "p99_ms", "achieved_qps",
+                                                    "gpix_s")}))
+    res["load_curve"] = curve
+    st = serve.CropServer(reps, slots=1, hbm_budget=32 << 30,
+                          device=dev).run_poisson(reqs, qps=args.serve_qps, seed=3000)
+    res["single_slot"] = dict(offered_qps=args.serve_qps, **{
+        k: v for k, v in st.summary().items() if k in ("p50_ms", "p99_ms", "achieved_qps",
+                                                         "gpix_s")})
     log(f"serve {res}")
     return res
 
@@ -265,6 +280,8 @@ def main():
                     help="crop requests in the serving run (0 skips it)")
     ap.add_argument("--serve-qps", type=float, default=150.0)
     ap.add_argument("--serve-reps", type=int, default=32)
+    ap.add_argument("--serve-load-qps", default="300,600",
+                    help="further offered rates of the serving run (comma list; empty skips)")
     ap.add_argument("--batch-steps", type=int, default=5,
                     help="timed steps of the batched-fitting comparison (0 skips it)")
     args = ap.parse_args()
