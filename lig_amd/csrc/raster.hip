@@ -243,12 +243,20 @@ __device__ __forceinline__ void fwd16_tile(
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
     float *__restrict__ out_img, float *__restrict__ out_wsum,
     int32_t *__restrict__ final_idx, const float *__restrict__ target,
-    float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg, float4 *s0,
-    float4 *s1, float2 *s2) {
+    float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg,
+    const int32_t *__restrict__ cum, float4 *s0, float4 *s1, float2 *s2) {
     // gtile: bins / partials index
     int tile = gtile;  // tile of its own image
+    float init = 0.0f;
     if (tab) {  // batched fit: the tile's image, its canvas and pixel buffers
-        const BatchImage &im = tab[batch_find<0>(tab, nimg, gtile)];
+        const int b = batch_find<0>(tab, nimg, gtile);
+        const BatchImage &im = tab[b];
+        if (cum) {
+            // an image none of whose Gaussians meets a tile renders ones, as
+            // the same image fitted alone (rasterize.py:80-90)
+            const int g0 = im.gauss_off, g1 = tab[b + 1].gauss_off;
+            if ((g1 > 0 ? cum[g1 - 1] : 0) == (g0 > 0 ? cum[g0 - 1] : 0)) init = 1.0f;
+        }
         tile = gtile - im.tile_off;
         tiles_x = im.tiles_x;
         img_w = im.w;
@@ -269,7 +277,8 @@ __device__ __forceinline__ void fwd16_tile(
     int fin[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        ar[q] = ag[q] = ab[q] = aw[q] = 0.0f;
+        ar[q] = ag[q] = ab[q] = init;
+        aw[q] = 0.0f;
         fin[q] = 0;
     }
 
@@ -370,13 +379,14 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
     float *__restrict__ out_img, float *__restrict__ out_wsum,
     int32_t *__restrict__ final_idx, const float *__restrict__ target,
-    float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg) {
+    float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg,
+    const int32_t *__restrict__ cum) {
     __shared__ float4 s0[DEPTH];
     __shared__ float4 s1[DEPTH];
     __shared__ float2 s2[DEPTH];  // (blue, quadrant mask bits)
     fwd16_tile<HAS_OPAC, TRAIN, DEPTH>((int)xcd_remap(blockIdx.x, gridDim.x), tiles_x, img_w,
                                        img_h, gids, bins, pack, out_img, out_wsum, final_idx,
-                                       target, loss_partial, tab, nimg, s0, s1, s2);
+                                       target, loss_partial, tab, nimg, cum, s0, s1, s2);
 }
 
 // Persistent form: each wave takes tiles from the per-XCD queue until none is
@@ -388,7 +398,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_queue_kernel(
     float *__restrict__ out_img, float *__restrict__ out_wsum,
     int32_t *__restrict__ final_idx, const float *__restrict__ target,
     float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg,
-    TileQueue queue) {
+    const int32_t *__restrict__ cum, TileQueue queue) {
     __shared__ float4 s0[DEPTH];
     __shared__ float4 s1[DEPTH];
     __shared__ float2 s2[DEPTH];
@@ -399,7 +409,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_queue_kernel(
         for (int k = 0; k < cnt; ++k)
             fwd16_tile<HAS_OPAC, TRAIN, DEPTH>(t + k, tiles_x, img_w, img_h, gids, bins, pack,
                                                out_img, out_wsum, final_idx, target,
-                                               loss_partial, tab, nimg, s0, s1, s2);
+                                               loss_partial, tab, nimg, cum, s0, s1, s2);
         t = queue_resolve(queue, pending, cnt);
     }
 }
@@ -1266,19 +1276,19 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
     else if (image_only && opacities)
         hipLaunchKernelGGL((raster_fwd16_kernel<true, true>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr);
     else if (image_only)
         hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr);
     else if (block_width == 16 && opacities)
         hipLaunchKernelGGL((raster_fwd16_kernel<true, false>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0);
+                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0, nullptr);
     else if (block_width == 16)
         hipLaunchKernelGGL((raster_fwd16_kernel<false, false>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0);
+                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0, nullptr);
     else
         launch_fwd<false>(px, grid, st, tiles_x, block_width, img_width, img_height,
                           gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, nullptr,
@@ -1311,17 +1321,18 @@ unsigned persistent_grid(KernelT kernel, int ntiles) {
 void launch_fit_forward(int depth, dim3 grid, hipStream_t st, int tiles_x, int img_w, int img_h,
                         const int32_t *gids, const int2 *bins, const GaussPack *pk,
                         float *out_img, const float *target, float *partial,
-                        const BatchImage *tab, int nimg, TileQueue queue = TileQueue{nullptr, 0}) {
+                        const BatchImage *tab, int nimg, const int32_t *cum = nullptr,
+                        TileQueue queue = TileQueue{nullptr, 0}) {
 #define LIG_FIT_FWD(D)                                                                        \
     if (queue.ctr) {                                                                          \
         auto kern = raster_fwd16_queue_kernel<false, true, D>;                                \
         hipLaunchKernelGGL(kern, dim3(persistent_grid(kern, queue.ntiles)), dim3(kThreads), 0, \
                            st, tiles_x, img_w, img_h, gids, bins, pk, out_img, nullptr,        \
-                           nullptr, target, partial, tab, nimg, queue);                       \
+                           nullptr, target, partial, tab, nimg, cum, queue);                  \
     } else {                                                                                  \
         hipLaunchKernelGGL((raster_fwd16_kernel<false, true, D>), grid, dim3(kThreads), 0, st, \
                            tiles_x, img_w, img_h, gids, bins, pk, out_img, nullptr, nullptr,   \
-                           target, partial, tab, nimg);                                        \
+                           target, partial, tab, nimg, cum);                                   \
     }
     if (depth == 256) {
         LIG_FIT_FWD(256)
@@ -1414,7 +1425,7 @@ int fit_render_batched(int n, const int32_t *gaussian_ids_sorted, const int32_t 
     }
     launch_fit_forward(batch_depth, dim3((unsigned)num_tiles), st, 1, 1, 1, gaussian_ids_sorted,
                        reinterpret_cast<const int2 *>(tile_bins), pk, out_img, target, partial,
-                       tab, num_images, q);
+                       tab, num_images, cum_tiles_hit, q);
     hipLaunchKernelGGL(image_loss_kernel, dim3((unsigned)num_images), dim3(kSumThreads), 0, st,
                        tab, (const float *)partial, loss_per_image);
     hipLaunchKernelGGL(total_loss_kernel, dim3(1), dim3(kSumThreads), 0, st, num_images,

@@ -143,3 +143,35 @@ def test_autotune_batched_picks_form_and_depth(device, tmp_path, monkeypatch):
     assert again["cached"] and again["choice"] == res["choice"]
     assert again["persistent"] == res["persistent"]
     bf.step()   # the tuned configuration runs
+
+
+def test_batched_images_without_intersections(device):
+    """An image with no Gaussians and one whose Gaussians all lie off its
+    canvas render ones, as each does fitted alone (rasterize.py:80-90), and
+    the other images are unaffected: losses and parameters match the plain
+    per-image fits."""
+    from lig_amd.batch import BatchedFit, make_targets
+    from lig_amd.fit import Level
+    sizes = [(40, 56), (33, 47), (64, 64), (20, 31)]
+    counts = [200, 0, 150, 80]
+    seeds = [21, 22, 23, 24]
+    targets = make_targets(sizes, seeds, device)
+    bf = BatchedFit(targets, counts, seeds, scale=1.5, lr=1e-2)
+    levels = [Level(c, h, w, s, 1.5, lr=1e-2, device=device)
+              for (h, w), c, s in zip(sizes, counts, seeds)]
+    with torch.no_grad():
+        bf.means[200:350] += 1.0e4
+        levels[2].means += 1.0e4
+    for step in range(2):
+        bf.step()
+        plain = [float(lv.step(t)) for lv, t in zip(levels, targets)]
+        for i, (a, b) in enumerate(zip(bf.last_losses.tolist(), plain)):
+            assert a == pytest.approx(b, rel=1e-6), (step, i)
+    for i in (1, 2):
+        want = float(((1.0 - targets[i]) ** 2).mean())
+        assert bf.last_losses[i].item() == pytest.approx(want, rel=1e-6)
+    o = 0
+    for i, (lv, c) in enumerate(zip(levels, counts)):
+        for name in ("means", "cholesky", "rgbs"):
+            assert torch.equal(getattr(bf, name)[o:o + c], getattr(lv, name)), (i, name)
+        o += c
