@@ -409,7 +409,7 @@ def main():
         import oracle
         oracle.build()
         threads = oracle.num_threads()
-        crops = {"l1": 512, "l2": 1024}
+        crops = {"l1": 1024, "l2": 2048}   # about 10 s of CPU work in all
         est = 0.0
         parts = []
         for name, lev, tgt in (("l1", l1, t1), ("l2", l2, t2)):
