@@ -318,8 +318,17 @@ def main():
     torch.cuda.synchronize()
     log(f"setup {time.perf_counter() - t0:.1f}s")
     from lig_amd import autotune
-    tuned = {"l1": autotune.tune_level(l1, t1, use_cache=False),
-             "l2": autotune.tune_level(l2, t2, use_cache=False)}
+    # tile size (16 fused / 8 through the API path) and LDS depth per level,
+    # timed on forward + backward
+    tuned = {}
+    for name, lv, tt in (("l1", l1, t1), ("l2", l2, t2)):
+        res = autotune.tune_level_tiles(lv, tt, use_cache=False)
+        if not lv.fused:
+            # the kernel roofline below times the fused 16 x 16 entry points
+            best16 = min((k for k in res["ms"] if k.startswith("16/")), key=res["ms"].get)
+            lv.fused, lv.block_width, lv.batch_depth = True, 16, int(best16.split("/")[1])
+            res["measured_with"] = best16
+        tuned[name] = res
     log(f"autotune {tuned}")
 
     for i in range(args.warmup):
@@ -455,7 +464,8 @@ def main():
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
-        "autotune": {k: {"batch_depth": v["choice"], "ms": v["ms"]} for k, v in tuned.items()},
+        "autotune": {k: {"tile/depth": v.get("measured_with", v["choice"]), "ms": v["ms"]}
+                     for k, v in tuned.items()},
         "batched": batched,
         "serve": serve,
     }

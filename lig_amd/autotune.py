@@ -13,9 +13,14 @@ For batched fitting the launch form is tuned too: one workgroup per tile
 fed by per-XCD tile queues with stealing (lig_fit_render_l2_batched_persistent),
 which wins when some images' tiles cost much more than others'.
 
-Tile size is not a knob here: the fused fit kernels are written for 16 x 16
-tiles (the reference API caps block_width at 16, rasterize.py:26); other tile
-widths run through the gsplat2d API path.
+Tile size: the fused fit kernels are written for 16 x 16 tiles, and the
+reference API caps block_width at 16 (rasterize.py:26), so 32 is not a
+candidate.  `tune_level_tiles` also times 8 x 8 tiles through the gsplat2d API
+path (Level(fused=False)) against the fused 16 x 16 step at every depth, on
+the whole forward + backward.  Images do not depend on the tile size (a
+pixel's sum runs over the same Gaussians in gid order); gradients are summed
+over a different number of per-tile partials, so their last bits do.  The
+batched kernels are 16 x 16 only.
 """
 from __future__ import annotations
 
@@ -128,3 +133,51 @@ def tune_batched(bf, reps: int = 3, use_cache: bool = True) -> dict:
     _cache[key] = [bf.batch_depth, bf.persistent]
     _store()
     return {"choice": bf.batch_depth, "persistent": bf.persistent, "ms": ms, "cached": False}
+
+
+def tune_level_tiles(level, target: torch.Tensor, reps: int = 2,
+                     use_cache: bool = True) -> dict:
+    """Pick the tile size and LDS depth of `level`'s step: the fused 16 x 16
+    step at each depth and 8 x 8 tiles through the API path, each timed with
+    HIP events over loss + backward (gradients cleared after, no optimiser
+    step, so the parameters do not change).  Sets level.fused, .block_width
+    and .batch_depth; returns {"choice": {...}, "ms": {candidate: ms}}."""
+    key = f"tiles:{level.H}x{level.W}:n{level.means.shape[0]}"
+    cands = {f"16/{d}": (True, 16, d) for d in DEPTHS}
+    cands["8/api"] = (False, 8, 0)
+
+    def apply(c):
+        level.fused, level.block_width, level.batch_depth = c
+
+    if use_cache:
+        if not _cache:
+            _load()
+        hit = _cache.get(key)
+        if isinstance(hit, str) and hit in cands:
+            apply(cands[hit])
+            return {"choice": hit, "ms": None, "cached": True}
+    dev = level.means.device
+    ms = {}
+    for name, c in cands.items():
+        apply(c)
+
+        def run():
+            loss = level.loss(target)
+            loss.backward()
+            for p in (level.means, level.cholesky, level.rgbs):
+                p.grad = None
+
+        run()   # warm-up
+        torch.cuda.synchronize(dev)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        e1.synchronize()
+        ms[name] = round(e0.elapsed_time(e1) / reps, 4)
+    choice = min(ms, key=ms.get)
+    apply(cands[choice])
+    _cache[key] = choice
+    _store()
+    return {"choice": choice, "ms": ms, "cached": False}

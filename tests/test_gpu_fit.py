@@ -166,3 +166,37 @@ def test_level_graph_matches_eager_steps(device):
         for key in ("exp_avg", "exp_avg_sq"):
             assert torch.equal(a.optimizer.state[pa][key], b.optimizer.state[pb][key])
         assert a.optimizer.state[pa]["step"] == b.optimizer.state[pb]["step"] == 6
+
+
+def test_tile_size_tuning(device, tmp_path, monkeypatch):
+    """tune_level_tiles times 16 x 16 (fused, each depth) and 8 x 8 (API path)
+    steps and keeps the fastest; the image does not depend on the tile size
+    and the gradients agree to FP32 summation order."""
+    from lig_amd import autotune, data, fit
+    monkeypatch.setenv("LIG_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_cache", {})
+    H, W = 96, 112
+    tgt = data.make_image(H, W, 4, device=device)
+    grads = []
+    for fused, bw in ((True, 16), (False, 8)):
+        lv = fit.Level(3000, H, W, 6, 1.5, device=device, block_width=bw, fused=fused)
+        assert lv.fused == fused
+        with torch.no_grad():
+            img = lv.render()
+        loss = lv.loss(tgt)
+        loss.backward()
+        grads.append((img, [p.grad.clone() for p in (lv.means, lv.cholesky, lv.rgbs)]))
+    assert torch.equal(grads[0][0], grads[1][0])
+    for a, b in zip(grads[0][1], grads[1][1]):
+        torch.testing.assert_close(a, b, rtol=1e-4, atol=1e-6 * float(b.abs().max()))
+    lv = fit.Level(3000, H, W, 6, 1.5, device=device)
+    before = [p.detach().clone() for p in (lv.means, lv.cholesky, lv.rgbs)]
+    res = autotune.tune_level_tiles(lv, tgt, reps=1)
+    assert set(res["ms"]) == {"16/64", "16/128", "16/256", "8/api"}
+    assert res["ms"][res["choice"]] == min(res["ms"].values())
+    assert (lv.block_width, lv.fused) == ((8, False) if res["choice"] == "8/api" else (16, True))
+    for a, b in zip(before, (lv.means, lv.cholesky, lv.rgbs)):
+        assert torch.equal(a, b)   # timing runs take no optimiser step
+    again = autotune.tune_level_tiles(lv, tgt, reps=1)
+    assert again["cached"] and again["choice"] == res["choice"]
+    lv.step(tgt)
