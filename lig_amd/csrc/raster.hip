@@ -1080,10 +1080,11 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
 constexpr int kSumThreads = 256;
 constexpr int kSumChunk = kSumThreads * 16;
 
+template <int THREADS = kSumThreads>
 __device__ __forceinline__ double block_sum_f64(double v, double *ws) {
     ws[threadIdx.x] = v;
     __syncthreads();
-    for (int s = kSumThreads / 2; s > 0; s >>= 1) {
+    for (int s = THREADS / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) ws[threadIdx.x] += ws[threadIdx.x + s];
         __syncthreads();
     }
@@ -1117,15 +1118,36 @@ __global__ __launch_bounds__(kSumThreads) void sum_blocks_kernel(int nblocks,
 
 // Batched fit losses: image b's mean over its 3 w h values of the per-tile
 // clamp + L2 sums of its tile range (FP64, fixed order), then their total.
-__global__ __launch_bounds__(kSumThreads) void image_loss_kernel(
+// One block per image.  Thread i sums the image's tile partials i, i + 1024,
+// ... into eight FP64 accumulators (k mod 8), eight loads in flight per step,
+// and adds them in a fixed order; then a fixed block tree.  A 262 K-tile
+// image takes tens of microseconds instead of one latency-bound chain.
+constexpr int kImgLossThreads = 1024;
+constexpr int kImgLossLanes = 8;
+
+__global__ __launch_bounds__(kImgLossThreads) void image_loss_kernel(
     const BatchImage *__restrict__ tab, const float *__restrict__ partial,
     float *__restrict__ loss_img) {
-    __shared__ double ws[kSumThreads];
+    __shared__ double ws[kImgLossThreads];
     const BatchImage im = tab[blockIdx.x];
     const int t1 = tab[blockIdx.x + 1].tile_off;
-    double acc = 0.0;
-    for (int t = im.tile_off + (int)threadIdx.x; t < t1; t += kSumThreads) acc += partial[t];
-    const double sum = block_sum_f64(acc, ws);
+    double acc[kImgLossLanes];
+#pragma unroll
+    for (int j = 0; j < kImgLossLanes; ++j) acc[j] = 0.0;
+    constexpr int kStep = kImgLossThreads * kImgLossLanes;
+    for (int t0 = im.tile_off + (int)threadIdx.x; t0 < t1; t0 += kStep) {
+        float v[kImgLossLanes];
+#pragma unroll
+        for (int j = 0; j < kImgLossLanes; ++j) {
+            const int t = t0 + j * kImgLossThreads;
+            v[j] = t < t1 ? partial[t] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < kImgLossLanes; ++j) acc[j] += (double)v[j];
+    }
+    const double mine = ((acc[0] + acc[1]) + (acc[2] + acc[3])) +
+                        ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    const double sum = block_sum_f64<kImgLossThreads>(mine, ws);
     if (threadIdx.x == 0) loss_img[blockIdx.x] = (float)(sum / (3.0 * (double)im.h * im.w));
 }
 
@@ -1426,7 +1448,7 @@ int fit_render_batched(int n, const int32_t *gaussian_ids_sorted, const int32_t 
     launch_fit_forward(batch_depth, dim3((unsigned)num_tiles), st, 1, 1, 1, gaussian_ids_sorted,
                        reinterpret_cast<const int2 *>(tile_bins), pk, out_img, target, partial,
                        tab, num_images, cum_tiles_hit, q);
-    hipLaunchKernelGGL(image_loss_kernel, dim3((unsigned)num_images), dim3(kSumThreads), 0, st,
+    hipLaunchKernelGGL(image_loss_kernel, dim3((unsigned)num_images), dim3(kImgLossThreads), 0, st,
                        tab, (const float *)partial, loss_per_image);
     hipLaunchKernelGGL(total_loss_kernel, dim3(1), dim3(kSumThreads), 0, st, num_images,
                        (const float *)loss_per_image, loss);
