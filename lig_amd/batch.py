@@ -220,9 +220,10 @@ class BatchedStepGraph:
     again and takes the remaining steps, so the parameters after `run(k)` are
     bit-identical to k calls of BatchedFit.step.
 
-    The optimiser's lr, betas, eps and weight decay are recorded in the graph:
-    build a new BatchedStepGraph after changing them.  Eager BatchedFit.step
-    calls may be interleaved with `run` (the step counter is kept in sync).
+    The optimiser's lr, betas, eps and weight decay, and the model's
+    batch_depth and persistent settings, are recorded in the graph: build a new
+    BatchedStepGraph after changing them.  Eager BatchedFit.step calls may be
+    interleaved with `run` (the step counter is kept in sync).
     """
 
     def __init__(self, bf: BatchedFit, capacity: int = None, headroom: float = 1.25,
