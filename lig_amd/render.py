@@ -55,15 +55,30 @@ class Projected:
     w: int
 
 
+def _project_scan(p: LevelParams, x0: float, y0: float, w: int, h: int, scale: float):
+    n = p.means.shape[0]
+    xys, ext, con, nt = _C.project_gaussians_view(n, p.cholesky, p.means, None, True, (x0, y0),
+                                                  (scale, scale), h, w, 16)
+    cum = _C.inclusive_cumsum_i32(nt)
+    return xys, ext, con, nt, cum
+
+
+def project_windows(views) -> list:
+    """Projections of several (params, x0, y0, w, h, scale) views, with one
+    device-to-host read for all their intersection counts."""
+    parts = [_project_scan(*v) for v in views]
+    lasts = [c[-1:] if c.numel() else torch.zeros(1, dtype=torch.int32, device=c.device)
+             for *_, c in parts]
+    counts = torch.cat(lasts).cpu().tolist() if parts else []
+    return [Projected(xys, ext, con, nt, cum, int(m), v[4], v[3])
+            for (xys, ext, con, nt, cum), m, v in zip(parts, counts, views)]
+
+
 def project_window(p: LevelParams, x0: float, y0: float, w: int, h: int,
                    scale: float = 1.0) -> Projected:
     """View projection and the inclusive scan of tiles hit (one device-to-host
     read of the intersection count, on the current stream)."""
-    n = p.means.shape[0]
-    xys, ext, con, nt = _C.project_gaussians_view(n, p.cholesky, p.means, None, True, (x0, y0),
-                                                  (scale, scale), h, w, 16)
-    num, cum = g.compute_cumulative_intersects(nt)
-    return Projected(xys, ext, con, nt, cum, int(num), h, w)
+    return project_windows([(p, x0, y0, w, h, scale)])[0]
 
 
 def raster_window(p: LevelParams, q: Projected) -> torch.Tensor:
@@ -154,8 +169,8 @@ class TwoLevel:
         """Projections of a crop's two windows (level-1 source region, level-2
         window), the upscaler ROI and the HBM the rest of the render needs."""
         (rx0, ry0, rx1, ry1), roi = self.l1_region(x0, y0, w, h, scale)
-        q1 = project_window(self.l1, float(rx0), float(ry0), rx1 - rx0, ry1 - ry0, 1.0)
-        q2 = project_window(self.l2, x0, y0, w, h, scale)
+        q1, q2 = project_windows([(self.l1, float(rx0), float(ry0), rx1 - rx0, ry1 - ry0, 1.0),
+                                  (self.l2, x0, y0, w, h, scale)])
         need = (window_bytes(self.l1, q1, True) + window_bytes(self.l2, q2, False) +
                 3 * 12 * w * h)
         return q1, q2, roi, need
