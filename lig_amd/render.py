@@ -17,7 +17,9 @@ path (upscale.py, gradient_aware_upscaler.cu:201-206), plus the level-2 window.
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass
+import threading
+from dataclasses import dataclass, field
+from typing import Optional
 
 import torch
 
@@ -132,15 +134,34 @@ class TwoLevel:
     l2: LevelParams
     lo: float
     hi: float
+    # the level-1 canvas rendered once (image, dx, dy, dxy), shared by every
+    # crop: a crop's level-1 source region is small (few tiles, thousands of
+    # Gaussians each), so rendering it per request left most of the device idle
+    _l1_render: Optional[tuple] = field(default=None, repr=False, compare=False)
+    _l1_lock: threading.Lock = field(default_factory=threading.Lock, repr=False, compare=False)
 
     @property
     def factor(self) -> int:
         return self.l2.W // self.l1.W
 
     @torch.no_grad()
+    def level1_render(self):
+        """(img, dx, dy, dxy) of the whole level-1 canvas, rendered on first use
+        and kept resident (48 bytes per level-1 pixel)."""
+        cached = self._l1_render
+        if cached is None:
+            with self._l1_lock:
+                if self._l1_render is None:
+                    out = render_view_ga(self.l1, 0.0, 0.0, self.l1.W, self.l1.H)
+                    torch.cuda.current_stream(self.l1.means.device).synchronize()
+                    self._l1_render = out
+                cached = self._l1_render
+        return cached
+
+    @torch.no_grad()
     def render_full(self) -> torch.Tensor:
         """The whole image, exactly the composition of fit.two_level_fit."""
-        img1, dx, dy, dxy = render_view_ga(self.l1, 0.0, 0.0, self.l1.W, self.l1.H)
+        img1, dx, dy, dxy = self.level1_render()
         up = torch.clamp(g.gradient_aware_upscale(img1, dx, dy, dxy, self.l2.H, self.l2.W), 0, 1)
         img2 = torch.clamp(render_view(self.l2, 0.0, 0.0, self.l2.W, self.l2.H), 0, 1)
         return compose(img2, up, self.lo, self.hi)
@@ -165,10 +186,17 @@ class TwoLevel:
         roi = (sx0 - rx0, sy0 - ry0, sx0 - rx0 + span_x, sy0 - ry0 + span_y)
         return (rx0, ry0, rx1, ry1), roi
 
-    def crop_plan(self, x0: float, y0: float, w: int, h: int, scale: float = 1.0):
-        """Projections of a crop's two windows (level-1 source region, level-2
-        window), the upscaler ROI and the HBM the rest of the render needs."""
+    def crop_plan(self, x0: float, y0: float, w: int, h: int, scale: float = 1.0,
+                  level1_cache: bool = True):
+        """The level-2 window's projection (and, without the level-1 cache,
+        the level-1 source region's), the upscaler ROI and the HBM the rest of
+        the render needs.  With the cache the ROI is in level-1 canvas
+        coordinates and level 1 is not rendered per request."""
         (rx0, ry0, rx1, ry1), roi = self.l1_region(x0, y0, w, h, scale)
+        if level1_cache:
+            (q2,) = project_windows([(self.l2, x0, y0, w, h, scale)])
+            roi = (roi[0] + rx0, roi[1] + ry0, roi[2] + rx0, roi[3] + ry0)
+            return None, q2, roi, window_bytes(self.l2, q2, False) + 3 * 12 * w * h
         q1, q2 = project_windows([(self.l1, float(rx0), float(ry0), rx1 - rx0, ry1 - ry0, 1.0),
                                   (self.l2, x0, y0, w, h, scale)])
         need = (window_bytes(self.l1, q1, True) + window_bytes(self.l2, q2, False) +
@@ -178,14 +206,14 @@ class TwoLevel:
     @torch.no_grad()
     def finish_crop(self, plan, w: int, h: int) -> torch.Tensor:
         q1, q2, roi, _ = plan
-        img1, dx, dy, dxy = raster_window_ga(self.l1, q1)
+        img1, dx, dy, dxy = self.level1_render() if q1 is None else raster_window_ga(self.l1, q1)
         up = torch.clamp(g.gradient_aware_upscale(img1, dx, dy, dxy, h, w, roi), 0, 1)
         img2 = torch.clamp(raster_window(self.l2, q2), 0, 1)
         return compose(img2, up, self.lo, self.hi)
 
     @torch.no_grad()
-    def render_crop(self, x0: float, y0: float, w: int, h: int,
-                    scale: float = 1.0) -> torch.Tensor:
+    def render_crop(self, x0: float, y0: float, w: int, h: int, scale: float = 1.0,
+                    level1_cache: bool = True) -> torch.Tensor:
         """h x w x 3 window of the composed image at `scale` output pixels per
         full-resolution pixel, in [0, 1]."""
-        return self.finish_crop(self.crop_plan(x0, y0, w, h, scale), w, h)
+        return self.finish_crop(self.crop_plan(x0, y0, w, h, scale, level1_cache), w, h)

@@ -104,10 +104,16 @@ def test_two_level_crop_matches_full(device, scale):
         wins = ((0, 0, 128, 128), (96, 200, 256, 128), (384, 384, 64, 64)) if scale == 1.0 \
             else ((0, 0, 128, 128), (96, 200, 128, 64), (384, 384, 64, 64))
         for x0, y0, w, h in wins:
-            crop = rep.render_crop(float(x0), float(y0), w, h, scale)
             i0, j0 = int(y0 * scale), int(x0 * scale)
             want = full[i0:i0 + h, j0:j0 + w]
-            assert crop.shape == want.shape
-            err = float((crop - want).abs().max())
-            print(f"two-level crop scale={scale} at ({x0},{y0}) {w}x{h}: max err {err:.2e}")
-            assert err <= 1e-5
+            # resident level-1 render (default) and the per-request level-1
+            # source region
+            for cache in (True, False):
+                crop = rep.render_crop(float(x0), float(y0), w, h, scale, level1_cache=cache)
+                assert crop.shape == want.shape
+                err = float((crop - want).abs().max())
+                print(f"two-level crop scale={scale} cache={cache} at ({x0},{y0}) {w}x{h}: "
+                      f"max err {err:.2e}")
+                assert err <= 1e-5
+                if cache:
+                    assert torch.equal(crop, want)
