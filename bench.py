@@ -259,11 +259,15 @@ def serve_run(args, dev, torch):
                                            if k in ("p50_ms", "p99_ms", "achieved_qps",
                                                     "gpix_s")}))
     res["load_curve"] = curve
-    st = serve.CropServer(reps, slots=1, hbm_budget=32 << 30,
-                          device=dev).run_poisson(reqs, qps=args.serve_qps, seed=3000)
-    res["single_slot"] = dict(offered_qps=args.serve_qps, **{
-        k: v for k, v in st.summary().items() if k in ("p50_ms", "p99_ms", "achieved_qps",
-                                                         "gpix_s")})
+    single = []
+    for q in sorted({args.serve_qps, max([args.serve_qps] + [c["offered_qps"] for c in curve
+                                                             if c["offered_qps"] <= 1200])}):
+        st = serve.CropServer(reps, slots=1, hbm_budget=32 << 30,
+                              device=dev).run_poisson(reqs, qps=q, seed=3000)
+        single.append(dict(offered_qps=q, **{
+            k: v for k, v in st.summary().items() if k in ("p50_ms", "p99_ms", "achieved_qps",
+                                                             "gpix_s")}))
+    res["single_slot"] = single
     log(f"serve {res}")
     return res
 
@@ -280,7 +284,7 @@ def main():
                     help="crop requests in the serving run (0 skips it)")
     ap.add_argument("--serve-qps", type=float, default=150.0)
     ap.add_argument("--serve-reps", type=int, default=32)
-    ap.add_argument("--serve-load-qps", default="300,600",
+    ap.add_argument("--serve-load-qps", default="300,600,1200,2400",
                     help="further offered rates of the serving run (comma list; empty skips)")
     ap.add_argument("--batch-steps", type=int, default=5,
                     help="timed steps of the batched-fitting comparison (0 skips it)")
