@@ -176,20 +176,31 @@ def compose(level2_img: torch.Tensor, prev_up: torch.Tensor, lo, hi) -> torch.Te
     return torch.clamp(level2_img * (hi - lo) + prev_up - 0.5 + lo, 0, 1)
 
 
+def fit_level(level: Level, target_hwc: torch.Tensor, steps: int):
+    """`steps` optimiser steps of `level`: replayed as a captured step
+    (Level.graph, the same bits; small images are otherwise bound by the
+    host's launch work) when the level is fused, else eager."""
+    if steps <= 0:
+        return
+    if level.fused:
+        level.graph(target_hwc).run(steps)
+    else:
+        for _ in range(steps):
+            level.step(target_hwc)
+
+
 def two_level_fit(gt_hwc: torch.Tensor, n1: int, n2: int, steps: int, seed: int = 0,
                   scale: float = 1.0, lr: float = 1e-3, factor: int = 4, device="cuda"):
     """Plain two-level fit; returns (level1, level2, final PSNR)."""
     H, W = gt_hwc.shape[:2]
     l1 = Level(n1, H // factor, W // factor, seed, scale, lr, device=device)
     t1 = downsample(gt_hwc, factor)
-    for _ in range(steps):
-        l1.step(t1)
+    fit_level(l1, t1, steps)
     with torch.no_grad():
         up = upscale_render(l1, H, W)
         t2, lo, hi = residual_target(gt_hwc, torch.clamp(up, 0, 1))
     l2 = Level(n2, H, W, seed + 1, scale, lr, device=device)
-    for _ in range(steps):
-        l2.step(t2)
+    fit_level(l2, t2, steps)
     with torch.no_grad():
         out = compose(torch.clamp(l2.render(), 0, 1), torch.clamp(up, 0, 1), lo, hi)
     return l1, l2, psnr(out, gt_hwc)

@@ -200,3 +200,17 @@ def test_tile_size_tuning(device, tmp_path, monkeypatch):
     again = autotune.tune_level_tiles(lv, tgt, reps=1)
     assert again["cached"] and again["choice"] == res["choice"]
     lv.step(tgt)
+
+
+def test_fit_level_captured_equals_eager(device):
+    """fit.fit_level (captured steps, as two_level_fit runs them) leaves the
+    parameters of the same number of eager steps, bit for bit."""
+    from lig_amd import data, fit
+    gt = data.make_image(64, 80, 2, device=device)
+    a = fit.Level(500, 64, 80, 3, 1.0, lr=1e-2, device=device)
+    b = fit.Level(500, 64, 80, 3, 1.0, lr=1e-2, device=device)
+    for _ in range(7):
+        a.step(gt)
+    fit.fit_level(b, gt, 7)
+    for name in ("means", "cholesky", "rgbs"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
