@@ -190,6 +190,19 @@ def batched_run(args, dev, torch):
         p_batch = bf.psnr()
     res["max_abs_dpsnr_db"] = float(np.abs(p_plain - p_batch).max())
     res["mean_psnr_db"] = round(float(p_batch.mean()), 3)
+    # the plain form with each image's step captured (Level.graph): k steps
+    # of every image per call, one image after another
+    graphs = [lv.graph(t) for lv, t in zip(plain, targets)]
+    for gph in graphs:
+        gph.run(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for gph in graphs:
+        gph.run(args.batch_steps)
+    torch.cuda.synchronize()
+    res["plain_graph_steps_s"] = round(args.batch_steps / (time.perf_counter() - t0), 3)
+    res["speedup_vs_plain_graph"] = round(res["batched_steps_s"] / res["plain_graph_steps_s"], 3)
+    del graphs
     # both launch forms of the raster stages (bit-identical): one workgroup per
     # tile, and the persistent grid fed by per-XCD tile queues
     chosen = bf.persistent
