@@ -269,6 +269,16 @@ int lig_fit_reduce_records(int n, int num_intersects, const int32_t *cum_tiles_h
                            const void *workspace, size_t workspace_bytes, float *v_xy,
                            float *v_conic, float *v_colors, void *stream);
 
+/* No reference counterpart (a tuning override).  The 16x16 backward kernels
+ * (API records or atomics, fit records, batched grid launch) may split each
+ * tile's 64-Gaussian batches over several blocks, "shares"; every record has
+ * its own slot, so the results do not depend on the split.  By default
+ * (shares = 0) the library picks the count per launch from the tile count,
+ * the mean pairs per tile and the resident blocks; 1..8 forces that count
+ * for every later launch in the process.  Returns the previous setting, or
+ * LIG_ERR_INVALID_ARGUMENT for a value outside 0..8. */
+int lig_set_backward_shares(int shares);
+
 /* ---- batched fit (many images, one launch per stage) ----------------------- */
 
 /* No reference counterpart.  B images fitted together: `batch_table` is a

@@ -72,6 +72,7 @@ _SIGNATURES = {
                  _vp]),
     "lig_fit_reduce_records": (
         _c_int, [_c_int, _c_int, _vp, _vp, _c_size, _vp, _vp, _vp, _vp]),
+    "lig_set_backward_shares": (_c_int, [_c_int]),
     "lig_adam_step": (
         _c_int, [ctypes.c_longlong, _vp, _vp, _vp, _vp, _c_float, _c_float, _c_float, _c_float,
                  _c_float, _c_int, _vp]),

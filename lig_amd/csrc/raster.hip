@@ -22,6 +22,9 @@
 // Reference behaviour followed (paths relative to the reference repository):
 //   gsplat2d/gsplat2d/cuda/csrc/forward.cu:173-279, backward.cu:8-183,
 //   gradient_aware_rasterizer.cu:118-495, bindings.cu:303-506.
+#include <atomic>
+#include <cmath>
+
 #include "lig_device.h"
 
 namespace lig {
@@ -29,6 +32,7 @@ namespace {
 
 constexpr int kThreads = LIG_WAVE;
 constexpr int kMaxPix = 256;  // 16 x 16
+constexpr int kMaxBwdSplit = 8;  // shares per tile of the 16x16 backward
 
 // (clamp(x, 0, 1) - t)^2: the caller's loss term per channel (gaussianlig.py:95-103,
 // utils.py:21-22).
@@ -824,7 +828,8 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
 // those of the gated path.
 template <bool HAS_OPAC, bool DET, bool TRAIN>
 __device__ __forceinline__ void bwd16_tile(
-    int gtile, int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
+    int gtile, int part, int nparts, int tiles_x, int img_w, int img_h,
+    const int32_t *__restrict__ gids,
     const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
     const GaussPack *__restrict__ pack, const int32_t *__restrict__ final_idx,
     const float *__restrict__ v_out, const float *__restrict__ v_wsum,
@@ -833,9 +838,18 @@ __device__ __forceinline__ void bwd16_tile(
     float *__restrict__ g_xy, float *__restrict__ g_xy_abs, float *__restrict__ g_conic,
     float *__restrict__ g_rgb, float *__restrict__ g_opac, const BatchImage *__restrict__ tab,
     int nimg, float4 *sv, int *sfin) {
-    // gtile: bins index
+    // gtile: bins index; this block takes part `part` of `nparts` equal shares
+    // of the tile's 64-Gaussian batches (each record has its own slot, so the
+    // split changes no result)
     const int lane = threadIdx.x;
-    const int2 range = bins[gtile];
+    int2 range = bins[gtile];
+    if (nparts > 1) {
+        const int nb = (range.y - range.x + kThreads - 1) / kThreads;
+        const int b0 = (int)((long long)part * nb / nparts);
+        const int b1 = (int)((long long)(part + 1) * nb / nparts);
+        range.y = min(range.y, range.x + b1 * kThreads);
+        range.x += b0 * kThreads;
+    }
     if (range.x >= range.y) return;
     int tile = gtile;  // tile of its own image
     if (TRAIN && tab) {  // batched fit: the tile's image, canvas, buffers and loss scale
@@ -926,7 +940,8 @@ __device__ __forceinline__ void bwd16_tile(
     }
 }
 
-// One tile per block: the tile xcd_remap(blockIdx.x).
+// Block xcd_remap(blockIdx.x) = tile * split + part: one share of a tile's
+// batches per block (split = 1: the whole tile).
 template <bool HAS_OPAC, bool DET, bool TRAIN>
 __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
@@ -937,13 +952,15 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     const float *__restrict__ grad_loss, float norm, GradRec *__restrict__ rec,
     float *__restrict__ g_xy, float *__restrict__ g_xy_abs, float *__restrict__ g_conic,
     float *__restrict__ g_rgb, float *__restrict__ g_opac, const BatchImage *__restrict__ tab,
-    int nimg) {
+    int nimg, int split) {
     __shared__ float4 sv[kMaxPix];
     __shared__ int sfin[TRAIN ? 1 : kMaxPix];
-    bwd16_tile<HAS_OPAC, DET, TRAIN>((int)xcd_remap(blockIdx.x, gridDim.x), tiles_x, img_w,
-                                     img_h, gids, slots, bins, pack, final_idx, v_out, v_wsum,
-                                     img, target, grad_loss, norm, rec, g_xy, g_xy_abs, g_conic,
-                                     g_rgb, g_opac, tab, nimg, sv, sfin);
+    const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int gtile = u / split;
+    bwd16_tile<HAS_OPAC, DET, TRAIN>(gtile, u - gtile * split, split, tiles_x, img_w, img_h,
+                                     gids, slots, bins, pack, final_idx, v_out, v_wsum, img,
+                                     target, grad_loss, norm, rec, g_xy, g_xy_abs, g_conic, g_rgb,
+                                     g_opac, tab, nimg, sv, sfin);
 }
 
 // Persistent form of the fit-step backward (batched): each wave takes tiles
@@ -960,7 +977,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_fit_queue_kernel(
     while (t >= 0) {
         const int pending = queue_issue(queue);
         for (int k = 0; k < cnt; ++k) {
-            bwd16_tile<false, true, true>(t + k, 1, 1, 1, gids, nullptr, bins, pack, nullptr,
+            bwd16_tile<false, true, true>(t + k, 0, 1, 1, 1, 1, gids, nullptr, bins, pack, nullptr,
                                           nullptr, nullptr, img, target, grad_loss, 0.0f, rec,
                                           nullptr, nullptr, nullptr, nullptr, nullptr, tab, nimg,
                                           sv, sfin);
@@ -1337,6 +1354,47 @@ unsigned persistent_grid(KernelT kernel, int ntiles) {
     return (unsigned)(g < ntiles ? g : ntiles);
 }
 
+// Shares per tile for the 16x16 backward (any number gives the same
+// records).  With R = the blocks resident at once:
+// * T * s <= R for some s >= 2 (a small level: every share runs at once, so
+//   the deepest tile sets the time): the largest such s up to 8 (512^2 x
+//   20 K Gaussians of 700 pairs per tile: 0.35 -> 0.11 ms);
+// * else a launch of T * s blocks runs in about ceil(T * s / R) rounds of
+//   (mean batches per tile / s + c) batch times, c being a share's fixed cost
+//   (staging the tile's pixel gradients; about 0.75 batch times, measured);
+//   the s that minimises this, keeping at least 1.5 batches per share, when
+//   it saves 3% (2048^2 x 1 M: s = 3, 10% faster; 8192^2 x 2 M: s = 1).
+std::atomic<int> g_bwd_shares{0};  // lig_set_backward_shares: 0 = chosen per launch
+
+template <typename KernelT>
+int bwd16_split(KernelT kernel, int ntiles, long long nisect) {
+    if (const int forced = g_bwd_shares.load(std::memory_order_relaxed)) return forced;
+    if (ntiles <= 0 || nisect <= 0) return 1;
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kThreads, 0);
+    const double resident = (double)(cus > 0 ? cus : 1) * (per > 0 ? per : 1);
+    const int one_round = (int)std::min<double>(kMaxBwdSplit, std::floor(resident / ntiles));
+    if (one_round >= 2) return one_round;
+    const double batches = (double)nisect / ((double)ntiles * kThreads);
+    constexpr double kShareCost = 0.75;
+    auto cost = [&](int sh) {
+        return std::ceil((double)ntiles * sh / resident) * (batches / sh + kShareCost);
+    };
+    int best = 1;
+    double best_cost = cost(1);
+    for (int sh = 2; sh <= kMaxBwdSplit && batches / sh >= 1.5; ++sh) {
+        if ((long long)ntiles * sh > INT32_MAX) break;
+        const double c = cost(sh);
+        if (c < 0.97 * best_cost) {
+            best = sh;
+            best_cost = c;
+        }
+    }
+    return best;
+}
+
 // Fit-mode (image + loss partials) forward at LDS batch depth `depth`; with
 // queue.ctr set, a persistent grid over queue.ntiles tiles (the counters
 // must be zero).
@@ -1485,12 +1543,26 @@ extern "C" int lig_fit_render_l2_batched_persistent(
                               workspace_bytes, stream, true);
 }
 
+extern "C" int lig_set_backward_shares(int shares) {
+    if (shares < 0 || shares > kMaxBwdSplit) return LIG_ERR_INVALID_ARGUMENT;
+    return g_bwd_shares.exchange(shares);
+}
+
 extern "C" size_t lig_rasterize_backward_workspace_size(int num_intersects) {
     return num_intersects > 0 ? (size_t)num_intersects * sizeof(GradRec) + 256 : 256;
 }
 
 namespace lig {
 namespace {
+
+// bwd16_split for the API backward's instantiation.
+int bwd16_split_for(int has_opacities, bool det, int ntiles, long long nisect) {
+    if (has_opacities)
+        return det ? bwd16_split(raster_bwd16_kernel<true, true, false>, ntiles, nisect)
+                   : bwd16_split(raster_bwd16_kernel<true, false, false>, ntiles, nisect);
+    return det ? bwd16_split(raster_bwd16_kernel<false, true, false>, ntiles, nisect)
+               : bwd16_split(raster_bwd16_kernel<false, false, false>, ntiles, nisect);
+}
 
 // Launches the tile kernel of the backward: records (det) or atomics.
 int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
@@ -1499,12 +1571,17 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
                           const float *v_output_dx, const float *v_output_dy,
                           const float *v_output_dxy, int has_opacities, const int32_t *slots,
                           GradRec *rec, float *v_xy, float *v_xy_abs, float *v_conic,
-                          float *v_colors, float *v_opacity, hipStream_t st) {
+                          float *v_colors, float *v_opacity, long long num_intersects,
+                          hipStream_t st) {
     const bool ga = v_output_dx != nullptr;
     const bool det = rec != nullptr;
     const int tiles_x = (img_width + block_width - 1) / block_width;
     const int tiles_y = (img_height + block_width - 1) / block_width;
     const dim3 grid((unsigned)(tiles_x * tiles_y));
+    const int split = block_width == 16 && !ga
+                          ? bwd16_split_for(has_opacities, det, tiles_x * tiles_y, num_intersects)
+                          : 1;
+    const dim3 grid16((unsigned)(tiles_x * tiles_y * split));
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
     const GaussPack *pk = reinterpret_cast<const GaussPack *>(packed);
 #define LIG_BWD_ARGS_PLAIN                                                                    \
@@ -1513,7 +1590,7 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
 #define LIG_BWD_ARGS_16                                                                      \
     tiles_x, img_width, img_height, gids, slots, bins, pk, final_idx, v_output, v_wsum,         \
         nullptr, nullptr, nullptr, 0.0f, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity,      \
-        nullptr, 0
+        nullptr, 0, split
 #define LIG_BWD_ARGS_GA                                                                      \
     tiles_x, block_width, img_width, img_height, gids, slots, bins, pk, final_idx, v_output,    \
         v_wsum, v_output_dx, v_output_dy, v_output_dxy, has_opacities, rec, v_xy, v_xy_abs,    \
@@ -1530,13 +1607,13 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
         const int sel = (has_opacities ? 4 : 0) | (det ? 2 : 0) | (block_width == 16 ? 1 : 0);
         switch (sel) {
             case 0: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 1: hipLaunchKernelGGL((raster_bwd16_kernel<false, false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 1: hipLaunchKernelGGL((raster_bwd16_kernel<false, false, false>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
             case 2: hipLaunchKernelGGL((raster_bwd_plain_kernel<false, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 3: hipLaunchKernelGGL((raster_bwd16_kernel<false, true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 3: hipLaunchKernelGGL((raster_bwd16_kernel<false, true, false>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
             case 4: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            case 5: hipLaunchKernelGGL((raster_bwd16_kernel<true, false, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 5: hipLaunchKernelGGL((raster_bwd16_kernel<true, false, false>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
             case 6: hipLaunchKernelGGL((raster_bwd_plain_kernel<true, true>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_PLAIN); break;
-            default: hipLaunchKernelGGL((raster_bwd16_kernel<true, true, false>), grid, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            default: hipLaunchKernelGGL((raster_bwd16_kernel<true, true, false>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
         }
     }
 #undef LIG_BWD_ARGS_PLAIN
@@ -1582,7 +1659,7 @@ extern "C" int lig_rasterize_backward_records(int n, int img_height, int img_wid
                                  tile_bins, packed, final_idx, v_output, v_wsum, v_output_dx,
                                  v_output_dy, v_output_dxy, has_opacities, emission_ids_sorted,
                                  rec, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                 as_stream(stream));
+                                 num_intersects, as_stream(stream));
 }
 
 extern "C" int lig_fit_backward_records_l2(int n, int img_height, int img_width,
@@ -1599,14 +1676,16 @@ extern "C" int lig_fit_backward_records_l2(int n, int img_height, int img_width,
         return LIG_ERR_WORKSPACE;
     GradRec *rec = reinterpret_cast<GradRec *>(reinterpret_cast<char *>(workspace) + 256);
     const int tiles_x = (img_width + 15) / 16;
-    const dim3 grid((unsigned)(tiles_x * ((img_height + 15) / 16)));
+    const int ntiles = tiles_x * ((img_height + 15) / 16);
+    const int split = bwd16_split(raster_bwd16_kernel<false, true, true>, ntiles, num_intersects);
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
     const GaussPack *pk = reinterpret_cast<const GaussPack *>(packed);
     const float norm = (float)(2.0 / (3.0 * (double)img_height * (double)img_width));
-    hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), grid, dim3(kThreads), 0,
-                       as_stream(stream), tiles_x, img_width, img_height, gaussian_ids_sorted,
-                       nullptr, bins, pk, nullptr, nullptr, nullptr, img, target, grad_loss, norm,
-                       rec, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+    hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), dim3((unsigned)(ntiles * split)),
+                       dim3(kThreads), 0, as_stream(stream), tiles_x, img_width, img_height,
+                       gaussian_ids_sorted, nullptr, bins, pk, nullptr, nullptr, nullptr, img,
+                       target, grad_loss, norm, rec, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, 0, split);
     return launch_status();
 }
 
@@ -1643,10 +1722,13 @@ int fit_backward_batched(int n, const int32_t *gaussian_ids_sorted, const int32_
                            dim3(kThreads), 0, st, gaussian_ids_sorted, bins, pk, img, target,
                            grad_loss, rec, tab, num_images, q);
     } else {
-        hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), dim3((unsigned)num_tiles),
-                           dim3(kThreads), 0, st, 1, 1, 1, gaussian_ids_sorted, nullptr, bins, pk,
-                           nullptr, nullptr, nullptr, img, target, grad_loss, 0.0f, rec, nullptr,
-                           nullptr, nullptr, nullptr, nullptr, tab, num_images);
+        const int split =
+            bwd16_split(raster_bwd16_kernel<false, true, true>, num_tiles, num_intersects);
+        hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>),
+                           dim3((unsigned)(num_tiles * split)), dim3(kThreads), 0, st, 1, 1, 1,
+                           gaussian_ids_sorted, nullptr, bins, pk, nullptr, nullptr, nullptr, img,
+                           target, grad_loss, 0.0f, rec, nullptr, nullptr, nullptr, nullptr,
+                           nullptr, tab, num_images, split);
     }
     return launch_status();
 }
@@ -1759,7 +1841,7 @@ extern "C" int lig_rasterize_backward(int n, int img_height, int img_width, int 
     return launch_backward_tiles(n, img_height, img_width, block_width, gaussian_ids_sorted,
                                  tile_bins, packed, final_idx, v_output, v_wsum, v_output_dx,
                                  v_output_dy, v_output_dxy, has_opacities, nullptr, nullptr, v_xy,
-                                 v_xy_abs, v_conic, v_colors, v_opacity, st);
+                                 v_xy_abs, v_conic, v_colors, v_opacity, num_intersects, st);
 }
 
 extern "C" int lig_count_pairs(int n, int img_height, int img_width, const float *xys,

@@ -34,6 +34,21 @@ def test_library_exports_every_declared_symbol():
     assert L.size_query("lig_bin_workspace_size", 0, 16) > 0
 
 
+def test_backward_shares_override_is_host_state():
+    """lig_set_backward_shares only sets host state: 0 (automatic) or 1..8,
+    returning the previous value; other values are rejected."""
+    import lig_amd._lib as L
+    lib = L.load()
+    prev = lib.lig_set_backward_shares(3)
+    try:
+        assert lib.lig_set_backward_shares(8) == 3
+        assert lib.lig_set_backward_shares(9) == -1
+        assert lib.lig_set_backward_shares(-1) == -1
+        assert lib.lig_set_backward_shares(0) == 8
+    finally:
+        lib.lig_set_backward_shares(prev)
+
+
 def test_invalid_arguments_rejected_without_gpu():
     import ctypes
     import lig_amd._lib as L
