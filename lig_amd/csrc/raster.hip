@@ -248,7 +248,7 @@ __device__ __forceinline__ void fwd16_tile(
     float *__restrict__ out_img, float *__restrict__ out_wsum,
     int32_t *__restrict__ final_idx, const float *__restrict__ target,
     float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg,
-    const int32_t *__restrict__ cum, float4 *s0, float4 *s1, float2 *s2) {
+    const int32_t *__restrict__ cum, float4 *s0, float4 *s1, float *s2) {
     // gtile: bins / partials index
     int tile = gtile;  // tile of its own image
     float init = 0.0f;
@@ -286,60 +286,78 @@ __device__ __forceinline__ void fwd16_tile(
         fin[q] = 0;
     }
 
+    // the tile's range is wave-uniform: batch bounds and loop control stay scalar
     const int2 range = bins[gtile];
-    for (int base = range.x; base < range.y; base += DEPTH) {
-        const int cnt = min(DEPTH, range.y - base);
-        int gq[DEPTH / kThreads];
+    const int r0 = __builtin_amdgcn_readfirstlane(range.x);
+    const int r1 = __builtin_amdgcn_readfirstlane(range.y);
+    constexpr int kSub = DEPTH / kThreads;
+    for (int base = r0; base < r1; base += DEPTH) {
+        const int cnt = min(DEPTH, r1 - base);
+        int gq[kSub];
 #pragma unroll
-        for (int k = 0; k < DEPTH / kThreads; ++k) {
+        for (int k = 0; k < kSub; ++k) {
             const int j = k * kThreads + lane;
             gq[k] = j < cnt ? gids[base + j] : -1;
         }
+        // qbits[k][q]: ballot over the k-th 64 staged Gaussians of "reaches
+        // quadrant q", so the loop below finds each Gaussian's quadrants with
+        // scalar bit tests and skips Gaussians that reach none
+        uint64_t qbits[kSub][4];
 #pragma unroll
-        for (int k = 0; k < DEPTH / kThreads; ++k) {
+        for (int k = 0; k < kSub; ++k) {
             const int j = k * kThreads + lane;
-            if (gq[k] < 0) continue;
-            const GaussPack gp = pack[gq[k]];
-            s0[j] = gp.p0;
-            s1[j] = gp.p1;
-            // quadrant q spans pixel centres [org + 8h + 0.5, org + 8h + 7.5]
-            const float xl = gp.p0.x - gp.p3.x, xh = gp.p0.x + gp.p3.x;
-            const float yl = gp.p0.y - gp.p3.y, yh = gp.p0.y + gp.p3.y;
-            const float bx0 = (float)x_org + 0.5f, by0 = (float)y_org + 0.5f;
-            // negated tests keep NaN boxes (NaN means) unculled, as the generic kernel
-            const bool cx0 = !(xl > bx0 + 7.0f) && !(xh < bx0);
-            const bool cx1 = !(xl > bx0 + 15.0f) && !(xh < bx0 + 8.0f);
-            const bool cy0 = !(yl > by0 + 7.0f) && !(yh < by0);
-            const bool cy1 = !(yl > by0 + 15.0f) && !(yh < by0 + 8.0f);
-            const int m = (cx0 && cy0 ? 1 : 0) | (cx1 && cy0 ? 2 : 0) | (cx0 && cy1 ? 4 : 0) |
-                          (cx1 && cy1 ? 8 : 0);
-            s2[j] = make_float2(gp.p2.x, __int_as_float(m));
+            int m = 0;
+            if (gq[k] >= 0) {
+                const GaussPack gp = pack[gq[k]];
+                s0[j] = gp.p0;
+                // without opacities the opacity slot carries blue
+                s1[j] = HAS_OPAC ? gp.p1 : make_float4(gp.p1.x, gp.p2.x, gp.p1.z, gp.p1.w);
+                if constexpr (HAS_OPAC) s2[j] = gp.p2.x;
+                // quadrant q spans pixel centres [org + 8h + 0.5, org + 8h + 7.5]
+                const float xl = gp.p0.x - gp.p3.x, xh = gp.p0.x + gp.p3.x;
+                const float yl = gp.p0.y - gp.p3.y, yh = gp.p0.y + gp.p3.y;
+                const float bx0 = (float)x_org + 0.5f, by0 = (float)y_org + 0.5f;
+                // negated tests keep NaN boxes (NaN means) unculled, as the generic kernel
+                const bool cx0 = !(xl > bx0 + 7.0f) && !(xh < bx0);
+                const bool cx1 = !(xl > bx0 + 15.0f) && !(xh < bx0 + 8.0f);
+                const bool cy0 = !(yl > by0 + 7.0f) && !(yh < by0);
+                const bool cy1 = !(yl > by0 + 15.0f) && !(yh < by0 + 8.0f);
+                m = (cx0 && cy0 ? 1 : 0) | (cx1 && cy0 ? 2 : 0) | (cx0 && cy1 ? 4 : 0) |
+                    (cx1 && cy1 ? 8 : 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) qbits[k][q] = __ballot((m >> q) & 1);
         }
         __syncthreads();
-        for (int t = 0; t < cnt; ++t) {
-            const float2 a2 = s2[t];
-            const int mask = __builtin_amdgcn_readfirstlane(__float_as_int(a2.y));
-            if (mask == 0) continue;
-            const float4 a0 = s0[t];
-            const float4 a1 = s1[t];
-            const int kidx = base + t;
-            const float dx0 = a0.x - X0, dx1 = a0.x - X1;
-            const RowTerms r0 = row_terms(a0.w, a1.x, a0.y - Y0);
-            const RowTerms r1 = row_terms(a0.w, a1.x, a0.y - Y1);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (!(mask & (1 << q))) continue;
-                const float s = pair_s(a0.z, (q & 2) ? r1 : r0, (q & 1) ? dx1 : dx0);
-                const float e = fast_exp2(-s);
-                float al = fminf(HAS_OPAC ? a1.y * e : e, clampv);
-                const bool ok = !(s < 0.0f) && !(al < amin);
-                al = ok ? al : 0.0f;
-                ar[q] = __builtin_fmaf(al, a1.z, ar[q]);
-                ag[q] = __builtin_fmaf(al, a1.w, ag[q]);
-                ab[q] = __builtin_fmaf(al, a2.x, ab[q]);
-                if constexpr (!TRAIN) {
-                    aw[q] += al;
-                    fin[q] = ok ? kidx : fin[q];
+        for (int k = 0; k < kSub; ++k) {
+            uint64_t todo = qbits[k][0] | qbits[k][1] | qbits[k][2] | qbits[k][3];
+            while (todo) {
+                const int bit = __builtin_ctzll(todo);
+                todo &= todo - 1;
+                const int t = k * kThreads + bit;
+                const float4 a0 = s0[t];
+                const float4 a1 = s1[t];
+                const float blue = HAS_OPAC ? s2[t] : a1.y;
+                const int kidx = base + t;
+                const float dx0 = a0.x - X0, dx1 = a0.x - X1;
+                const RowTerms rw0 = row_terms(a0.w, a1.x, a0.y - Y0);
+                const RowTerms rw1 = row_terms(a0.w, a1.x, a0.y - Y1);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (!((qbits[k][q] >> bit) & 1)) continue;
+                    const float s = pair_s(a0.z, (q & 2) ? rw1 : rw0, (q & 1) ? dx1 : dx0);
+                    const float e = fast_exp2(-s);
+                    float al = vmin_f32(HAS_OPAC ? a1.y * e : e, clampv);
+                    const bool ok = !(s < 0.0f) && !(al < amin);
+                    al = ok ? al : 0.0f;
+                    ar[q] = __builtin_fmaf(al, a1.z, ar[q]);
+                    ag[q] = __builtin_fmaf(al, a1.w, ag[q]);
+                    ab[q] = __builtin_fmaf(al, blue, ab[q]);
+                    if constexpr (!TRAIN) {
+                        aw[q] += al;
+                        fin[q] = ok ? kidx : fin[q];
+                    }
                 }
             }
         }
@@ -387,7 +405,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
     const int32_t *__restrict__ cum) {
     __shared__ float4 s0[DEPTH];
     __shared__ float4 s1[DEPTH];
-    __shared__ float2 s2[DEPTH];  // (blue, quadrant mask bits)
+    __shared__ float s2[HAS_OPAC ? DEPTH : 1];  // blue (with opacities)
     fwd16_tile<HAS_OPAC, TRAIN, DEPTH>((int)xcd_remap(blockIdx.x, gridDim.x), tiles_x, img_w,
                                        img_h, gids, bins, pack, out_img, out_wsum, final_idx,
                                        target, loss_partial, tab, nimg, cum, s0, s1, s2);
@@ -405,7 +423,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_queue_kernel(
     const int32_t *__restrict__ cum, TileQueue queue) {
     __shared__ float4 s0[DEPTH];
     __shared__ float4 s1[DEPTH];
-    __shared__ float2 s2[DEPTH];
+    __shared__ float s2[HAS_OPAC ? DEPTH : 1];
     int cnt;
     int t = queue_resolve(queue, queue_issue(queue), cnt);
     while (t >= 0) {
