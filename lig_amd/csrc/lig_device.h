@@ -172,6 +172,8 @@ struct __align__(16) GaussPack {
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kHalfLog2e = 0.5f * kLog2e;
 constexpr float kAlphaMin = 1.0f / 255.0f;
+// log2(255): alpha >= 1/255 without opacity needs exp2(-s) >= 1/255
+constexpr float kAcceptLog2 = 7.994353436858858f;
 
 // sigma*log2(e) of a pair from the scaled conic (a', b', c'), evaluated with
 // explicit FMAs so that every kernel computes bit-identical values for the

@@ -226,6 +226,51 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
     }
 }
 
+// Quadrants of a 16x16 tile (bit q: x half q & 1, y half q >> 1) that the
+// accept region of a Gaussian can reach.  In the forward's own terms the region
+// is {u : s(u) <= T}, s(u) = a' u_x^2 + b2' u_x u_y + c' u_y^2 with the staged
+// conic (a', b2', c') = (a, 2b, c) log2(e) / 2, u = mean - pixel centre, and
+// T = log2(255 o); (bx0, by0) is the tile's first pixel centre.  The form's
+// minimum over a quadrant's rectangle of centres is zero when the mean is
+// inside it and otherwise the least of its four edges' minima (along
+// u_x = const the form is least at u_y = -b2' u_x / (2 c'), clamped to the
+// edge; likewise across).  A quadrant is dropped when that minimum exceeds T
+// by more than 0.36 + 2^-10 T (0.5 + 2^-10 t in units of the unscaled
+// d^T Q d <= t).  Where the conic's condition number is below 1e4
+// (tr^2 <= 1e4 det), a' + c' <= 46 (a + c <= 64) and the mean lies within
+// 65536 px, the FP32 error of this minimum and of the per-pixel s together stay
+// below 0.22 (terms at most 1e4 T; centre differences off by at most 2^-8 px),
+// so a dropped quadrant holds no pixel the accept test passes.  Other
+// Gaussians keep all four quadrants.
+__device__ __forceinline__ int reach_quadrants(float mx, float my, float ap, float bp2, float cp,
+                                               float T, float bx0, float by0) {
+    const float bp = 0.5f * bp2;
+    const float det = ap * cp - bp * bp, tr = ap + cp;
+    if (!(ap > 0.0f && cp > 0.0f && det > 0.0f && tr <= 46.0f && tr * tr <= 1e4f * det &&
+          T > 0.0f && T < 1e4f && fabsf(mx) <= 65536.0f && fabsf(my) <= 65536.0f))
+        return 0xF;
+    const float lim = __builtin_fmaf(T, 0x1p-10f, T) + 0.36f;
+    const float ra = -bp / ap, rc = -bp / cp;
+    auto form = [&](float x, float y) {
+        return __builtin_fmaf(ap * x, x, y * __builtin_fmaf(bp2, x, cp * y));
+    };
+    int m = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        // the quadrant's rectangle of centre offsets [x0, x1] x [y0, y1]
+        const float x0 = (bx0 + (float)((q & 1) * 8)) - mx, x1 = x0 + 7.0f;
+        const float y0 = (by0 + (float)((q >> 1) * 8)) - my, y1 = y0 + 7.0f;
+        const bool inside = x0 <= 0.0f && x1 >= 0.0f && y0 <= 0.0f && y1 >= 0.0f;
+        const float e0 = form(x0, fminf(fmaxf(rc * x0, y0), y1));
+        const float e1 = form(x1, fminf(fmaxf(rc * x1, y0), y1));
+        const float e2 = form(fminf(fmaxf(ra * y0, x0), x1), y0);
+        const float e3 = form(fminf(fmaxf(ra * y1, x0), x1), y1);
+        const float least = fminf(fminf(e0, e1), fminf(e2, e3));
+        if (inside || !(least > lim)) m |= 1 << q;
+    }
+    return m;
+}
+
 // Forward for 16x16 tiles.  Lane l owns pixel (l % 8, l / 8) of each 8x8
 // quadrant q (q = 0..3: x half q & 1, y half q >> 1), so the four pixels of a
 // lane share two columns and two rows (dx, dy, b'dy and c'dy are computed
@@ -303,10 +348,11 @@ __device__ __forceinline__ void fwd16_tile(
         // quadrant q", so the loop below finds each Gaussian's quadrants with
         // scalar bit tests and skips Gaussians that reach none
         uint64_t qbits[kSub][4];
+        int box[kSub];
 #pragma unroll
         for (int k = 0; k < kSub; ++k) {
             const int j = k * kThreads + lane;
-            int m = 0;
+            box[k] = 0;
             if (gq[k] >= 0) {
                 const GaussPack gp = pack[gq[k]];
                 s0[j] = gp.p0;
@@ -322,8 +368,23 @@ __device__ __forceinline__ void fwd16_tile(
                 const bool cx1 = !(xl > bx0 + 15.0f) && !(xh < bx0 + 8.0f);
                 const bool cy0 = !(yl > by0 + 7.0f) && !(yh < by0);
                 const bool cy1 = !(yl > by0 + 15.0f) && !(yh < by0 + 8.0f);
-                m = (cx0 && cy0 ? 1 : 0) | (cx1 && cy0 ? 2 : 0) | (cx0 && cy1 ? 4 : 0) |
-                    (cx1 && cy1 ? 8 : 0);
+                box[k] = (cx0 && cy0 ? 1 : 0) | (cx1 && cy0 ? 2 : 0) | (cx0 && cy1 ? 4 : 0) |
+                         (cx1 && cy1 ? 8 : 0);
+            }
+        }
+        // then the region itself, not only its box (about a fifth of the
+        // quadrants the box reaches hold no accepted pixel), from the lane's
+        // own staged entries
+#pragma unroll
+        for (int k = 0; k < kSub; ++k) {
+            const int j = k * kThreads + lane;
+            int m = box[k];
+            if (m) {
+                const float4 a0 = s0[j];
+                const float4 a1 = s1[j];
+                const float T = HAS_OPAC ? __log2f(255.0f * a1.y) : kAcceptLog2;
+                m &= reach_quadrants(a0.x, a0.y, a0.z, a0.w, a1.x, T, (float)x_org + 0.5f,
+                                     (float)y_org + 0.5f);
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) qbits[k][q] = __ballot((m >> q) & 1);
@@ -396,7 +457,7 @@ __device__ __forceinline__ void fwd16_tile(
 
 // One tile per block: the tile xcd_remap(blockIdx.x).
 template <bool HAS_OPAC, bool TRAIN, int DEPTH = kThreads>
-__global__ __launch_bounds__(kThreads) void raster_fwd16_kernel(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(TRAIN ? 8 : 1))) void raster_fwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
     float *__restrict__ out_img, float *__restrict__ out_wsum,

@@ -114,6 +114,36 @@ def test_raster_forward_backward(device, oracle_lib, n, H, W, bw, op, seed, scal
         assert e64 <= 1e-4, (name, e64)
 
 
+@pytest.mark.parametrize("offdiag,op", [(0.9, False), (0.999, False), (0.95, True)])
+def test_forward_region_cull_keeps_accepted_pairs(device, oracle_lib, offdiag, op):
+    """The 16 x 16 forward drops the quadrants of a tile that a Gaussian's
+    bounding box reaches but its accept region does not; on rotated, strongly
+    elongated, large and tiny Gaussians (opacities up to 1.5 and below 1/255,
+    means off the canvas) every accepted pair is still drawn: image and final
+    index as the FP64 oracle's on pixels without an ambiguous pair."""
+    from lig_amd.gsplat2d import cuda as C
+    n, H, W, bw = 500, 80, 96, 16
+    s = random_scene(n, H, W, 77 + int(op), opacities=op, scale=(0.3, 12.0), offdiag=offdiag,
+                     margin=30.0)
+    if op:
+        s["opac"][::7] = 1.5
+        s["opac"][3::11] = 0.002
+    p = _prepare(oracle_lib, s, bw)
+    img, wsum, _, _, _, fin, _ = C.rasterize_forward_packed(
+        H, W, bw, _t(p["gs"], device, torch.int32), _t(p["bins"], device, torch.int32),
+        _t(p["xys"], device), _t(p["con"], device), _t(s["colors"], device),
+        _t(s["opac"], device), False)
+    ref = oracle_lib.rasterize_forward(p["gs"], p["bins"], p["xys"], p["con"], s["colors"],
+                                       s["opac"], H, W, bw)
+    clean = ref["ambig"] == 0
+    print(f"offdiag {offdiag} op {op}: {int(p['m'])} intersections, clean {clean.mean():.3f}",
+          flush=True)
+    assert clean.mean() > 0.9
+    np.testing.assert_array_equal(_np(fin)[clean], ref["final_idx"][clean])
+    assert np.abs(_np(img) - ref["img"])[clean].max() <= 1e-5
+    assert np.abs(_np(wsum) - ref["wsum"])[clean].max() <= 1e-5
+
+
 def test_raster_api_end_to_end(device, oracle_lib):
     """project -> rasterize -> loss -> backward through the autograd wrappers."""
     import lig_amd.gsplat2d as g
