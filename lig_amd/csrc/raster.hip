@@ -457,7 +457,7 @@ __device__ __forceinline__ void fwd16_tile(
 
 // One tile per block: the tile xcd_remap(blockIdx.x).
 template <bool HAS_OPAC, bool TRAIN, int DEPTH = kThreads>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(TRAIN ? 8 : 1))) void raster_fwd16_kernel(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(TRAIN && DEPTH <= 128 ? 8 : 1))) void raster_fwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
     float *__restrict__ out_img, float *__restrict__ out_wsum,
@@ -624,6 +624,9 @@ __device__ __forceinline__ void emit_grads(bool det, GradRec *__restrict__ rec, 
 // longest range (all lanes active for the shuffles; missing records read 0).
 constexpr int kSegLanes = 16;
 constexpr int kSegThreads = 256;
+#ifndef LIG_SEG_UNROLL
+#define LIG_SEG_UNROLL 2
+#endif
 
 // Sum over a 16-lane DPP row, valid in the row's first lane: quad swaps
 // (xor 1, xor 2), then the half-row and row mirrors; a fixed pairing.
@@ -646,6 +649,11 @@ __device__ __forceinline__ float row16_sum(float v) {
     return v;
 }
 
+// kSegUnroll chunks are loaded before any is reduced (more loads in flight);
+// their sums are still added in chunk order, and chunks past a group's range
+// (as those past the wave's longest range) add zeros.
+constexpr int kSegUnroll = LIG_SEG_UNROLL;
+
 template <int NF>
 __device__ __forceinline__ void seg_sum(const float4 *__restrict__ src, int lo, int hi,
                                         float (&s)[NF]) {
@@ -656,26 +664,30 @@ __device__ __forceinline__ void seg_sum(const float4 *__restrict__ src, int lo, 
         nchunk = max(nchunk, __shfl_xor(nchunk, off, LIG_WAVE));
 #pragma unroll
     for (int k = 0; k < NF; ++k) s[k] = 0.0f;
-    for (int c = 0; c < nchunk; ++c) {
-        const int r = lo + c * kSegLanes + sub;
-        float v[NF];
-        if (r < hi) {
+    for (int c = 0; c < nchunk; c += kSegUnroll) {
+        float v[kSegUnroll][NF];
 #pragma unroll
-            for (int q = 0; q < NF / 4; ++q) {
-                const float4 x = src[(size_t)r * (NF / 4) + q];
-                v[4 * q] = x.x;
-                v[4 * q + 1] = x.y;
-                v[4 * q + 2] = x.z;
-                v[4 * q + 3] = x.w;
+        for (int u = 0; u < kSegUnroll; ++u) {
+            const int r = lo + (c + u) * kSegLanes + sub;
+            if (r < hi) {
+#pragma unroll
+                for (int q = 0; q < NF / 4; ++q) {
+                    const float4 x = src[(size_t)r * (NF / 4) + q];
+                    v[u][4 * q] = x.x;
+                    v[u][4 * q + 1] = x.y;
+                    v[u][4 * q + 2] = x.z;
+                    v[u][4 * q + 3] = x.w;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < NF; ++k) v[u][k] = 0.0f;
             }
-        } else {
-#pragma unroll
-            for (int k = 0; k < NF; ++k) v[k] = 0.0f;
         }
 #pragma unroll
-        for (int k = 0; k < NF; ++k) v[k] = row16_sum(v[k]);
+        for (int u = 0; u < kSegUnroll; ++u) {
 #pragma unroll
-        for (int k = 0; k < NF; ++k) s[k] += v[k];
+            for (int k = 0; k < NF; ++k) s[k] += row16_sum(v[u][k]);
+        }
     }
 }
 
