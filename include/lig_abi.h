@@ -239,7 +239,12 @@ int lig_reduce_gradient_records(int n, int num_intersects, const int32_t *cum_ti
  * per tile in FP32 and over tiles in FP64 (deterministic).  batch_depth
  * (64, 128, 256; 0 = 64) is the number of Gaussians staged in LDS per batch;
  * it changes speed only, not results.  Workspace:
- * lig_fit_render_workspace_size(H, W).
+ * lig_fit_render_workspace_size(H, W), which also holds the tiles' launch
+ * order: heaviest (most pairs) first, so that long tiles start early and the
+ * XCDs get like loads; the order changes speed only.  The backward below and
+ * the batched calls order their tiles the same way (the backward when its
+ * workspace has room after the 32-byte records, as it has whenever
+ * num_intersects >= num_tiles / 4 + 96).
  *
  * lig_fit_backward_records_l2: the records half of the deterministic backward
  * for upstream gradient dL/dI = (0 <= img <= 1) ? (2/(3HW))(img - target) *

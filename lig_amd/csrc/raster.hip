@@ -455,7 +455,8 @@ __device__ __forceinline__ void fwd16_tile(
     }
 }
 
-// One tile per block: the tile xcd_remap(blockIdx.x).
+// One tile per block: the tile order[blockIdx.x], or without an order (none
+// given, or order[0] = -1) the tile xcd_remap(blockIdx.x).
 template <bool HAS_OPAC, bool TRAIN, int DEPTH = kThreads>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(TRAIN && DEPTH <= 128 ? 8 : 1))) void raster_fwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
@@ -463,11 +464,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(TRAIN 
     float *__restrict__ out_img, float *__restrict__ out_wsum,
     int32_t *__restrict__ final_idx, const float *__restrict__ target,
     float *__restrict__ loss_partial, const BatchImage *__restrict__ tab, int nimg,
-    const int32_t *__restrict__ cum) {
+    const int32_t *__restrict__ cum, const int32_t *__restrict__ order) {
     __shared__ float4 s0[DEPTH];
     __shared__ float4 s1[DEPTH];
     __shared__ float s2[HAS_OPAC ? DEPTH : 1];  // blue (with opacities)
-    fwd16_tile<HAS_OPAC, TRAIN, DEPTH>((int)xcd_remap(blockIdx.x, gridDim.x), tiles_x, img_w,
+    const int gtile = order && order[0] >= 0 ? order[blockIdx.x]
+                                             : (int)xcd_remap(blockIdx.x, gridDim.x);
+    fwd16_tile<HAS_OPAC, TRAIN, DEPTH>(gtile, tiles_x, img_w,
                                        img_h, gids, bins, pack, out_img, out_wsum, final_idx,
                                        target, loss_partial, tab, nimg, cum, s0, s1, s2);
 }
@@ -1031,8 +1034,10 @@ __device__ __forceinline__ void bwd16_tile(
     }
 }
 
-// Block xcd_remap(blockIdx.x) = tile * split + part: one share of a tile's
-// batches per block (split = 1: the whole tile).
+// One share of a tile's batches per block (split = 1: the whole tile): block
+// b takes share b % split of tile order[b / split], or without an order (none
+// given, or order[0] = -1), with u = xcd_remap(b), share u % split of tile
+// u / split.
 template <bool HAS_OPAC, bool DET, bool TRAIN>
 __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
@@ -1043,12 +1048,20 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     const float *__restrict__ grad_loss, float norm, GradRec *__restrict__ rec,
     float *__restrict__ g_xy, float *__restrict__ g_xy_abs, float *__restrict__ g_conic,
     float *__restrict__ g_rgb, float *__restrict__ g_opac, const BatchImage *__restrict__ tab,
-    int nimg, int split) {
+    int nimg, int split, const int32_t *__restrict__ order) {
     __shared__ float4 sv[kMaxPix];
     __shared__ int sfin[TRAIN ? 1 : kMaxPix];
-    const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
-    const int gtile = u / split;
-    bwd16_tile<HAS_OPAC, DET, TRAIN>(gtile, u - gtile * split, split, tiles_x, img_w, img_h,
+    int gtile, part;
+    if (order && order[0] >= 0) {  // the shares of a tile are consecutive blocks
+        const int k = (int)blockIdx.x / split;
+        part = (int)blockIdx.x - k * split;
+        gtile = order[k];
+    } else {
+        const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
+        gtile = u / split;
+        part = u - gtile * split;
+    }
+    bwd16_tile<HAS_OPAC, DET, TRAIN>(gtile, part, split, tiles_x, img_w, img_h,
                                      gids, slots, bins, pack, final_idx, v_out, v_wsum, img,
                                      target, grad_loss, norm, rec, g_xy, g_xy_abs, g_conic, g_rgb,
                                      g_opac, tab, nimg, sv, sfin);
@@ -1269,6 +1282,109 @@ __global__ __launch_bounds__(kSumThreads) void total_loss_kernel(int nimg,
     if (threadIdx.x == 0) total[0] = (float)sum;
 }
 
+// Launch order of a tile grid.  Heaviest first: a tile's class falls with
+// its pair count, four classes per octave (128 in all); order_hist_kernel
+// counts the classes and order_scatter_kernel writes each tile at its class's
+// next free place (tiles of one class in no fixed order, which changes no
+// result: each tile's work and outputs are its own).  Launched so (block b
+// takes tile order[b]; the hardware deals consecutive blocks to the eight
+// XCDs in turn), long tiles start first and every XCD gets a like mix.  The
+// default mapping, xcd_remap, gives each XCD a contiguous band of tiles, which
+// keeps a band's Gaussians in that XCD's L2 but leaves the XCDs that own a
+// dense region running long after the others.  So the heaviest-first order is
+// taken when the caller asks for it (a launch of few rounds, whose tail it
+// shortens) or when the bands' loads (pairs + kTileCost per tile) differ by
+// more than a tenth; otherwise order[0] = -1 and the kernels keep xcd_remap.
+constexpr int kOrderClasses = 128;
+constexpr int kOrderThreads = 256;
+constexpr int kOrderPer = 8;    // tiles per thread
+constexpr int kTileCost = 16;   // a tile's fixed cost, in pairs
+
+__device__ __forceinline__ int order_class(int2 bin) {
+    const unsigned c = (unsigned)max(bin.y - bin.x, 0) + 1u;
+    const int lg = 31 - __clz(c);
+    const unsigned frac = lg >= 2 ? (c >> (lg - 2)) & 3u : (c << (2 - lg)) & 3u;
+    return kOrderClasses - 1 - (4 * lg + (int)frac);
+}
+
+// xcd_remap's band (XCD share) of tile t of nt
+__device__ __forceinline__ int xcd_band(int t, int nt) {
+    const int q = nt >> 3, r = nt & 7;
+    return t < r * (q + 1) ? t / (q + 1) : r + (t - r * (q + 1)) / max(q, 1);
+}
+
+__global__ __launch_bounds__(kOrderThreads) void order_hist_kernel(int ntiles,
+                                                                   const int2 *__restrict__ bins,
+                                                                   int *__restrict__ hist,
+                                                                   long long *__restrict__ band) {
+    __shared__ int lh[kOrderClasses];
+    __shared__ long long lb[8];
+    for (int i = threadIdx.x; i < kOrderClasses; i += kOrderThreads) lh[i] = 0;
+    if (threadIdx.x < 8) lb[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kOrderPer; ++k) {
+        const int t = (blockIdx.x * kOrderPer + k) * kOrderThreads + threadIdx.x;
+        if (t < ntiles) {
+            const int2 b = bins[t];
+            atomicAdd(&lh[order_class(b)], 1);
+            atomicAdd((unsigned long long *)&lb[xcd_band(t, ntiles)],
+                      (unsigned long long)(max(b.y - b.x, 0) + kTileCost));
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kOrderClasses; i += kOrderThreads)
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+    if (threadIdx.x < 8 && lb[threadIdx.x])
+        atomicAdd((unsigned long long *)&band[threadIdx.x], (unsigned long long)lb[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(kOrderThreads) void order_scatter_kernel(
+    int ntiles, const int2 *__restrict__ bins, const int *__restrict__ hist,
+    const long long *__restrict__ band, int force, int *__restrict__ cursor,
+    int32_t *__restrict__ order) {
+    if (!force) {
+        long long mx = 0, sum = 0;
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            mx = max(mx, band[x]);
+            sum += band[x];
+        }
+        if (mx * 80 <= sum * 11) {  // max <= 1.1 x mean: keep the bands
+            if (blockIdx.x == 0 && threadIdx.x == 0) order[0] = -1;
+            return;
+        }
+    }
+    __shared__ int pre[kOrderClasses];  // exclusive start of each class
+    __shared__ int lcnt[kOrderClasses];
+    __shared__ int lbase[kOrderClasses];
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int i = 0; i < kOrderClasses; ++i) {
+            pre[i] = run;
+            run += hist[i];
+        }
+    }
+    for (int i = threadIdx.x; i < kOrderClasses; i += kOrderThreads) lcnt[i] = 0;
+    __syncthreads();
+    int cls[kOrderPer], rank[kOrderPer];
+#pragma unroll
+    for (int k = 0; k < kOrderPer; ++k) {
+        const int t = (blockIdx.x * kOrderPer + k) * kOrderThreads + threadIdx.x;
+        cls[k] = t < ntiles ? order_class(bins[t]) : -1;
+        rank[k] = cls[k] >= 0 ? atomicAdd(&lcnt[cls[k]], 1) : 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kOrderClasses; i += kOrderThreads)
+        lbase[i] = lcnt[i] ? atomicAdd(&cursor[i], lcnt[i]) : 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kOrderPer; ++k) {
+        const int t = (blockIdx.x * kOrderPer + k) * kOrderThreads + threadIdx.x;
+        if (cls[k] >= 0) order[pre[cls[k]] + lbase[cls[k]] + rank[k]] = t;
+    }
+}
+
 __global__ void zero_f32_kernel(float *__restrict__ p, size_t count) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = 0.0f;
@@ -1406,19 +1522,19 @@ extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int b
     else if (image_only && opacities)
         hipLaunchKernelGGL((raster_fwd16_kernel<true, true>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr);
+                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
     else if (image_only)
         hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr);
+                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
     else if (block_width == 16 && opacities)
         hipLaunchKernelGGL((raster_fwd16_kernel<true, false>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0, nullptr);
+                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
     else if (block_width == 16)
         hipLaunchKernelGGL((raster_fwd16_kernel<false, false>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0, nullptr);
+                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
     else
         launch_fwd<false>(px, grid, st, tiles_x, block_width, img_width, img_height,
                           gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, nullptr,
@@ -1486,6 +1602,56 @@ int bwd16_split(KernelT kernel, int ntiles, long long nisect) {
     return best;
 }
 
+// Workspace of launch_tile_order: the order, then band loads, class counts
+// and cursors.
+size_t order_ws_bytes(int ntiles) {
+    return (((size_t)ntiles * sizeof(int32_t) + 255) & ~size_t(255)) + 8 * sizeof(long long) +
+           2 * kOrderClasses * sizeof(int);
+}
+
+// Blocks of `kernel` resident on the device at once.
+template <typename KernelT>
+long long resident_blocks(KernelT kernel) {
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kThreads, 0);
+    return (long long)(cus > 0 ? cus : 1) * (per > 0 ? per : 1);
+}
+
+// Launch order of `ntiles` tiles (order_hist_kernel / order_scatter_kernel)
+// in `ws`, order_ws_bytes(ntiles) bytes, 8-byte aligned.  `blocks` blocks
+// will take the tiles and `resident` of them run at once: under 8 rounds the
+// heaviest-first order is always taken.
+const int32_t *launch_tile_order(int ntiles, const int2 *bins, long long blocks,
+                                 long long resident, void *ws, hipStream_t st) {
+    int32_t *order = reinterpret_cast<int32_t *>(ws);
+    long long *band = reinterpret_cast<long long *>(
+        reinterpret_cast<char *>(ws) + (((size_t)ntiles * sizeof(int32_t) + 255) & ~size_t(255)));
+    int *hist = reinterpret_cast<int *>(band + 8);
+    int *cursor = hist + kOrderClasses;
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(512), 0, st, reinterpret_cast<float *>(band),
+                       (size_t)(16 + 2 * kOrderClasses));
+    const dim3 grid((unsigned)ceil_div(ntiles, kOrderThreads * kOrderPer));
+    const int force = blocks < 8 * resident ? 1 : 0;
+    hipLaunchKernelGGL(order_hist_kernel, grid, dim3(kOrderThreads), 0, st, ntiles, bins, hist,
+                       band);
+    hipLaunchKernelGGL(order_scatter_kernel, grid, dim3(kOrderThreads), 0, st, ntiles, bins,
+                       (const int *)hist, (const long long *)band, force, cursor, order);
+    return order;
+}
+
+// The order in the slack of a fit-backward workspace after its 32-byte
+// records (null when it does not fit: the launch then takes xcd_remap order).
+const int32_t *order_after_fit_records(int ntiles, int split, const int2 *bins, void *ws,
+                                       size_t ws_bytes, long long nisect, hipStream_t st) {
+    const size_t off = (256 + (size_t)nisect * 32 + 255) & ~size_t(255);
+    if (off + order_ws_bytes(ntiles) > ws_bytes) return nullptr;
+    return launch_tile_order(ntiles, bins, (long long)ntiles * split,
+                             resident_blocks(raster_bwd16_kernel<false, true, true>),
+                             reinterpret_cast<char *>(ws) + off, st);
+}
+
 // Fit-mode (image + loss partials) forward at LDS batch depth `depth`; with
 // queue.ctr set, a persistent grid over queue.ntiles tiles (the counters
 // must be zero).
@@ -1493,7 +1659,8 @@ void launch_fit_forward(int depth, dim3 grid, hipStream_t st, int tiles_x, int i
                         const int32_t *gids, const int2 *bins, const GaussPack *pk,
                         float *out_img, const float *target, float *partial,
                         const BatchImage *tab, int nimg, const int32_t *cum = nullptr,
-                        TileQueue queue = TileQueue{nullptr, 0}) {
+                        TileQueue queue = TileQueue{nullptr, 0},
+                        const int32_t *order = nullptr) {
 #define LIG_FIT_FWD(D)                                                                        \
     if (queue.ctr) {                                                                          \
         auto kern = raster_fwd16_queue_kernel<false, true, D>;                                \
@@ -1503,7 +1670,7 @@ void launch_fit_forward(int depth, dim3 grid, hipStream_t st, int tiles_x, int i
     } else {                                                                                  \
         hipLaunchKernelGGL((raster_fwd16_kernel<false, true, D>), grid, dim3(kThreads), 0, st, \
                            tiles_x, img_w, img_h, gids, bins, pk, out_img, nullptr, nullptr,   \
-                           target, partial, tab, nimg, cum);                                   \
+                           target, partial, tab, nimg, cum, order);                            \
     }
     if (depth == 256) {
         LIG_FIT_FWD(256)
@@ -1514,6 +1681,13 @@ void launch_fit_forward(int depth, dim3 grid, hipStream_t st, int tiles_x, int i
     }
 #undef LIG_FIT_FWD
 }
+
+// Resident blocks of the fit-mode forward at LDS batch depth `depth`.
+long long fit_forward_resident(int depth) {
+    if (depth == 256) return resident_blocks(raster_fwd16_kernel<false, true, 256>);
+    if (depth == 128) return resident_blocks(raster_fwd16_kernel<false, true, 128>);
+    return resident_blocks(raster_fwd16_kernel<false, true, 64>);
+}
 }  // namespace
 }  // namespace lig
 
@@ -1521,8 +1695,9 @@ extern "C" size_t lig_fit_render_workspace_size(int img_height, int img_width) {
     if (img_height <= 0 || img_width <= 0) return 256;
     const size_t tiles = (size_t)((img_height + 15) / 16) * (size_t)((img_width + 15) / 16);
     const size_t blocks = (tiles + kSumChunk - 1) / kSumChunk;
-    // [block sums, FP64][tile partials, FP32]
-    return blocks * sizeof(double) + tiles * sizeof(float) + 256;
+    // [block sums, FP64][tile partials, FP32][launch order at a 256-byte boundary]
+    return ((blocks * sizeof(double) + tiles * sizeof(float) + 255) & ~size_t(255)) +
+           order_ws_bytes((int)tiles);
 }
 
 extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
@@ -1549,8 +1724,13 @@ extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
     const int nsum = (tiles + kSumChunk - 1) / kSumChunk;
     double *bsum = reinterpret_cast<double *>(workspace);
     float *partial = reinterpret_cast<float *>(bsum + nsum);
+    const size_t ooff = ((size_t)nsum * sizeof(double) + (size_t)tiles * sizeof(float) + 255) &
+                        ~size_t(255);
+    const int32_t *order = launch_tile_order(tiles, bins, tiles, fit_forward_resident(batch_depth),
+                                             reinterpret_cast<char *>(workspace) + ooff, st);
     launch_fit_forward(batch_depth, dim3((unsigned)tiles), st, tiles_x, img_width, img_height,
-                       gaussian_ids_sorted, bins, pk, out_img, target, partial, nullptr, 0);
+                       gaussian_ids_sorted, bins, pk, out_img, target, partial, nullptr, 0,
+                       nullptr, TileQueue{nullptr, 0}, order);
     const double numel = 3.0 * (double)img_height * (double)img_width;
     hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)nsum), dim3(kSumThreads), 0, st, tiles,
                        (const float *)partial, bsum);
@@ -1560,9 +1740,10 @@ extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
 }
 
 extern "C" size_t lig_fit_batched_workspace_size(int num_tiles) {
-    // [tile partials, FP32][tile-queue counters at a 256-byte boundary]
+    // [tile partials, FP32][tile-queue counters at a 256-byte boundary][launch order]
     const size_t t = num_tiles > 0 ? (size_t)num_tiles : 0;
-    return ((t * sizeof(float) + 255) & ~size_t(255)) + kQueueInts * sizeof(int);
+    return ((t * sizeof(float) + 255) & ~size_t(255)) + kQueueInts * sizeof(int) +
+           order_ws_bytes((int)t);
 }
 
 namespace lig {
@@ -1587,16 +1768,22 @@ int fit_render_batched(int n, const int32_t *gaussian_ids_sorted, const int32_t 
     hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
                        colors, nullptr, extents, cum_tiles_hit, 0, 0, tab, num_images, pk);
     float *partial = reinterpret_cast<float *>(workspace);
+    char *after_partials =
+        reinterpret_cast<char *>(workspace) + (((size_t)num_tiles * sizeof(float) + 255) & ~size_t(255));
     TileQueue q{nullptr, num_tiles};
+    const int32_t *order = nullptr;
     if (persistent) {
-        q.ctr = reinterpret_cast<int *>(reinterpret_cast<char *>(workspace) +
-                                        (((size_t)num_tiles * sizeof(float) + 255) & ~size_t(255)));
+        q.ctr = reinterpret_cast<int *>(after_partials);
         hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(kQueueInts, 256)), dim3(256), 0, st,
                            reinterpret_cast<float *>(q.ctr), (size_t)kQueueInts);
+    } else {
+        order = launch_tile_order(num_tiles, reinterpret_cast<const int2 *>(tile_bins),
+                                  num_tiles, fit_forward_resident(batch_depth),
+                                  after_partials + kQueueInts * sizeof(int), st);
     }
     launch_fit_forward(batch_depth, dim3((unsigned)num_tiles), st, 1, 1, 1, gaussian_ids_sorted,
                        reinterpret_cast<const int2 *>(tile_bins), pk, out_img, target, partial,
-                       tab, num_images, cum_tiles_hit, q);
+                       tab, num_images, cum_tiles_hit, q, order);
     hipLaunchKernelGGL(image_loss_kernel, dim3((unsigned)num_images), dim3(kImgLossThreads), 0, st,
                        tab, (const float *)partial, loss_per_image);
     hipLaunchKernelGGL(total_loss_kernel, dim3(1), dim3(kSumThreads), 0, st, num_images,
@@ -1681,7 +1868,7 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
 #define LIG_BWD_ARGS_16                                                                      \
     tiles_x, img_width, img_height, gids, slots, bins, pk, final_idx, v_output, v_wsum,         \
         nullptr, nullptr, nullptr, 0.0f, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity,      \
-        nullptr, 0, split
+        nullptr, 0, split, nullptr
 #define LIG_BWD_ARGS_GA                                                                      \
     tiles_x, block_width, img_width, img_height, gids, slots, bins, pk, final_idx, v_output,    \
         v_wsum, v_output_dx, v_output_dy, v_output_dxy, has_opacities, rec, v_xy, v_xy_abs,    \
@@ -1772,11 +1959,14 @@ extern "C" int lig_fit_backward_records_l2(int n, int img_height, int img_width,
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
     const GaussPack *pk = reinterpret_cast<const GaussPack *>(packed);
     const float norm = (float)(2.0 / (3.0 * (double)img_height * (double)img_width));
+    hipStream_t st = as_stream(stream);
+    const int32_t *order = order_after_fit_records(ntiles, split, bins, workspace,
+                                                   workspace_bytes, num_intersects, st);
     hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>), dim3((unsigned)(ntiles * split)),
-                       dim3(kThreads), 0, as_stream(stream), tiles_x, img_width, img_height,
+                       dim3(kThreads), 0, st, tiles_x, img_width, img_height,
                        gaussian_ids_sorted, nullptr, bins, pk, nullptr, nullptr, nullptr, img,
                        target, grad_loss, norm, rec, nullptr, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, 0, split);
+                       nullptr, 0, split, order);
     return launch_status();
 }
 
@@ -1815,11 +2005,13 @@ int fit_backward_batched(int n, const int32_t *gaussian_ids_sorted, const int32_
     } else {
         const int split =
             bwd16_split(raster_bwd16_kernel<false, true, true>, num_tiles, num_intersects);
+        const int32_t *order = order_after_fit_records(num_tiles, split, bins, workspace,
+                                                       workspace_bytes, num_intersects, st);
         hipLaunchKernelGGL((raster_bwd16_kernel<false, true, true>),
                            dim3((unsigned)(num_tiles * split)), dim3(kThreads), 0, st, 1, 1, 1,
                            gaussian_ids_sorted, nullptr, bins, pk, nullptr, nullptr, nullptr, img,
                            target, grad_loss, 0.0f, rec, nullptr, nullptr, nullptr, nullptr,
-                           nullptr, tab, num_images, split);
+                           nullptr, tab, num_images, split, order);
     }
     return launch_status();
 }
