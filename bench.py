@@ -299,6 +299,8 @@ def main():
     ap.add_argument("--serve-reps", type=int, default=32)
     ap.add_argument("--serve-load-qps", default="300,600,1200,2400",
                     help="further offered rates of the serving run (comma list; empty skips)")
+    ap.add_argument("--skew-steps", type=int, default=3,
+                    help="timed loss+backward calls on the density-skewed level 2 (0 skips)")
     ap.add_argument("--batch-steps", type=int, default=5,
                     help="timed steps of the batched-fitting comparison (0 skips it)")
     args = ap.parse_args()
@@ -455,6 +457,33 @@ def main():
                          "on a central crop of each level, scaled by image area / crop area; "
                          + "; ".join(parts)}
 
+    # the level-2 kernels on a density-skewed copy of the level (half the
+    # Gaussians pulled into the top quarter of the canvas: 2.5x the mean
+    # density there, 0.5x elsewhere), where the kernels' launch order has to
+    # balance the XCDs; loss + backward only, parameters untouched
+    skew = None
+    if args.skew_steps > 0:
+        sk = fit.Level(cfg["n2"], cfg["H"], cfg["W"], cfg["seed"] * 10 + 2, cfg["scale"],
+                       device=dev)
+        sk.batch_depth = l2.batch_depth
+        with torch.no_grad():
+            sk.means[: cfg["n2"] // 2, 1] *= 0.25
+        sf = KernelTimer("lig_fit_render_l2", dev)
+        sb = KernelTimer("lig_fit_backward_records_l2", dev)
+        for i in range(args.skew_steps + 1):
+            for p_ in (sk.means, sk.cholesky, sk.rgbs):
+                p_.grad = None
+            if i == 0:
+                sk.loss(t2).backward()
+                continue
+            with sf, sb:
+                sk.loss(t2).backward()
+        skew = {"workload": "the 8192^2 level with half its Gaussians in the top quarter",
+                "raster_fwd_ms": round(sf.mean_ms(), 3), "raster_bwd_ms": round(sb.mean_ms(), 3),
+                "uniform_fwd_ms": round(fwd_ms, 3), "uniform_bwd_ms": round(bwd_ms, 3)}
+        log(f"skewed level 2 {skew}")
+        del sk
+
     del l1, l2, gt, t1, t2
     batched = batched_run(args, dev, torch) if args.batch_steps > 0 else None
     serve = serve_run(args, dev, torch) if args.serve_requests > 0 else None
@@ -481,6 +510,7 @@ def main():
         "render_api_gpix_s": round(P / render_api_s / 1e9, 4),
         "workload_counts": counts,
         "kernel_ms_l2": {k: round(v["ms"], 3) for k, v in kernels.items()},
+        "skewed_l2": skew,
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
