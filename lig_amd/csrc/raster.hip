@@ -814,7 +814,8 @@ template <bool HAS_OPAC, bool ABS = true>
 __device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, float bp,
                                            const RowTerms &rt, float op, float cr, float cg,
                                            float cb, float dx, float lx, float amin,
-                                           float clampv, const float4 *__restrict__ wp) {
+                                           float clampv, const float4 *__restrict__ wp,
+                                           bool col0 = false) {
     const float4 w = *wp;
     const float s = pair_s(ap, rt, dx);
     const float vis = fast_exp2(-s);
@@ -828,8 +829,10 @@ __device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, 
         if constexpr (HAS_OPAC) A.o = __builtin_fmaf(vis, va, A.o);
         const float vsig = -al * va;
         A.q0 += vsig;
-        A.q1 = __builtin_fmaf(vsig, lx, A.q1);
-        A.q2 = __builtin_fmaf(vsig, lx * lx, A.q2);
+        if (!col0) {  // col0: lx = 0 (known at compile time), no lx moments
+            A.q1 = __builtin_fmaf(vsig, lx, A.q1);
+            A.q2 = __builtin_fmaf(vsig, lx * lx, A.q2);
+        }
         if constexpr (ABS) {
             const float u = __builtin_fmaf(ap, dx, rt.bdy);
             const float v = __builtin_fmaf(bp, dx, rt.cdy);
@@ -994,25 +997,28 @@ __device__ __forceinline__ void bwd16_tile(
             }
             const float dy = my - ((float)(y_org + ly) + 0.5f);
             const RowTerms rt = row_terms(bp2, cp, dy);
-            const float4 *row = sv + in_vgpr(ly * 16);
+            // the row's LDS address as one byte offset (immediate offsets per pixel)
+            const float4 *row = reinterpret_cast<const float4 *>(
+                reinterpret_cast<const char *>(sv) + in_vgpr(ly * 16 * (int)sizeof(float4)));
             if (full == 0xffffu) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb, dxs[lx],
-                                                 (float)lx, amin, clampv, row + lx);
+                                                 (float)lx, amin, clampv, row + lx, lx == 0);
             } else if (full == any) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
                         plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb,
-                                                     dxs[lx], (float)lx, amin, clampv, row + lx);
+                                                     dxs[lx], (float)lx, amin, clampv, row + lx,
+                                                     lx == 0);
             } else {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
                         plain_pair<HAS_OPAC>(A, kidx <= __builtin_amdgcn_readlane(fcol, lx), ap,
                                              bp, rt, op, cr, cg, cb, dxs[lx], (float)lx, amin,
-                                             clampv, row + lx);
+                                             clampv, row + lx, lx == 0);
             }
             plain_row_end(A, dxs[0], dy);
         }
