@@ -226,24 +226,24 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
     }
 }
 
-// Quadrants of a 16x16 tile (bit q: x half q & 1, y half q >> 1) that the
-// accept region of a Gaussian can reach.  In the forward's own terms the region
-// is {u : s(u) <= T}, s(u) = a' u_x^2 + b2' u_x u_y + c' u_y^2 with the staged
-// conic (a', b2', c') = (a, 2b, c) log2(e) / 2, u = mean - pixel centre, and
-// T = log2(255 o); (bx0, by0) is the tile's first pixel centre.  The form's
-// minimum over a quadrant's rectangle of centres is zero when the mean is
-// inside it and otherwise the least of its four edges' minima (along
+// Column strips of a 16x16 tile (bit k: columns 4k .. 4k + 3, all 16 rows)
+// that the accept region of a Gaussian can reach.  In the forward's own terms
+// the region is {u : s(u) <= T}, s(u) = a' u_x^2 + b2' u_x u_y + c' u_y^2 with
+// the staged conic (a', b2', c') = (a, 2b, c) log2(e) / 2, u = mean - pixel
+// centre, and T = log2(255 o); (bx0, by0) is the tile's first pixel centre.
+// The form's minimum over a strip's rectangle of centres is zero when the mean
+// is inside it and otherwise the least of its four edges' minima (along
 // u_x = const the form is least at u_y = -b2' u_x / (2 c'), clamped to the
-// edge; likewise across).  A quadrant is dropped when that minimum exceeds T
-// by more than 0.36 + 2^-10 T (0.5 + 2^-10 t in units of the unscaled
+// edge; likewise across).  A strip is dropped when that minimum exceeds T by
+// more than 0.36 + 2^-10 T (0.5 + 2^-10 t in units of the unscaled
 // d^T Q d <= t).  Where the conic's condition number is below 1e4
 // (tr^2 <= 1e4 det), a' + c' <= 46 (a + c <= 64) and the mean lies within
 // 65536 px, the FP32 error of this minimum and of the per-pixel s together stay
 // below 0.22 (terms at most 1e4 T; centre differences off by at most 2^-8 px),
-// so a dropped quadrant holds no pixel the accept test passes.  Other
-// Gaussians keep all four quadrants.
-__device__ __forceinline__ int reach_quadrants(float mx, float my, float ap, float bp2, float cp,
-                                               float T, float bx0, float by0) {
+// so a dropped strip holds no pixel the accept test passes.  Other Gaussians
+// keep all four strips.
+__device__ __forceinline__ int reach_strips(float mx, float my, float ap, float bp2, float cp,
+                                            float T, float bx0, float by0) {
     const float bp = 0.5f * bp2;
     const float det = ap * cp - bp * bp, tr = ap + cp;
     if (!(ap > 0.0f && cp > 0.0f && det > 0.0f && tr <= 46.0f && tr * tr <= 1e4f * det &&
@@ -254,30 +254,31 @@ __device__ __forceinline__ int reach_quadrants(float mx, float my, float ap, flo
     auto form = [&](float x, float y) {
         return __builtin_fmaf(ap * x, x, y * __builtin_fmaf(bp2, x, cp * y));
     };
+    // rows of every strip: centre offsets [y0, y1]
+    const float y0 = by0 - my, y1 = y0 + 15.0f;
+    const bool in_y = y0 <= 0.0f && y1 >= 0.0f;
     int m = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        // the quadrant's rectangle of centre offsets [x0, x1] x [y0, y1]
-        const float x0 = (bx0 + (float)((q & 1) * 8)) - mx, x1 = x0 + 7.0f;
-        const float y0 = (by0 + (float)((q >> 1) * 8)) - my, y1 = y0 + 7.0f;
-        const bool inside = x0 <= 0.0f && x1 >= 0.0f && y0 <= 0.0f && y1 >= 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        const float x0 = (bx0 + (float)(4 * k)) - mx, x1 = x0 + 3.0f;
+        const bool inside = in_y && x0 <= 0.0f && x1 >= 0.0f;
         const float e0 = form(x0, fminf(fmaxf(rc * x0, y0), y1));
         const float e1 = form(x1, fminf(fmaxf(rc * x1, y0), y1));
         const float e2 = form(fminf(fmaxf(ra * y0, x0), x1), y0);
         const float e3 = form(fminf(fmaxf(ra * y1, x0), x1), y1);
         const float least = fminf(fminf(e0, e1), fminf(e2, e3));
-        if (inside || !(least > lim)) m |= 1 << q;
+        if (inside || !(least > lim)) m |= 1 << k;
     }
     return m;
 }
 
-// Forward for 16x16 tiles.  Lane l owns pixel (l % 8, l / 8) of each 8x8
-// quadrant q (q = 0..3: x half q & 1, y half q >> 1), so the four pixels of a
-// lane share two columns and two rows (dx, dy, b'dy and c'dy are computed
-// once per Gaussian), and a Gaussian's cull box is tested against each
-// quadrant once, in the staging lane: quadrants it cannot reach are skipped
-// by a wave-uniform branch.  Pair evaluation and accumulation order are those
-// of raster_fwd_kernel, so the outputs are identical.
+// Forward for 16x16 tiles.  Lane l owns the four pixels (l % 4 + 4k, l / 4),
+// k = 0..3, of one row: one row's dy, b'dy and c'dy serve all four, and pixel
+// k of every lane lies in column strip k (columns 4k .. 4k + 3).  A Gaussian's
+// cull box and accept region are tested against each strip once, in the
+// staging lane; strips it cannot reach are skipped by a wave-uniform branch.
+// Pair evaluation and accumulation order are those of raster_fwd_kernel, so
+// the outputs are identical.
 //
 // TRAIN (fit step and render-only): no wsum / final index; with a target the
 // tile's clamp + L2 sum, over its pixels and channels of (clamp(I, 0, 1) - t)^2,
@@ -317,9 +318,12 @@ __device__ __forceinline__ void fwd16_tile(
     const int ty = tile / tiles_x;
     const int tx = tile - ty * tiles_x;
     const int x_org = tx * 16, y_org = ty * 16;
-    const int qx0 = lane & 7, qy0 = lane >> 3;
-    const float X0 = (float)(x_org + qx0) + 0.5f, X1 = X0 + 8.0f;
-    const float Y0 = (float)(y_org + qy0) + 0.5f, Y1 = Y0 + 8.0f;
+    const int px0 = lane & 3, py = lane >> 2;
+    // centres of the lane's pixels (exact: integers + 0.5)
+    float X[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) X[k] = (float)(x_org + px0 + 4 * k) + 0.5f;
+    const float Y = (float)(y_org + py) + 0.5f;
 
     const float amin = in_vgpr(kAlphaMin), clampv = in_vgpr(0.999f);
     float ar[4], ag[4], ab[4], aw[4];
@@ -345,8 +349,8 @@ __device__ __forceinline__ void fwd16_tile(
             gq[k] = j < cnt ? gids[base + j] : -1;
         }
         // qbits[k][q]: ballot over the k-th 64 staged Gaussians of "reaches
-        // quadrant q", so the loop below finds each Gaussian's quadrants with
-        // scalar bit tests and skips Gaussians that reach none
+        // strip q", so the loop below finds each Gaussian's strips with scalar
+        // bit tests and skips Gaussians that reach none
         uint64_t qbits[kSub][4];
         int box[kSub];
 #pragma unroll
@@ -359,22 +363,24 @@ __device__ __forceinline__ void fwd16_tile(
                 // without opacities the opacity slot carries blue
                 s1[j] = HAS_OPAC ? gp.p1 : make_float4(gp.p1.x, gp.p2.x, gp.p1.z, gp.p1.w);
                 if constexpr (HAS_OPAC) s2[j] = gp.p2.x;
-                // quadrant q spans pixel centres [org + 8h + 0.5, org + 8h + 7.5]
+                // strip k spans pixel centres x in [org + 4k + 0.5, org + 4k + 3.5]
                 const float xl = gp.p0.x - gp.p3.x, xh = gp.p0.x + gp.p3.x;
                 const float yl = gp.p0.y - gp.p3.y, yh = gp.p0.y + gp.p3.y;
                 const float bx0 = (float)x_org + 0.5f, by0 = (float)y_org + 0.5f;
                 // negated tests keep NaN boxes (NaN means) unculled, as the generic kernel
-                const bool cx0 = !(xl > bx0 + 7.0f) && !(xh < bx0);
-                const bool cx1 = !(xl > bx0 + 15.0f) && !(xh < bx0 + 8.0f);
-                const bool cy0 = !(yl > by0 + 7.0f) && !(yh < by0);
-                const bool cy1 = !(yl > by0 + 15.0f) && !(yh < by0 + 8.0f);
-                box[k] = (cx0 && cy0 ? 1 : 0) | (cx1 && cy0 ? 2 : 0) | (cx0 && cy1 ? 4 : 0) |
-                         (cx1 && cy1 ? 8 : 0);
+                const bool cy = !(yl > by0 + 15.0f) && !(yh < by0);
+                int b = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    b |= (cy && !(xl > bx0 + (float)(4 * q + 3)) && !(xh < bx0 + (float)(4 * q)))
+                             ? 1 << q
+                             : 0;
+                box[k] = b;
             }
         }
         // then the region itself, not only its box (about a fifth of the
-        // quadrants the box reaches hold no accepted pixel), from the lane's
-        // own staged entries
+        // strips the box reaches hold no accepted pixel), from the lane's own
+        // staged entries
 #pragma unroll
         for (int k = 0; k < kSub; ++k) {
             const int j = k * kThreads + lane;
@@ -383,8 +389,8 @@ __device__ __forceinline__ void fwd16_tile(
                 const float4 a0 = s0[j];
                 const float4 a1 = s1[j];
                 const float T = HAS_OPAC ? __log2f(255.0f * a1.y) : kAcceptLog2;
-                m &= reach_quadrants(a0.x, a0.y, a0.z, a0.w, a1.x, T, (float)x_org + 0.5f,
-                                     (float)y_org + 0.5f);
+                m &= reach_strips(a0.x, a0.y, a0.z, a0.w, a1.x, T, (float)x_org + 0.5f,
+                                  (float)y_org + 0.5f);
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) qbits[k][q] = __ballot((m >> q) & 1);
@@ -401,13 +407,11 @@ __device__ __forceinline__ void fwd16_tile(
                 const float4 a1 = s1[t];
                 const float blue = HAS_OPAC ? s2[t] : a1.y;
                 const int kidx = base + t;
-                const float dx0 = a0.x - X0, dx1 = a0.x - X1;
-                const RowTerms rw0 = row_terms(a0.w, a1.x, a0.y - Y0);
-                const RowTerms rw1 = row_terms(a0.w, a1.x, a0.y - Y1);
+                const RowTerms rw = row_terms(a0.w, a1.x, a0.y - Y);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     if (!((qbits[k][q] >> bit) & 1)) continue;
-                    const float s = pair_s(a0.z, (q & 2) ? rw1 : rw0, (q & 1) ? dx1 : dx0);
+                    const float s = pair_s(a0.z, rw, a0.x - X[q]);
                     const float e = fast_exp2(-s);
                     float al = vmin_f32(HAS_OPAC ? a1.y * e : e, clampv);
                     const bool ok = !(s < 0.0f) && !(al < amin);
@@ -428,8 +432,8 @@ __device__ __forceinline__ void fwd16_tile(
     float lsum = 0.0f;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int i = y_org + (q >> 1) * 8 + qy0;
-        const int j = x_org + (q & 1) * 8 + qx0;
+        const int i = y_org + py;
+        const int j = x_org + px0 + 4 * q;
         if (i < img_h && j < img_w) {
             const size_t pid = (size_t)i * img_w + j;
             out_img[3 * pid + 0] = ar[q];
