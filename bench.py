@@ -356,24 +356,25 @@ def main():
         torch.cuda.synchronize()
         log(f"warmup {i + 1}/{args.warmup}")
 
-    lvl_ms = {"l1": 0.0, "l2": 0.0}
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    # per-level events are recorded every step and read after the timed
+    # region, so the host never waits inside it
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
     fwd_t = KernelTimer("lig_fit_render_l2", dev)
     bwd_t = KernelTimer("lig_fit_backward_records_l2", dev)
     torch.cuda.synchronize()
     start = time.perf_counter()
     for i in range(args.steps):
+        ev = evs[i]
         ev[0].record()
         l1.step(t1)
         ev[1].record()
         with fwd_t, bwd_t:
             l2.step(t2)
         ev[2].record()
-        ev[2].synchronize()
-        lvl_ms["l1"] += ev[0].elapsed_time(ev[1])
-        lvl_ms["l2"] += ev[1].elapsed_time(ev[2])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - start
+    lvl_ms = {"l1": sum(ev[0].elapsed_time(ev[1]) for ev in evs),
+              "l2": sum(ev[1].elapsed_time(ev[2]) for ev in evs)}
     steps_per_s = args.steps / elapsed
     log(f"timed {args.steps} steps in {elapsed:.3f}s")
 
