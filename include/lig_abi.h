@@ -173,6 +173,27 @@ int lig_rasterize_forward(int n, int img_height, int img_width, int block_width,
                           float *out_wsum, int32_t *final_idx, float *out_dx, float *out_dy,
                           float *out_dxy, void *stream);
 
+/* No reference counterpart: lig_rasterize_forward with a workspace of
+ * lig_tile_order_workspace_size(num_tiles) bytes (or more), in which the
+ * 16 x 16 plain forward (no out_dx) orders its tiles' launch: heaviest (most
+ * pairs) first when the launch is short or the image's density is uneven
+ * enough that contiguous per-XCD shares of the tiles would leave some XCDs
+ * running long after the others.  The order changes speed only, never a
+ * result.  The backward does the same when its workspace
+ * (lig_rasterize_backward_records / lig_rasterize_backward with emission ids)
+ * has lig_tile_order_workspace_size(num_tiles) bytes to spare after the
+ * records: pass lig_rasterize_backward_workspace_size(num_intersects) + 256 +
+ * lig_tile_order_workspace_size(num_tiles). */
+size_t lig_tile_order_workspace_size(int num_tiles);
+int lig_rasterize_forward_ws(int n, int img_height, int img_width, int block_width,
+                             const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
+                             const float *xys, const float *conics, const float *colors,
+                             const float *opacities, const float *extents,
+                             const int32_t *cum_tiles_hit, void *packed, float *out_img,
+                             float *out_wsum, int32_t *final_idx, float *out_dx, float *out_dy,
+                             float *out_dxy, void *workspace, size_t workspace_bytes,
+                             void *stream);
+
 /* Replaces rasterize_backward_tensor (csrc/bindings.cu:403-506, kernels
  * csrc/backward.cu:8-183 and csrc/gradient_aware_rasterizer.cu:118-349).
  * v_output [H,W,3], v_wsum [H,W] or NULL (= zeros); v_output_dx/dy/dxy

@@ -50,6 +50,10 @@ _SIGNATURES = {
     "lig_rasterize_forward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                  _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lig_rasterize_forward_ws": (
+        _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "lig_tile_order_workspace_size": (_c_size, [_c_int]),
     "lig_rasterize_backward_workspace_size": (_c_size, [_c_int]),
     "lig_rasterize_backward": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -1490,67 +1490,6 @@ extern "C" int lig_pack_gaussians(int n, const float *xys, const float *conics,
     return launch_status();
 }
 
-extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int block_width,
-                                     const int32_t *gaussian_ids_sorted,
-                                     const int32_t *tile_bins, const float *xys,
-                                     const float *conics, const float *colors,
-                                     const float *opacities, const float *extents,
-                                     const int32_t *cum_tiles_hit, void *packed, float *out_img,
-                                     float *out_wsum, int32_t *final_idx, float *out_dx,
-                                     float *out_dy, float *out_dxy, void *stream) {
-    if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
-        return LIG_ERR_INVALID_ARGUMENT;
-    const bool ga = out_dx != nullptr;
-    // image only (no wsum / final_idx): 16 x 16 tiles, plain forward
-    const bool image_only = !out_wsum && !final_idx;
-    if (!tile_bins || !out_img || (!image_only && (!out_wsum || !final_idx)) ||
-        (image_only && (ga || block_width != 16)))
-        return LIG_ERR_INVALID_ARGUMENT;
-    if (ga && (!out_dy || !out_dxy)) return LIG_ERR_INVALID_ARGUMENT;
-    if (n > 0 && (!gaussian_ids_sorted || !xys || !conics || !colors || !packed))
-        return LIG_ERR_INVALID_ARGUMENT;
-    // record addressing for the backward without emission ids: 16 x 16 tiles,
-    // no opacities (so no zero-extent phantom slots)
-    const bool slots_in_pack = extents != nullptr;
-    if (slots_in_pack && (!cum_tiles_hit || block_width != 16 || opacities))
-        return LIG_ERR_INVALID_ARGUMENT;
-    hipStream_t st = as_stream(stream);
-    GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
-    const int tiles_x = (img_width + block_width - 1) / block_width;
-    const int tiles_y = (img_height + block_width - 1) / block_width;
-    if (n > 0)
-        hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
-                           colors, opacities, extents, cum_tiles_hit, tiles_x, tiles_y, nullptr,
-                           0, pk);
-    const dim3 grid((unsigned)(tiles_x * tiles_y));
-    const int px = (block_width * block_width + LIG_WAVE - 1) / LIG_WAVE;
-    const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
-    if (ga)
-        launch_fwd<true>(px, grid, st, tiles_x, block_width, img_width, img_height,
-                         gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, out_dx,
-                         out_dy, out_dxy);
-    else if (image_only && opacities)
-        hipLaunchKernelGGL((raster_fwd16_kernel<true, true>), grid, dim3(kThreads), 0, st,
-                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
-    else if (image_only)
-        hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), grid, dim3(kThreads), 0, st,
-                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
-    else if (block_width == 16 && opacities)
-        hipLaunchKernelGGL((raster_fwd16_kernel<true, false>), grid, dim3(kThreads), 0, st,
-                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
-    else if (block_width == 16)
-        hipLaunchKernelGGL((raster_fwd16_kernel<false, false>), grid, dim3(kThreads), 0, st,
-                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
-                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
-    else
-        launch_fwd<false>(px, grid, st, tiles_x, block_width, img_width, img_height,
-                          gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, nullptr,
-                          nullptr, nullptr);
-    return launch_status();
-}
 
 namespace lig {
 namespace {
@@ -1698,8 +1637,115 @@ long long fit_forward_resident(int depth) {
     if (depth == 128) return resident_blocks(raster_fwd16_kernel<false, true, 128>);
     return resident_blocks(raster_fwd16_kernel<false, true, 64>);
 }
+int rasterize_forward_impl(int n, int img_height, int img_width, int block_width,
+                                     const int32_t *gaussian_ids_sorted,
+                                     const int32_t *tile_bins, const float *xys,
+                                     const float *conics, const float *colors,
+                                     const float *opacities, const float *extents,
+                                     const int32_t *cum_tiles_hit, void *packed, float *out_img,
+                                     float *out_wsum, int32_t *final_idx, float *out_dx,
+                                     float *out_dy, float *out_dxy, void *workspace,
+                                     size_t workspace_bytes, void *stream) {
+    if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
+        return LIG_ERR_INVALID_ARGUMENT;
+    const bool ga = out_dx != nullptr;
+    // image only (no wsum / final_idx): 16 x 16 tiles, plain forward
+    const bool image_only = !out_wsum && !final_idx;
+    if (!tile_bins || !out_img || (!image_only && (!out_wsum || !final_idx)) ||
+        (image_only && (ga || block_width != 16)))
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (ga && (!out_dy || !out_dxy)) return LIG_ERR_INVALID_ARGUMENT;
+    if (n > 0 && (!gaussian_ids_sorted || !xys || !conics || !colors || !packed))
+        return LIG_ERR_INVALID_ARGUMENT;
+    // record addressing for the backward without emission ids: 16 x 16 tiles,
+    // no opacities (so no zero-extent phantom slots)
+    const bool slots_in_pack = extents != nullptr;
+    if (slots_in_pack && (!cum_tiles_hit || block_width != 16 || opacities))
+        return LIG_ERR_INVALID_ARGUMENT;
+    hipStream_t st = as_stream(stream);
+    GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
+    const int tiles_x = (img_width + block_width - 1) / block_width;
+    const int tiles_y = (img_height + block_width - 1) / block_width;
+    if (n > 0)
+        hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
+                           colors, opacities, extents, cum_tiles_hit, tiles_x, tiles_y, nullptr,
+                           0, pk);
+    const dim3 grid((unsigned)(tiles_x * tiles_y));
+    const int px = (block_width * block_width + LIG_WAVE - 1) / LIG_WAVE;
+    const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
+    // 16 x 16 plain forward: launch order in the workspace when one is given
+    const int ntiles = tiles_x * tiles_y;
+    const int32_t *order = nullptr;
+    if (!ga && block_width == 16 && workspace && workspace_bytes >= order_ws_bytes(ntiles)) {
+        const long long resident =
+            image_only ? (opacities ? resident_blocks(raster_fwd16_kernel<true, true>)
+                                    : resident_blocks(raster_fwd16_kernel<false, true>))
+                       : (opacities ? resident_blocks(raster_fwd16_kernel<true, false>)
+                                    : resident_blocks(raster_fwd16_kernel<false, false>));
+        order = launch_tile_order(ntiles, bins, ntiles, resident, workspace, st);
+    }
+    if (ga)
+        launch_fwd<true>(px, grid, st, tiles_x, block_width, img_width, img_height,
+                         gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, out_dx,
+                         out_dy, out_dxy);
+    else if (image_only && opacities)
+        hipLaunchKernelGGL((raster_fwd16_kernel<true, true>), grid, dim3(kThreads), 0, st,
+                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
+                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, order);
+    else if (image_only)
+        hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), grid, dim3(kThreads), 0, st,
+                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
+                           nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, order);
+    else if (block_width == 16 && opacities)
+        hipLaunchKernelGGL((raster_fwd16_kernel<true, false>), grid, dim3(kThreads), 0, st,
+                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
+                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0, nullptr, order);
+    else if (block_width == 16)
+        hipLaunchKernelGGL((raster_fwd16_kernel<false, false>), grid, dim3(kThreads), 0, st,
+                           tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
+                           out_wsum, final_idx, nullptr, nullptr, nullptr, 0, nullptr, order);
+    else
+        launch_fwd<false>(px, grid, st, tiles_x, block_width, img_width, img_height,
+                          gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, nullptr,
+                          nullptr, nullptr);
+    return launch_status();
+}
+
 }  // namespace
 }  // namespace lig
+
+extern "C" size_t lig_tile_order_workspace_size(int num_tiles) {
+    return num_tiles > 0 ? order_ws_bytes(num_tiles) : 0;
+}
+
+extern "C" int lig_rasterize_forward(int n, int img_height, int img_width, int block_width,
+                                     const int32_t *gaussian_ids_sorted,
+                                     const int32_t *tile_bins, const float *xys,
+                                     const float *conics, const float *colors,
+                                     const float *opacities, const float *extents,
+                                     const int32_t *cum_tiles_hit, void *packed, float *out_img,
+                                     float *out_wsum, int32_t *final_idx, float *out_dx,
+                                     float *out_dy, float *out_dxy, void *stream) {
+    return rasterize_forward_impl(n, img_height, img_width, block_width, gaussian_ids_sorted,
+                                  tile_bins, xys, conics, colors, opacities, extents,
+                                  cum_tiles_hit, packed, out_img, out_wsum, final_idx, out_dx,
+                                  out_dy, out_dxy, nullptr, 0, stream);
+}
+
+extern "C" int lig_rasterize_forward_ws(int n, int img_height, int img_width, int block_width,
+                                        const int32_t *gaussian_ids_sorted,
+                                        const int32_t *tile_bins, const float *xys,
+                                        const float *conics, const float *colors,
+                                        const float *opacities, const float *extents,
+                                        const int32_t *cum_tiles_hit, void *packed,
+                                        float *out_img, float *out_wsum, int32_t *final_idx,
+                                        float *out_dx, float *out_dy, float *out_dxy,
+                                        void *workspace, size_t workspace_bytes, void *stream) {
+    return rasterize_forward_impl(n, img_height, img_width, block_width, gaussian_ids_sorted,
+                                  tile_bins, xys, conics, colors, opacities, extents,
+                                  cum_tiles_hit, packed, out_img, out_wsum, final_idx, out_dx,
+                                  out_dy, out_dxy, workspace, workspace_bytes, stream);
+}
 
 extern "C" size_t lig_fit_render_workspace_size(int img_height, int img_width) {
     if (img_height <= 0 || img_width <= 0) return 256;
@@ -1860,7 +1906,7 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
                           const float *v_output_dxy, int has_opacities, const int32_t *slots,
                           GradRec *rec, float *v_xy, float *v_xy_abs, float *v_conic,
                           float *v_colors, float *v_opacity, long long num_intersects,
-                          hipStream_t st) {
+                          hipStream_t st, void *order_ws = nullptr, size_t order_ws_len = 0) {
     const bool ga = v_output_dx != nullptr;
     const bool det = rec != nullptr;
     const int tiles_x = (img_width + block_width - 1) / block_width;
@@ -1870,6 +1916,18 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
                           ? bwd16_split_for(has_opacities, det, tiles_x * tiles_y, num_intersects)
                           : 1;
     const dim3 grid16((unsigned)(tiles_x * tiles_y * split));
+    // 16 x 16 plain backward: launch order in the caller's spare workspace
+    const int32_t *order = nullptr;
+    if (block_width == 16 && !ga && order_ws &&
+        order_ws_len >= order_ws_bytes(tiles_x * tiles_y)) {
+        const long long resident =
+            has_opacities ? (det ? resident_blocks(raster_bwd16_kernel<true, true, false>)
+                                 : resident_blocks(raster_bwd16_kernel<true, false, false>))
+                          : (det ? resident_blocks(raster_bwd16_kernel<false, true, false>)
+                                 : resident_blocks(raster_bwd16_kernel<false, false, false>));
+        order = launch_tile_order(tiles_x * tiles_y, reinterpret_cast<const int2 *>(tile_bins),
+                                  (long long)tiles_x * tiles_y * split, resident, order_ws, st);
+    }
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
     const GaussPack *pk = reinterpret_cast<const GaussPack *>(packed);
 #define LIG_BWD_ARGS_PLAIN                                                                    \
@@ -1878,7 +1936,7 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
 #define LIG_BWD_ARGS_16                                                                      \
     tiles_x, img_width, img_height, gids, slots, bins, pk, final_idx, v_output, v_wsum,         \
         nullptr, nullptr, nullptr, 0.0f, rec, v_xy, v_xy_abs, v_conic, v_colors, v_opacity,      \
-        nullptr, 0, split, nullptr
+        nullptr, 0, split, order
 #define LIG_BWD_ARGS_GA                                                                      \
     tiles_x, block_width, img_width, img_height, gids, slots, bins, pk, final_idx, v_output,    \
         v_wsum, v_output_dx, v_output_dy, v_output_dxy, has_opacities, rec, v_xy, v_xy_abs,    \
@@ -1943,11 +2001,16 @@ extern "C" int lig_rasterize_backward_records(int n, int img_height, int img_wid
         return LIG_ERR_WORKSPACE;
     if (n == 0) return LIG_OK;
     GradRec *rec = reinterpret_cast<GradRec *>(reinterpret_cast<char *>(workspace) + 256);
+    // room after the records (a workspace sized with lig_tile_order_workspace_size
+    // on top) holds the launch order
+    const size_t off = (256 + (size_t)num_intersects * sizeof(GradRec) + 255) & ~size_t(255);
+    void *ows = workspace_bytes > off ? reinterpret_cast<char *>(workspace) + off : nullptr;
     return launch_backward_tiles(n, img_height, img_width, block_width, gaussian_ids_sorted,
                                  tile_bins, packed, final_idx, v_output, v_wsum, v_output_dx,
                                  v_output_dy, v_output_dxy, has_opacities, emission_ids_sorted,
                                  rec, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                 num_intersects, as_stream(stream));
+                                 num_intersects, as_stream(stream), ows,
+                                 ows ? workspace_bytes - off : 0);
 }
 
 extern "C" int lig_fit_backward_records_l2(int n, int img_height, int img_width,

@@ -112,16 +112,29 @@ def rasterize_image(img_height, img_width, gaussian_ids_sorted, tile_bins, xys, 
     H, W = int(img_height), int(img_width)
     out_img = _empty((H, W, 3), xys)
     packed = _workspace(L.size_query("lig_packed_gaussians_bytes", n), xys)
+    ows = _order_workspace(H, W, 16, xys)
     with torch.cuda.device(xys.device):
-        L.call("lig_rasterize_forward", n, H, W, 16, L.ptr(gaussian_ids_sorted),
+        L.call("lig_rasterize_forward_ws", n, H, W, 16, L.ptr(gaussian_ids_sorted),
                L.ptr(tile_bins), L.ptr(xys), L.ptr(conics), L.ptr(colors), L.ptr(opacities),
                None, None, L.ptr(packed), L.ptr(out_img), None, None, None, None, None,
-               L.stream_of(xys))
+               L.ptr(ows), ows.numel(), L.stream_of(xys))
     return out_img
 
 
 def _contig(t):
     return t.contiguous() if t is not None else None
+
+
+def _num_tiles(img_height, img_width, block_width):
+    bw = int(block_width)
+    return ((int(img_height) + bw - 1) // bw) * ((int(img_width) + bw - 1) // bw)
+
+
+def _order_workspace(img_height, img_width, block_width, like):
+    """Workspace in which a 16 x 16 raster launch orders its tiles
+    (lig_tile_order_workspace_size; speed only, no result depends on it)."""
+    return _workspace(L.size_query("lig_tile_order_workspace_size",
+                                   _num_tiles(img_height, img_width, block_width)), like)
 
 
 def project_gaussians_backward(num_points, extent, conics, v_xy, v_conic):
@@ -310,11 +323,13 @@ def pack_gaussians(xys, conics, colors, opacities):
 
 def rasterize_forward_packed(img_height, img_width, block_width, gaussian_ids_sorted, tile_bins,
                              xys, conics, colors, opacities, compute_upscale_gradients,
-                             extents=None, cum_tiles_hit=None):
+                             extents=None, cum_tiles_hit=None, ordered=True):
     """Forward that also returns the packed records for the backward:
     (out_img, out_wsum, out_dx, out_dy, out_dxy, final_idx, packed).  With
     extents and cum_tiles_hit (16 x 16 tiles, no opacities) the pack also
-    addresses the backward's records, which then need no emission ids."""
+    addresses the backward's records, which then need no emission ids.
+    ordered=False launches the tiles in the fixed per-XCD order (same
+    results; the launch order only changes speed)."""
     for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),
                       ("tile_bins", tile_bins, _i32), ("xys", xys, _f32),
                       ("conics", conics, _f32), ("colors", colors, _f32),
@@ -331,11 +346,13 @@ def rasterize_forward_packed(img_height, img_width, block_width, gaussian_ids_so
     if compute_upscale_gradients:
         dx, dy, dxy = (_empty((H, W, 3), xys) for _ in range(3))
     packed = _workspace(L.size_query("lig_packed_gaussians_bytes", n), xys)
+    ows = _order_workspace(H, W, block_width, xys) if ordered else None
     with torch.cuda.device(xys.device):
-        L.call("lig_rasterize_forward", n, H, W, int(block_width), L.ptr(gaussian_ids_sorted),
-               L.ptr(tile_bins), L.ptr(xys), L.ptr(conics), L.ptr(colors), L.ptr(opacities),
-               L.ptr(extents), L.ptr(cum_tiles_hit), L.ptr(packed), L.ptr(out_img),
-               L.ptr(out_wsum), L.ptr(final_idx), L.ptr(dx), L.ptr(dy), L.ptr(dxy),
+        L.call("lig_rasterize_forward_ws", n, H, W, int(block_width),
+               L.ptr(gaussian_ids_sorted), L.ptr(tile_bins), L.ptr(xys), L.ptr(conics),
+               L.ptr(colors), L.ptr(opacities), L.ptr(extents), L.ptr(cum_tiles_hit),
+               L.ptr(packed), L.ptr(out_img), L.ptr(out_wsum), L.ptr(final_idx), L.ptr(dx),
+               L.ptr(dy), L.ptr(dxy), L.ptr(ows), ows.numel() if ordered else 0,
                L.stream_of(xys))
     return out_img, out_wsum, dx, dy, dxy, final_idx, packed
 
@@ -344,11 +361,13 @@ def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_s
                               packed, num_points, final_idx, v_output, v_render_wsum,
                               has_opacities, v_output_dx=None, v_output_dy=None,
                               v_output_dxy=None, emission_ids_sorted=None, cum_tiles_hit=None,
-                              slots_in_pack=False):
+                              slots_in_pack=False, ordered=True):
     """Backward over packed records; v_render_wsum may be None (zeros).  With
     cum_tiles_hit and either emission_ids_sorted (from bin_and_sort_fast) or
     slots_in_pack (packed by a forward given extents and cum_tiles_hit) the
-    per-Gaussian sums are reduced deterministically; otherwise by atomics."""
+    per-Gaussian sums are reduced deterministically; otherwise by atomics.
+    The deterministic form orders its tiles' launch in spare workspace unless
+    ordered=False (same results)."""
     v_output, v_render_wsum, v_output_dx, v_output_dy, v_output_dxy = map(
         _contig, (v_output, v_render_wsum, v_output_dx, v_output_dy, v_output_dxy))
     for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),
@@ -381,7 +400,11 @@ def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_s
     with torch.cuda.device(like.device):
         if det:
             m = int(gaussian_ids_sorted.numel())
-            ws = _workspace(L.size_query("lig_rasterize_backward_workspace_size", m), like)
+            nbytes = L.size_query("lig_rasterize_backward_workspace_size", m)
+            if ordered:
+                nbytes += 256 + L.size_query("lig_tile_order_workspace_size",
+                                             _num_tiles(img_height, img_width, block_width))
+            ws = _workspace(nbytes, like)
             L.call("lig_rasterize_backward_records", *common, L.ptr(emission_ids_sorted), m,
                    L.ptr(ws), ws.numel(), L.stream_of(like))
             L.call("lig_reduce_gradient_records", n, m, L.ptr(cum_tiles_hit), hop, L.ptr(ws),

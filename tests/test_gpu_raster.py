@@ -144,6 +144,41 @@ def test_forward_region_cull_keeps_accepted_pairs(device, oracle_lib, offdiag, o
     assert np.abs(_np(wsum) - ref["wsum"])[clean].max() <= 1e-5
 
 
+@pytest.mark.parametrize("H,skew,op", [(4096, True, False), (4096, True, True), (96, False, True)])
+def test_api_launch_order_changes_no_bits(device, H, skew, op):
+    """The API raster calls order their 16 x 16 tiles' launch in workspace
+    (heaviest first when the launch is short or the per-XCD shares' loads
+    differ); ordered and fixed-order launches give the same bits: image,
+    weight sum, final index, and the deterministic backward's gradients."""
+    import lig_amd.gsplat2d as g
+    from lig_amd import fit
+    from lig_amd.gsplat2d import cuda as C
+    n = 60000 if H > 1000 else 3000
+    print(f"api launch order H={H} skew={skew} op={op}", flush=True)
+    lv = fit.Level(n, H, H, 8, 3.0, device=device)
+    with torch.no_grad():
+        if skew:
+            lv.means[: n // 2, 1] *= 0.25
+        xys, ext, con, nt = lv.project()
+    opac = torch.rand(n, device=device) * 0.9 + 0.05 if op else None
+    num, cum = g.compute_cumulative_intersects(nt)
+    gids, bins, eids = C.bin_and_sort_fast(n, num, xys, ext, cum, H, H, 16,
+                                           with_emission_ids=True)
+    runs = []
+    v_out = torch.randn(H, H, 3, device=device)
+    v_w = torch.randn(H, H, device=device)
+    for ordered in (True, False):
+        img, wsum, _, _, _, fin, packed = C.rasterize_forward_packed(
+            H, H, 16, gids, bins, xys, con, lv.rgbs.detach(), opac, False, ordered=ordered)
+        grads = C.rasterize_backward_packed(H, H, 16, gids, bins, packed, n, fin, v_out, v_w, op,
+                                            emission_ids_sorted=eids, cum_tiles_hit=cum,
+                                            ordered=ordered)
+        runs.append((img, wsum, fin) + tuple(x for x in grads if x is not None))
+    assert torch.count_nonzero(runs[0][3]) > 0
+    for k, (a, b) in enumerate(zip(*runs)):
+        assert torch.equal(a, b), k
+
+
 def test_raster_api_end_to_end(device, oracle_lib):
     """project -> rasterize -> loss -> backward through the autograd wrappers."""
     import lig_amd.gsplat2d as g
