@@ -1574,6 +1574,7 @@ long long resident_blocks(KernelT kernel) {
 // heaviest-first order is always taken.
 const int32_t *launch_tile_order(int ntiles, const int2 *bins, long long blocks,
                                  long long resident, void *ws, hipStream_t st) {
+    if (ntiles <= 0 || !ws) return nullptr;
     int32_t *order = reinterpret_cast<int32_t *>(ws);
     long long *band = reinterpret_cast<long long *>(
         reinterpret_cast<char *>(ws) + (((size_t)ntiles * sizeof(int32_t) + 255) & ~size_t(255)));
