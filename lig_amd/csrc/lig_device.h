@@ -216,15 +216,4 @@ __device__ __forceinline__ float in_vgpr(float x) {
     return r;
 }
 
-// fminf of two values that are results of arithmetic or of in_vgpr (never a
-// signalling NaN), as one v_min_f32.  In IEEE mode fminf makes the compiler
-// quiet an operand it cannot prove canonical (a v_max x, x) before the min in
-// every basic block that uses it; for these operands that is a no-op, and the
-// instruction's own NaN rule (a quiet NaN operand yields the other) is fminf's.
-__device__ __forceinline__ float vmin_f32(float a, float b) {
-    float r;
-    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-
 }  // namespace lig

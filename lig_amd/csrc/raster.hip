@@ -325,7 +325,9 @@ __device__ __forceinline__ void fwd16_tile(
     for (int k = 0; k < 4; ++k) X[k] = (float)(x_org + px0 + 4 * k) + 0.5f;
     const float Y = (float)(y_org + py) + 0.5f;
 
-    const float amin = in_vgpr(kAlphaMin), clampv = in_vgpr(0.999f);
+    // the clamp is a literal operand: fminf against a VGPR the compiler cannot
+    // prove canonical would add a quieting v_max before every min
+    const float amin = in_vgpr(kAlphaMin);
     float ar[4], ag[4], ab[4], aw[4];
     int fin[4];
 #pragma unroll
@@ -413,7 +415,7 @@ __device__ __forceinline__ void fwd16_tile(
                     if (!((qbits[k][q] >> bit) & 1)) continue;
                     const float s = pair_s(a0.z, rw, a0.x - X[q]);
                     const float e = fast_exp2(-s);
-                    float al = vmin_f32(HAS_OPAC ? a1.y * e : e, clampv);
+                    float al = fminf(HAS_OPAC ? a1.y * e : e, 0.999f);
                     const bool ok = !(s < 0.0f) && !(al < amin);
                     al = ok ? al : 0.0f;
                     ar[q] = __builtin_fmaf(al, a1.z, ar[q]);
@@ -812,8 +814,9 @@ __device__ __forceinline__ void plain_row_end(PlainAcc &A, float D, float dy) {
 // One pair of the plain backward.  Rejected pairs (lane test, sigma < 0,
 // alpha < 1/255) leave the accumulators untouched, as the reference's
 // `continue`: the accumulation runs under the lane mask of accepted pairs and
-// is skipped when no lane accepts.  amin = 1/255 and clampv = 0.99 come in
-// VGPRs.
+// is skipped when no lane accepts.  amin = 1/255 comes in a VGPR; clampv =
+// 0.99 is passed as a literal (a VGPR clamp the compiler cannot prove
+// canonical costs a quieting v_max before the min).
 template <bool HAS_OPAC, bool ABS = true>
 __device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, float bp,
                                            const RowTerms &rt, float op, float cr, float cg,
@@ -885,7 +888,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
     stage_tile_grads<false>(bw, x_org, y_org, img_w, img_h, final_idx, v_out, v_wsum, nullptr,
                             nullptr, nullptr, sv, sfin, nullptr, nullptr, nullptr);
 
-    const float amin = in_vgpr(kAlphaMin), clampv = in_vgpr(0.99f);
+    const float amin = in_vgpr(kAlphaMin), clampv = 0.99f;
     for (int base = range.x; base < range.y; base += kThreads) {
         const bool act = base + lane < range.y;
         const int kidx = act ? base + lane : INT32_MAX;
@@ -982,14 +985,26 @@ __device__ __forceinline__ void bwd16_tile(
         const GaussPack gp = pack[g];
         const float mx = gp.p0.x, my = gp.p0.y, ap = gp.p0.z, bp2 = gp.p0.w, bp = 0.5f * bp2;
         const float cp = gp.p1.x, op = gp.p1.y, cr = gp.p1.z, cg = gp.p1.w, cb = gp.p2.x;
-        // exact: x_org + lx + 0.5 is representable, one rounding in the difference
-        float dxs[16];
+        // exact: x_org + lx + 0.5 is representable, one rounding in the difference.
+        // The fit step holds the 16 differences in VGPRs; the API path (more
+        // accumulators, the final-index gating) keeps the 16 column centres
+        // wave-uniform in SGPRs and forms each difference where it is used,
+        // the same expression, to stay within 72 VGPRs.
+        float dxs[TRAIN ? 16 : 1];
+        float xcs[TRAIN ? 1 : 16];
 #pragma unroll
-        for (int lx = 0; lx < 16; ++lx) dxs[lx] = mx - ((float)(x_org + lx) + 0.5f);
+        for (int lx = 0; lx < 16; ++lx) {
+            if constexpr (TRAIN)
+                dxs[lx] = mx - ((float)(x_org + lx) + 0.5f);
+            else
+                xcs[lx] = __int_as_float(__builtin_amdgcn_readfirstlane(
+                    __float_as_int((float)(x_org + lx) + 0.5f)));
+        }
+        auto dx_of = [&](int lx) { return TRAIN ? dxs[TRAIN ? lx : 0] : mx - xcs[TRAIN ? 0 : lx]; };
         PlainAcc A;
         plain_acc_zero(A);
         const int full_from = base + kThreads - 1;
-        const float amin = in_vgpr(kAlphaMin), clampv = in_vgpr(0.99f);
+        const float amin = in_vgpr(kAlphaMin), clampv = 0.99f;
         for (int ly = 0; ly < nrows; ++ly) {
             int fcol = 0;
             uint32_t any = cols, full = cols;
@@ -1007,24 +1022,24 @@ __device__ __forceinline__ void bwd16_tile(
             if (full == 0xffffu) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
-                    plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb, dxs[lx],
+                    plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb, dx_of(lx),
                                                  (float)lx, amin, clampv, row + lx, lx == 0);
             } else if (full == any) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
                         plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb,
-                                                     dxs[lx], (float)lx, amin, clampv, row + lx,
+                                                     dx_of(lx), (float)lx, amin, clampv, row + lx,
                                                      lx == 0);
             } else {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
                         plain_pair<HAS_OPAC>(A, kidx <= __builtin_amdgcn_readlane(fcol, lx), ap,
-                                             bp, rt, op, cr, cg, cb, dxs[lx], (float)lx, amin,
+                                             bp, rt, op, cr, cg, cb, dx_of(lx), (float)lx, amin,
                                              clampv, row + lx, lx == 0);
             }
-            plain_row_end(A, dxs[0], dy);
+            plain_row_end(A, dx_of(0), dy);
         }
         if constexpr (TRAIN) {
             if (act) {
