@@ -1000,7 +1000,12 @@ __device__ __forceinline__ void bwd16_tile(
                 xcs[lx] = __int_as_float(__builtin_amdgcn_readfirstlane(
                     __float_as_int((float)(x_org + lx) + 0.5f)));
         }
-        auto dx_of = [&](int lx) { return TRAIN ? dxs[TRAIN ? lx : 0] : mx - xcs[TRAIN ? 0 : lx]; };
+        auto dx_of = [&](int lx) {
+            if constexpr (TRAIN)
+                return dxs[lx];
+            else
+                return mx - xcs[lx];
+        };
         PlainAcc A;
         plain_acc_zero(A);
         const int full_from = base + kThreads - 1;
