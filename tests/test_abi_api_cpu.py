@@ -49,6 +49,17 @@ def test_backward_shares_override_is_host_state():
         lib.lig_set_backward_shares(prev)
 
 
+def test_tile_order_workspace_size():
+    """The launch-order workspace holds one int per tile plus the class
+    counts, band loads and cursors; none for an empty grid."""
+    import lig_amd._lib as L
+    assert L.size_query("lig_tile_order_workspace_size", 0) == 0
+    for t in (1, 7, 16384, 262144):
+        n = L.size_query("lig_tile_order_workspace_size", t)
+        assert n >= 4 * t + 8 * 8 + 2 * 128 * 4, (t, n)
+        assert n % 8 == 0
+
+
 def test_invalid_arguments_rejected_without_gpu():
     import ctypes
     import lig_amd._lib as L
