@@ -1591,10 +1591,11 @@ long long resident_blocks(KernelT kernel) {
 // Launch order of `ntiles` tiles (order_hist_kernel / order_scatter_kernel)
 // in `ws`, order_ws_bytes(ntiles) bytes, 8-byte aligned.  `blocks` blocks
 // will take the tiles and `resident` of them run at once: under 8 rounds the
-// heaviest-first order is always taken.
+// heaviest-first order is always taken; within one round every block starts
+// at once and the order cannot matter, so none is computed (null).
 const int32_t *launch_tile_order(int ntiles, const int2 *bins, long long blocks,
                                  long long resident, void *ws, hipStream_t st) {
-    if (ntiles <= 0 || !ws) return nullptr;
+    if (ntiles <= 0 || !ws || blocks <= resident) return nullptr;
     int32_t *order = reinterpret_cast<int32_t *>(ws);
     long long *band = reinterpret_cast<long long *>(
         reinterpret_cast<char *>(ws) + (((size_t)ntiles * sizeof(int32_t) + 255) & ~size_t(255)));
