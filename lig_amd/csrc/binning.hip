@@ -493,11 +493,13 @@ __global__ __launch_bounds__(kEmitThreads) void emit_isect_i64_kernel(
 // reference's zero fill leaves them).  The sort's values are the slots
 // themselves (implicit identity), so the sorted order maps back to each
 // Gaussian's contiguous slot range.  Each wave computes the tile rectangles
-// of its 64 Gaussians, then covers their joint slot range 64 slots at a time:
-// a lane finds the Gaussian that owns its slot by a binary search over the
-// lanes' range starts (non-decreasing, as cum is), decodes the tile from the
-// slot's offset in that Gaussian's row-major rectangle, and every store is
-// coalesced.
+// of its 64 Gaussians, parks them in LDS, then covers their joint slot range
+// 64 slots at a time: a lane finds the Gaussian that owns its slot by
+// counting the range starts (non-decreasing, as cum is) at or before it, in
+// two levels (every eighth start from scalar registers, then the seven inside
+// the chosen group of eight from LDS), decodes the tile from the slot's offset
+// in that Gaussian's row-major rectangle (row by a reciprocal estimate with an
+// exact correction), and every store is coalesced.
 //
 // Batched (tab != NULL, 16 x 16 tiles): each Gaussian's rectangle is taken in
 // its own image's tile grid and its keys are offset by the image's first
@@ -533,29 +535,61 @@ __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
             w = max(x1 - x0, 1);
         }
     }
-    const int wstart = __shfl(start, 0, LIG_WAVE);
+    // the wave's rectangles in LDS; only this wave reads its rows, and a
+    // wave's LDS operations complete in issue order
+    constexpr int kWaves = kEmitThreads / LIG_WAVE;
+    __shared__ __attribute__((aligned(16))) int s_start[kWaves][LIG_WAVE];  // read as int4
+    __shared__ int4 s_rect[kWaves][LIG_WAVE];  // (start, x0, y0, w)
+    __shared__ int4 s_grid[kWaves][LIG_WAVE];  // (row length, tile offset, 1/w bits, -)
+    const int wv = threadIdx.x / LIG_WAVE;
+    const float rcp = w > 0 ? __builtin_amdgcn_rcpf((float)w) : 0.0f;
+    s_start[wv][lane] = start;
+    s_rect[wv][lane] = make_int4(start, x0, y0, w);
+    s_grid[wv][lane] = make_int4(row_len, toff, __float_as_int(rcp), 0);
+    __builtin_amdgcn_wave_barrier();
+    // every eighth start, wave-uniform
+    int b8[LIG_WAVE / 8];
+#pragma unroll
+    for (int i = 1; i < LIG_WAVE / 8; ++i) b8[i] = __builtin_amdgcn_readlane(start, 8 * i);
+    const int wstart = __builtin_amdgcn_readfirstlane(start);
     for (int base = wstart; base < wend; base += LIG_WAVE) {
         const int pos = base + lane;
-        // owner: the last lane whose range starts at or before pos; its range
-        // ends where the next lane's starts, past pos
-        int j = 0;
+        // owner: the last lane whose range starts at or before pos (its range
+        // ends where the next lane's starts, past pos); starts are
+        // non-decreasing, so that is the count of starts 1..63 at or before pos
+        int j8 = 0;
 #pragma unroll
-        for (int step = LIG_WAVE / 2; step > 0; step >>= 1) {
-            if (__shfl(start, j + step, LIG_WAVE) <= pos) j += step;
-        }
-        const int s0 = __shfl(start, j, LIG_WAVE);
-        const int rx0 = __shfl(x0, j, LIG_WAVE);
-        const int ry0 = __shfl(y0, j, LIG_WAVE);
-        const int rw = __shfl(w, j, LIG_WAVE);
-        const uint32_t rlen = (uint32_t)__shfl(row_len, j, LIG_WAVE);
-        const uint32_t roff = (uint32_t)__shfl(toff, j, LIG_WAVE);
+        for (int i = 1; i < LIG_WAVE / 8; ++i) j8 += b8[i] <= pos ? 8 : 0;
+        const int4 lo4 = *reinterpret_cast<const int4 *>(&s_start[wv][j8]);
+        const int4 hi4 = *reinterpret_cast<const int4 *>(&s_start[wv][j8 + 4]);
+        const int j = j8 + (lo4.y <= pos) + (lo4.z <= pos) + (lo4.w <= pos) + (hi4.x <= pos) +
+                      (hi4.y <= pos) + (hi4.z <= pos) + (hi4.w <= pos);
+        const int4 rc = s_rect[wv][j];
+        const int4 gr = s_grid[wv][j];
         if (pos < wend) {
             uint32_t key = 0;
             int gid = 0;
+            const int rw = rc.w;
             if (rw > 0) {
-                const int k = pos - s0;
-                const int row = k / rw;
-                key = roff + (uint32_t)(ry0 + row) * rlen + (uint32_t)(rx0 + k - row * rw);
+                const int k = pos - rc.x;
+                // row = k / rw: the estimate is within one of it while k < 2^22,
+                // and the exact division covers anything further off
+                int row = (int)((float)k * __int_as_float(gr.z));
+                int col = k - row * rw;
+                if (col >= rw) {
+                    ++row;
+                    col -= rw;
+                }
+                if (col < 0) {
+                    --row;
+                    col += rw;
+                }
+                if ((unsigned)col >= (unsigned)rw) {
+                    row = k / rw;
+                    col = k - row * rw;
+                }
+                key = (uint32_t)gr.y + (uint32_t)(rc.z + row) * (uint32_t)gr.x +
+                      (uint32_t)(rc.y + col);
                 gid = g0 + j;
             }
             keys[pos] = key;

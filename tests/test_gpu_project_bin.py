@@ -118,6 +118,32 @@ def test_binning_bit_exact(device, oracle_lib, n, H, W, bw, op, chol, seed):
         np.testing.assert_array_equal(_np(a), b, err_msg=name)
 
 
+def test_binning_wide_rectangles(device, oracle_lib):
+    """Tile rectangles from one tile up to the whole 256 x 256 grid, so the
+    emission's row/column decode of a slot sees offsets up to 65535 and
+    widths from 1 to 256."""
+    from lig_amd.gsplat2d import cuda as C
+    import lig_amd.gsplat2d as g
+    H = W = 4096
+    bw = 16
+    s = random_scene(1500, H, W, 8, cholesky=True, scale=(0.5, 300.0), margin=200.0)
+    s["par"][:4, 0] = 3000.0  # four Gaussians whose rectangles are the whole grid
+    s["par"][:4, 2] = 3000.0
+    s["par"][4:40, 0] = 0.6   # tall, one-tile-wide rectangles
+    s["par"][4:40, 1] = 0.0
+    print("binning wide rectangles 4096^2 n=1500", flush=True)
+    xys, ext, con, nt = oracle_lib.project_gaussians_forward(s["par"], s["means"], H, W, bw,
+                                                             None, cholesky=True)
+    m, cum, isect, gids, ks, gs, bins = oracle_lib.bin_and_sort(xys, ext, nt, H, W, bw)
+    assert int(nt.max()) == (H // bw) * (W // bw)
+    num_int, cum_d = g.compute_cumulative_intersects(_t(nt, device, torch.int32))
+    assert num_int == m
+    gids_d, bins_d = C.bin_and_sort_fast(1500, m, _t(xys, device), _t(ext, device), cum_d, H,
+                                         W, bw)
+    np.testing.assert_array_equal(_np(gids_d), gs)
+    np.testing.assert_array_equal(_np(bins_d), bins)
+
+
 def test_sort_signed_keys_and_large(device, oracle_lib):
     """Signed 64-bit keys (negative depths sign-extend) and a multi-block sort."""
     from lig_amd.gsplat2d import cuda as C
