@@ -133,6 +133,51 @@ def _pmc_means(path: str) -> dict:
     return out
 
 
+def valu_issue(path: str, top: int = 16, cus: int = 256, xcds: int = 8) -> str:
+    """VALU issue rate per (kernel, launch size) from one rocprofv3 run with
+    `--pmc SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS SQ_WAVES SQ_BUSY_CYCLES
+    GRBM_GUI_ACTIVE GRBM_COUNT --kernel-trace`.  GRBM_COUNT counts GPU clocks
+    over the dispatch summed over the `xcds` dies, so a CU's cycles are
+    GRBM_COUNT / xcds and the clock is that over the dispatch's duration;
+    the rates are wave-level VALU instructions per second (G) and per CU per
+    cycle."""
+    import glob
+    import sqlite3
+    dbs = [path] if path.endswith(".db") else sorted(
+        glob.glob(f"{path}/**/*_results.db", recursive=True))
+    per = {}
+    meta = {}
+    for db in dbs:
+        con = sqlite3.connect(db)
+        for did, name, grid, cname, val, dur in con.execute(
+                "select dispatch_id, kernel_name, grid_size, counter_name, value, duration "
+                "from counters_collection"):
+            per.setdefault((db, did), {})[cname] = float(val)
+            meta[(db, did)] = (_short(name), int(grid), float(dur))
+    groups = {}
+    for key, cs in per.items():
+        name, grid, dur = meta[key]
+        groups.setdefault((name, grid), []).append((cs, dur))
+    lines = [f"{'kernel':<44} {'threads':>9} {'disp':>5} {'dur_us':>9} {'VALU':>10} "
+             f"{'SALU':>9} {'LDS':>9} {'VALU_G/s':>8} {'clk_GHz':>7} {'VALU/CU/clk':>11}"]
+    for (name, grid), lst in sorted(groups.items(),
+                                    key=lambda kv: -sum(d for _, d in kv[1]))[:top]:
+        n = len(lst)
+
+        def mean(c):
+            return sum(cs.get(c, 0.0) for cs, _ in lst) / n
+
+        dur = sum(d for _, d in lst) / n
+        cyc = mean("GRBM_COUNT") / xcds
+        rate = mean("SQ_INSTS_VALU") / (cus * cyc) if cyc > 0 else float("nan")
+        clk = cyc / dur if dur > 0 else float("nan")
+        lines.append(f"{name[:44]:<44} {grid:>9} {n:>5} {dur / 1e3:>9.1f} "
+                     f"{mean('SQ_INSTS_VALU'):>10.4g} {mean('SQ_INSTS_SALU'):>9.3g} "
+                     f"{mean('SQ_INSTS_LDS'):>9.3g} {mean('SQ_INSTS_VALU') / dur:>8.1f} "
+                     f"{clk:>7.2f} {rate:>11.3f}")
+    return "\n".join(lines)
+
+
 # kernels whose HBM traffic bench.py reports: (name prefix, launch threads)
 TRAFFIC_KERNELS = {
     "raster_bwd": ("raster_bwd16_kernel<false, true, true>", 16777216,
@@ -179,7 +224,7 @@ if __name__ == "__main__":
     import argparse
     import glob
     ap = argparse.ArgumentParser(description="rocprofv3 helpers")
-    ap.add_argument("cmd", choices=["summarize", "pmc", "copy-probe", "traffic"])
+    ap.add_argument("cmd", choices=["summarize", "pmc", "copy-probe", "traffic", "valu"])
     ap.add_argument("path", nargs="?", help="rocprofv3 output directory or *_results.db file")
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--match", default="")
@@ -194,6 +239,8 @@ if __name__ == "__main__":
     a = ap.parse_args()
     if a.cmd == "copy-probe":
         copy_probe()
+    elif a.cmd == "valu":
+        print(valu_issue(a.path, a.top))
     elif a.cmd == "traffic":
         import json
         print(json.dumps(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes,
