@@ -272,6 +272,33 @@ __device__ __forceinline__ int reach_strips(float mx, float my, float ap, float 
     return m;
 }
 
+// Rows of a 16 x 16 tile that a Gaussian can reach: bit r set unless the form
+// stays above the accept limit along the whole row r (pixel centres bx0 ..
+// bx0 + 15 at height by0 + r), the least value taken at the clamped
+// minimiser along the row, with the margin and the guards of reach_strips
+// (all rows when a guard fails).  The set bits form one run, the form's
+// minimum over a segment being convex in the row offset.
+__device__ __forceinline__ uint32_t reach_rows(float mx, float my, float ap, float bp2, float cp,
+                                               float T, float bx0, float by0) {
+    const float bp = 0.5f * bp2;
+    const float det = ap * cp - bp * bp, tr = ap + cp;
+    if (!(ap > 0.0f && cp > 0.0f && det > 0.0f && tr <= 46.0f && tr * tr <= 1e4f * det &&
+          T > 0.0f && T < 1e4f && fabsf(mx) <= 65536.0f && fabsf(my) <= 65536.0f))
+        return 0xFFFFu;
+    const float lim = __builtin_fmaf(T, 0x1p-10f, T) + 0.36f;
+    const float ra = -bp / ap;
+    const float x0 = bx0 - mx, x1 = x0 + 15.0f, y0 = by0 - my;
+    uint32_t m = 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const float y = y0 + (float)r;
+        const float x = fminf(fmaxf(ra * y, x0), x1);
+        const float e = __builtin_fmaf(ap * x, x, y * __builtin_fmaf(bp2, x, cp * y));
+        if (!(e > lim)) m |= 1u << r;
+    }
+    return m;
+}
+
 // Forward for 16x16 tiles.  Lane l owns the four pixels (l % 4 + 4k, l / 4),
 // k = 0..3, of one row: one row's dy, b'dy and c'dy serve all four, and pixel
 // k of every lane lies in column strip k (columns 4k .. 4k + 3).  A Gaussian's
@@ -916,6 +943,123 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
     }
 }
 
+// Fit-step backward, row grouping.  A lane's record depends only on its own
+// Gaussian and the tile's pixels (the batch it shares changes nothing), so a
+// chunk of up to kFitChunk of the tile's Gaussians is reordered before it is
+// batched: Gaussians that reach no row of the tile get their all-zero record
+// at once (the accumulators start at +0 and a rejected pair leaves them
+// untouched), the rest are sorted, stably, by (first, last) reachable row, and
+// each batch walks only the rows some member reaches.  Rows are still taken
+// in increasing order, so every record keeps its bits.
+constexpr int kFitChunk = 512;            // Gaussians sorted together (8 batches)
+constexpr int kFitCntWords = 128;         // 256 16-bit class counters, two per word
+constexpr int kFitLdsInts = kFitCntWords + kFitChunk / 2;  // counters + uint16 order
+constexpr uint32_t kNoRows = 0xF0;        // class of a Gaussian reaching no row
+
+// Sorts entries c0 .. c0 + cn - 1 of the tile's list into ord (chunk-local
+// indices) and returns how many were sorted; cnt[j] ends holding the rows that
+// sorted batch j (ord[64j .. 64j + 63]) walks: the union of the row runs of
+// the classes it overlaps.  One wave; LDS operations of a wave complete in
+// issue order.  Kept out of line: inlined, its scheduling region raises the
+// register count of the whole kernel past the seven-wave budget.
+template <bool HAS_OPAC>
+__device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const int32_t *__restrict__ gids,
+                                              const GaussPack *__restrict__ pack, int x_org,
+                                              int y_org, int tx, int ty,
+                                              FitRec *__restrict__ rec, uint32_t *cnt,
+                                              uint16_t *ord) {
+    const int lane = threadIdx.x;
+    for (int w = lane; w < kFitCntWords; w += kThreads) cnt[w] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    constexpr int kRounds = kFitChunk / kThreads;
+    static_assert(kRounds <= 8, "one class byte per round in a 64-bit word");
+    uint64_t kw = 0;  // the lane's class per round, one byte each
+#pragma unroll 2
+    for (int k = 0; k < kRounds; ++k) {
+        const int i = lane + kThreads * k;
+        uint32_t key = kNoRows;
+        if (i < cn) {
+            const int g = gids[c0 + i];
+            const float4 p0 = pack[g].p0, p1 = pack[g].p1;
+            const float T = HAS_OPAC ? __log2f(255.0f * p1.y) : kAcceptLog2;
+            const uint32_t m = reach_rows(p0.x, p0.y, p0.z, p0.w, p1.x, T, (float)x_org + 0.5f,
+                                          (float)y_org + 0.5f);
+            if (m) {
+                key = ((uint32_t)__builtin_ctz(m) << 4) | (31u - (uint32_t)__builtin_clz(m));
+                atomicAdd(&cnt[key >> 1], 1u << (16 * (key & 1u)));
+            } else {
+                const float4 p3 = pack[g].p3;
+                const int slot = __float_as_int(p3.z) + ty * __float_as_int(p3.w) + tx;
+                FitRec z;
+                z.r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                z.r1 = z.r0;
+                rec[slot] = z;
+            }
+        }
+        kw |= (uint64_t)key << (8 * k);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // exclusive scan of the 256 counts: lane l holds classes 4l .. 4l + 3
+    const uint32_t w0 = cnt[2 * lane], w1 = cnt[2 * lane + 1];
+    const int n0 = (int)(w0 & 0xFFFFu), n1 = (int)(w0 >> 16), n2 = (int)(w1 & 0xFFFFu),
+              n3 = (int)(w1 >> 16);
+    const int local = n0 + n1 + n2 + n3;
+    int incl = local;
+#pragma unroll
+    for (int off = 1; off < LIG_WAVE; off <<= 1) {
+        const int y = __shfl_up(incl, off, LIG_WAVE);
+        if (lane >= off) incl += y;
+    }
+    const int total = __shfl(incl, LIG_WAVE - 1, LIG_WAVE);
+    const uint32_t s0 = (uint32_t)(incl - local), s1 = s0 + n0, s2 = s1 + n1, s3 = s2 + n2;
+    cnt[2 * lane] = s0 | (s1 << 16);
+    cnt[2 * lane + 1] = s2 | (s3 << 16);
+    __builtin_amdgcn_wave_barrier();
+    // stable ranking, round by round in index order: lanes of one class are
+    // matched by ballots, the class's first lane advances its counter
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (LIG_WAVE - lane));
+#pragma unroll
+    for (int k = 0; k < kRounds; ++k) {
+        if (kThreads * k >= cn) break;  // wave-uniform
+        const uint32_t key = (uint32_t)(kw >> (8 * k)) & 0xFFu;
+        const bool valid = key != kNoRows;
+        uint64_t mm = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (key >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            mm &= bit ? bb : ~bb;
+        }
+        const int rank = __popcll(mm & lt);
+        const int before = (int)((cnt[key >> 1] >> (16 * (key & 1u))) & 0xFFFFu);
+        if (valid) ord[before + rank] = (uint16_t)(lane + kThreads * k);
+        if (valid && rank == 0)
+            atomicAdd(&cnt[key >> 1], (uint32_t)__popcll(mm) << (16 * (key & 1u)));
+        __builtin_amdgcn_wave_barrier();
+    }
+    // batch row masks from the class ends: lane l holds classes 4l .. 4l + 3
+    const uint32_t e01 = cnt[2 * lane], e23 = cnt[2 * lane + 1];
+    const int prev = lane == 0 ? 0 : (int)(cnt[2 * lane - 1] >> 16);
+    __builtin_amdgcn_wave_barrier();
+    if (lane < kRounds) cnt[lane] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    const int ends[4] = {(int)(e01 & 0xFFFFu), (int)(e01 >> 16), (int)(e23 & 0xFFFFu),
+                         (int)(e23 >> 16)};
+    int start = prev;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (start < ends[q]) {
+            const int key = 4 * lane + q, r0 = key >> 4, r1 = key & 15;
+            const uint32_t rows = ((2u << r1) - 1u) & ~((1u << r0) - 1u);
+            for (int j = start / kThreads; j <= (ends[q] - 1) / kThreads; ++j)
+                atomicOr(&cnt[j], rows);
+        }
+        start = ends[q];
+    }
+    __builtin_amdgcn_wave_barrier();
+    return total;
+}
+
 // 16 x 16 tiles.  Per batch and pixel row, two ballots over the row's final
 // indices give wave-uniform bit masks: pixels some lane of the batch reaches
 // (fp >= base) and pixels every lane reaches (fp >= base + 63).  Only pixels
@@ -978,10 +1122,9 @@ __device__ __forceinline__ void bwd16_tile(
     const int nrows = TRAIN ? min(16, img_h - y_org) : 16;
     const uint32_t cols = img_w - x_org >= 16 ? 0xffffu : ((1u << (img_w - x_org)) - 1u);
 
-    for (int base = range.x; base < range.y; base += kThreads) {
-        const bool act = base + lane < range.y;
-        const int kidx = act ? base + lane : INT32_MAX;
-        const int g = act ? gids[base + lane] : 0;
+    // one batch: lanes act (Gaussian g, list position kidx) over the rows in
+    // `rows` (bit r: tile row r), in increasing order
+    auto run_batch = [&](int base, bool act, int kidx, int g, uint32_t rows) {
         const GaussPack gp = pack[g];
         const float mx = gp.p0.x, my = gp.p0.y, ap = gp.p0.z, bp2 = gp.p0.w, bp = 0.5f * bp2;
         const float cp = gp.p1.x, op = gp.p1.y, cr = gp.p1.z, cg = gp.p1.w, cb = gp.p2.x;
@@ -1010,7 +1153,9 @@ __device__ __forceinline__ void bwd16_tile(
         plain_acc_zero(A);
         const int full_from = base + kThreads - 1;
         const float amin = in_vgpr(kAlphaMin), clampv = 0.99f;
-        for (int ly = 0; ly < nrows; ++ly) {
+        while (rows) {
+            const int ly = __builtin_ctz(rows);
+            rows &= rows - 1u;
             int fcol = 0;
             uint32_t any = cols, full = cols;
             if constexpr (!TRAIN) {
@@ -1061,6 +1206,30 @@ __device__ __forceinline__ void bwd16_tile(
             plain_emit<HAS_OPAC, DET>(A, gp, act, slot, g, rec, g_xy, g_xy_abs, g_conic, g_rgb,
                                       g_opac);
         }
+    };
+    const uint32_t all_rows = (1u << nrows) - 1u;
+    if constexpr (TRAIN) {
+        uint32_t *cnt = reinterpret_cast<uint32_t *>(sfin);
+        uint16_t *ord = reinterpret_cast<uint16_t *>(sfin + kFitCntWords);
+        for (int c0 = range.x; c0 < range.y; c0 += kFitChunk) {
+            const int cn = min(kFitChunk, range.y - c0);
+            const int m = fit_chunk_sort<HAS_OPAC>(c0, cn, gids, pack, x_org, y_org, tx, ty,
+                                                   reinterpret_cast<FitRec *>(rec), cnt, ord);
+            for (int b = 0; b < m; b += kThreads) {
+                const bool act = b + lane < m;
+                const int g = act ? gids[c0 + (int)ord[b + lane]] : 0;
+                const uint32_t rows = (uint32_t)__builtin_amdgcn_readfirstlane(
+                                          (int)cnt[b / kThreads]) & all_rows;
+                run_batch(b, act, INT32_MAX, g, rows);
+            }
+            __builtin_amdgcn_wave_barrier();  // the next chunk rewrites cnt and ord
+        }
+    } else {
+        for (int base = range.x; base < range.y; base += kThreads) {
+            const bool act = base + lane < range.y;
+            run_batch(base, act, act ? base + lane : INT32_MAX, act ? gids[base + lane] : 0,
+                      all_rows);
+        }
     }
 }
 
@@ -1080,7 +1249,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     float *__restrict__ g_rgb, float *__restrict__ g_opac, const BatchImage *__restrict__ tab,
     int nimg, int split, const int32_t *__restrict__ order) {
     __shared__ float4 sv[kMaxPix];
-    __shared__ int sfin[TRAIN ? 1 : kMaxPix];
+    __shared__ int sfin[TRAIN ? kFitLdsInts : kMaxPix];  // TRAIN: row-group sort
     int gtile, part;
     if (order && order[0] >= 0) {  // the shares of a tile are consecutive blocks
         const int k = (int)blockIdx.x / split;
@@ -1105,7 +1274,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_fit_queue_kernel(
     const float *__restrict__ target, const float *__restrict__ grad_loss,
     GradRec *__restrict__ rec, const BatchImage *__restrict__ tab, int nimg, TileQueue queue) {
     __shared__ float4 sv[kMaxPix];
-    __shared__ int sfin[1];
+    __shared__ int sfin[kFitLdsInts];  // row-group sort
     int cnt;
     int t = queue_resolve(queue, queue_issue(queue), cnt);
     while (t >= 0) {

@@ -80,6 +80,36 @@ def test_fused_step_matches_api_chain(device, n, H, W, scale, seed):
         assert torch.equal(getattr(a, name), getattr(b, name)), name
 
 
+def test_fused_step_matches_api_chain_dense_tiles(device):
+    """Thousands of Gaussians per tile, each tile's backward in one block: the
+    fit backward groups each chunk of 512 of a tile's Gaussians by the rows
+    they reach (several chunks per tile here, Gaussians reaching no row among
+    them) and still gives the API chain's gradients bit for bit."""
+    import lig_amd._lib as L
+    from lig_amd import data
+    n, H, W, seed = 12000, 96, 96, 6
+    print(f"fused-vs-api dense n={n} H={H} W={W} seed={seed} shares=1", flush=True)
+    lib = L.load()
+    prev = lib.lig_set_backward_shares(1)
+    try:
+        tgt = data.make_image(H, W, seed, device=device)
+        a, b = _level_pair(n, H, W, seed, 3.0, device)
+        with torch.no_grad():
+            b.rgbs.mul_(0.01)
+            a.rgbs.copy_(b.rgbs)
+            nt = a.project()[3]
+        assert int(nt.sum()) > 36 * 2 * 512  # 36 tiles: several 512-Gaussian chunks each
+        la, lb = a.loss(tgt), b.loss(tgt)
+        la.backward()
+        lb.backward()
+        for name in ("means", "cholesky", "rgbs"):
+            ga, gb = getattr(a, name).grad, getattr(b, name).grad
+            assert torch.count_nonzero(gb) > 0, name
+            assert torch.equal(ga, gb), (name, float((ga - gb).abs().max()))
+    finally:
+        lib.lig_set_backward_shares(prev)
+
+
 @pytest.mark.parametrize("skew", [False, True])
 def test_launch_order_changes_no_bits(device, skew):
     """A 4096 x 4096 level (65536 tiles, many rounds of resident blocks):
