@@ -31,6 +31,7 @@ extern "C" {
 #define LIG_ERR_WORKSPACE (-2)
 #define LIG_ERR_LAUNCH (-3)
 #define LIG_ERR_UNSUPPORTED (-4)
+#define LIG_ERR_TOO_LARGE (-5)
 
 /* Human-readable name of a status code. */
 const char *lig_strerror(int status);
@@ -140,6 +141,28 @@ int lig_bin_and_sort_gaussians(int n, int num_intersects, const float *xys,
                                int32_t *gaussian_ids_sorted, int32_t *tile_bins,
                                int32_t *emission_ids_sorted, void *workspace,
                                size_t workspace_bytes, void *stream);
+
+/* No counterpart in the reference.  lig_bin_and_sort_gaussians without the
+ * workspace argument, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the workspace is a fixed scratch array of
+ * LIG_BIN_NOALLOC_SCRATCH_BYTES in the library's device memory, one per GPU,
+ * held from the library's first use on that GPU until the process ends.  The
+ * call allocates nothing.  Same arguments (minus the workspace) and the same
+ * results bit for bit.  Sizes whose lig_bin_workspace_size(num_intersects,
+ * tiles) exceeds the scratch (about 2.7 M intersections at 16 x 16 tiles) are
+ * rejected with LIG_ERR_TOO_LARGE before anything is launched.  The scratch
+ * used is the current device's; a non-NULL stream must belong to it
+ * (LIG_ERR_INVALID_ARGUMENT otherwise).  Nothing in the scratch needs to start
+ * at zero; every call on a GPU shares that GPU's scratch, so calls that overlap
+ * on one GPU (different streams, or different host threads) race: the caller
+ * serialises them. */
+#define LIG_BIN_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int lig_bin_and_sort_gaussians_noalloc(int n, int num_intersects, const float *xys,
+                                       const float *extents, const int32_t *cum_tiles_hit,
+                                       int img_height, int img_width, int block_width,
+                                       int32_t *gaussian_ids_sorted, int32_t *tile_bins,
+                                       int32_t *emission_ids_sorted, void *stream);
 
 /* ---- rasterisation ------------------------------------------------------- */
 

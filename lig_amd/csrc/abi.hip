@@ -8,6 +8,7 @@ extern "C" const char *lig_strerror(int status) {
         case LIG_ERR_WORKSPACE: return "workspace missing or too small";
         case LIG_ERR_LAUNCH: return "kernel launch failed";
         case LIG_ERR_UNSUPPORTED: return "unsupported configuration";
+        case LIG_ERR_TOO_LARGE: return "input exceeds the entry point's fixed scratch size";
         default: return "unknown status";
     }
 }

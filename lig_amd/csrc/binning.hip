@@ -9,6 +9,8 @@
 // Reference behaviour followed (paths relative to the reference repository):
 //   gsplat2d/gsplat2d/utils.py:61-100 (cumsum, .item, torch.sort, gather),
 //   gsplat2d/gsplat2d/cuda/csrc/forward.cu:109-170, bindings.cu:239-301.
+#include <mutex>
+
 #include "lig_device.h"
 
 namespace lig {
@@ -746,20 +748,19 @@ extern "C" size_t lig_bin_workspace_size(int num_intersects, int num_tiles) {
     return 4 * align_up(m * 4) + sort_ws_bytes<uint32_t>(m, bits_for(num_tiles)) + 256;
 }
 
-extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float *xys,
-                                          const float *extents, const int32_t *cum_tiles_hit,
-                                          int img_height, int img_width, int block_width,
-                                          int32_t *gaussian_ids_sorted, int32_t *tile_bins,
-                                          int32_t *emission_ids_sorted, void *workspace,
-                                          size_t workspace_bytes, void *stream) {
-    if (n < 0 || num_intersects < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 ||
-        block_width > 16 || !tile_bins)
-        return LIG_ERR_INVALID_ARGUMENT;
+namespace lig {
+namespace {
+
+// Body of lig_bin_and_sort_gaussians and of its no-allocation form: the
+// arguments are checked, the workspace is `workspace_bytes` at `workspace`.
+int bin_and_sort(int n, int num_intersects, const float *xys, const float *extents,
+                 const int32_t *cum_tiles_hit, int img_height, int img_width, int block_width,
+                 int32_t *gaussian_ids_sorted, int32_t *tile_bins, int32_t *emission_ids_sorted,
+                 void *workspace, size_t workspace_bytes, hipStream_t st) {
     const int tx = (img_width + block_width - 1) / block_width;
     const int ty = (img_height + block_width - 1) / block_width;
     const int nt = tx * ty;
     if (workspace_bytes < lig_bin_workspace_size(num_intersects, nt)) return LIG_ERR_WORKSPACE;
-    hipStream_t st = as_stream(stream);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * (size_t)nt, 256)), dim3(256), 0, st,
                        tile_bins, 2 * (size_t)nt, 0);
     if (num_intersects == 0 || n == 0) return launch_status();
@@ -802,6 +803,91 @@ extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float
                        (const uint32_t *)slots_sorted, (const int32_t *)slot_gid, nt, tile_bins,
                        gaussian_ids_sorted);
     return launch_status();
+}
+
+bool bin_args_ok(int n, int num_intersects, int img_height, int img_width, int block_width,
+                 const int32_t *tile_bins) {
+    return n >= 0 && num_intersects >= 0 && img_height > 0 && img_width > 0 &&
+           block_width >= 1 && block_width <= 16 && tile_bins;
+}
+
+}  // namespace
+}  // namespace lig
+
+extern "C" int lig_bin_and_sort_gaussians(int n, int num_intersects, const float *xys,
+                                          const float *extents, const int32_t *cum_tiles_hit,
+                                          int img_height, int img_width, int block_width,
+                                          int32_t *gaussian_ids_sorted, int32_t *tile_bins,
+                                          int32_t *emission_ids_sorted, void *workspace,
+                                          size_t workspace_bytes, void *stream) {
+    if (!bin_args_ok(n, num_intersects, img_height, img_width, block_width, tile_bins))
+        return LIG_ERR_INVALID_ARGUMENT;
+    return bin_and_sort(n, num_intersects, xys, extents, cum_tiles_hit, img_height, img_width,
+                        block_width, gaussian_ids_sorted, tile_bins, emission_ids_sorted,
+                        workspace, workspace_bytes, as_stream(stream));
+}
+
+// Scratch of lig_bin_and_sort_gaussians_noalloc: each device that runs the
+// library holds its own copy for the life of the process.  Host code never
+// passes the array's name to a kernel (on the host it is a stand-in, not a
+// device address); it uses the device address hipGetSymbolAddress returns,
+// looked up once per device and kept in g_scratch_addr.
+static __device__ __align__(256) unsigned char scratch[LIG_BIN_NOALLOC_SCRATCH_BYTES];
+
+namespace lig {
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxScratchDevices];
+
+// Device address of the current device's scratch (looked up on the first
+// call for that device).
+int scratch_address(void **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return LIG_ERR_LAUNCH;
+    if (dev < 0 || dev >= kMaxScratchDevices) return LIG_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+            return LIG_ERR_LAUNCH;
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return LIG_OK;
+}
+
+}  // namespace
+}  // namespace lig
+
+extern "C" int lig_bin_and_sort_gaussians_noalloc(int n, int num_intersects, const float *xys,
+                                                  const float *extents,
+                                                  const int32_t *cum_tiles_hit, int img_height,
+                                                  int img_width, int block_width,
+                                                  int32_t *gaussian_ids_sorted,
+                                                  int32_t *tile_bins,
+                                                  int32_t *emission_ids_sorted, void *stream) {
+    if (!bin_args_ok(n, num_intersects, img_height, img_width, block_width, tile_bins))
+        return LIG_ERR_INVALID_ARGUMENT;
+    const int nt = ((img_width + block_width - 1) / block_width) *
+                   ((img_height + block_width - 1) / block_width);
+    if (lig_bin_workspace_size(num_intersects, nt) > LIG_BIN_NOALLOC_SCRATCH_BYTES)
+        return LIG_ERR_TOO_LARGE;
+    hipStream_t st = as_stream(stream);
+    if (st) {
+        // the scratch is the current device's: the stream must belong to it
+        int cur = -1, sdev = -1;
+        if (hipGetDevice(&cur) != hipSuccess || hipStreamGetDevice(st, &sdev) != hipSuccess)
+            return LIG_ERR_LAUNCH;
+        if (cur != sdev) return LIG_ERR_INVALID_ARGUMENT;
+    }
+    void *ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc != LIG_OK) return rc;
+    return bin_and_sort(n, num_intersects, xys, extents, cum_tiles_hit, img_height, img_width,
+                        block_width, gaussian_ids_sorted, tile_bins, emission_ids_sorted, ws,
+                        LIG_BIN_NOALLOC_SCRATCH_BYTES, st);
 }
 
 namespace lig {

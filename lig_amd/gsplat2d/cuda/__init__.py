@@ -308,6 +308,29 @@ def bin_and_sort_fast(num_points, num_intersects, xys, extents, cum_tiles_hit, i
     return (gids, bins, eids) if with_emission_ids else (gids, bins)
 
 
+# Fixed scratch of lig_bin_and_sort_gaussians_noalloc (LIG_BIN_NOALLOC_SCRATCH_BYTES).
+BIN_NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def bin_and_sort_noalloc(num_points, num_intersects, xys, extents, cum_tiles_hit, img_height,
+                         img_width, block_width, gaussian_ids_sorted, tile_bins,
+                         emission_ids_sorted=None):
+    """bin_and_sort_fast into caller-provided outputs, with the library's fixed
+    per-device scratch as workspace (lig_bin_and_sort_gaussians_noalloc): no
+    device memory is allocated.  Calls that overlap on one GPU race."""
+    for name, t in (("xys", xys), ("extents", extents)):
+        _check(name, t)
+    for name, t in (("cum_tiles_hit", cum_tiles_hit), ("gaussian_ids_sorted",
+                    gaussian_ids_sorted), ("tile_bins", tile_bins),
+                    ("emission_ids_sorted", emission_ids_sorted)):
+        _check(name, t, _i32)
+    with torch.cuda.device(xys.device):
+        L.call("lig_bin_and_sort_gaussians_noalloc", int(num_points), int(num_intersects),
+               L.ptr(xys), L.ptr(extents), L.ptr(cum_tiles_hit), int(img_height),
+               int(img_width), int(block_width), L.ptr(gaussian_ids_sorted), L.ptr(tile_bins),
+               L.ptr(emission_ids_sorted), L.stream_of(xys))
+
+
 def pack_gaussians(xys, conics, colors, opacities):
     """Per-Gaussian 48-byte records read by the rasteriser kernels."""
     for n_, t in (("xys", xys), ("conics", conics), ("colors", colors),
