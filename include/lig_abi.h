@@ -475,6 +475,16 @@ int lig_adan_step(long long n, float *param, const float *grad, float *exp_avg,
 
 /* ---- gradient-aware upscaler ---------------------------------------------- */
 
+/* Replaces torch.nn.functional.interpolate(mode="bilinear", align_corners=False)
+ * of train.py's level composition (train.py:103-105, 193): src [src_h,src_w,3]
+ * resized to a virtual full_h x full_w image, of which the out_h x out_w
+ * window at (off_y, off_x) is written to output [out_h,out_w,3] (the whole
+ * image when the window is).  Destination index d samples source coordinate
+ * max((src/full) * (d + 0.5) - 0.5, 0), the upper neighbour clamped to the
+ * last index.  The window must lie inside the full image. */
+int lig_bilinear_resize(int src_h, int src_w, int full_h, int full_w, int off_y, int off_x,
+                        int out_h, int out_w, const float *src, float *output, void *stream);
+
 /* Replaces gradient_aware_upscale_forward_tensor (csrc/bindings.cu:508-559,
  * kernel csrc/gradient_aware_upscaler.cu:181-246).  render/dx/dy/dxy
  * [src_h,src_w,3] -> output [dst_h,dst_w,3]; roi = (x1,y1,x2,y2), x2>=x1, y2>=y1. */

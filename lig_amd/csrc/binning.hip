@@ -297,10 +297,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
         const int rank = __popcll(m & lt_mask);
         const int before = valid ? cnt_w[d] : 0;
         lpos[r] = before + rank;
-        // one lane per digit advances the wave's counter; LDS operations of a
-        // wave complete in order, so the next round reads the new value
+        // one lane per digit advances the wave's counter; the hand-off below
+        // makes the new value visible to the next round
         if (valid && rank == 0) cnt_w[d] = before + __popcll(m);
-        __builtin_amdgcn_wave_barrier();
+        wave_lds_sync();
     }
     __syncthreads();
     // block exclusive scan of the digit totals; per-(wave, digit) starts
@@ -537,8 +537,8 @@ __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
             w = max(x1 - x0, 1);
         }
     }
-    // the wave's rectangles in LDS; only this wave reads its rows, and a
-    // wave's LDS operations complete in issue order
+    // the wave's rectangles in LDS; only this wave reads its rows (handed
+    // over by wave_lds_sync, no block barrier)
     constexpr int kWaves = kEmitThreads / LIG_WAVE;
     __shared__ __attribute__((aligned(16))) int s_start[kWaves][LIG_WAVE];  // read as int4
     __shared__ int4 s_rect[kWaves][LIG_WAVE];  // (start, x0, y0, w)
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
     s_start[wv][lane] = start;
     s_rect[wv][lane] = make_int4(start, x0, y0, w);
     s_grid[wv][lane] = make_int4(row_len, toff, __float_as_int(rcp), 0);
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     // every eighth start, wave-uniform
     int b8[LIG_WAVE / 8];
 #pragma unroll

@@ -10,6 +10,18 @@
 
 namespace lig {
 
+// Hand-off of LDS data between the lanes of one wave: every LDS write the
+// wave issued before the call is visible to every lane's LDS reads after it.
+// The wave barrier alone only keeps the compiler from moving code across it;
+// the wavefront-scope release/acquire pair makes the ordering part of the
+// memory model (also for accesses through generic pointers), not a
+// consequence of LDS operations completing in issue order.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Status of the last launch on this thread, mapped to the ABI codes.
 inline int launch_status() {
     hipError_t e = hipGetLastError();

@@ -959,8 +959,8 @@ constexpr uint32_t kNoRows = 0xF0;        // class of a Gaussian reaching no row
 // Sorts entries c0 .. c0 + cn - 1 of the tile's list into ord (chunk-local
 // indices) and returns how many were sorted; cnt[j] ends holding the rows that
 // sorted batch j (ord[64j .. 64j + 63]) walks: the union of the row runs of
-// the classes it overlaps.  One wave; LDS operations of a wave complete in
-// issue order.  Kept out of line: inlined, its scheduling region raises the
+// the classes it overlaps.  One wave; its lanes hand LDS data to each other
+// through wave_lds_sync.  Kept out of line: inlined, its scheduling region raises the
 // register count of the whole kernel past the seven-wave budget.
 template <bool HAS_OPAC>
 __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const int32_t *__restrict__ gids,
@@ -970,7 +970,7 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
                                               uint16_t *ord) {
     const int lane = threadIdx.x;
     for (int w = lane; w < kFitCntWords; w += kThreads) cnt[w] = 0u;
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     constexpr int kRounds = kFitChunk / kThreads;
     static_assert(kRounds <= 8, "one class byte per round in a 64-bit word");
     uint64_t kw = 0;  // the lane's class per round, one byte each
@@ -998,7 +998,7 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
         }
         kw |= (uint64_t)key << (8 * k);
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     // exclusive scan of the 256 counts: lane l holds classes 4l .. 4l + 3
     const uint32_t w0 = cnt[2 * lane], w1 = cnt[2 * lane + 1];
     const int n0 = (int)(w0 & 0xFFFFu), n1 = (int)(w0 >> 16), n2 = (int)(w1 & 0xFFFFu),
@@ -1014,7 +1014,7 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
     const uint32_t s0 = (uint32_t)(incl - local), s1 = s0 + n0, s2 = s1 + n1, s3 = s2 + n2;
     cnt[2 * lane] = s0 | (s1 << 16);
     cnt[2 * lane + 1] = s2 | (s3 << 16);
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     // stable ranking, round by round in index order: lanes of one class are
     // matched by ballots, the class's first lane advances its counter
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (LIG_WAVE - lane));
@@ -1035,14 +1035,14 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
         if (valid) ord[before + rank] = (uint16_t)(lane + kThreads * k);
         if (valid && rank == 0)
             atomicAdd(&cnt[key >> 1], (uint32_t)__popcll(mm) << (16 * (key & 1u)));
-        __builtin_amdgcn_wave_barrier();
+        wave_lds_sync();
     }
     // batch row masks from the class ends: lane l holds classes 4l .. 4l + 3
     const uint32_t e01 = cnt[2 * lane], e23 = cnt[2 * lane + 1];
     const int prev = lane == 0 ? 0 : (int)(cnt[2 * lane - 1] >> 16);
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     if (lane < kRounds) cnt[lane] = 0u;
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     const int ends[4] = {(int)(e01 & 0xFFFFu), (int)(e01 >> 16), (int)(e23 & 0xFFFFu),
                          (int)(e23 >> 16)};
     int start = prev;
@@ -1056,7 +1056,7 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
         }
         start = ends[q];
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     return total;
 }
 
@@ -1222,7 +1222,7 @@ __device__ __forceinline__ void bwd16_tile(
                                           (int)cnt[b / kThreads]) & all_rows;
                 run_batch(b, act, INT32_MAX, g, rows);
             }
-            __builtin_amdgcn_wave_barrier();  // the next chunk rewrites cnt and ord
+            wave_lds_sync();  // the next chunk rewrites cnt and ord
         }
     } else {
         for (int base = range.x; base < range.y; base += kThreads) {
@@ -1292,6 +1292,11 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_fit_queue_kernel(
 
 // Gradient-aware backward (gradient_aware_rasterizer.cu:201-315 per pair),
 // d' = pixel - mean, sigma_x = a d'x + b d'y, sigma_y = b d'x + c d'y.
+// xys.absgrad follows the reference's reduction: v_xy is summed over each
+// 32-thread warp of the bw x bw block (thread rank p = ly * bw + lx) and the
+// absolute value of each warp's sum is added (gradient_aware_rasterizer.cu:
+// 170, 320, 339-341).  A lane walks its Gaussian's pixels in rank order, so a
+// group is a run of 32 consecutive p; pixels outside the image add 0.
 template <bool DET>
 __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
     int tiles_x, int bw, int img_w, int img_h, const int32_t *__restrict__ gids,
@@ -1331,6 +1336,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
         float a_r = 0.f, a_g = 0.f, a_b = 0.f;
         float a_c0 = 0.f, a_c1 = 0.f, a_c2 = 0.f;
         float a_x = 0.f, a_y = 0.f, a_ax = 0.f, a_ay = 0.f, a_o = 0.f;
+        float w_x = 0.f, w_y = 0.f;  // v_xy of the current 32-rank group
 
         for (int ly = 0; ly < bw; ++ly) {
             const float dy = my - ((float)(y_org + ly) + 0.5f);
@@ -1338,6 +1344,12 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
             float pxf = (float)x_org + 0.5f;
             for (int lx = 0; lx < bw; ++lx, pxf += 1.0f) {
                 const int p = ly * bw + lx;
+                if ((p & 31) == 0) {
+                    a_ax += fabsf(w_x);
+                    a_ay += fabsf(w_y);
+                    w_x = 0.f;
+                    w_y = 0.f;
+                }
                 const int fp = sfin[p];
                 if (fp < base) continue;
                 const float dx = mx - pxf;
@@ -1381,11 +1393,13 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
                               vay * (0.5f * ey2 * sy - ey) + vaxy * (-0.5f * ey2 * P + ey * sx));
                 a_x += gx;
                 a_y += gy;
-                a_ax += fabsf(gx);
-                a_ay += fabsf(gy);
+                w_x += gx;
+                w_y += gy;
                 a_o = __builtin_fmaf(vs_, va, a_o);
             }
         }
+        a_ax += fabsf(w_x);
+        a_ay += fabsf(w_y);
         if (act)
             emit_grads(DET, rec, DET ? record_slot(slots, base + lane, gp, tx, ty) : 0, g, a_r,
                        a_g, a_b, a_c0, a_c1, a_c2, a_x, a_y, a_ax, a_ay, a_o, has_opac != 0,

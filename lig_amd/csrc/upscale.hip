@@ -195,10 +195,56 @@ bool roi_ok(int src_h, int src_w, int dst_h, int dst_w, float x1, float y1, floa
     return true;
 }
 
+// Bilinear resize with the half-pixel grid of torch.nn.functional.interpolate
+// (mode="bilinear", align_corners=False), the resampling train.py:103-105 and
+// :193 apply between levels: destination index d of a full_h x full_w output
+// samples source coordinate max(scale * (d + 0.5) - 0.5, 0), scale = src /
+// full, between floor and floor + 1 (clamped to the last index).  The kernel
+// writes the out_h x out_w window at (off_y, off_x) of that output, so a crop
+// holds exactly the full output's values at its pixels.
+__global__ __launch_bounds__(256) void bilinear_kernel(int src_h, int src_w, float scale_y,
+                                                       float scale_x, int off_y, int off_x,
+                                                       int out_h, int out_w,
+                                                       const float *__restrict__ src,
+                                                       float *__restrict__ out) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= out_w || y >= out_h) return;
+    const float fy = fmaxf(scale_y * ((float)(y + off_y) + 0.5f) - 0.5f, 0.0f);
+    const float fx = fmaxf(scale_x * ((float)(x + off_x) + 0.5f) - 0.5f, 0.0f);
+    const int y0 = min((int)fy, src_h - 1), x0 = min((int)fx, src_w - 1);
+    const int y1 = y0 + (y0 < src_h - 1 ? 1 : 0), x1 = x0 + (x0 < src_w - 1 ? 1 : 0);
+    const float ly = fy - (float)y0, lx = fx - (float)x0;
+    const float hy = 1.0f - ly, hx = 1.0f - lx;
+    const size_t i00 = 3 * ((size_t)y0 * src_w + x0), i01 = 3 * ((size_t)y0 * src_w + x1);
+    const size_t i10 = 3 * ((size_t)y1 * src_w + x0), i11 = 3 * ((size_t)y1 * src_w + x1);
+    const size_t o = 3 * ((size_t)y * out_w + x);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+        out[o + ch] = hy * (hx * src[i00 + ch] + lx * src[i01 + ch]) +
+                      ly * (hx * src[i10 + ch] + lx * src[i11 + ch]);
+}
+
 }  // namespace
 }  // namespace lig
 
 using namespace lig;
+
+extern "C" int lig_bilinear_resize(int src_h, int src_w, int full_h, int full_w, int off_y,
+                                   int off_x, int out_h, int out_w, const float *src,
+                                   float *output, void *stream) {
+    if (src_h <= 0 || src_w <= 0 || full_h <= 0 || full_w <= 0 || out_h < 0 || out_w < 0 ||
+        off_y < 0 || off_x < 0 || (long long)off_y + out_h > full_h ||
+        (long long)off_x + out_w > full_w)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (out_h == 0 || out_w == 0) return LIG_OK;
+    if (!src || !output) return LIG_ERR_INVALID_ARGUMENT;
+    const float sy = (float)src_h / (float)full_h, sx = (float)src_w / (float)full_w;
+    const dim3 grid(ceil_div(out_w, 64), ceil_div(out_h, 4));
+    hipLaunchKernelGGL(bilinear_kernel, grid, dim3(256), 0, as_stream(stream), src_h, src_w, sy,
+                       sx, off_y, off_x, out_h, out_w, src, output);
+    return launch_status();
+}
 
 extern "C" int lig_gradient_aware_upscale_forward(int src_h, int src_w, int dst_h, int dst_w,
                                                   float roi_x1, float roi_y1, float roi_x2,

@@ -152,10 +152,16 @@ def evaluate(out_hwc: torch.Tensor, gt_hwc: torch.Tensor):
 
 
 def downsample(img_hwc: torch.Tensor, factor: int, mode: str = "bicubic") -> torch.Tensor:
+    """H x W x 3 -> H/factor x W/factor x 3: antialiased bicubic, or "area"
+    (the mean of each factor x factor block, train.py:95-98's
+    interpolate(scale_factor=2^-k, mode='area'))."""
     x = img_hwc.permute(2, 0, 1)[None]
     H, W = img_hwc.shape[0] // factor, img_hwc.shape[1] // factor
-    kw = dict(antialias=True, align_corners=False) if mode == "bicubic" else {}
-    y = F.interpolate(x, size=(H, W), mode=mode, **kw)
+    if mode == "area":
+        y = F.interpolate(x, scale_factor=1.0 / factor, mode="area")
+    else:
+        kw = dict(antialias=True, align_corners=False) if mode == "bicubic" else {}
+        y = F.interpolate(x, size=(H, W), mode=mode, **kw)
     return y[0].permute(1, 2, 0).contiguous()
 
 
@@ -192,20 +198,43 @@ def fit_level(level: Level, target_hwc: torch.Tensor, steps: int):
             level.step(target_hwc)
 
 
+def bilinear_render(level: Level, H: int, W: int) -> torch.Tensor:
+    """train.py:103-105: the clamped level render resized to H x W by bilinear
+    interpolation (half-pixel grid, lig_bilinear_resize)."""
+    return _C.bilinear_resize(torch.clamp(level.render(), 0, 1).contiguous(), H, W)
+
+
+FLOWS = ("lig", "train")
+
+
 def two_level_fit(gt_hwc: torch.Tensor, n1: int, n2: int, steps: int, seed: int = 0,
-                  scale: float = 1.0, lr: float = 1e-3, factor: int = 4, device="cuda"):
-    """Plain two-level fit; returns (level1, level2, final PSNR)."""
+                  scale: float = 1.0, lr: float = 1e-3, factor: int = 4, device="cuda",
+                  flow: str = "lig"):
+    """Plain two-level fit; returns (level1, level2, final PSNR).
+
+    flow "lig": level 1 fits the antialiased bicubic downsample and is carried
+    up by the gradient-aware Hermite upscaler from its render and analytic
+    derivatives (the upscaler the path provides, upscale.py:50-82).  flow
+    "train": train.py's own resampling, level 1 fits the area downsample
+    (train.py:95-98) and the previous estimate is the bilinear resize of its
+    clamped render (train.py:103-105).  Both then fit level 2 to the
+    normalised residual (train.py:113-135) and compose (train.py:150-154)."""
+    if flow not in FLOWS:
+        raise ValueError(f"flow must be one of {FLOWS}, got {flow!r}")
     H, W = gt_hwc.shape[:2]
     l1 = Level(n1, H // factor, W // factor, seed, scale, lr, device=device)
-    t1 = downsample(gt_hwc, factor)
+    t1 = downsample(gt_hwc, factor, "bicubic" if flow == "lig" else "area")
     fit_level(l1, t1, steps)
     with torch.no_grad():
-        up = upscale_render(l1, H, W)
-        t2, lo, hi = residual_target(gt_hwc, torch.clamp(up, 0, 1))
+        if flow == "lig":
+            prev = torch.clamp(upscale_render(l1, H, W), 0, 1)
+        else:
+            prev = bilinear_render(l1, H, W)
+        t2, lo, hi = residual_target(gt_hwc, prev)
     l2 = Level(n2, H, W, seed + 1, scale, lr, device=device)
     fit_level(l2, t2, steps)
     with torch.no_grad():
-        out = compose(torch.clamp(l2.render(), 0, 1), torch.clamp(up, 0, 1), lo, hi)
+        out = compose(torch.clamp(l2.render(), 0, 1), prev, lo, hi)
     return l1, l2, psnr(out, gt_hwc)
 
 
@@ -287,18 +316,32 @@ class LIG(nn.Module):
 
 def save_checkpoint(path: str, levels: List[nn.Module], store_min: Optional[list] = None,
                     store_max: Optional[list] = None) -> None:
-    """train.py:172-176 format: {'state_dict': [per level], 'store_max', 'store_min'}."""
+    """The dict layout of train.py:172-176, {'state_dict': [one state dict per
+    level], 'store_max': [...], 'store_min': [...]}.  Each level's entries are
+    its own parameters: {means, cov2d, rgbs} for the Gaussian2D mirror (the
+    reference's keys, gaussianlig.py:52-58), {means, cholesky, rgbs} for a
+    Level; render.TwoLevel.from_checkpoint serves either."""
     torch.save({"state_dict": [m.state_dict() for m in levels],
                 "store_max": store_max or [], "store_min": store_min or []}, path)
 
 
 def load_checkpoint(path: str, levels: List[nn.Module], map_location=None):
-    """train.py:51-67: per-level state dicts by key; returns (store_min, store_max)."""
+    """train.py:51-67: per-level state dicts by key; returns (store_min,
+    store_max).  Where train.py keeps a level's own values for keys the
+    checkpoint lacks (so a checkpoint of another parameterisation loads as a
+    silent no-op), this raises KeyError naming the missing and unexpected
+    keys, and ValueError when the level counts differ."""
     ck = torch.load(path, map_location=map_location, weights_only=True)
-    for level, sd in zip(levels, ck["state_dict"]):
-        model_dict = level.state_dict()
-        model_dict.update({k: v for k, v in sd.items() if k in model_dict})
-        level.load_state_dict(model_dict)
+    sds = ck["state_dict"]
+    if len(sds) != len(levels):
+        raise ValueError(f"checkpoint has {len(sds)} levels, the model {len(levels)}")
+    for i, (level, sd) in enumerate(zip(levels, sds)):
+        want = set(level.state_dict())
+        missing, unexpected = sorted(want - set(sd)), sorted(set(sd) - want)
+        if missing or unexpected:
+            raise KeyError(f"level {i}: checkpoint keys do not match the model "
+                           f"(missing {missing}, unexpected {unexpected})")
+        level.load_state_dict(sd)
     return ck["store_min"], ck["store_max"]
 
 def _main(argv=None):

@@ -224,6 +224,23 @@ def _roi4(roi):
     return tuple(float(r) for r in roi)
 
 
+def bilinear_resize(src, full_h, full_w, window=None):
+    """src [h, w, 3] resized bilinearly to full_h x full_w with the half-pixel
+    grid of torch's interpolate(mode="bilinear", align_corners=False); with
+    window = (x0, y0, w, h) only that window of the result is computed
+    (lig_bilinear_resize).  -> [h, w, 3] (the window) or [full_h, full_w, 3]."""
+    _check("src", src)
+    if src.dim() != 3 or src.shape[2] != 3:
+        raise ValueError("src must be [H, W, 3]")
+    x0, y0, w, h = window if window is not None else (0, 0, full_w, full_h)
+    out = _empty((h, w, 3), src)
+    with torch.cuda.device(src.device):
+        L.call("lig_bilinear_resize", int(src.shape[0]), int(src.shape[1]), int(full_h),
+               int(full_w), int(y0), int(x0), int(h), int(w), L.ptr(src), L.ptr(out),
+               L.stream_of(src))
+    return out
+
+
 def gradient_aware_upscale_forward(render, dx, dy, dxy, dst_h, dst_w, roi):
     """bindings.cu:508-559 -> output [dst_h, dst_w, 3]."""
     for n_, t in (("render", render), ("dx", dx), ("dy", dy), ("dxy", dxy)):

@@ -10,9 +10,14 @@ window (image-only forward: no wsum, no final index).
 `TwoLevel` holds a fitted two-level representation (train.py:92-158): level 1
 on an H/f x W/f canvas, level 2 on H x W fitted to the normalised residual, and
 the normalisation bounds.  `render_crop` composes a window of the full image as
-train.py:150-154 does for the whole image: the level-1 render of the window's
-source region with its gradients, upscaled by the gradient-aware upscaler's ROI
-path (upscale.py, gradient_aware_upscaler.cu:201-206), plus the level-2 window.
+train.py:150-154 does for the whole image.  Two ways of carrying level 1 up to
+full resolution: "hermite" (the gradient-aware upscaler's ROI path, upscale.py,
+gradient_aware_upscaler.cu:201-206, on the level-1 render and its analytic
+derivatives) and "bilinear" (train.py's own composition, train.py:103-105 and
+187-196: bilinear resampling of the clamped level-1 render).
+`TwoLevel.from_checkpoint` loads the checkpoint train.py writes
+(train.py:172-176: per-level {means, cov2d, rgbs} state dicts of the
+reference's Gaussian2D, gaussianlig.py:52-58, and store_min / store_max).
 """
 from __future__ import annotations
 
@@ -29,19 +34,47 @@ from .fit import compose
 from .gsplat2d import cuda as _C
 
 
+KINDS = ("cholesky", "cov2d")
+
+
 @dataclass
 class LevelParams:
-    """Cholesky-parameterised Gaussians of one level on an H x W canvas."""
+    """Gaussians of one level on an H x W canvas: means [n, 2], rgbs [n, 3]
+    and `par` [n, 3], the Cholesky factors (l11, l21, l22) when kind is
+    "cholesky" or the covariance (a, b, c) of the reference's Gaussian2D
+    (gaussianlig.py:56, which may be indefinite) when kind is "cov2d"."""
     means: torch.Tensor
-    cholesky: torch.Tensor
+    par: torch.Tensor
     rgbs: torch.Tensor
     H: int
     W: int
+    kind: str = "cholesky"
+
+    def __post_init__(self):
+        if self.kind not in KINDS:
+            raise ValueError(f"kind must be one of {KINDS}, got {self.kind!r}")
+
+    @property
+    def cholesky(self) -> torch.Tensor:
+        if self.kind != "cholesky":
+            raise AttributeError("a covariance-parameterised level has no Cholesky factors")
+        return self.par
+
+    @property
+    def cov2d(self) -> torch.Tensor:
+        if self.kind != "cov2d":
+            raise AttributeError("a Cholesky-parameterised level has no cov2d parameter")
+        return self.par
 
     @classmethod
     def of(cls, level) -> "LevelParams":
-        return cls(level.means.detach(), level.cholesky.detach(), level.rgbs.detach(), level.H,
-                   level.W)
+        """The parameters of a fit.Level (Cholesky) or of a Gaussian2D (cov2d)."""
+        if hasattr(level, "cholesky"):
+            par, kind = level.cholesky, "cholesky"
+        else:
+            par, kind = level.cov2d, "cov2d"
+        return cls(level.means.detach(), par.detach(), level.rgbs.detach(), level.H, level.W,
+                   kind)
 
 
 @dataclass
@@ -59,8 +92,8 @@ class Projected:
 
 def _project_scan(p: LevelParams, x0: float, y0: float, w: int, h: int, scale: float):
     n = p.means.shape[0]
-    xys, ext, con, nt = _C.project_gaussians_view(n, p.cholesky, p.means, None, True, (x0, y0),
-                                                  (scale, scale), h, w, 16)
+    xys, ext, con, nt = _C.project_gaussians_view(n, p.par, p.means, None, p.kind == "cholesky",
+                                                  (x0, y0), (scale, scale), h, w, 16)
     cum = _C.inclusive_cumsum_i32(nt)
     return xys, ext, con, nt, cum
 
@@ -128,41 +161,133 @@ def render_view_ga(p: LevelParams, x0: float, y0: float, w: int, h: int):
     return raster_window_ga(p, project_window(p, x0, y0, w, h, 1.0))
 
 
+UPSAMPLE = ("hermite", "bilinear")
+
+
+def _tile_span(o: int, n: int, full: int):
+    """[a, b): the 16-pixel-tile-aligned span of an axis of `full` pixels that
+    covers [o, o + n), cut at the image's edge when the window ends inside it."""
+    a = (o // 16) * 16
+    b = -(-(o + n) // 16) * 16
+    if o + n <= full:
+        b = min(b, full)
+    return a, b
+
+
+def _aligned_view(p: LevelParams, x0: float, y0: float, w: int, h: int, scale: float):
+    """The view (p, x0', y0', w', h', scale) to render for the window (x0, y0,
+    w, h) at `scale`, and the window's pixel offset inside it.  A Gaussian is
+    drawn only inside its tile rectangle, and an indefinite covariance (which
+    the reference's cov2d levels allow, gaussianlig.py:56) can reach pixels
+    beyond that rectangle, so a window only holds the full image's pixels when
+    its tiles are the full image's tiles: the view is widened to the 16-pixel
+    grid of the H*scale x W*scale image.  Windows off that pixel grid render
+    as asked."""
+    ox, oy = x0 * scale, y0 * scale
+    fw, fh = p.W * scale, p.H * scale
+    if any(v != int(v) for v in (ox, oy, fw, fh)):
+        return (p, x0, y0, w, h, scale), (0, 0)
+    ax, bx = _tile_span(int(ox), w, int(fw))
+    ay, by = _tile_span(int(oy), h, int(fh))
+    return (p, ax / scale, ay / scale, bx - ax, by - ay, scale), (int(ox) - ax, int(oy) - ay)
+
+
 @dataclass
 class TwoLevel:
     l1: LevelParams
     l2: LevelParams
     lo: float
     hi: float
-    # the level-1 canvas rendered once (image, dx, dy, dxy), shared by every
-    # crop: a crop's level-1 source region is small (few tiles, thousands of
-    # Gaussians each), so rendering it per request left most of the device idle
+    upsample: str = "hermite"
+    # the level-1 canvas rendered once and shared by every crop ("hermite":
+    # image, dx, dy, dxy; "bilinear": the clamped image): a crop's level-1
+    # source region is small (few tiles, thousands of Gaussians each), so
+    # rendering it per request left most of the device idle
     _l1_render: Optional[tuple] = field(default=None, repr=False, compare=False)
     _l1_lock: threading.Lock = field(default_factory=threading.Lock, repr=False, compare=False)
+
+    def __post_init__(self):
+        if self.upsample not in UPSAMPLE:
+            raise ValueError(f"upsample must be one of {UPSAMPLE}, got {self.upsample!r}")
 
     @property
     def factor(self) -> int:
         return self.l2.W // self.l1.W
 
+    @classmethod
+    def from_checkpoint(cls, path: str, H: int, W: int, device="cuda:0",
+                        upsample: str = "bilinear") -> "TwoLevel":
+        """A two-level representation from a checkpoint in train.py's format
+        (train.py:172-176; loaded as train.py:51-67 does): 'state_dict' holds
+        one state dict per level with keys {means, cov2d, rgbs} (the
+        reference's Gaussian2D) or {means, cholesky, rgbs} (fit.Level), and
+        'store_min' / 'store_max' the level-2 normalisation bounds.  The
+        checkpoint does not store level sizes: level s of an n-level model is
+        int(H * 2^(s + 1 - n)) x int(W * 2^(s + 1 - n)) (gaussianlig.py:20-22),
+        so H, W are the full image's.  upsample="bilinear" composes as
+        train.py does; the loader executes nothing from the file
+        (torch.load(weights_only=True))."""
+        ck = torch.load(path, map_location=device, weights_only=True)
+        sds = ck["state_dict"]
+        if len(sds) != 2:
+            raise ValueError(f"a two-level representation needs 2 levels, the checkpoint has "
+                             f"{len(sds)}")
+        levels = []
+        for s, sd in enumerate(sds):
+            kinds = [k for k in KINDS if k in sd]
+            if len(kinds) != 1 or set(sd) != {"means", kinds[0], "rgbs"}:
+                raise KeyError(f"level {s}: expected keys means, rgbs and one of {KINDS}, got "
+                               f"{sorted(sd)}")
+            f = 2.0 ** (s + 1 - len(sds))
+            levels.append(LevelParams(*(sd[k].detach().float().contiguous().to(device)
+                                        for k in ("means", kinds[0], "rgbs")),
+                                      int(H * f), int(W * f), kinds[0]))
+        if not ck["store_min"] or not ck["store_max"]:
+            raise KeyError("the checkpoint holds no store_min / store_max for level 2")
+        return cls(levels[0], levels[1], float(ck["store_min"][0]), float(ck["store_max"][0]),
+                   upsample=upsample)
+
     @torch.no_grad()
     def level1_render(self):
-        """(img, dx, dy, dxy) of the whole level-1 canvas, rendered on first use
-        and kept resident (48 bytes per level-1 pixel)."""
+        """The whole level-1 canvas, rendered on first use and kept resident:
+        (img, dx, dy, dxy) for "hermite" (48 bytes per level-1 pixel), the
+        image clamped to [0, 1] (as Gaussian2D.forward returns it,
+        gaussianlig.py:95) for "bilinear"."""
         cached = self._l1_render
         if cached is None:
             with self._l1_lock:
                 if self._l1_render is None:
-                    out = render_view_ga(self.l1, 0.0, 0.0, self.l1.W, self.l1.H)
+                    if self.upsample == "hermite":
+                        out = render_view_ga(self.l1, 0.0, 0.0, self.l1.W, self.l1.H)
+                    else:
+                        out = torch.clamp(render_view(self.l1, 0.0, 0.0, self.l1.W, self.l1.H),
+                                          0, 1)
                     torch.cuda.current_stream(self.l1.means.device).synchronize()
                     self._l1_render = out
                 cached = self._l1_render
         return cached
 
+    def _bilinear_window(self, x0: float, y0: float, w: int, h: int, scale: float):
+        """The window of the clamped level-1 render bilinearly resized to the
+        H*scale x W*scale image (the crop of train.py's resampled estimate)."""
+        fh, fw = self.l2.H * scale, self.l2.W * scale
+        ox, oy = x0 * scale, y0 * scale
+        if fh != int(fh) or fw != int(fw) or ox != int(ox) or oy != int(oy):
+            raise ValueError(f"bilinear composition needs whole output pixels: image "
+                             f"{fw} x {fh}, window origin ({ox}, {oy}) at scale {scale}")
+        return _C.bilinear_resize(self.level1_render(), int(fh), int(fw),
+                                  (int(ox), int(oy), w, h))
+
     @torch.no_grad()
     def render_full(self) -> torch.Tensor:
-        """The whole image, exactly the composition of fit.two_level_fit."""
-        img1, dx, dy, dxy = self.level1_render()
-        up = torch.clamp(g.gradient_aware_upscale(img1, dx, dy, dxy, self.l2.H, self.l2.W), 0, 1)
+        """The whole image: fit.two_level_fit's composition ("hermite"), or
+        train.py's (train.py:187-196) for "bilinear"."""
+        if self.upsample == "bilinear":
+            up = self._bilinear_window(0.0, 0.0, self.l2.W, self.l2.H, 1.0)
+        else:
+            img1, dx, dy, dxy = self.level1_render()
+            up = torch.clamp(g.gradient_aware_upscale(img1, dx, dy, dxy, self.l2.H, self.l2.W),
+                             0, 1)
         img2 = torch.clamp(render_view(self.l2, 0.0, 0.0, self.l2.W, self.l2.H), 0, 1)
         return compose(img2, up, self.lo, self.hi)
 
@@ -178,37 +303,51 @@ class TwoLevel:
                              f"level-1 pixels ({span_x} x {span_y})")
         sx0, sy0 = x0 / f, y0 / f
         # source samples lie in [s0 - 0.5, s0 + span - 0.5); the Hermite patch
-        # reads the two corners around each, so one pixel of margin suffices
+        # reads the two corners around each, so one pixel of margin suffices;
+        # the region is then widened to level 1's 16-pixel tile grid (see
+        # _aligned_view)
         rx0 = max(0, math.floor(sx0 - 0.5) - 1)
         ry0 = max(0, math.floor(sy0 - 0.5) - 1)
         rx1 = min(self.l1.W, math.floor(sx0 + span_x - 0.5) + 3)
         ry1 = min(self.l1.H, math.floor(sy0 + span_y - 0.5) + 3)
+        rx0, rx1 = _tile_span(rx0, rx1 - rx0, self.l1.W)
+        ry0, ry1 = _tile_span(ry0, ry1 - ry0, self.l1.H)
         roi = (sx0 - rx0, sy0 - ry0, sx0 - rx0 + span_x, sy0 - ry0 + span_y)
         return (rx0, ry0, rx1, ry1), roi
 
     def crop_plan(self, x0: float, y0: float, w: int, h: int, scale: float = 1.0,
                   level1_cache: bool = True):
         """The level-2 window's projection (and, without the level-1 cache,
-        the level-1 source region's), the upscaler ROI and the HBM the rest of
-        the render needs.  With the cache the ROI is in level-1 canvas
-        coordinates and level 1 is not rendered per request."""
+        the level-1 source region's), the upscaler ROI, the HBM the rest of
+        the render needs, and the window's offset in its tile-aligned level-2
+        view (plan[3] is the byte count).  With the cache the ROI is in
+        level-1 canvas coordinates and level 1 is not rendered per request."""
+        v2, off = _aligned_view(self.l2, x0, y0, w, h, scale)
+        if self.upsample == "bilinear":
+            (q2,) = project_windows([v2])
+            return (None, q2, (x0, y0, scale), window_bytes(self.l2, q2, False) + 3 * 12 * w * h,
+                    off)
         (rx0, ry0, rx1, ry1), roi = self.l1_region(x0, y0, w, h, scale)
         if level1_cache:
-            (q2,) = project_windows([(self.l2, x0, y0, w, h, scale)])
+            (q2,) = project_windows([v2])
             roi = (roi[0] + rx0, roi[1] + ry0, roi[2] + rx0, roi[3] + ry0)
-            return None, q2, roi, window_bytes(self.l2, q2, False) + 3 * 12 * w * h
+            return None, q2, roi, window_bytes(self.l2, q2, False) + 3 * 12 * w * h, off
         q1, q2 = project_windows([(self.l1, float(rx0), float(ry0), rx1 - rx0, ry1 - ry0, 1.0),
-                                  (self.l2, x0, y0, w, h, scale)])
+                                  v2])
         need = (window_bytes(self.l1, q1, True) + window_bytes(self.l2, q2, False) +
                 3 * 12 * w * h)
-        return q1, q2, roi, need
+        return q1, q2, roi, need, off
 
     @torch.no_grad()
     def finish_crop(self, plan, w: int, h: int) -> torch.Tensor:
-        q1, q2, roi, _ = plan
-        img1, dx, dy, dxy = self.level1_render() if q1 is None else raster_window_ga(self.l1, q1)
-        up = torch.clamp(g.gradient_aware_upscale(img1, dx, dy, dxy, h, w, roi), 0, 1)
-        img2 = torch.clamp(raster_window(self.l2, q2), 0, 1)
+        q1, q2, roi, _, (ox, oy) = plan
+        if self.upsample == "bilinear":
+            up = self._bilinear_window(roi[0], roi[1], w, h, roi[2])
+        else:
+            img1, dx, dy, dxy = (self.level1_render() if q1 is None
+                                 else raster_window_ga(self.l1, q1))
+            up = torch.clamp(g.gradient_aware_upscale(img1, dx, dy, dxy, h, w, roi), 0, 1)
+        img2 = torch.clamp(raster_window(self.l2, q2)[oy:oy + h, ox:ox + w], 0, 1)
         return compose(img2, up, self.lo, self.hi)
 
     @torch.no_grad()

@@ -415,9 +415,12 @@ static void add_plain_pair(double *acc, double alpha, double vis, double dx, dou
     acc[10] += vis * v_alpha;
 }
 
-/* gradient_aware_rasterizer.cu:228-315, d = p - mu */
-static void add_ga_pair(double *acc, double alpha, double vis, double dx, double dy,
-                        const double *conic, const double *c, const double *vI,
+/* gradient_aware_rasterizer.cu:228-315, d = p - mu.  The position gradient
+ * is added to acc[6..7] and to grp[0..1], the running sum of the pixel's
+ * 32-thread group: the reference adds |group sum| to xys.absgrad per group
+ * (gradient_aware_rasterizer.cu:320,339-341), not |v_xy| per pixel. */
+static void add_ga_pair(double *acc, double *grp, double alpha, double vis, double dx,
+                        double dy, const double *conic, const double *c, const double *vI,
                         const double *vdx, const double *vdy, const double *vdxy, double vW) {
     double a = conic[0], b = conic[1], cc = conic[2];
     double sx = a * dx + b * dy, sy = b * dx + cc * dy;
@@ -441,7 +444,7 @@ static void add_ga_pair(double *acc, double alpha, double vis, double dx, double
     acc[5] += alpha * (va * (-0.5 * dy2) + vax * (0.5 * dy2 * sx) + vay * (0.5 * dy2 * sy - dy) +
                        vaxy * (-0.5 * dy2 * P + dy * sx));
     acc[6] += gx; acc[7] += gy;
-    acc[8] += fabs(gx); acc[9] += fabs(gy);
+    grp[0] += gx; grp[1] += gy;
     acc[10] += vis * va;
 }
 
@@ -450,9 +453,14 @@ static void add_ga_pair(double *acc, double alpha, double vis, double dx, double
  * order (the "reference gradient"), FP64 accumulation.  v_dx == NULL selects
  * the plain kernel.  grads: [N, 11] doubles laid out as
  * rgb(3) conic(3) xy(2) xy_abs(2) opacity(1), zero-filled by this function.
+ * xy_abs: the plain kernel sums |v_xy| per pixel (backward.cu:144-150); the GA
+ * kernel sums v_xy over each 32-thread warp of the bw x bw block (thread rank
+ * ty*bw + tx, backward.cu:67 / gradient_aware_rasterizer.cu:170) and adds the
+ * absolute value of that sum per warp (gradient_aware_rasterizer.cu:320,
+ * 339-341); threads outside the image contribute 0.
  * ambig_g (may be NULL) counts ambiguous pairs per Gaussian.
- * Accumulation is deterministic: threads own disjoint tile rows and are
- * reduced in a fixed order. */
+ * Accumulation is deterministic: threads own disjoint tiles and are reduced
+ * in a fixed order. */
 void orc_rasterize_bwd(int num_points, int img_h, int img_w, int bw, const int32_t *gids,
                        const int32_t *bins, const float *xys, const float *conics,
                        const float *colors, const float *opac, const int32_t *final_idx,
@@ -463,6 +471,8 @@ void orc_rasterize_bwd(int num_points, int img_h, int img_w, int bw, const int32
                  gids, bins, xys, conics, colors, opac};
     int ga = v_dx != NULL;
     double clampv = ga ? 0.999 : 0.99;
+    int ntiles = s.tx * s.ty;
+    int npix = bw * bw;
     int nthr = 1;
 #ifdef _OPENMP
     nthr = omp_get_max_threads();
@@ -478,75 +488,87 @@ void orc_rasterize_bwd(int num_points, int img_h, int img_w, int bw, const int32
 #endif
         double *acc = bufs + per * (size_t)tid;
         int32_t *am = amb + (size_t)num_points * (size_t)tid;
-#pragma omp for schedule(static)
-        for (int i = 0; i < img_h; ++i) {
-            for (int j = 0; j < img_w; ++j) {
-                int tile = (i / bw) * s.tx + (j / bw);
-                int lo = bins[2 * tile], hi = bins[2 * tile + 1];
-                size_t pid = (size_t)i * img_w + j;
-                int fin = final_idx[pid];
-                double vo[3] = {v_out[3 * pid], v_out[3 * pid + 1], v_out[3 * pid + 2]};
-                double vw = v_wsum[pid];
-                double vx[3] = {0, 0, 0}, vy[3] = {0, 0, 0}, vxy[3] = {0, 0, 0};
-                if (ga)
-                    for (int ch = 0; ch < 3; ++ch) {
-                        vx[ch] = v_dx[3 * pid + ch];
-                        vy[ch] = v_dy[3 * pid + ch];
-                        vxy[ch] = v_dxy[3 * pid + ch];
-                    }
+#pragma omp for schedule(dynamic, 4)
+        for (int tile = 0; tile < ntiles; ++tile) {
+            int lo = bins[2 * tile], hi = bins[2 * tile + 1];
+            if (lo >= hi) continue;
+            int i0 = (tile / s.tx) * bw, j0 = (tile % s.tx) * bw;
+            /* 32-thread groups of the block, by thread rank r = ly * bw + lx */
+            for (int r0 = 0; r0 < npix; r0 += 32) {
+                int r1 = imin(r0 + 32, npix);
                 /* back to front as the reference; order only matters for FP sums */
                 for (int k = hi - 1; k >= lo; --k) {
-                    if (k > fin) continue;
                     int g = gids[k];
-                    double conic[3] = {conics[3 * g], conics[3 * g + 1], conics[3 * g + 2]};
-                    double col[3] = {colors[3 * g], colors[3 * g + 1], colors[3 * g + 2]};
-                    double alpha, vis, dx, dy;
-                    int valid;
-                    if (mode == 0) {
-                        pair64_t p = eval_pair64(&s, g, j + 0.5, i + 0.5, clampv);
-                        am[g] += p.ambiguous;
-                        valid = p.valid; alpha = p.alpha; vis = p.vis;
-                        dx = p.dx; dy = p.dy;
-                    } else {
-                        pair32_t p = eval_pair32(&s, g, (float)j + 0.5f, (float)i + 0.5f,
-                                                 (float)clampv, ga);
-                        valid = p.valid; alpha = p.alpha; vis = p.vis;
-                        dx = p.dx; dy = p.dy;
-                    }
-                    if (!valid) continue;
                     double *gacc = acc + (size_t)g * NG;
-                    if (!ga) {
-                        if (mode == 1) {
-                            /* backward.cu:127-150 in FP32 */
-                            float fa = (float)alpha, fv = (float)vis;
-                            float fdx = (float)dx, fdy = (float)dy;
-                            float fo[3] = {v_out[3 * pid], v_out[3 * pid + 1], v_out[3 * pid + 2]};
-                            const float *c = colors + 3 * g;
-                            const float *cn = conics + 3 * g;
-                            float v_alpha = 0.0f;
-                            v_alpha += c[0] * fo[0];
-                            v_alpha += c[1] * fo[1];
-                            v_alpha += c[2] * fo[2];
-                            v_alpha += v_wsum[pid];
-                            float v_op = fv * v_alpha;
-                            float v_sigma = -fa * v_alpha;
-                            float vc0 = 0.5f * v_sigma * fdx * fdx;
-                            float vc1 = v_sigma * fdx * fdy;
-                            float vc2 = 0.5f * v_sigma * fdy * fdy;
-                            float gx = v_sigma * (cn[0] * fdx + cn[1] * fdy);
-                            float gy = v_sigma * (cn[1] * fdx + cn[2] * fdy);
-                            for (int ch = 0; ch < 3; ++ch) gacc[ch] += (double)(fa * fo[ch]);
-                            gacc[3] += vc0; gacc[4] += vc1; gacc[5] += vc2;
-                            gacc[6] += gx; gacc[7] += gy;
-                            gacc[8] += fabsf(gx); gacc[9] += fabsf(gy);
-                            gacc[10] += v_op;
+                    double grp[2] = {0.0, 0.0};
+                    for (int r = r0; r < r1; ++r) {
+                        int i = i0 + r / bw, j = j0 + r % bw;
+                        if (i >= img_h || j >= img_w) continue;
+                        size_t pid = (size_t)i * img_w + j;
+                        if (k > final_idx[pid]) continue;
+                        double vo[3] = {v_out[3 * pid], v_out[3 * pid + 1], v_out[3 * pid + 2]};
+                        double vw = v_wsum[pid];
+                        double conic[3] = {conics[3 * g], conics[3 * g + 1], conics[3 * g + 2]};
+                        double col[3] = {colors[3 * g], colors[3 * g + 1], colors[3 * g + 2]};
+                        double alpha, vis, dx, dy;
+                        int valid;
+                        if (mode == 0) {
+                            pair64_t p = eval_pair64(&s, g, j + 0.5, i + 0.5, clampv);
+                            am[g] += p.ambiguous;
+                            valid = p.valid; alpha = p.alpha; vis = p.vis;
+                            dx = p.dx; dy = p.dy;
                         } else {
-                            add_plain_pair(gacc, alpha, vis, dx, dy, conic, col, vo, vw);
+                            pair32_t p = eval_pair32(&s, g, (float)j + 0.5f, (float)i + 0.5f,
+                                                     (float)clampv, ga);
+                            valid = p.valid; alpha = p.alpha; vis = p.vis;
+                            dx = p.dx; dy = p.dy;
                         }
-                    } else {
-                        /* GA: d = p - mu */
-                        if (mode == 0) { dx = -dx; dy = -dy; }
-                        add_ga_pair(gacc, alpha, vis, dx, dy, conic, col, vo, vx, vy, vxy, vw);
+                        if (!valid) continue;
+                        if (!ga) {
+                            if (mode == 1) {
+                                /* backward.cu:127-150 in FP32 */
+                                float fa = (float)alpha, fv = (float)vis;
+                                float fdx = (float)dx, fdy = (float)dy;
+                                float fo[3] = {v_out[3 * pid], v_out[3 * pid + 1],
+                                               v_out[3 * pid + 2]};
+                                const float *c = colors + 3 * g;
+                                const float *cn = conics + 3 * g;
+                                float v_alpha = 0.0f;
+                                v_alpha += c[0] * fo[0];
+                                v_alpha += c[1] * fo[1];
+                                v_alpha += c[2] * fo[2];
+                                v_alpha += v_wsum[pid];
+                                float v_op = fv * v_alpha;
+                                float v_sigma = -fa * v_alpha;
+                                float vc0 = 0.5f * v_sigma * fdx * fdx;
+                                float vc1 = v_sigma * fdx * fdy;
+                                float vc2 = 0.5f * v_sigma * fdy * fdy;
+                                float gx = v_sigma * (cn[0] * fdx + cn[1] * fdy);
+                                float gy = v_sigma * (cn[1] * fdx + cn[2] * fdy);
+                                for (int ch = 0; ch < 3; ++ch) gacc[ch] += (double)(fa * fo[ch]);
+                                gacc[3] += vc0; gacc[4] += vc1; gacc[5] += vc2;
+                                gacc[6] += gx; gacc[7] += gy;
+                                gacc[8] += fabsf(gx); gacc[9] += fabsf(gy);
+                                gacc[10] += v_op;
+                            } else {
+                                add_plain_pair(gacc, alpha, vis, dx, dy, conic, col, vo, vw);
+                            }
+                        } else {
+                            double vx[3], vy[3], vxy[3];
+                            for (int ch = 0; ch < 3; ++ch) {
+                                vx[ch] = v_dx[3 * pid + ch];
+                                vy[ch] = v_dy[3 * pid + ch];
+                                vxy[ch] = v_dxy[3 * pid + ch];
+                            }
+                            /* GA: d = p - mu */
+                            if (mode == 0) { dx = -dx; dy = -dy; }
+                            add_ga_pair(gacc, grp, alpha, vis, dx, dy, conic, col, vo, vx, vy,
+                                        vxy, vw);
+                        }
+                    }
+                    if (ga) {
+                        gacc[8] += fabs(grp[0]);
+                        gacc[9] += fabs(grp[1]);
                     }
                 }
             }

@@ -175,3 +175,39 @@ def test_batched_images_without_intersections(device):
         for name in ("means", "cholesky", "rgbs"):
             assert torch.equal(getattr(bf, name)[o:o + c], getattr(lv, name)), (i, name)
         o += c
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batched_dense_tiles_match_api_chain(device, persistent):
+    """Dense tiles (over a thousand Gaussians per tile, so the fit backward
+    groups several 512-Gaussian chunks by their rows, with partial edge rows
+    on the 100 x 130 image) in the batched grid and persistent-queue launches:
+    every image's loss and its parameters after two Adam steps equal those of
+    its plain fit through the reference API chain (Level(fused=False))."""
+    from lig_amd.batch import BatchedFit, make_targets
+    from lig_amd.fit import Level
+    sizes = [(100, 130), (96, 96), (40, 56)]
+    counts = [20000, 12000, 2000]
+    seeds = [31, 32, 33]
+    print(f"batched dense persistent={persistent}", flush=True)
+    targets = make_targets(sizes, seeds, device)
+    bf = BatchedFit(targets, counts, seeds, scale=3.0, lr=1e-2)
+    bf.persistent = persistent
+    levels = [Level(c, h, w, s, 3.0, lr=1e-2, device=device, fused=False)
+              for (h, w), c, s in zip(sizes, counts, seeds)]
+    with torch.no_grad():
+        bf.rgbs.mul_(0.01)   # keep part of each image inside (0, 1)
+        for lv in levels:
+            lv.rgbs.mul_(0.01)
+        nt = levels[0].project()[3]
+    assert int(nt.sum()) > 63 * 2 * 512
+    for step in range(2):
+        bf.step()
+        plain = [float(lv.step(t)) for lv, t in zip(levels, targets)]
+        for i, (a, b) in enumerate(zip(bf.last_losses.tolist(), plain)):
+            assert a == pytest.approx(b, rel=1e-6), (step, i)
+    o = 0
+    for i, (lv, c) in enumerate(zip(levels, counts)):
+        for name in ("means", "cholesky", "rgbs"):
+            assert torch.equal(getattr(bf, name)[o:o + c], getattr(lv, name)), (i, name)
+        o += c

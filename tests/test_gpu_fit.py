@@ -80,17 +80,21 @@ def test_fused_step_matches_api_chain(device, n, H, W, scale, seed):
         assert torch.equal(getattr(a, name), getattr(b, name)), name
 
 
-def test_fused_step_matches_api_chain_dense_tiles(device):
-    """Thousands of Gaussians per tile, each tile's backward in one block: the
-    fit backward groups each chunk of 512 of a tile's Gaussians by the rows
-    they reach (several chunks per tile here, Gaussians reaching no row among
-    them) and still gives the API chain's gradients bit for bit."""
+@pytest.mark.parametrize("n,H,W,seed,shares", [
+    (12000, 96, 96, 6, 1),      # each tile's backward in one block
+    (20000, 100, 130, 7, 0),    # partial edge rows and columns, shares chosen per launch
+])
+def test_fused_step_matches_api_chain_dense_tiles(device, n, H, W, seed, shares):
+    """Thousands of Gaussians per tile: the fit backward groups each chunk of
+    512 of a tile's Gaussians by the rows they reach (several chunks per tile
+    here, Gaussians reaching no row among them; on the 100 x 130 canvas the
+    edge tiles' row masks cut the groups, and the tile's batches may be split
+    over several blocks) and still gives the API chain's gradients bit for bit."""
     import lig_amd._lib as L
     from lig_amd import data
-    n, H, W, seed = 12000, 96, 96, 6
-    print(f"fused-vs-api dense n={n} H={H} W={W} seed={seed} shares=1", flush=True)
+    print(f"fused-vs-api dense n={n} H={H} W={W} seed={seed} shares={shares}", flush=True)
     lib = L.load()
-    prev = lib.lig_set_backward_shares(1)
+    prev = lib.lig_set_backward_shares(shares)
     try:
         tgt = data.make_image(H, W, seed, device=device)
         a, b = _level_pair(n, H, W, seed, 3.0, device)
@@ -98,7 +102,8 @@ def test_fused_step_matches_api_chain_dense_tiles(device):
             b.rgbs.mul_(0.01)
             a.rgbs.copy_(b.rgbs)
             nt = a.project()[3]
-        assert int(nt.sum()) > 36 * 2 * 512  # 36 tiles: several 512-Gaussian chunks each
+        ntiles = ((H + 15) // 16) * ((W + 15) // 16)
+        assert int(nt.sum()) > ntiles * 2 * 512  # several 512-Gaussian chunks per tile
         la, lb = a.loss(tgt), b.loss(tgt)
         la.backward()
         lb.backward()

@@ -257,3 +257,111 @@ def test_backward_matches_finite_differences(ga):
             assert abs(an - fd) <= 1e-5 * max(1.0, abs(fd)), (g, name, k, an, fd)
             checked += 1
     assert checked > 60
+
+
+def _pixel_vxy_fd(mu, cn, col, op, H, W, w_img, w_ws, ga_w, h=1e-6):
+    """Per-pixel, per-Gaussian position gradient [n, H, W, 2] by FP64 central
+    differences of each Gaussian's own tile-free forward (the forward is a sum
+    over Gaussians, so pixel p's gradient w.r.t. mean g involves g alone)."""
+    n = mu.shape[0]
+    out = np.zeros((n, H, W, 2))
+    ga = ga_w is not None
+
+    def per_pixel(m, g):
+        img, ws, dx, dy, dxy = _brute_forward(m, cn[g:g + 1], col[g:g + 1], op[g:g + 1], H, W,
+                                              ga=ga)
+        v = (img * w_img).sum(-1) + ws * w_ws
+        if ga:
+            v = v + (dx * ga_w[0]).sum(-1) + (dy * ga_w[1]).sum(-1) + (dxy * ga_w[2]).sum(-1)
+        return v
+
+    for g in range(n):
+        base = _pairs(mu[g:g + 1], cn[g:g + 1], op[g:g + 1], H, W)
+        for k in range(2):
+            plus, minus = mu[g:g + 1].copy(), mu[g:g + 1].copy()
+            plus[0, k] += h
+            minus[0, k] -= h
+            for m in (plus, minus):
+                assert np.array_equal(_pairs(m, cn[g:g + 1], op[g:g + 1], H, W), base)
+            out[g, :, :, k] = (per_pixel(plus, g) - per_pixel(minus, g)) / (2 * h)
+    return out
+
+
+def _group_abs(vxy_px, H, W, bw):
+    """sum over tiles and 32-thread groups (thread rank ty*bw + tx) of
+    |sum of the group's per-pixel gradients|; pixels outside the image add 0."""
+    n = vxy_px.shape[0]
+    tx, ty = (W + bw - 1) // bw, (H + bw - 1) // bw
+    out = np.zeros((n, 2))
+    ranks = np.arange(bw * bw)
+    for t in range(tx * ty):
+        i0, j0 = (t // tx) * bw, (t % tx) * bw
+        for r0 in range(0, bw * bw, 32):
+            r = ranks[r0:r0 + 32]
+            i, j = i0 + r // bw, j0 + r % bw
+            inside = (i < H) & (j < W)
+            out += np.abs(vxy_px[:, i[inside], j[inside], :].sum(1))
+    return out
+
+
+@pytest.mark.parametrize("bw", [16, 8, 12, 5])
+def test_absgrad_group_semantics(bw):
+    """xys.absgrad pinned independently of the oracle's pair math: per-pixel
+    position gradients by FD, then the plain kernel's sum of per-pixel |v_xy|
+    (backward.cu:144-150) and the gradient-aware kernel's sum over 32-thread
+    groups of |group sum| (gradient_aware_rasterizer.cu:320,339-341).  The scene
+    is chosen so that the two differ by far more than the tolerance."""
+    H, W = 26, 30
+    s = random_scene(10, H, W, 30 + bw, opacities=True, scale=(1.5, 4.0))
+    s["opac"] = np.clip(s["opac"], 0.1, 0.9).astype(np.float32)   # never clamped
+    xys, ext, con, nt = _scene_proj(s, bw)
+    assert np.all(nt > 0)
+    m, cum, isect, gids, ks, gs, bins = oracle.bin_and_sort(xys, ext, nt, H, W, bw)
+    fwd = oracle.rasterize_forward(gs, bins, xys, con, s["colors"], s["opac"], H, W, bw)
+    rng = np.random.default_rng(bw)
+    w_img = rng.normal(size=(H, W, 3)).astype(np.float32)
+    w_ws = rng.normal(size=(H, W)).astype(np.float32)
+    ga_w = [rng.normal(size=(H, W, 3)).astype(np.float32) for _ in range(3)]
+    mu, cn = xys.astype(np.float64), con.astype(np.float64)
+    col, op = s["colors"].astype(np.float64), s["opac"].astype(np.float64)
+    for ga in (False, True):
+        gw = ga_w if ga else None
+        got = oracle.rasterize_backward(gs, bins, xys, con, s["colors"], s["opac"],
+                                        fwd["final_idx"], w_img, w_ws, H, W, bw,
+                                        *(gw or [None] * 3))
+        px = _pixel_vxy_fd(mu, cn, col, op, H, W, w_img.astype(np.float64),
+                           w_ws.astype(np.float64),
+                           None if gw is None else [a.astype(np.float64) for a in gw])
+        per_pixel = np.abs(px).sum((1, 2))
+        grouped = _group_abs(px, H, W, bw)
+        want = grouped if ga else per_pixel
+        np.testing.assert_allclose(got["v_xy_abs"], want, rtol=1e-5, atol=1e-6,
+                                   err_msg=f"ga={ga} bw={bw}")
+        np.testing.assert_allclose(got["v_xy"], px.sum((1, 2)), rtol=1e-5, atol=1e-6)
+        if ga:
+            # the two semantics must be far apart for this pin to mean anything
+            assert grouped.sum() < 0.9 * per_pixel.sum(), (grouped.sum(), per_pixel.sum())
+
+
+@pytest.mark.parametrize("bw,op", [(16, True), (8, False), (5, True)])
+def test_final_idx_is_last_contributing_bin_position(bw, op):
+    """final_idx (forward.cu:217,269,274): for each pixel, the largest position
+    k of its tile's bin whose Gaussian contributes (tile-free predicate), else 0."""
+    H, W = 30, 34
+    s = random_scene(60, H, W, 40 + bw, opacities=op, margin=5.0)
+    xys, ext, con, nt = _scene_proj(s, bw)
+    m, cum, isect, gids, ks, gs, bins = oracle.bin_and_sort(xys, ext, nt, H, W, bw)
+    fwd = oracle.rasterize_forward(gs, bins, xys, con, s["colors"], s["opac"], H, W, bw)
+    opv = np.ones(60) if s["opac"] is None else s["opac"].astype(np.float64)
+    pairs = _pairs(xys.astype(np.float64), con.astype(np.float64), opv, H, W)
+    tx = (W + bw - 1) // bw
+    want = np.zeros((H, W), np.int32)
+    for i in range(H):
+        for j in range(W):
+            t = (i // bw) * tx + j // bw
+            for k in range(bins[t, 0], bins[t, 1]):
+                if pairs[gs[k], i, j]:
+                    want[i, j] = k
+    clean = fwd["ambig"] == 0
+    np.testing.assert_array_equal(fwd["final_idx"][clean], want[clean])
+    assert clean.mean() > 0.95 and want.max() > 0
