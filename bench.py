@@ -109,37 +109,38 @@ def measured_traffic(config, kernel):
     return int(k["traffic_bytes"]), os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(level, target, crop, oracle, torch):
-    """Oracle (FP64 CPU port) fit step on a crop x crop window of `level`:
-    project, bin, rasterise forward, loss gradient, rasterise backward,
-    projection backward.  Returns seconds for the crop."""
+def cpu_baseline(level, target, oracle, torch):
+    """Oracle (FP64 CPU port) fit step of a whole level: Cholesky projection,
+    binning, rasteriser forward, clamp + L2 loss gradient, rasteriser
+    backward, projection backward, Adam (torch's update on the CPU).
+    Returns (seconds, Gaussians, intersections)."""
     import numpy as np
     H, W = level.H, level.W
-    y0, x0 = (H - crop) // 2, (W - crop) // 2
     with torch.no_grad():
-        xys, ext, con, nt = level.project()
-        xs, ys = xys[:, 0], xys[:, 1]
-        ex, ey = ext[:, 0], ext[:, 1]
-        sel = (nt > 0) & (xs + ex >= x0) & (xs - ex <= x0 + crop) & (ys + ey >= y0) & \
-              (ys - ey <= y0 + crop)
-        idx = torch.nonzero(sel).flatten()
-        means = (level.means[idx] - torch.tensor([x0, y0], device=xys.device,
-                                                 dtype=torch.float32)).cpu().numpy()
-        chol = level.cholesky[idx].cpu().numpy()
-        rgbs = level.rgbs[idx].cpu().numpy()
-        tgt = target[y0:y0 + crop, x0:x0 + crop].cpu().numpy().astype(np.float64)
+        means = level.means.detach().cpu().numpy()
+        chol = level.cholesky.detach().cpu().numpy()
+        rgbs = level.rgbs.detach().cpu().numpy()
+        tgt = target.cpu().numpy().astype(np.float64)
+    params = [torch.from_numpy(a.copy()).requires_grad_(True) for a in (means, chol, rgbs)]
+    opt = torch.optim.Adam(params, lr=1e-3)
     t0 = time.perf_counter()
-    pxy, pext, pcon, pnt = oracle.project_gaussians_forward(chol, means, crop, crop, 16, None,
+    pxy, pext, pcon, pnt = oracle.project_gaussians_forward(chol, means, H, W, 16, None,
                                                             cholesky=True)
-    m, cum, isect, gids, ks, gs, bins = oracle.bin_and_sort(pxy, pext, pnt, crop, crop, 16)
-    fwd = oracle.rasterize_forward(gs, bins, pxy, pcon, rgbs, None, crop, crop, 16)
+    m, cum, isect, gids, ks, gs, bins = oracle.bin_and_sort(pxy, pext, pnt, H, W, 16)
+    del isect, gids, ks
+    fwd = oracle.rasterize_forward(gs, bins, pxy, pcon, rgbs, None, H, W, 16)
     img = fwd["img"]
     inside = (img > 0) & (img < 1)
-    v_out = (2.0 / img.size) * (np.clip(img, 0, 1) - tgt) * inside
+    v_out = ((2.0 / img.size) * (np.clip(img, 0, 1) - tgt) * inside).astype(np.float32)
+    del img, inside
     bwd = oracle.rasterize_backward(gs, bins, pxy, pcon, rgbs, None, fwd["final_idx"], v_out,
-                                    np.zeros((crop, crop), np.float32), crop, crop, 16)
-    oracle.project_gaussians_backward(pext, pcon, bwd["v_xy"], bwd["v_conic"], cholesky=chol)
-    return time.perf_counter() - t0, int(idx.numel()), m
+                                    np.zeros((H, W), np.float32), H, W, 16)
+    v_chol, v_mean = oracle.project_gaussians_backward(pext, pcon, bwd["v_xy"], bwd["v_conic"],
+                                                       cholesky=chol)
+    for p_, g_ in zip(params, (v_mean, v_chol, bwd["v_rgb"])):
+        p_.grad = torch.from_numpy(np.ascontiguousarray(g_, dtype=np.float32))
+    opt.step()
+    return time.perf_counter() - t0, int(means.shape[0]), m
 
 
 def batched_run(args, dev, torch):
@@ -219,6 +220,47 @@ def batched_run(args, dev, torch):
     res["graph_captures"] = gs.captures
     del gs, bf
     torch.cuda.empty_cache()
+    # the two forms at a fitted PSNR: fresh parameters, K steps of each
+    # image's captured plain step (one image after another) against K replays
+    # of the captured batched step, lr 1e-2; mean PSNR of each form after
+    if args.batch_fit_steps > 0:
+        K = args.batch_fit_steps
+        lr = 1e-2
+        plain_f = [fit.Level(c, h, w, s, scale, lr=lr, device=dev) for (h, w), c, s in
+                   zip(sizes, counts, seeds)]
+        for lv, lp in zip(plain_f, plain):
+            lv.batch_depth = lp.batch_depth
+        bff = batch.BatchedFit(targets, counts, seeds, scale=scale, lr=lr)
+        bff.batch_depth, bff.persistent = tuned["choice"], tuned["persistent"]
+        graphs = [lv.graph(t) for lv, t in zip(plain_f, targets)]
+        gsf = batch.BatchedStepGraph(bff)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for gph in graphs:
+            gph.run(K)
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        gsf.run(K)
+        torch.cuda.synchronize()
+        batch_s = time.perf_counter() - t0
+        with torch.no_grad():
+            pp = np.array([fit.psnr(torch.clamp(lv.render(), 0, 1), t)
+                           for lv, t in zip(plain_f, targets)])
+            bff.loss()
+            pb = bff.psnr()
+        res["fitted"] = {"steps": K, "lr": lr,
+                         "plain_graph_steps_s": round(K / plain_s, 3),
+                         "batched_graph_steps_s": round(K / batch_s, 3),
+                         "speedup": round(plain_s / batch_s, 3),
+                         "mean_psnr_db_plain": round(float(pp.mean()), 4),
+                         "mean_psnr_db_batched": round(float(pb.mean()), 4),
+                         "max_abs_dpsnr_db": float(np.abs(pp - pb).max()),
+                         "plain_captures": int(sum(gph.captures for gph in graphs)),
+                         "batched_captures": gsf.captures}
+        log(f"batched fitted {res['fitted']}")
+        del graphs, gsf, bff, plain_f
+        torch.cuda.empty_cache()
     # skewed batch: the same images with 8 of them at 20x the Gaussian density,
     # where whole XCD shares of the tile list cost far more than others
     skew = list(counts)
@@ -303,6 +345,10 @@ def main():
                     help="timed loss+backward calls on the density-skewed level 2 (0 skips)")
     ap.add_argument("--batch-steps", type=int, default=5,
                     help="timed steps of the batched-fitting comparison (0 skips it)")
+    ap.add_argument("--batch-fit-steps", type=int, default=300,
+                    help="steps of the batched-vs-plain comparison at a fitted PSNR (0 skips)")
+    ap.add_argument("--api-steps", type=int, default=5,
+                    help="timed steps of the reference-API chain at the headline workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -436,27 +482,50 @@ def main():
                 "algorithmic_bytes": dom["bytes"],
                 "hbm_gbs_algorithmic": round(dom["bytes"] / (dom["ms"] * 1e-3) / 1e9, 1)}
 
+    # the rate an unchanged train.py gets through the reference API (the
+    # alias of lig_amd.gsplat2d): Level(fused=False) runs project ->
+    # rasterize_gaussians -> clamp + L2 -> backward -> Adam per level
+    api_chain = None
+    if args.api_steps > 0:
+        for lv in (l1, l2):
+            lv.fused = False
+        for _ in range(2):
+            l1.step(t1)
+            l2.step(t2)
+        torch.cuda.synchronize()
+        a0 = time.perf_counter()
+        for _ in range(args.api_steps):
+            l1.step(t1)
+            l2.step(t2)
+        torch.cuda.synchronize()
+        api_s = time.perf_counter() - a0
+        for lv in (l1, l2):
+            lv.fused = True
+        api_chain = {"steps_s": round(args.api_steps / api_s, 3),
+                     "ms_per_step": round(api_s / args.api_steps * 1e3, 3),
+                     "steps": args.api_steps,
+                     "path": "gsplat2d API: project_gaussians_cholesky -> rasterize_gaussians "
+                             "-> clamp + L2 -> backward -> Adam (Level(fused=False))"}
+        log(f"api chain {api_chain}")
+
     cpu = None
     if args.cpu_baseline:
         import oracle
         oracle.build()
         threads = oracle.num_threads()
-        crops = {"l1": 1024, "l2": 2048}   # about 10 s of CPU work in all
-        est = 0.0
+        total = 0.0
         parts = []
         for name, lev, tgt in (("l1", l1, t1), ("l2", l2, t2)):
-            cr = min(crops[name], lev.H, lev.W)
-            secs, ng, m = cpu_baseline(lev, tgt, cr, oracle, torch)
-            scale = (lev.H * lev.W) / (cr * cr)
-            est += secs * scale
-            parts.append(f"{name}: {cr}x{cr} crop ({ng} Gaussians, {m} intersections) "
-                         f"{secs:.2f}s x{scale:.0f}")
+            secs, ng, m = cpu_baseline(lev, tgt, oracle, torch)
+            total += secs
+            parts.append(f"{name}: whole {lev.W}x{lev.H} level ({ng} Gaussians, {m} "
+                         f"intersections) {secs:.2f}s")
             log(f"cpu baseline {parts[-1]}")
-        cpu = {"value": round(1.0 / est, 6), "unit": "fit steps/s", "cores": threads,
+        cpu = {"value": round(1.0 / total, 6), "unit": "fit steps/s", "cores": threads,
                "kind": "port",
-               "sample": "oracle (C, FP64 raster, OpenMP) project+bin+raster fwd/bwd+proj bwd "
-                         "on a central crop of each level, scaled by image area / crop area; "
-                         + "; ".join(parts)}
+               "sample": "one whole two-level fit step: the oracle (C, FP64 raster, OpenMP) "
+                         "project+bin+raster fwd/loss grad/raster bwd/proj bwd and torch's "
+                         "CPU Adam on each full level; " + "; ".join(parts)}
 
     # the level-2 kernels on a density-skewed copy of the level (half the
     # Gaussians pulled into the top quarter of the canvas: 2.5x the mean
@@ -515,6 +584,7 @@ def main():
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
+        "api_chain": api_chain,
         "autotune": {k: {"tile/depth": v.get("measured_with", v["choice"]), "ms": v["ms"]}
                      for k, v in tuned.items()},
         "batched": batched,
