@@ -45,6 +45,7 @@ _SIGNATURES = {
     "lig_bin_and_sort_gaussians": (
         _c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
                  _c_size, _vp]),
+    "lig_bin_rows_workspace_size": (_c_size, [_c_int, _c_int, _c_int, _c_int, _c_int]),
     "lig_bin_and_sort_gaussians_noalloc": (
         _c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
     "lig_packed_gaussians_bytes": (_c_size, [_c_int]),

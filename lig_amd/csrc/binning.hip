@@ -9,6 +9,8 @@
 // Reference behaviour followed (paths relative to the reference repository):
 //   gsplat2d/gsplat2d/utils.py:61-100 (cumsum, .item, torch.sort, gather),
 //   gsplat2d/gsplat2d/cuda/csrc/forward.cu:109-170, bindings.cu:239-301.
+#include <algorithm>
+#include <cmath>
 #include <mutex>
 
 #include "lig_device.h"
@@ -662,6 +664,488 @@ int bits_for(int num_tiles) {
     return b;
 }
 
+// ---------------------------------------------------------------------------
+// Row-expansion binning (one image, at most kRowMaxTiles tiles per row).
+//
+// A Gaussian's tiles form a rectangle, so its intersections are laid out by
+// tile rows: one row entry (row, gid) per tile row a Gaussian spans, about
+// sqrt(tiles per Gaussian) times fewer than intersections (12.6 M against
+// 79 M at 8192^2 x 2 M).  The entries are emitted gid-major and sorted
+// stably by row, which puts each row's Gaussians in gid order.  Each row's
+// entries are split into P segments (P chosen by the host from the
+// intersection count); a counting pass writes, for every (row, segment,
+// tile), how many of the segment's entries cover the tile.  Within a row, a
+// prefix over segments per tile and a scan of the tile totals, plus a scan of
+// the row totals, give every segment's first output position in every tile
+// and the bins.  The scatter pass expands each segment's entries (rounds of
+// 64, one per lane) into their tiles; a lane's rank in a tile is the tile's
+// cursor plus the number of lower lanes covering it (a 64-bit lane mask per
+// tile, built with LDS atomics).  The output is the stable (tile, emission)
+// order of the radix path, bit for bit.
+// ---------------------------------------------------------------------------
+constexpr int kRowMaxTiles = 1024;     // tiles per row the LDS tables hold
+constexpr int kRowMaxRows = 32767;     // rows per image (15-bit row fields)
+constexpr uint32_t kPhantomBit = 0x80000000u;
+
+// Tile rectangle of a Gaussian: x = x0 | x1 << 16, y = y0 | y1 << 16, bit 31
+// of y set for a zero-extent Gaussian with a tile (its one slot is binned as
+// tile 0, gid 0: the reference's zero fill).
+__device__ __forceinline__ void unpack_rect(uint2 rc, int &x0, int &y0, int &x1, int &y1,
+                                            bool &ph) {
+    x0 = (int)(rc.x & 0xFFFFu);
+    x1 = (int)(rc.x >> 16);
+    y0 = (int)(rc.y & 0xFFFFu);
+    y1 = (int)((rc.y >> 16) & 0x7FFFu);
+    ph = (rc.y & kPhantomBit) != 0;
+}
+
+// Per Gaussian: its tile rectangle and the number of tile rows it spans (the
+// rows of its emission slots; 1 for a phantom slot, 0 without a slot).
+__global__ __launch_bounds__(256) void rect_rows_kernel(int n, const float *__restrict__ xys,
+                                                        const float *__restrict__ extents,
+                                                        const int32_t *__restrict__ cum,
+                                                        int tiles_x, int tiles_y,
+                                                        int block_width,
+                                                        uint2 *__restrict__ rect,
+                                                        int32_t *__restrict__ nrows) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int cnt = cum[g] - (g == 0 ? 0 : cum[g - 1]);
+    if (cnt <= 0) {
+        rect[g] = make_uint2(0u, 0u);
+        nrows[g] = 0;
+        return;
+    }
+    const float ex = extents[2 * g], ey = extents[2 * g + 1];
+    if (ex > 0.0f && ey > 0.0f) {
+        int x0, y0, x1, y1;
+        tile_bbox(xys[2 * g], xys[2 * g + 1], ex, ey, tiles_x, tiles_y, block_width, x0, y0, x1,
+                  y1);
+        rect[g] = make_uint2((uint32_t)x0 | ((uint32_t)x1 << 16),
+                             (uint32_t)y0 | ((uint32_t)y1 << 16));
+        nrows[g] = y1 - y0;
+    } else {
+        rect[g] = make_uint2(1u << 16, (1u << 16) | kPhantomBit);
+        nrows[g] = 1;
+    }
+}
+
+// Row entries in gid-major order: key = tile row, value = gid (phantom bit
+// set for a phantom slot, whose key is row 0).  A wave covers its 64
+// Gaussians' joint entry range 64 entries at a time; a lane finds the owner of
+// its entry by binary search over the wave's starts, so the stores are
+// coalesced.  `cap` bounds the entries written (they never exceed the
+// intersection count the buffers hold).
+__global__ __launch_bounds__(256) void emit_rows_kernel(int n, int cap,
+                                                        const int32_t *__restrict__ ecum,
+                                                        const uint2 *__restrict__ rect,
+                                                        uint32_t *__restrict__ rowkey,
+                                                        uint32_t *__restrict__ rowval) {
+    const int lane = threadIdx.x & (LIG_WAVE - 1);
+    const int wv = threadIdx.x / LIG_WAVE;
+    const int g0 = blockIdx.x * blockDim.x + wv * LIG_WAVE;
+    if (g0 >= n) return;  // wave-uniform
+    const int g = g0 + lane;
+    const int wstart = g0 == 0 ? 0 : ecum[g0 - 1];
+    const int wend = min(ecum[min(g0 + LIG_WAVE, n) - 1], cap);
+    __shared__ int s_start[256 / LIG_WAVE][LIG_WAVE];
+    __shared__ uint32_t s_key[256 / LIG_WAVE][LIG_WAVE];
+    int start = wend;
+    uint32_t key = 0u;
+    if (g < n) {
+        start = g == 0 ? 0 : ecum[g - 1];
+        int x0, y0, x1, y1;
+        bool ph;
+        unpack_rect(rect[g], x0, y0, x1, y1, ph);
+        key = ph ? kPhantomBit : (uint32_t)y0;
+    }
+    s_start[wv][lane] = start;
+    s_key[wv][lane] = key;
+    wave_lds_sync();
+    for (int base = wstart; base < wend; base += LIG_WAVE) {
+        const int pos = base + lane;
+        if (pos < wend) {
+            // owner: the last lane whose range starts at or before pos
+            int lo = 0, hi = LIG_WAVE - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_start[wv][mid] <= pos)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const uint32_t k = s_key[wv][lo];
+            const bool ph = (k & kPhantomBit) != 0;
+            rowkey[pos] = ph ? 0u : k + (uint32_t)(pos - s_start[wv][lo]);
+            rowval[pos] = (uint32_t)(g0 + lo) | (ph ? kPhantomBit : 0u);
+        }
+    }
+}
+
+// Row r's entries [lo, hi) of segment s (of P).
+__device__ __forceinline__ void row_segment(const int2 *__restrict__ rowbins, int r, int s, int P,
+                                            int &lo, int &hi) {
+    const int2 rb = rowbins[r];
+    const int cnt = rb.y - rb.x;
+    const int len = (cnt + P - 1) / P;
+    lo = rb.x + min(cnt, s * len);
+    hi = rb.x + min(cnt, (s + 1) * len);
+}
+
+__device__ __forceinline__ int wave_exclusive_sum(int v) {
+    const int lane = threadIdx.x & (LIG_WAVE - 1);
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < LIG_WAVE; off <<= 1) {
+        const int y = __shfl_up(x, off, LIG_WAVE);
+        if (lane >= off) x += y;
+    }
+    return x - v;
+}
+
+// Counting pass: one wave per (row, segment), a row's segments on one XCD.
+// counts[(r P + s) tiles_x + t] = the segment's entries whose column range
+// covers tile t (a difference array over the row's columns in LDS, then its
+// prefix sum).
+__global__ __launch_bounds__(LIG_WAVE) void row_count_kernel(
+    int P, int tiles_x, const int2 *__restrict__ rowbins, const uint32_t *__restrict__ rowval,
+    const uint2 *__restrict__ rect, int32_t *__restrict__ counts) {
+    extern __shared__ int diff[];  // tiles_x + 1
+    const int lane = threadIdx.x;
+    const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int r = u / P;
+    const int s = u - r * P;
+    int lo, hi;
+    row_segment(rowbins, r, s, P, lo, hi);
+    for (int t = lane; t <= tiles_x; t += LIG_WAVE) diff[t] = 0;
+    wave_lds_sync();
+    for (int j = lo + lane; j < hi; j += LIG_WAVE) {
+        const uint32_t v = rowval[j];
+        const uint2 rc = rect[v & ~kPhantomBit];
+        int x0, y0, x1, y1;
+        bool ph;
+        unpack_rect(rc, x0, y0, x1, y1, ph);
+        atomicAdd(&diff[x0], 1);
+        atomicAdd(&diff[x1], -1);
+    }
+    wave_lds_sync();
+    // prefix: chunks of LIG_WAVE consecutive columns, lanes on consecutive columns
+    int carry = 0;
+    int32_t *out = counts + ((size_t)r * P + s) * tiles_x;
+    for (int t0 = 0; t0 < tiles_x; t0 += LIG_WAVE) {
+        const int t = t0 + lane;
+        const int d = t < tiles_x ? diff[t] : 0;
+        const int incl = wave_exclusive_sum(d) + d;
+        if (t < tiles_x) out[t] = carry + incl;
+        carry += __shfl(incl, LIG_WAVE - 1, LIG_WAVE);
+    }
+}
+
+// Per row (one block): each tile's column of segment counts becomes its
+// exclusive prefix over segments (scol, same layout); the tile totals go to
+// tt, their exclusive prefix within the row to te, the row total to
+// rowtot[r] (rowtot[rows] = 0 for the scan that follows).
+constexpr int kPrefixThreads = 256;
+__global__ __launch_bounds__(kPrefixThreads) void row_prefix_kernel(
+    int P, int tiles_x, int rows, const int32_t *__restrict__ counts, int32_t *__restrict__ scol,
+    int32_t *__restrict__ tt, int32_t *__restrict__ te, int32_t *__restrict__ rowtot) {
+    __shared__ int wsum[kPrefixThreads / LIG_WAVE];
+    const int r = blockIdx.x;
+    const size_t rbase = (size_t)r * P * tiles_x;
+    constexpr int kPer = kRowMaxTiles / kPrefixThreads;
+    int tot[kPer];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        // thread i takes columns i * kPer .. i * kPer + kPer - 1 (row order for the scan)
+        const int t = threadIdx.x * kPer + k;
+        int run = 0;
+        if (t < tiles_x) {
+            for (int s = 0; s < P; ++s) {
+                const size_t i = rbase + (size_t)s * tiles_x + t;
+                const int v = counts[i];
+                scol[i] = run;
+                run += v;
+            }
+        }
+        tot[k] = run;
+        mine += run;
+    }
+    int total;
+    int excl = block_exclusive_scan_n<kPrefixThreads>(mine, wsum, &total);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int t = threadIdx.x * kPer + k;
+        if (t < tiles_x) {
+            tt[(size_t)r * tiles_x + t] = tot[k];
+            te[(size_t)r * tiles_x + t] = excl;
+        }
+        excl += tot[k];
+    }
+    if (threadIdx.x == 0) {
+        rowtot[r] = total;
+        if (r == 0) rowtot[rows] = 0;
+    }
+}
+
+// Bins: tile (r, t) holds [rowoff[r] + te, + tt), (0, 0) when empty.
+__global__ void row_bins_kernel(int num_tiles, int tiles_x, const int32_t *__restrict__ rowoff,
+                                const int32_t *__restrict__ tt, const int32_t *__restrict__ te,
+                                int32_t *__restrict__ bins) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tiles) return;
+    const int c = tt[t];
+    const int st = rowoff[t / tiles_x] + te[t];
+    bins[2 * t] = c > 0 ? st : 0;
+    bins[2 * t + 1] = c > 0 ? st + c : 0;
+}
+
+// Scatter pass: one wave per (row, segment), a row's segments on one XCD.
+// The segment's output for tile t is the run [S(t, s), S(t, s) + c(t, s)) of
+// the tile's bin (c from consecutive scanned counts); the runs are numbered
+// consecutively in LDS (lstart).  The wave takes the segment's entries in
+// rounds of 64 (gid order): each lane marks its tiles in per-tile 64-bit lane
+// masks, then puts its gid (0 for a phantom) -- and its emission slot when
+// asked -- at the tile's run start + cursor + popcount(mask & lower lanes) in
+// an LDS buffer, with the tile's index beside it; the lowest lane of each mask
+// advances the cursor and clears the mask.  The buffer is then written out in
+// order, so consecutive lanes store consecutive addresses of each run
+// (scattered 4-byte stores straight to memory cost 1.1 ms of the 8192^2
+// level's 1.5).  Runs beyond the buffer are taken in further windows; a
+// single run larger than the buffer is stored directly.
+constexpr int kScatterLds = 65536;     // LDS bytes per block, at most
+#ifndef LIG_SCATTER_BUF
+#define LIG_SCATTER_BUF 2048
+#endif
+constexpr int kScatterBufMax = LIG_SCATTER_BUF;  // buffer entries, at most
+
+// Buffer entries of the scatter for a row of tiles_x tiles.
+int scatter_buf(int tiles_x, bool slots) {
+    const int fixed = tiles_x * 20 + 4;        // masks, cursors, segment starts, run starts
+    const int per = 4 + 2 + (slots ? 4 : 0);   // gid, tile index, slot
+    return std::min(kScatterBufMax, (kScatterLds - fixed) / per);
+}
+
+template <bool SLOTS>
+__global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
+    int P, int tiles_x, int m, int bufn, const int2 *__restrict__ rowbins,
+    const uint32_t *__restrict__ rowval, const uint2 *__restrict__ rect,
+    const int32_t *__restrict__ counts, const int32_t *__restrict__ scol,
+    const int32_t *__restrict__ te, const int32_t *__restrict__ rowoff,
+    const int32_t *__restrict__ cum, int32_t *__restrict__ gids_out,
+    int32_t *__restrict__ slots_out) {
+    extern __shared__ unsigned long long smem[];
+    unsigned long long *mask = smem;                       // tiles_x
+    int *cur = reinterpret_cast<int *>(smem + tiles_x);    // tiles_x
+    int *segst = cur + tiles_x;                            // tiles_x
+    int *lstart = segst + tiles_x;                         // tiles_x + 1
+    int *bg = lstart + tiles_x + 1;                        // bufn
+    int *bs = bg + bufn;                                   // bufn (SLOTS)
+    uint16_t *bt = reinterpret_cast<uint16_t *>(bs + (SLOTS ? bufn : 0));  // bufn
+    const int lane = threadIdx.x;
+    const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int r = u / P;
+    const int s = u - r * P;
+    int lo, hi;
+    row_segment(rowbins, r, s, P, lo, hi);
+    if (lo >= hi) return;  // wave-uniform
+    // run lengths and starts of the segment, lanes on consecutive tiles; then
+    // the runs numbered consecutively (lstart)
+    const size_t seg_base = ((size_t)r * P + s) * tiles_x;
+    const size_t row_tiles = (size_t)r * tiles_x;
+    const int roff = rowoff[r];
+    int carry = 0;
+    for (int t0 = 0; t0 < tiles_x; t0 += LIG_WAVE) {
+        const int t = t0 + lane;
+        int c = 0;
+        if (t < tiles_x) {
+            c = counts[seg_base + t];
+            segst[t] = roff + te[row_tiles + t] + scol[seg_base + t];
+            mask[t] = 0ull;
+            cur[t] = 0;
+        }
+        const int ex = wave_exclusive_sum(c);
+        if (t < tiles_x) lstart[t] = carry + ex;
+        carry += __shfl(ex + c, LIG_WAVE - 1, LIG_WAVE);
+    }
+    if (lane == 0) lstart[tiles_x] = carry;
+    wave_lds_sync();
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (LIG_WAVE - lane));
+    const unsigned long long bit = 1ull << lane;
+    for (int wa = 0; wa < tiles_x;) {
+        // window [wa, wb): the most tiles whose runs fit the buffer (one at least)
+        int a = wa + 1, b = tiles_x;
+        while (a < b) {
+            const int mid = (a + b + 1) >> 1;
+            if (lstart[mid] - lstart[wa] <= bufn) a = mid; else b = mid - 1;
+        }
+        const int wb = a;
+        const int base_q = lstart[wa];
+        const int nq = lstart[wb] - base_q;
+        const bool direct = nq > bufn;
+        if (nq > 0) {
+            for (int base = lo; base < hi; base += LIG_WAVE) {
+                const int j = base + lane;
+                int x0 = 0, y0 = 0, x1 = 0, y1 = 0, g = 0;
+                bool ph = false;
+                if (j < hi) {
+                    const uint32_t v = rowval[j];
+                    g = (int)(v & ~kPhantomBit);
+                    unpack_rect(rect[g], x0, y0, x1, y1, ph);
+                }
+                const int t0 = max(x0, wa), t1 = min(x1, wb);
+                for (int t = t0; t < t1; ++t) atomicOr(&mask[t], bit);
+                wave_lds_sync();
+                int slot0 = 0;
+                if (SLOTS && t1 > t0) {
+                    slot0 = g == 0 ? 0 : cum[g - 1];
+                    if (!ph) slot0 += (r - y0) * (x1 - x0) - x0;
+                }
+                const int gid = ph ? 0 : g;
+                for (int t = t0; t < t1; ++t) {
+                    const int rank = cur[t] + __popcll(mask[t] & lt);
+                    const int slot = ph ? slot0 : slot0 + t;
+                    if (direct) {
+                        const int pos = segst[t] + rank;
+                        if (pos < m) {
+                            gids_out[pos] = gid;
+                            if (SLOTS) slots_out[pos] = slot;
+                        }
+                    } else {
+                        const int q = lstart[t] - base_q + rank;
+                        bg[q] = gid;
+                        if (SLOTS) bs[q] = slot;
+                        bt[q] = (uint16_t)t;
+                    }
+                }
+                wave_lds_sync();
+                for (int t = t0; t < t1; ++t) {
+                    const unsigned long long mt = mask[t];
+                    if ((mt & lt) == 0ull) {  // the lowest lane covering t
+                        cur[t] += __popcll(mt);
+                        mask[t] = 0ull;
+                    }
+                }
+                wave_lds_sync();
+            }
+            if (!direct) {  // the window's runs in order: consecutive lanes, consecutive slots
+                for (int q = lane; q < nq; q += LIG_WAVE) {
+                    const int t = bt[q];
+                    const int pos = segst[t] + (q + base_q - lstart[t]);
+                    if (pos < m) {
+                        gids_out[pos] = bg[q];
+                        if (SLOTS) slots_out[pos] = bs[q];
+                    }
+                }
+                wave_lds_sync();  // the next window reuses the buffer
+            }
+        }
+        wa = wb;
+    }
+}
+
+// Segments per row: enough that a segment's output (its entries' tiles) fills
+// about three quarters of the scatter's buffer on an average row.
+int row_segments(int m, int tiles_x, int tiles_y) {
+    const double target = 0.75 * scatter_buf(tiles_x, false);
+    const double p = std::ceil((double)m / ((double)tiles_y * target));
+    return (int)std::min(4096.0, std::max(1.0, p));
+}
+
+// Workspace of the row path (0 when it does not apply): rectangles, row
+// counts and their scan, the row entries and their sorted copies, the row
+// sort's workspace, the row bins, the (row, segment, tile) counts and their
+// segment prefixes, tile totals and their row prefixes, row totals and their
+// scan.
+size_t rows_ws_bytes(int n, int m, int tiles_x, int tiles_y, int *P_out) {
+    if (n <= 0 || m <= 0 || tiles_x < 1 || tiles_y < 1 || tiles_x > kRowMaxTiles ||
+        tiles_y > kRowMaxRows)
+        return 0;
+    const int P = row_segments(m, tiles_x, tiles_y);
+    const size_t T = (size_t)tiles_x * tiles_y;
+    const size_t nctr = (size_t)P * T;
+    if (nctr > (size_t)INT32_MAX) return 0;
+    if (P_out) *P_out = P;
+    const size_t nn = (size_t)n, mm = (size_t)m, ty = (size_t)tiles_y;
+    return align_up(8 * nn) + align_up(4 * nn) + align_up(scan_ws_elems(nn) * 4) +
+           4 * align_up(4 * mm) + sort_ws_bytes<uint32_t>(mm, bits_for(tiles_y)) +
+           align_up(8 * ty) + 2 * align_up(4 * nctr) + 2 * align_up(4 * T) +
+           align_up(4 * (ty + 1)) + align_up(scan_ws_elems(ty + 1) * 4) + 256;
+}
+
+// The row path of bin_and_sort (workspace checked by the caller).
+int bin_rows(int n, int m, const float *xys, const float *extents, const int32_t *cum,
+             int tiles_x, int tiles_y, int block_width, int32_t *gaussian_ids_sorted,
+             int32_t *tile_bins, int32_t *emission_ids_sorted, char *ws, hipStream_t st) {
+    int P = 1;
+    rows_ws_bytes(n, m, tiles_x, tiles_y, &P);
+    const size_t nn = (size_t)n, mm = (size_t)m, T = (size_t)tiles_x * tiles_y;
+    const size_t nctr = (size_t)P * T, ty = (size_t)tiles_y;
+    char *p = ws;
+    auto take = [&p](size_t bytes) {
+        char *q = p;
+        p += align_up(bytes);
+        return q;
+    };
+    uint2 *rect = reinterpret_cast<uint2 *>(take(8 * nn));
+    int32_t *ecum = reinterpret_cast<int32_t *>(take(4 * nn));
+    int32_t *scan_ws = reinterpret_cast<int32_t *>(take(scan_ws_elems(nn) * 4));
+    uint32_t *rowkey = reinterpret_cast<uint32_t *>(take(4 * mm));
+    uint32_t *rowval = reinterpret_cast<uint32_t *>(take(4 * mm));
+    uint32_t *rowkey_s = reinterpret_cast<uint32_t *>(take(4 * mm));
+    uint32_t *rowval_s = reinterpret_cast<uint32_t *>(take(4 * mm));
+    char *sort_ws = take(sort_ws_bytes<uint32_t>(mm, bits_for(tiles_y)));
+    int32_t *rowbins = reinterpret_cast<int32_t *>(take(8 * ty));
+    int32_t *counts = reinterpret_cast<int32_t *>(take(4 * nctr));
+    int32_t *scol = reinterpret_cast<int32_t *>(take(4 * nctr));
+    int32_t *tt = reinterpret_cast<int32_t *>(take(4 * T));
+    int32_t *te = reinterpret_cast<int32_t *>(take(4 * T));
+    int32_t *rowoff = reinterpret_cast<int32_t *>(take(4 * (ty + 1)));
+    int32_t *row_ws = reinterpret_cast<int32_t *>(take(scan_ws_elems(ty + 1) * 4));
+
+    hipLaunchKernelGGL(rect_rows_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, n, xys,
+                       extents, cum, tiles_x, tiles_y, block_width, rect, ecum);
+    scan_i32(ecum, ecum, nn, true, scan_ws, st);
+    const int32_t *r_dev = ecum + (n - 1);  // row entries (at most m)
+    hipLaunchKernelGGL(emit_rows_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, n, m,
+                       (const int32_t *)ecum, (const uint2 *)rect, rowkey, rowval);
+    int rc = radix_sort<uint32_t>(rowkey, rowval, rowkey_s, rowval_s, mm, bits_for(tiles_y), 0u,
+                                  sort_ws, st, r_dev);
+    if (rc != LIG_OK) return rc;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * ty, 256)), dim3(256), 0, st, rowbins,
+                       2 * ty, 0);
+    hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(mm, 1024)), dim3(256), 0,
+                       st, m, r_dev, (const uint32_t *)rowkey_s, tiles_y, rowbins);
+    const unsigned segs = (unsigned)tiles_y * (unsigned)P;
+    hipLaunchKernelGGL(row_count_kernel, dim3(segs), dim3(LIG_WAVE), (tiles_x + 1) * 4, st, P,
+                       tiles_x, (const int2 *)rowbins, (const uint32_t *)rowval_s,
+                       (const uint2 *)rect, counts);
+    hipLaunchKernelGGL(row_prefix_kernel, dim3(tiles_y), dim3(kPrefixThreads), 0, st, P, tiles_x,
+                       tiles_y, (const int32_t *)counts, scol, tt, te, rowoff);
+    scan_i32(rowoff, rowoff, ty + 1, false, row_ws, st);
+    hipLaunchKernelGGL(row_bins_kernel, dim3(ceil_div(T, 256)), dim3(256), 0, st, (int)T,
+                       tiles_x, (const int32_t *)rowoff, (const int32_t *)tt,
+                       (const int32_t *)te, tile_bins);
+    const bool slots = emission_ids_sorted != nullptr;
+    const int bufn = scatter_buf(tiles_x, slots);
+    const size_t lds = (size_t)tiles_x * 20 + 4 + (size_t)bufn * (6 + (slots ? 4 : 0));
+    if (slots)
+        hipLaunchKernelGGL(row_scatter_kernel<true>, dim3(segs), dim3(LIG_WAVE), lds, st, P,
+                           tiles_x, m, bufn, (const int2 *)rowbins,
+                           (const uint32_t *)rowval_s, (const uint2 *)rect,
+                           (const int32_t *)counts, (const int32_t *)scol, (const int32_t *)te,
+                           (const int32_t *)rowoff, cum, gaussian_ids_sorted,
+                           emission_ids_sorted);
+    else
+        hipLaunchKernelGGL(row_scatter_kernel<false>, dim3(segs), dim3(LIG_WAVE), lds, st, P,
+                           tiles_x, m, bufn, (const int2 *)rowbins,
+                           (const uint32_t *)rowval_s, (const uint2 *)rect,
+                           (const int32_t *)counts, (const int32_t *)scol, (const int32_t *)te,
+                           (const int32_t *)rowoff, cum, gaussian_ids_sorted,
+                           emission_ids_sorted);
+    return launch_status();
+}
+
 }  // namespace
 }  // namespace lig
 
@@ -748,6 +1232,14 @@ extern "C" size_t lig_bin_workspace_size(int num_intersects, int num_tiles) {
     return 4 * align_up(m * 4) + sort_ws_bytes<uint32_t>(m, bits_for(num_tiles)) + 256;
 }
 
+extern "C" size_t lig_bin_rows_workspace_size(int n, int num_intersects, int img_height,
+                                              int img_width, int block_width) {
+    if (img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16) return 0;
+    const int tx = (img_width + block_width - 1) / block_width;
+    const int ty = (img_height + block_width - 1) / block_width;
+    return rows_ws_bytes(n, num_intersects, tx, ty, nullptr);
+}
+
 namespace lig {
 namespace {
 
@@ -760,12 +1252,19 @@ int bin_and_sort(int n, int num_intersects, const float *xys, const float *exten
     const int tx = (img_width + block_width - 1) / block_width;
     const int ty = (img_height + block_width - 1) / block_width;
     const int nt = tx * ty;
-    if (workspace_bytes < lig_bin_workspace_size(num_intersects, nt)) return LIG_ERR_WORKSPACE;
+    const size_t rows_bytes = rows_ws_bytes(n, num_intersects, tx, ty, nullptr);
+    const bool rows_path = rows_bytes > 0 && workspace_bytes >= rows_bytes;
+    if (!rows_path && workspace_bytes < lig_bin_workspace_size(num_intersects, nt))
+        return LIG_ERR_WORKSPACE;
     hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * (size_t)nt, 256)), dim3(256), 0, st,
                        tile_bins, 2 * (size_t)nt, 0);
     if (num_intersects == 0 || n == 0) return launch_status();
     if (!xys || !extents || !cum_tiles_hit || !gaussian_ids_sorted || !workspace)
         return LIG_ERR_INVALID_ARGUMENT;
+    if (rows_path)
+        return bin_rows(n, num_intersects, xys, extents, cum_tiles_hit, tx, ty, block_width,
+                        gaussian_ids_sorted, tile_bins, emission_ids_sorted,
+                        reinterpret_cast<char *>(workspace), st);
     const size_t m = (size_t)num_intersects;
     char *p = reinterpret_cast<char *>(workspace);
     auto take = [&p, m]() {

@@ -303,9 +303,27 @@ def sort_isect_pairs(isect_ids, gaussian_ids):
     return ks, vs
 
 
+def bin_workspace_bytes(num_points, num_intersects, img_height, img_width, block_width,
+                        path="auto"):
+    """Workspace of lig_bin_and_sort_gaussians: the row path's when it applies
+    (path "auto"), the radix path's (lig_bin_workspace_size) otherwise or when
+    path is "radix" -- the library takes the row path whenever the workspace
+    holds it, so "radix" only selects the radix path where its workspace is the
+    smaller of the two (both give the same outputs)."""
+    tx = (img_width + block_width - 1) // block_width
+    ty = (img_height + block_width - 1) // block_width
+    radix = L.size_query("lig_bin_workspace_size", int(num_intersects), tx * ty)
+    if path == "radix":
+        return radix
+    rows = L.size_query("lig_bin_rows_workspace_size", int(num_points), int(num_intersects),
+                        int(img_height), int(img_width), int(block_width))
+    return max(radix, rows)
+
+
 def bin_and_sort_fast(num_points, num_intersects, xys, extents, cum_tiles_hit, img_height,
-                      img_width, block_width, with_emission_ids=False):
-    """Fused emit + tile-key radix sort + bins for the all-zero-depth path.
+                      img_width, block_width, with_emission_ids=False, path="auto"):
+    """Fused binning for the all-zero-depth path (emission and sort by tile
+    rows, or the tile-key radix sort; the same outputs).
     -> (gaussian_ids_sorted [I] int32, tile_bins [T,2] int32), plus the emission
     slot of each sorted intersection [I] int32 when with_emission_ids."""
     _check("xys", xys)
@@ -316,7 +334,8 @@ def bin_and_sort_fast(num_points, num_intersects, xys, extents, cum_tiles_hit, i
     gids = _empty((num_intersects,), xys, _i32)
     bins = _empty((tx * ty, 2), xys, _i32)
     eids = _empty((num_intersects,), xys, _i32) if with_emission_ids else None
-    ws = _workspace(L.size_query("lig_bin_workspace_size", int(num_intersects), tx * ty), xys)
+    ws = _workspace(bin_workspace_bytes(num_points, num_intersects, img_height, img_width,
+                                        block_width, path), xys)
     with torch.cuda.device(xys.device):
         L.call("lig_bin_and_sort_gaussians", int(num_points), int(num_intersects), L.ptr(xys),
                L.ptr(extents), L.ptr(cum_tiles_hit), int(img_height), int(img_width),

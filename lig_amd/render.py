@@ -142,7 +142,7 @@ def window_bytes(p: LevelParams, q: Projected, ga: bool) -> int:
     """HBM the rest of a window's render allocates after projection: binning
     workspace and outputs, packed records, images."""
     tiles = ((q.h + 15) // 16) * ((q.w + 15) // 16)
-    ws = L.size_query("lig_bin_workspace_size", max(q.num_intersects, 0), tiles)
+    ws = _C.bin_workspace_bytes(p.means.shape[0], max(q.num_intersects, 0), q.h, q.w, 16)
     pix = q.h * q.w
     per_pix = (12 + 4 + 36 + 4) if ga else 12
     return int(ws + 4 * q.num_intersects + 8 * tiles + 64 * p.means.shape[0] + per_pix * pix)

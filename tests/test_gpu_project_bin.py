@@ -118,6 +118,90 @@ def test_binning_bit_exact(device, oracle_lib, n, H, W, bw, op, chol, seed):
         np.testing.assert_array_equal(_np(a), b, err_msg=name)
 
 
+@pytest.mark.parametrize("n,H,W,bw,op,chol,seed", CASES)
+def test_binning_row_path_matches_radix_path(device, oracle_lib, n, H, W, bw, op, chol, seed):
+    """The row path of lig_bin_and_sort_gaussians (one entry per (tile row,
+    Gaussian), sorted by row, expanded into tiles with per-tile counts and
+    lane-mask ranks) and the radix path (32-bit tile keys, LSD sort) give the
+    same gids, bins and emission slots, phantom slots included."""
+    import lig_amd._lib as L
+    import lig_amd.gsplat2d as g
+    from lig_amd.gsplat2d import cuda as C
+    s = random_scene(n, H, W, seed, opacities=op, cholesky=chol, margin=20.0)
+    if op:
+        s["opac"][::7] = 0.001
+    print(f"row vs radix binning n={n} H={H} W={W} bw={bw} seed={seed}", flush=True)
+    xys, ext, con, nt = oracle_lib.project_gaussians_forward(s["par"], s["means"], H, W, bw,
+                                                             s["opac"], cholesky=chol)
+    num, cum = g.compute_cumulative_intersects(_t(nt, device, torch.int32))
+    tiles = ((W + bw - 1) // bw) * ((H + bw - 1) // bw)
+    rows_b = L.size_query("lig_bin_rows_workspace_size", n, num, H, W, bw)
+    radix_b = L.size_query("lig_bin_workspace_size", num, tiles)
+    assert rows_b > 0
+    m, _, _, _, _, gs, bins = oracle_lib.bin_and_sort(xys, ext, nt, H, W, bw)
+    for eids in (False, True):
+        a = C.bin_and_sort_fast(n, num, _t(xys, device), _t(ext, device), cum, H, W, bw,
+                                with_emission_ids=eids)
+        np.testing.assert_array_equal(_np(a[0]), gs)
+        np.testing.assert_array_equal(_np(a[1]), bins)
+        if radix_b < rows_b:
+            b = C.bin_and_sort_fast(n, num, _t(xys, device), _t(ext, device), cum, H, W, bw,
+                                    with_emission_ids=eids, path="radix")
+            for x, y in zip(a, b):
+                assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("kind", ["windows", "direct"])
+def test_binning_row_path_buffer_overflow(device, oracle_lib, kind):
+    """The row path's scatter stages a segment's output in an LDS buffer of
+    2048 entries: a segment whose output exceeds it is taken in several
+    windows ("windows": 3000 Gaussians each covering two tiles of one tile
+    row), and a single tile run larger than the buffer is stored directly
+    ("direct": 5000 Gaussians on one tile).  Same gids, bins and emission
+    slots as the oracle and the radix path."""
+    import lig_amd._lib as L
+    import lig_amd.gsplat2d as g
+    from lig_amd.gsplat2d import cuda as C
+    H, W, bw = 64, 1024, 16
+    rng = np.random.default_rng(17)
+    if kind == "windows":
+        n = 3000
+        means = np.stack([rng.uniform(8, W - 8, n), rng.uniform(20, 28, n)], 1)
+        chol = np.stack([rng.uniform(2.0, 2.4, n), np.zeros(n), rng.uniform(0.5, 0.7, n)], 1)
+    else:
+        n = 5000
+        means = np.stack([rng.uniform(36, 44, n), rng.uniform(36, 44, n)], 1)
+        chol = np.stack([rng.uniform(0.4, 0.6, n), np.zeros(n), rng.uniform(0.4, 0.6, n)], 1)
+    means, chol = means.astype(np.float32), chol.astype(np.float32)
+    xys, ext, con, nt = oracle_lib.project_gaussians_forward(chol, means, H, W, bw, None,
+                                                             cholesky=True)
+    m, _, isect, _, _, gs, bins = oracle_lib.bin_and_sort(xys, ext, nt, H, W, bw)
+    num, cum = g.compute_cumulative_intersects(_t(nt, device, torch.int32))
+    assert num == m
+    counts = np.diff(bins, axis=1).ravel()
+    print(f"row path {kind}: {m} intersections, largest bin {counts.max()}", flush=True)
+    if kind == "direct":
+        assert counts.max() > 2048
+    else:
+        assert m > 2 * 2048 and counts.max() < 2048
+    tiles = (W // bw) * (H // bw)
+    rows_b = L.size_query("lig_bin_rows_workspace_size", n, m, H, W, bw)
+    radix_b = L.size_query("lig_bin_workspace_size", m, tiles)
+    assert rows_b > 0
+    for eids in (False, True):
+        a = C.bin_and_sort_fast(n, m, _t(xys, device), _t(ext, device), cum, H, W, bw,
+                                with_emission_ids=eids)
+        np.testing.assert_array_equal(_np(a[0]), gs)
+        np.testing.assert_array_equal(_np(a[1]), bins)
+        if eids:
+            np.testing.assert_array_equal(_np(a[2]), np.argsort(isect, kind="stable"))
+        if radix_b < rows_b:
+            b = C.bin_and_sort_fast(n, m, _t(xys, device), _t(ext, device), cum, H, W, bw,
+                                    with_emission_ids=eids, path="radix")
+            for x, y in zip(a, b):
+                assert torch.equal(x, y)
+
+
 def test_binning_wide_rectangles(device, oracle_lib):
     """Tile rectangles from one tile up to the whole 256 x 256 grid, so the
     emission's row/column decode of a slot sees offsets up to 65535 and
