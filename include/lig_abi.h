@@ -401,6 +401,70 @@ int lig_fit_backward_records_l2_batched_persistent(
     int num_images, int num_tiles, int num_intersects, void *workspace, size_t workspace_bytes,
     void *stream);
 
+/* ---- batched crop rendering (serving) -------------------------------------- */
+
+/* One view of a batched crop render (64 bytes): n Gaussians (par [n,3]
+ * Cholesky factors when cholesky != 0, else covariances; means2d [n,2]; rgbs
+ * [n,3]) seen through lig_project_gaussians_view's transform with this origin
+ * and scale on an img_height x img_width canvas, written to slots
+ * [gauss_off, gauss_off + n) of the batch's concatenated arrays. */
+typedef struct LigView {
+    const float *par;
+    const float *means2d;
+    const float *rgbs;
+    int32_t n, cholesky;
+    float origin_x, origin_y, scale_x, scale_y;
+    int32_t img_height, img_width;
+    int32_t gauss_off, reserved;
+} LigView;
+
+/* One crop of the batched composition (128 bytes), train.py:150-154 /
+ * 187-196 for a window: out[h,w,3] = clamp(clamp(L2) * span + up - 0.5 + lo,
+ * 0, 1) evaluated left to right in FP32 without contraction, where L2 is the
+ * window at (off_x, off_y) of the level-2 view render `level2` (row pitch
+ * view_w pixels; NULL: a view with no intersections, whose render is ones,
+ * rasterize.py:80-90), span = (float)(hi - lo), lo = (float)lo, and `up` the
+ * level-1 estimate: mode 0, lig_gradient_aware_upscale_forward of (f, fx,
+ * fy, fxy) [src_h,src_w,3] over roi to h x w, clamped to [0, 1]; mode 1,
+ * lig_bilinear_resize of f to full_h x full_w, window at (win_x, win_y).
+ * block_off: the crop's first 256-pixel block in the launch (crops sorted by
+ * it, each owning ceil(w*h/256) blocks). */
+typedef struct LigCrop {
+    const float *level2;
+    const float *f, *fx, *fy, *fxy;
+    float *out;
+    int32_t view_w, off_x, off_y, w, h, mode, src_h, src_w;
+    float roi_x1, roi_y1, roi_x2, roi_y2;
+    int32_t full_h, full_w, win_x, win_y;
+    float span, lo;
+    int32_t block_off, reserved;
+} LigCrop;
+
+/* No reference counterpart: the crops a server has queued rendered as one
+ * batch, one launch per stage.  Views may come from different
+ * representations.  lig_project_views_batched: lig_project_gaussians_view for
+ * every view of the device array `views` (sorted by gauss_off, ranges back to
+ * back from 0, n_total = their sum; no opacities), bit for bit, and, when
+ * colors is not NULL, each slot's rgbs copied to colors [n_total,3]; a view
+ * with a non-positive size or scale emits nothing (validate on the host).
+ * Binning: lig_bin_batched with the views as the batch's images.
+ * lig_render_batched: the image-only sum render of every image of the batch
+ * table (lig_fit_render_l2_batched's image, without target or loss), the
+ * same bits as lig_rasterize_forward's image of each view alone; workspace
+ * lig_fit_batched_workspace_size(num_tiles).  lig_compose_crops: the crops
+ * of the device array `crops` in one launch of total_blocks 256-pixel
+ * blocks; the same bits as the per-crop chain (upscale or resize, clamps and
+ * train.py's composition as FP32 tensor operations). */
+int lig_project_views_batched(int num_views, const LigView *views, int n_total,
+                              int block_width, float *xys, float *extents, float *conics,
+                              int32_t *num_tiles_hit, float *colors, void *stream);
+int lig_render_batched(int n, const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
+                       const float *xys, const float *extents, const int32_t *cum_tiles_hit,
+                       const float *conics, const float *colors, void *packed,
+                       const void *batch_table, int num_images, int num_tiles, float *out_img,
+                       void *workspace, size_t workspace_bytes, void *stream);
+int lig_compose_crops(int num_crops, const LigCrop *crops, int total_blocks, void *stream);
+
 /* ---- captured fit step (no host read of the intersection count) ------------ */
 
 /* No reference counterpart (the reference reads the count on the host every

@@ -41,14 +41,13 @@ __device__ __forceinline__ bool cov_bounds(float ca, float cb, float cc, float o
 // One Gaussian of the forward projection; writes every output slot (zeros
 // where the reference leaves its zero fill).
 __device__ __forceinline__ void project_one(int i, float ca, float cb, float cc, float mx,
-                                            float my, const float *__restrict__ opacities,
-                                            int tiles_x, int tiles_y, int block_width,
+                                            float my, float o, int tiles_x, int tiles_y,
+                                            int block_width,
                                             float *__restrict__ xys,
                                             float *__restrict__ extents,
                                             float *__restrict__ conics,
                                             int32_t *__restrict__ num_tiles_hit) {
     float ka = 0.0f, kb = 0.0f, kc = 0.0f, ex = 0.0f, ey = 0.0f;
-    const float o = opacities ? opacities[i] : 1.0f;
     const bool ok = cov_bounds(ca, cb, cc, o, ka, kb, kc, ex, ey);
     int area = 0;
     if (ok) {
@@ -75,8 +74,8 @@ __global__ __launch_bounds__(kThreads) void project_cov_kernel(
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     project_one(i, cov2d[3 * i], cov2d[3 * i + 1], cov2d[3 * i + 2], means2d[2 * i],
-                means2d[2 * i + 1], opacities, tiles_x, tiles_y, block_width, xys, extents,
-                conics, num_tiles_hit);
+                means2d[2 * i + 1], opacities ? opacities[i] : 1.0f, tiles_x, tiles_y,
+                block_width, xys, extents, conics, num_tiles_hit);
 }
 
 __global__ __launch_bounds__(kThreads) void project_chol_kernel(
@@ -91,8 +90,9 @@ __global__ __launch_bounds__(kThreads) void project_chol_kernel(
     const float ca = l11 * l11;
     const float cb = l11 * l21;
     const float cc = l21 * l21 + l22 * l22;
-    project_one(i, ca, cb, cc, means2d[2 * i], means2d[2 * i + 1], opacities, tiles_x, tiles_y,
-                block_width, xys, extents, conics, num_tiles_hit);
+    project_one(i, ca, cb, cc, means2d[2 * i], means2d[2 * i + 1],
+                opacities ? opacities[i] : 1.0f, tiles_x, tiles_y, block_width, xys, extents,
+                conics, num_tiles_hit);
 }
 
 // Batched fit: Gaussian i belongs to the image of `tab` whose Gaussian range
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void project_batched_chol_kernel(
     const float ca = l11 * l11;
     const float cb = l11 * l21;
     const float cc = l21 * l21 + l22 * l22;
-    project_one(i, ca, cb, cc, means2d[2 * i], means2d[2 * i + 1], nullptr, im.tiles_x,
+    project_one(i, ca, cb, cc, means2d[2 * i], means2d[2 * i + 1], 1.0f, im.tiles_x,
                 im.tiles_y, 16, xys, extents, conics, num_tiles_hit);
 }
 
@@ -120,29 +120,82 @@ __global__ __launch_bounds__(kThreads) void project_batched_chol_kernel(
 //   cov' = S cov S  (a sx sx, b sx sy, c sy sy),
 // the transform of the reference's scale_to (minimal_trainer_upscale.py:139-159)
 // with an origin, in FP32 without contraction; then the projection above.
+// Input j of (par, means2d) is written to output slot i.
+__device__ __forceinline__ void project_view_one(int i, int j, int cholesky,
+                                                 const float *__restrict__ par,
+                                                 const float *__restrict__ means2d, float o,
+                                                 float ox, float oy, float sx, float sy,
+                                                 int tiles_x, int tiles_y, int block_width,
+                                                 float *__restrict__ xys,
+                                                 float *__restrict__ extents,
+                                                 float *__restrict__ conics,
+                                                 int32_t *__restrict__ num_tiles_hit) {
+#pragma clang fp contract(off)
+    const float mx = (means2d[2 * j] - ox) * sx;
+    const float my = (means2d[2 * j + 1] - oy) * sy;
+    float ca, cb, cc;
+    if (cholesky) {
+        const float l11 = par[3 * j] * sx, l21 = par[3 * j + 1] * sy, l22 = par[3 * j + 2] * sy;
+        ca = l11 * l11;
+        cb = l11 * l21;
+        cc = l21 * l21 + l22 * l22;
+    } else {
+        ca = (par[3 * j] * sx) * sx;
+        cb = (par[3 * j + 1] * sx) * sy;
+        cc = (par[3 * j + 2] * sy) * sy;
+    }
+    project_one(i, ca, cb, cc, mx, my, o, tiles_x, tiles_y, block_width, xys, extents, conics,
+                num_tiles_hit);
+}
+
 __global__ __launch_bounds__(kThreads) void project_view_kernel(
     int n, int cholesky, const float *__restrict__ par, const float *__restrict__ means2d,
     const float *__restrict__ opacities, float ox, float oy, float sx, float sy, int tiles_x,
     int tiles_y, int block_width, float *__restrict__ xys, float *__restrict__ extents,
     float *__restrict__ conics, int32_t *__restrict__ num_tiles_hit) {
-#pragma clang fp contract(off)
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    const float mx = (means2d[2 * i] - ox) * sx;
-    const float my = (means2d[2 * i + 1] - oy) * sy;
-    float ca, cb, cc;
-    if (cholesky) {
-        const float l11 = par[3 * i] * sx, l21 = par[3 * i + 1] * sy, l22 = par[3 * i + 2] * sy;
-        ca = l11 * l11;
-        cb = l11 * l21;
-        cc = l21 * l21 + l22 * l22;
-    } else {
-        ca = (par[3 * i] * sx) * sx;
-        cb = (par[3 * i + 1] * sx) * sy;
-        cc = (par[3 * i + 2] * sy) * sy;
+    project_view_one(i, i, cholesky, par, means2d, opacities ? opacities[i] : 1.0f, ox, oy, sx,
+                     sy, tiles_x, tiles_y, block_width, xys, extents, conics, num_tiles_hit);
+}
+
+// Several views in one launch: output slot i belongs to the view whose
+// [gauss_off, gauss_off + n) range holds it (views sorted by gauss_off,
+// ranges back to back from 0) and is that view's Gaussian i - gauss_off.
+// A view with a non-positive size or scale emits nothing.
+__global__ __launch_bounds__(kThreads) void project_views_kernel(
+    int num_views, const LigView *__restrict__ views, int n_total, int block_width,
+    float *__restrict__ xys, float *__restrict__ extents, float *__restrict__ conics,
+    int32_t *__restrict__ num_tiles_hit, float *__restrict__ colors) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n_total) return;
+    int lo = 0, hi = num_views - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (views[mid].gauss_off <= i) lo = mid; else hi = mid - 1;
     }
-    project_one(i, ca, cb, cc, mx, my, opacities, tiles_x, tiles_y, block_width, xys, extents,
-                conics, num_tiles_hit);
+    const LigView &v = views[lo];
+    const int j = i - v.gauss_off;
+    const bool ok = j < v.n && v.img_height > 0 && v.img_width > 0 && v.scale_x > 0.0f &&
+                    v.scale_y > 0.0f;
+    if (ok) {
+        const int tx = (v.img_width + block_width - 1) / block_width;
+        const int ty = (v.img_height + block_width - 1) / block_width;
+        project_view_one(i, j, v.cholesky, v.par, v.means2d, 1.0f, v.origin_x, v.origin_y,
+                         v.scale_x, v.scale_y, tx, ty, block_width, xys, extents, conics,
+                         num_tiles_hit);
+    } else {
+        xys[2 * i] = xys[2 * i + 1] = 0.0f;
+        extents[2 * i] = extents[2 * i + 1] = 0.0f;
+        conics[3 * i] = conics[3 * i + 1] = conics[3 * i + 2] = 0.0f;
+        num_tiles_hit[i] = 0;
+    }
+    if (colors) {
+        const float *c = v.rgbs + 3 * (ok ? j : 0);
+        colors[3 * i] = ok ? c[0] : 0.0f;
+        colors[3 * i + 1] = ok ? c[1] : 0.0f;
+        colors[3 * i + 2] = ok ? c[2] : 0.0f;
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void cov2d_bounds_kernel(
@@ -293,6 +346,20 @@ extern "C" int lig_project_gaussians_view(int n, const float *par, const float *
                        as_stream(stream), n, cholesky, par, means2d, opacities, origin_x,
                        origin_y, scale_x, scale_y, tx, ty, block_width, xys, extents, conics,
                        num_tiles_hit);
+    return launch_status();
+}
+
+extern "C" int lig_project_views_batched(int num_views, const LigView *views, int n_total,
+                                         int block_width, float *xys, float *extents,
+                                         float *conics, int32_t *num_tiles_hit, float *colors,
+                                         void *stream) {
+    if (num_views < 1 || n_total < 0 || block_width < 1 || block_width > 16)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (n_total == 0) return LIG_OK;
+    if (!views || !xys || !extents || !conics || !num_tiles_hit) return LIG_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(project_views_kernel, dim3(ceil_div(n_total, kThreads)), dim3(kThreads),
+                       0, as_stream(stream), num_views, views, n_total, block_width, xys,
+                       extents, conics, num_tiles_hit, colors);
     return launch_status();
 }
 

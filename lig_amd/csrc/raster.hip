@@ -2082,6 +2082,39 @@ extern "C" int lig_fit_render_l2_batched_persistent(
                               workspace_bytes, stream, true);
 }
 
+// Image-only render of a batch of views (crop serving): the fit-mode forward
+// with no target and no loss partials, launched over the batch's tiles in the
+// heaviest-first order when the launch is short or uneven.  Record addressing
+// is packed as for the fit (extents and cum given) but not used.
+extern "C" int lig_render_batched(int n, const int32_t *gaussian_ids_sorted,
+                                  const int32_t *tile_bins, const float *xys,
+                                  const float *extents, const int32_t *cum_tiles_hit,
+                                  const float *conics, const float *colors, void *packed,
+                                  const void *batch_table, int num_images, int num_tiles,
+                                  float *out_img, void *workspace, size_t workspace_bytes,
+                                  void *stream) {
+    if (n <= 0 || num_images < 1 || num_tiles < 1 || !gaussian_ids_sorted || !tile_bins ||
+        !xys || !extents || !cum_tiles_hit || !conics || !colors || !packed || !batch_table ||
+        !out_img)
+        return LIG_ERR_INVALID_ARGUMENT;
+    if (!workspace || workspace_bytes < lig_fit_batched_workspace_size(num_tiles))
+        return LIG_ERR_WORKSPACE;
+    hipStream_t st = as_stream(stream);
+    const BatchImage *tab = reinterpret_cast<const BatchImage *>(batch_table);
+    GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
+    hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
+                       colors, nullptr, extents, cum_tiles_hit, 0, 0, tab, num_images, pk);
+    char *after_partials = reinterpret_cast<char *>(workspace) +
+                           (((size_t)num_tiles * sizeof(float) + 255) & ~size_t(255));
+    const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
+    const int32_t *order = launch_tile_order(num_tiles, bins, num_tiles, fit_forward_resident(256),
+                                             after_partials + kQueueInts * sizeof(int), st);
+    launch_fit_forward(256, dim3((unsigned)num_tiles), st, 1, 1, 1, gaussian_ids_sorted, bins, pk,
+                       out_img, nullptr, nullptr, tab, num_images, cum_tiles_hit,
+                       TileQueue{nullptr, num_tiles}, order);
+    return launch_status();
+}
+
 extern "C" int lig_set_backward_shares(int shares) {
     if (shares < 0 || shares > kMaxBwdSplit) return LIG_ERR_INVALID_ARGUMENT;
     return g_bwd_shares.exchange(shares);

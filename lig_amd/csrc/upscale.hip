@@ -43,11 +43,45 @@ __device__ __forceinline__ void axis_map(const Axis &ax, int d, int &c0, int &c1
 
 // Hermite basis (C_inv^T [1, t, t^2, t^3]).
 __device__ __forceinline__ void hermite(float t, float h[4]) {
+#pragma clang fp contract(off)
     const float t2 = t * t, t3 = t2 * t;
     h[0] = 1.0f - 3.0f * t2 + 2.0f * t3;
     h[1] = 3.0f * t2 - 2.0f * t3;
     h[2] = t - 2.0f * t2 + t3;
     h[3] = t3 - t2;
+}
+
+// Destination pixel (x, y) of the Hermite patch upscale (upscaler.cu:181-246).
+// This and bilinear_pixel evaluate without FP contraction, so every kernel
+// that inlines them gets the same bits (contraction choices depend on the
+// surrounding code).
+__device__ __forceinline__ void hermite_pixel(const Axis &ax, const Axis &ay, int x, int y,
+                                              const float *__restrict__ f,
+                                              const float *__restrict__ fx,
+                                              const float *__restrict__ fy,
+                                              const float *__restrict__ fxy, float v[3]) {
+#pragma clang fp contract(off)
+    int x0, x1, y0, y1;
+    float tx, ty, hx[4], hy[4];
+    axis_map(ax, x, x0, x1, tx);
+    axis_map(ay, y, y0, y1, ty);
+    hermite(tx, hx);
+    hermite(ty, hy);
+    const int w = ax.src;
+    const size_t i00 = 3 * ((size_t)y0 * w + x0), i01 = 3 * ((size_t)y0 * w + x1);
+    const size_t i10 = 3 * ((size_t)y1 * w + x0), i11 = 3 * ((size_t)y1 * w + x1);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const float c0 = f[i00 + ch] * hy[0] + f[i10 + ch] * hy[1] + fy[i00 + ch] * hy[2] +
+                         fy[i10 + ch] * hy[3];
+        const float c1 = f[i01 + ch] * hy[0] + f[i11 + ch] * hy[1] + fy[i01 + ch] * hy[2] +
+                         fy[i11 + ch] * hy[3];
+        const float c2 = fx[i00 + ch] * hy[0] + fx[i10 + ch] * hy[1] + fxy[i00 + ch] * hy[2] +
+                         fxy[i10 + ch] * hy[3];
+        const float c3 = fx[i01 + ch] * hy[0] + fx[i11 + ch] * hy[1] + fxy[i01 + ch] * hy[2] +
+                         fxy[i11 + ch] * hy[3];
+        v[ch] = hx[0] * c0 + hx[1] * c1 + hx[2] * c2 + hx[3] * c3;
+    }
 }
 
 __global__ __launch_bounds__(256) void upscale_fwd_kernel(Axis ax, Axis ay,
@@ -59,28 +93,11 @@ __global__ __launch_bounds__(256) void upscale_fwd_kernel(Axis ax, Axis ay,
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= ax.dst || y >= ay.dst) return;
-    int x0, x1, y0, y1;
-    float tx, ty, hx[4], hy[4];
-    axis_map(ax, x, x0, x1, tx);
-    axis_map(ay, y, y0, y1, ty);
-    hermite(tx, hx);
-    hermite(ty, hy);
-    const int w = ax.src;
-    const size_t i00 = 3 * ((size_t)y0 * w + x0), i01 = 3 * ((size_t)y0 * w + x1);
-    const size_t i10 = 3 * ((size_t)y1 * w + x0), i11 = 3 * ((size_t)y1 * w + x1);
+    float v[3];
+    hermite_pixel(ax, ay, x, y, f, fx, fy, fxy, v);
     const size_t o = 3 * ((size_t)y * ax.dst + x);
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        const float c0 = f[i00 + ch] * hy[0] + f[i10 + ch] * hy[1] + fy[i00 + ch] * hy[2] +
-                         fy[i10 + ch] * hy[3];
-        const float c1 = f[i01 + ch] * hy[0] + f[i11 + ch] * hy[1] + fy[i01 + ch] * hy[2] +
-                         fy[i11 + ch] * hy[3];
-        const float c2 = fx[i00 + ch] * hy[0] + fx[i10 + ch] * hy[1] + fxy[i00 + ch] * hy[2] +
-                         fxy[i10 + ch] * hy[3];
-        const float c3 = fx[i01 + ch] * hy[0] + fx[i11 + ch] * hy[1] + fxy[i01 + ch] * hy[2] +
-                         fxy[i11 + ch] * hy[3];
-        out[o + ch] = hx[0] * c0 + hx[1] * c1 + hx[2] * c2 + hx[3] * c3;
-    }
+    for (int ch = 0; ch < 3; ++ch) out[o + ch] = v[ch];
 }
 
 // For each source index s along an axis, the destination range [lo, hi] whose
@@ -202,6 +219,26 @@ bool roi_ok(int src_h, int src_w, int dst_h, int dst_w, float x1, float y1, floa
 // full, between floor and floor + 1 (clamped to the last index).  The kernel
 // writes the out_h x out_w window at (off_y, off_x) of that output, so a crop
 // holds exactly the full output's values at its pixels.
+// Pixel (gx, gy) of the full_h x full_w bilinear resize of src (scale =
+// src / full per axis).
+__device__ __forceinline__ void bilinear_pixel(int src_h, int src_w, float scale_y,
+                                               float scale_x, int gy, int gx,
+                                               const float *__restrict__ src, float v[3]) {
+#pragma clang fp contract(off)
+    const float fy = fmaxf(scale_y * ((float)gy + 0.5f) - 0.5f, 0.0f);
+    const float fx = fmaxf(scale_x * ((float)gx + 0.5f) - 0.5f, 0.0f);
+    const int y0 = min((int)fy, src_h - 1), x0 = min((int)fx, src_w - 1);
+    const int y1 = y0 + (y0 < src_h - 1 ? 1 : 0), x1 = x0 + (x0 < src_w - 1 ? 1 : 0);
+    const float ly = fy - (float)y0, lx = fx - (float)x0;
+    const float hy = 1.0f - ly, hx = 1.0f - lx;
+    const size_t i00 = 3 * ((size_t)y0 * src_w + x0), i01 = 3 * ((size_t)y0 * src_w + x1);
+    const size_t i10 = 3 * ((size_t)y1 * src_w + x0), i11 = 3 * ((size_t)y1 * src_w + x1);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+        v[ch] = hy * (hx * src[i00 + ch] + lx * src[i01 + ch]) +
+                ly * (hx * src[i10 + ch] + lx * src[i11 + ch]);
+}
+
 __global__ __launch_bounds__(256) void bilinear_kernel(int src_h, int src_w, float scale_y,
                                                        float scale_x, int off_y, int off_x,
                                                        int out_h, int out_w,
@@ -210,19 +247,53 @@ __global__ __launch_bounds__(256) void bilinear_kernel(int src_h, int src_w, flo
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= out_w || y >= out_h) return;
-    const float fy = fmaxf(scale_y * ((float)(y + off_y) + 0.5f) - 0.5f, 0.0f);
-    const float fx = fmaxf(scale_x * ((float)(x + off_x) + 0.5f) - 0.5f, 0.0f);
-    const int y0 = min((int)fy, src_h - 1), x0 = min((int)fx, src_w - 1);
-    const int y1 = y0 + (y0 < src_h - 1 ? 1 : 0), x1 = x0 + (x0 < src_w - 1 ? 1 : 0);
-    const float ly = fy - (float)y0, lx = fx - (float)x0;
-    const float hy = 1.0f - ly, hx = 1.0f - lx;
-    const size_t i00 = 3 * ((size_t)y0 * src_w + x0), i01 = 3 * ((size_t)y0 * src_w + x1);
-    const size_t i10 = 3 * ((size_t)y1 * src_w + x0), i11 = 3 * ((size_t)y1 * src_w + x1);
+    float v[3];
+    bilinear_pixel(src_h, src_w, scale_y, scale_x, y + off_y, x + off_x, src, v);
     const size_t o = 3 * ((size_t)y * out_w + x);
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch)
-        out[o + ch] = hy * (hx * src[i00 + ch] + lx * src[i01 + ch]) +
-                      ly * (hx * src[i10 + ch] + lx * src[i11 + ch]);
+    for (int ch = 0; ch < 3; ++ch) out[o + ch] = v[ch];
+}
+
+// clamp(v, 0, 1) as torch.clamp computes it (NaN passes through).
+__device__ __forceinline__ float clamp01(float v) {
+    return v != v ? v : fminf(fmaxf(v, 0.0f), 1.0f);
+}
+
+// Batched crop composition: block b belongs to the crop whose block range
+// holds it; its 256 threads take 256 consecutive pixels of the crop in
+// row-major order.
+__global__ __launch_bounds__(256) void compose_crops_kernel(int num_crops,
+                                                            const LigCrop *__restrict__ crops) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = num_crops - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (crops[mid].block_off <= b) lo = mid; else hi = mid - 1;
+    }
+    const LigCrop &c = crops[lo];
+    const long long p = (long long)(b - c.block_off) * 256 + threadIdx.x;
+    if (p >= (long long)c.w * c.h) return;
+    const int y = (int)(p / c.w), x = (int)(p - (long long)y * c.w);
+    float up[3];
+    if (c.mode == 0) {
+        const Axis ax{c.w, c.src_w, c.roi_x1, c.roi_x2};
+        const Axis ay{c.h, c.src_h, c.roi_y1, c.roi_y2};
+        hermite_pixel(ax, ay, x, y, c.f, c.fx, c.fy, c.fxy, up);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) up[ch] = clamp01(up[ch]);
+    } else {
+        const float sy = (float)c.src_h / (float)c.full_h, sx = (float)c.src_w / (float)c.full_w;
+        bilinear_pixel(c.src_h, c.src_w, sy, sx, y + c.win_y, x + c.win_x, c.f, up);
+    }
+    const float *l2 = c.level2 ? c.level2 + 3 * ((size_t)(y + c.off_y) * c.view_w + x + c.off_x)
+                               : nullptr;
+    float *o = c.out + 3 * p;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+#pragma clang fp contract(off)
+        const float a = clamp01(l2 ? l2[ch] : 1.0f);
+        o[ch] = clamp01(((a * c.span + up[ch]) - 0.5f) + c.lo);
+    }
 }
 
 }  // namespace
@@ -243,6 +314,15 @@ extern "C" int lig_bilinear_resize(int src_h, int src_w, int full_h, int full_w,
     const dim3 grid(ceil_div(out_w, 64), ceil_div(out_h, 4));
     hipLaunchKernelGGL(bilinear_kernel, grid, dim3(256), 0, as_stream(stream), src_h, src_w, sy,
                        sx, off_y, off_x, out_h, out_w, src, output);
+    return launch_status();
+}
+
+extern "C" int lig_compose_crops(int num_crops, const LigCrop *crops, int total_blocks,
+                                  void *stream) {
+    if (num_crops < 1 || total_blocks < 0 || !crops) return LIG_ERR_INVALID_ARGUMENT;
+    if (total_blocks == 0) return LIG_OK;
+    hipLaunchKernelGGL(compose_crops_kernel, dim3((unsigned)total_blocks), dim3(256), 0,
+                       as_stream(stream), num_crops, crops);
     return launch_status();
 }
 

@@ -26,6 +26,7 @@ import threading
 from dataclasses import dataclass, field
 from typing import Optional
 
+import numpy as np
 import torch
 
 from . import _lib as L
@@ -207,6 +208,7 @@ class TwoLevel:
     _l1_lock: threading.Lock = field(default_factory=threading.Lock, repr=False, compare=False)
 
     def __post_init__(self):
+        self.lo, self.hi = float(self.lo), float(self.hi)
         if self.upsample not in UPSAMPLE:
             raise ValueError(f"upsample must be one of {UPSAMPLE}, got {self.upsample!r}")
 
@@ -356,3 +358,196 @@ class TwoLevel:
         """h x w x 3 window of the composed image at `scale` output pixels per
         full-resolution pixel, in [0, 1]."""
         return self.finish_crop(self.crop_plan(x0, y0, w, h, scale, level1_cache), w, h)
+
+    def crop_geometry(self, x0: float, y0: float, w: int, h: int, scale: float = 1.0):
+        """Where a crop's inputs come from, without touching the device: its
+        tile-aligned level-2 view (params, x0', y0', w', h', scale), the
+        window's offset in that view, and the level-1 source -- ("hermite",
+        ROI on the level-1 canvas) or ("bilinear", full_h, full_w, win_x,
+        win_y).  Raises ValueError where render_crop would."""
+        if int(w) != w or int(h) != h or w < 1 or h < 1 or not scale > 0:
+            raise ValueError(f"crop {w} x {h} at scale {scale}: sizes must be positive "
+                             f"integers and the scale positive")
+        v2, off = _aligned_view(self.l2, x0, y0, int(w), int(h), scale)
+        if self.upsample == "bilinear":
+            fh, fw = self.l2.H * scale, self.l2.W * scale
+            ox, oy = x0 * scale, y0 * scale
+            if fh != int(fh) or fw != int(fw) or ox != int(ox) or oy != int(oy):
+                raise ValueError(f"bilinear composition needs whole output pixels: image "
+                                 f"{fw} x {fh}, window origin ({ox}, {oy}) at scale {scale}")
+            if ox < 0 or oy < 0 or ox + w > fw or oy + h > fh:
+                raise ValueError(f"window ({ox}, {oy}) + {w} x {h} leaves the {fw} x {fh} "
+                                 f"image")
+            return v2, off, ("bilinear", int(fh), int(fw), int(ox), int(oy))
+        (rx0, ry0, _, _), roi = self.l1_region(x0, y0, w, h, scale)
+        return v2, off, ("hermite", (roi[0] + rx0, roi[1] + ry0, roi[2] + rx0, roi[3] + ry0))
+
+
+# -- batched crops (serving) ---------------------------------------------------
+
+_VIEW_DTYPE = np.dtype([("par", "<u8"), ("means2d", "<u8"), ("rgbs", "<u8"), ("n", "<i4"),
+                        ("cholesky", "<i4"), ("origin_x", "<f4"), ("origin_y", "<f4"),
+                        ("scale_x", "<f4"), ("scale_y", "<f4"), ("img_height", "<i4"),
+                        ("img_width", "<i4"), ("gauss_off", "<i4"), ("reserved", "<i4")])
+_CROP_DTYPE = np.dtype([("level2", "<u8"), ("f", "<u8"), ("fx", "<u8"), ("fy", "<u8"),
+                        ("fxy", "<u8"), ("out", "<u8"), ("view_w", "<i4"), ("off_x", "<i4"),
+                        ("off_y", "<i4"), ("w", "<i4"), ("h", "<i4"), ("mode", "<i4"),
+                        ("src_h", "<i4"), ("src_w", "<i4"), ("roi_x1", "<f4"),
+                        ("roi_y1", "<f4"), ("roi_x2", "<f4"), ("roi_y2", "<f4"),
+                        ("full_h", "<i4"), ("full_w", "<i4"), ("win_x", "<i4"),
+                        ("win_y", "<i4"), ("span", "<f4"), ("lo", "<f4"),
+                        ("block_off", "<i4"), ("reserved", "<i4")])
+assert _VIEW_DTYPE.itemsize == 64 and _CROP_DTYPE.itemsize == 128
+
+
+def _to_device(arr: np.ndarray, dev) -> torch.Tensor:
+    """A host table copied to the device on the current stream (pinned
+    staging, so the copy is asynchronous)."""
+    return torch.from_numpy(arr.view(np.uint8).copy()).pin_memory().to(dev, non_blocking=True)
+
+
+class CropBatch:
+    """Several crops, of any of the resident representations, rendered as one
+    batch: one projection launch over every crop's level-2 view
+    (lig_project_views_batched), one host read of the views' intersection
+    counts, one batched binning (lig_bin_batched: each view an image of the
+    batch table, as in batched fitting), one render of every view
+    (lig_render_batched) and one composition launch for all crops
+    (lig_compose_crops).  Each crop gets the bits TwoLevel.render_crop gives
+    it alone (level 1 from the shared level-1 render).  Like crop_plan /
+    finish_crop: construction projects and reads the counts, `need` is the
+    HBM the rest allocates, `finish()` renders."""
+
+    def __init__(self, items):
+        """items: (TwoLevel, x0, y0, w, h, scale) per crop."""
+        if not items:
+            raise ValueError("an empty crop batch")
+        self.items = [(rep, float(x0), float(y0), int(w), int(h), float(scale))
+                      for rep, x0, y0, w, h, scale in items]
+        self.geo = [rep.crop_geometry(x0, y0, w, h, scale)
+                    for rep, x0, y0, w, h, scale in self.items]
+        self.dev = self.items[0][0].l2.means.device
+        views = np.zeros(len(self.items), _VIEW_DTYPE)
+        goff = 0
+        self.gauss_off = []
+        for k, ((p, vx, vy, vw, vh, vs), _, _) in enumerate(self.geo):
+            if p.means.device != self.dev:
+                raise ValueError("a crop batch renders on one device")
+            n = int(p.means.shape[0])
+            views[k] = (p.par.data_ptr(), p.means.data_ptr(), p.rgbs.data_ptr(), n,
+                        int(p.kind == "cholesky"), vx, vy, vs, vs, vh, vw, goff, 0)
+            self.gauss_off.append(goff)
+            goff += n
+        if goff >= 2 ** 31:
+            raise ValueError(f"a crop batch of {goff} Gaussians exceeds int32 slots")
+        self.n = goff
+        dev = self.dev
+        self.xys = torch.empty(self.n, 2, device=dev)
+        self.ext = torch.empty(self.n, 2, device=dev)
+        self.con = torch.empty(self.n, 3, device=dev)
+        self.nt = torch.empty(self.n, dtype=torch.int32, device=dev)
+        self.colors = torch.empty(self.n, 3, device=dev)
+        self._views = _to_device(views, dev)   # kept until the launches are done
+        s = L.stream_of(self.xys)
+        with torch.cuda.device(dev):
+            L.call("lig_project_views_batched", len(self.items), L.ptr(self._views), self.n, 16,
+                   L.ptr(self.xys), L.ptr(self.ext), L.ptr(self.con), L.ptr(self.nt),
+                   L.ptr(self.colors), s)
+        if self.n:
+            self.cum = _C.inclusive_cumsum_i32(self.nt)
+            ends = torch.tensor([self.gauss_off[k] + int(gm[0][0].means.shape[0]) - 1
+                                 for k, gm in enumerate(self.geo)], dtype=torch.int64)
+            ends_d = ends.clamp(min=0).pin_memory().to(dev, non_blocking=True)
+            upto = self.cum.index_select(0, ends_d).cpu().tolist()
+        else:
+            self.cum = torch.zeros(0, dtype=torch.int32, device=dev)
+            upto = [0] * len(self.items)
+        before = 0
+        self.counts = []
+        for k, gm in enumerate(self.geo):
+            c = upto[k] if gm[0][0].means.shape[0] > 0 else before
+            self.counts.append(int(c) - before)
+            before = int(c)
+        self.num_intersects = before
+        sizes = [(gm[0][4], gm[0][3]) for gm in self.geo]
+        self.num_tiles = sum(((h + 15) // 16) * ((w + 15) // 16) for h, w in sizes)
+        self.view_pix = sum(h * w for h, w in sizes)
+        self.out_pix = sum(w * h for _, _, _, w, h, _ in self.items)
+        m = self.num_intersects
+        self.need = int((L.size_query("lig_bin_workspace_size", m, self.num_tiles) if m else 0) +
+                        4 * m + 8 * self.num_tiles + 64 * self.n +
+                        L.size_query("lig_fit_batched_workspace_size", self.num_tiles) +
+                        12 * self.view_pix + 12 * self.out_pix + 128 * len(self.items))
+
+    @torch.no_grad()
+    def finish(self) -> list:
+        """The crops, h x w x 3 each, in [0, 1] (views of one output buffer).
+        Afterwards `view_images` holds each crop's level-2 view render (when
+        any view has intersections)."""
+        from .batch import batch_table
+        dev = self.dev
+        s = L.stream_of(self.xys)
+        img = None
+        geo = self.geo
+        if self.num_intersects > 0:
+            counts_n = [int(gm[0][0].means.shape[0]) for gm in geo]
+            tab, tiles, pix = batch_table([(gm[0][4], gm[0][3]) for gm in geo], counts_n)
+            tab_d = _to_device(tab, dev)
+            m = self.num_intersects
+            gids = torch.empty(m, dtype=torch.int32, device=dev)
+            bins = torch.empty(tiles, 2, dtype=torch.int32, device=dev)
+            ws = torch.empty(max(L.size_query("lig_bin_workspace_size", m, tiles), 1),
+                             dtype=torch.uint8, device=dev)
+            img = torch.empty(pix * 3, device=dev)
+            packed = torch.empty(L.size_query("lig_packed_gaussians_bytes", self.n),
+                                 dtype=torch.uint8, device=dev)
+            rws = torch.empty(L.size_query("lig_fit_batched_workspace_size", tiles),
+                              dtype=torch.uint8, device=dev)
+            with torch.cuda.device(dev):
+                L.call("lig_bin_batched", self.n, m, L.ptr(self.xys), L.ptr(self.ext),
+                       L.ptr(self.cum), L.ptr(tab_d), len(geo), tiles, L.ptr(gids), L.ptr(bins),
+                       L.ptr(ws), ws.numel(), s)
+                L.call("lig_render_batched", self.n, L.ptr(gids), L.ptr(bins), L.ptr(self.xys),
+                       L.ptr(self.ext), L.ptr(self.cum), L.ptr(self.con), L.ptr(self.colors),
+                       L.ptr(packed), L.ptr(tab_d), len(geo), tiles, L.ptr(img), L.ptr(rws),
+                       rws.numel(), s)
+            pix_off = np.cumsum([0] + [gm[0][4] * gm[0][3] for gm in geo])
+            self.view_images = [img[3 * int(pix_off[k]):3 * int(pix_off[k + 1])]
+                                .view(gm[0][4], gm[0][3], 3) for k, gm in enumerate(geo)]
+        out = torch.empty(self.out_pix * 3, device=dev)
+        crops = np.zeros(len(self.items), _CROP_DTYPE)
+        results, o, blk = [], 0, 0
+        keep = []
+        for k, ((rep, x0, y0, w, h, scale), ((p, vx, vy, vw, vh, vs), (ox, oy), src)) in \
+                enumerate(zip(self.items, geo)):
+            l2 = 0 if img is None or self.counts[k] == 0 else \
+                img.data_ptr() + 12 * int(pix_off[k])
+            l1 = rep.level1_render()
+            keep.append(l1)
+            rec = dict(level2=l2, out=out.data_ptr() + 12 * o, view_w=vw, off_x=ox, off_y=oy,
+                       w=w, h=h, src_h=rep.l1.H, src_w=rep.l1.W, span=rep.hi - rep.lo,
+                       lo=rep.lo, block_off=blk)   # span in FP64, rounded once, as torch does
+            if src[0] == "hermite":
+                f, fx, fy, fxy = l1
+                rec.update(mode=0, f=f.data_ptr(), fx=fx.data_ptr(), fy=fy.data_ptr(),
+                           fxy=fxy.data_ptr(), roi_x1=src[1][0], roi_y1=src[1][1],
+                           roi_x2=src[1][2], roi_y2=src[1][3])
+            else:
+                _, fh, fw, wx, wy = src
+                rec.update(mode=1, f=l1.data_ptr(), full_h=fh, full_w=fw, win_x=wx, win_y=wy)
+            for key, val in rec.items():
+                crops[key][k] = val
+            results.append(out[3 * o:3 * (o + w * h)].view(h, w, 3))
+            o += w * h
+            blk += -(-(w * h) // 256)
+        crops_d = _to_device(crops, dev)
+        with torch.cuda.device(dev):
+            L.call("lig_compose_crops", len(self.items), L.ptr(crops_d), blk, s)
+        return results
+
+
+@torch.no_grad()
+def render_crops(items) -> list:
+    """The crops (TwoLevel, x0, y0, w, h, scale) rendered as one batch
+    (CropBatch): the same bits as render_crop of each."""
+    return CropBatch(items).finish()

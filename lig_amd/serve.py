@@ -4,12 +4,16 @@ Resident two-level representations (lig_amd.render.TwoLevel) stay in HBM.
 Requests (representation, window origin, size, scale) arrive over time; a
 fixed number of worker slots, each with its own HIP stream, take requests as
 they arrive and run them concurrently on the device, so work of requests
-admitted while others are in flight overlaps.  Admission is by HBM budget:
-a request first projects its two windows (small, and gives its exact
-intersection counts), then reserves the bytes the rest of its render needs
-and waits while the reservations in flight would exceed the budget.  A
-request's render uses only its own buffers and deterministic kernels, so its
-result is the same whether it runs alone or in the mix.
+admitted while others are in flight overlaps.  With max_batch > 1 a slot
+also takes the requests already queued behind the one it takes (up to
+max_batch) and renders them as one batch: one launch per stage for all of
+them, the crops being images of one batch table (render.CropBatch).
+Admission is by HBM budget: a request (or batch) first projects its windows
+(small, and gives the exact intersection counts), then reserves the bytes
+the rest of its render needs and waits while the reservations in flight
+would exceed the budget.  A render uses only its own buffers and
+deterministic kernels, so each result is the same whether it runs alone, in
+a batch or in the mix.
 
 Reported: per-request latency (arrival to completion) p50 / p99, output
 Gpix/s, achieved request rate, peak reserved bytes.
@@ -25,7 +29,7 @@ from typing import List, Optional
 import numpy as np
 import torch
 
-from .render import TwoLevel
+from .render import CropBatch, TwoLevel
 
 
 @dataclass
@@ -73,10 +77,15 @@ class ServeStats:
     wall_s: float = 0.0
     requests: int = 0
     peak_reserved: int = 0
+    batch_sizes: List[int] = field(default_factory=list)
 
     def summary(self) -> dict:
         lat = np.array(self.latencies_ms) if self.latencies_ms else np.zeros(1)
-        return {"requests": self.requests,
+        extra = {}
+        if self.batch_sizes:
+            extra = {"mean_batch": round(float(np.mean(self.batch_sizes)), 2),
+                     "max_batch_seen": int(max(self.batch_sizes))}
+        return {**extra, "requests": self.requests,
                 "p50_ms": round(float(np.percentile(lat, 50)), 3),
                 "p99_ms": round(float(np.percentile(lat, 99)), 3),
                 "mean_ms": round(float(lat.mean()), 3),
@@ -86,12 +95,22 @@ class ServeStats:
 
 
 class CropServer:
-    """Continuous-batching crop renderer over resident representations."""
+    """Continuous-batching crop renderer over resident representations.
+
+    max_batch = 1: each slot renders one request at a time (its own
+    projection, binning, raster and composition launches).  max_batch > 1:
+    a slot that takes a request also takes up to max_batch - 1 more that are
+    already queued and renders them together as one render.CropBatch (one
+    launch per stage for all of them); every result has the same bits either
+    way."""
 
     def __init__(self, reps: List[TwoLevel], slots: int = 4, hbm_budget: int = 16 << 30,
-                 keep_results: bool = False, device="cuda:0"):
+                 keep_results: bool = False, device="cuda:0", max_batch: int = 1):
+        if max_batch < 1:
+            raise ValueError("max_batch must be at least 1")
         self.reps = reps
         self.slots = slots
+        self.max_batch = int(max_batch)
         self.budget = HbmBudget(hbm_budget)
         self.keep_results = keep_results
         self.device = torch.device(device)
@@ -99,12 +118,73 @@ class CropServer:
         self._done: List[Request] = []
         self._lock = threading.Lock()
         self._threads: List[threading.Thread] = []
+        self.batch_sizes: List[int] = []
 
     # -- worker -----------------------------------------------------------
+    def _take(self):
+        """The next request and up to max_batch - 1 more already queued;
+        (batch, stop) where stop says a stop marker was taken."""
+        req = self._q.get()
+        if req is None:
+            return [], True
+        batch = [req]
+        while len(batch) < self.max_batch:
+            try:
+                nxt = self._q.get_nowait()
+            except queue.Empty:
+                break
+            if nxt is None:
+                return batch, True
+            batch.append(nxt)
+        return batch, False
+
+    def _finish(self, reqs: List[Request], outs=None, error: Optional[BaseException] = None):
+        now = time.perf_counter()
+        for k, req in enumerate(reqs):
+            req.done = now
+            req.error = error
+            req.result = outs[k] if (outs is not None and self.keep_results) else None
+        with self._lock:
+            self._done.extend(reqs)
+
+    def _run_batch(self, batch: List[Request], stream):
+        ok = []
+        for req in batch:   # geometry errors are the request's own
+            try:
+                self.reps[req.rep].crop_geometry(req.x0, req.y0, req.w, req.h, req.scale)
+                ok.append(req)
+            except BaseException as e:
+                self._finish([req], error=e)
+        if not ok:
+            return
+        try:
+            cb = CropBatch([(self.reps[r.rep], r.x0, r.y0, r.w, r.h, r.scale) for r in ok])
+            need = cb.need
+            for r in ok:
+                r.reserved = need
+            self.budget.reserve(need)
+            try:
+                outs = cb.finish()
+                stream.synchronize()
+            finally:
+                del cb
+                self.budget.release(need)
+            with self._lock:
+                self.batch_sizes.append(len(ok))
+            self._finish(ok, outs)
+        except BaseException as e:
+            self._finish(ok, error=e)
+
     def _worker(self):
         torch.cuda.set_device(self.device)
         stream = torch.cuda.Stream(device=self.device)   # default priority
         with torch.cuda.stream(stream):
+            while self.max_batch > 1:
+                batch, stop = self._take()
+                if batch:
+                    self._run_batch(batch, stream)
+                if stop:
+                    return
             while True:
                 req = self._q.get()
                 if req is None:
@@ -172,6 +252,7 @@ class CropServer:
         st.pixels = sum(r.w * r.h for r in done)
         st.requests = len(done)
         st.peak_reserved = self.budget.peak
+        st.batch_sizes = list(self.batch_sizes)
         return st
 
 

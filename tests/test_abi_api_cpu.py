@@ -142,3 +142,33 @@ def test_c1_image_is_deterministic():
     assert img.shape == (512, 512, 3) and float(img.min()) == 0.0 and float(img.max()) == 1.0
     digest = hashlib.sha256(img.numpy().tobytes()).hexdigest()
     assert digest == "5c88d446e10635477bdef2e44d34e5aa2050dc40a16ad78f1ac9e6bcfdb3bfe0"
+
+
+def test_crop_batch_tables_match_the_c_structs(tmp_path):
+    """The host tables of batched crop rendering (render._VIEW_DTYPE,
+    render._CROP_DTYPE) have the field offsets and sizes of LigView and
+    LigCrop as a C compiler lays them out from lig_abi.h."""
+    import subprocess
+    from lig_amd import render
+    fields = {"LigView": render._VIEW_DTYPE, "LigCrop": render._CROP_DTYPE}
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "lig_abi.h"',
+           "int main(void) {"]
+    for st, dt in fields.items():
+        src.append(f'printf("{st} sizeof %zu\\n", sizeof({st}));')
+        for name in dt.names:
+            if name == "reserved":
+                continue
+            src.append(f'printf("{st} {name} %zu\\n", offsetof({st}, {name}));')
+    src += ["return 0;", "}"]
+    c = os.path.join(tmp_path, "layout.c")
+    exe = os.path.join(tmp_path, "layout")
+    open(c, "w").write("\n".join(src))
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", exe, c],
+                   check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(line.split()[:2]): int(line.split()[2]) for line in out if line}
+    for st, dt in fields.items():
+        assert got[(st, "sizeof")] == dt.itemsize
+        for name in dt.names:
+            if name != "reserved":
+                assert got[(st, name)] == dt.fields[name][1], (st, name)
