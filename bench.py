@@ -284,7 +284,8 @@ def batched_run(args, dev, torch):
 
 def serve_run(args, dev, torch):
     """Crop serving (C5): resident two-level representations, Poisson crop
-    requests, continuous batching over 4 stream slots, HBM admission."""
+    requests, continuous batching over 4 stream slots, HBM admission; the
+    load curve per request and with a slot's queued requests batched."""
     from lig_amd import serve
     torch.cuda.empty_cache()
     t0 = time.perf_counter()
@@ -294,6 +295,9 @@ def serve_run(args, dev, torch):
     warm = serve.CropServer(reps, slots=4, device=dev)
     warm.run_poisson([serve.Request(i, 0.0, 0.0, 256, 256, 1.0) for i in range(len(reps))],
                      qps=1e4)
+    serve.CropServer(reps, slots=args.serve_batch_slots, device=dev,
+                     max_batch=args.serve_batch).run_poisson(
+        [serve.Request(r.rep, r.x0, r.y0, r.w, r.h, r.scale) for r in reqs[:64]], qps=1e4)
     torch.cuda.synchronize()
     log(f"serve setup {time.perf_counter() - t0:.1f}s ({len(reps)} representations)")
     srv = serve.CropServer(reps, slots=4, hbm_budget=32 << 30, device=dev)
@@ -314,6 +318,18 @@ def serve_run(args, dev, torch):
                                            if k in ("p50_ms", "p99_ms", "achieved_qps",
                                                     "gpix_s")}))
     res["load_curve"] = curve
+    # the same streams with queued crops batched per slot (render.CropBatch:
+    # one launch per stage for a slot's queued requests)
+    bcurve = []
+    for q in [float(x) for x in args.serve_load_qps.split(",") if x]:
+        st = serve.CropServer(reps, slots=args.serve_batch_slots, hbm_budget=32 << 30,
+                              device=dev, max_batch=args.serve_batch).run_poisson(
+                                  reqs, qps=q, seed=3000)
+        bcurve.append(dict(offered_qps=q, **{k: v for k, v in st.summary().items()
+                                            if k in ("p50_ms", "p99_ms", "achieved_qps",
+                                                     "gpix_s", "mean_batch")}))
+    res["batched"] = {"slots": args.serve_batch_slots, "max_batch": args.serve_batch,
+                      "load_curve": bcurve}
     single = []
     for q in sorted({args.serve_qps, max([args.serve_qps] + [c["offered_qps"] for c in curve
                                                              if c["offered_qps"] <= 1200])}):
@@ -339,8 +355,11 @@ def main():
                     help="crop requests in the serving run (0 skips it)")
     ap.add_argument("--serve-qps", type=float, default=150.0)
     ap.add_argument("--serve-reps", type=int, default=32)
-    ap.add_argument("--serve-load-qps", default="300,600,1200,2400",
+    ap.add_argument("--serve-load-qps", default="300,600,1200,2000,2400",
                     help="further offered rates of the serving run (comma list; empty skips)")
+    ap.add_argument("--serve-batch", type=int, default=4,
+                    help="most queued requests a slot renders as one batch in the batched curve")
+    ap.add_argument("--serve-batch-slots", type=int, default=4)
     ap.add_argument("--skew-steps", type=int, default=3,
                     help="timed loss+backward calls on the density-skewed level 2 (0 skips)")
     ap.add_argument("--batch-steps", type=int, default=5,

@@ -445,13 +445,16 @@ typedef struct LigCrop {
  * representations.  lig_project_views_batched: lig_project_gaussians_view for
  * every view of the device array `views` (sorted by gauss_off, ranges back to
  * back from 0, n_total = their sum; no opacities), bit for bit, and, when
- * colors is not NULL, each slot's rgbs copied to colors [n_total,3]; a view
- * with a non-positive size or scale emits nothing (validate on the host).
+ * colors is not NULL, the rgbs of each slot whose Gaussian hits a tile copied
+ * to colors [n_total,3] (other slots are not written: nothing reads them); a
+ * view with a non-positive size or scale emits nothing (validate on the host).
  * Binning: lig_bin_batched with the views as the batch's images.
  * lig_render_batched: the image-only sum render of every image of the batch
  * table (lig_fit_render_l2_batched's image, without target or loss), the
- * same bits as lig_rasterize_forward's image of each view alone; workspace
- * lig_fit_batched_workspace_size(num_tiles).  lig_compose_crops: the crops
+ * same bits as lig_rasterize_forward's image of each view alone; it builds
+ * the records of Gaussian 0 and of the Gaussians with tile intersections
+ * only; batch_depth as for lig_fit_render_l2_batched (a speed setting);
+ * workspace lig_fit_batched_workspace_size(num_tiles).  lig_compose_crops: the crops
  * of the device array `crops` in one launch of total_blocks 256-pixel
  * blocks; the same bits as the per-crop chain (upscale or resize, clamps and
  * train.py's composition as FP32 tensor operations). */
@@ -462,7 +465,7 @@ int lig_render_batched(int n, const int32_t *gaussian_ids_sorted, const int32_t 
                        const float *xys, const float *extents, const int32_t *cum_tiles_hit,
                        const float *conics, const float *colors, void *packed,
                        const void *batch_table, int num_images, int num_tiles, float *out_img,
-                       void *workspace, size_t workspace_bytes, void *stream);
+                       int batch_depth, void *workspace, size_t workspace_bytes, void *stream);
 int lig_compose_crops(int num_crops, const LigCrop *crops, int total_blocks, void *stream);
 
 /* ---- captured fit step (no host read of the intersection count) ------------ */

@@ -102,8 +102,8 @@ _SIGNATURES = {
     "lig_project_views_batched": (
         _c_int, [_c_int, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lig_render_batched": (
-        _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp,
-                 _c_size, _vp]),
+        _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_int,
+                 _vp, _c_size, _vp]),
     "lig_compose_crops": (_c_int, [_c_int, _vp, _c_int, _vp]),
     "lig_bilinear_resize": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),

@@ -40,7 +40,7 @@ __device__ __forceinline__ bool cov_bounds(float ca, float cb, float cc, float o
 
 // One Gaussian of the forward projection; writes every output slot (zeros
 // where the reference leaves its zero fill).
-__device__ __forceinline__ void project_one(int i, float ca, float cb, float cc, float mx,
+__device__ __forceinline__ bool project_one(int i, float ca, float cb, float cc, float mx,
                                             float my, float o, int tiles_x, int tiles_y,
                                             int block_width,
                                             float *__restrict__ xys,
@@ -64,6 +64,7 @@ __device__ __forceinline__ void project_one(int i, float ca, float cb, float cc,
     xys[2 * i + 0] = hit ? mx : 0.0f;
     xys[2 * i + 1] = hit ? my : 0.0f;
     num_tiles_hit[i] = hit ? area : 0;
+    return hit;
 }
 
 __global__ __launch_bounds__(kThreads) void project_cov_kernel(
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(kThreads) void project_batched_chol_kernel(
 // the transform of the reference's scale_to (minimal_trainer_upscale.py:139-159)
 // with an origin, in FP32 without contraction; then the projection above.
 // Input j of (par, means2d) is written to output slot i.
-__device__ __forceinline__ void project_view_one(int i, int j, int cholesky,
+__device__ __forceinline__ bool project_view_one(int i, int j, int cholesky,
                                                  const float *__restrict__ par,
                                                  const float *__restrict__ means2d, float o,
                                                  float ox, float oy, float sx, float sy,
@@ -144,8 +145,8 @@ __device__ __forceinline__ void project_view_one(int i, int j, int cholesky,
         cb = (par[3 * j + 1] * sx) * sy;
         cc = (par[3 * j + 2] * sy) * sy;
     }
-    project_one(i, ca, cb, cc, mx, my, o, tiles_x, tiles_y, block_width, xys, extents, conics,
-                num_tiles_hit);
+    return project_one(i, ca, cb, cc, mx, my, o, tiles_x, tiles_y, block_width, xys, extents,
+                       conics, num_tiles_hit);
 }
 
 __global__ __launch_bounds__(kThreads) void project_view_kernel(
@@ -162,40 +163,66 @@ __global__ __launch_bounds__(kThreads) void project_view_kernel(
 // Several views in one launch: output slot i belongs to the view whose
 // [gauss_off, gauss_off + n) range holds it (views sorted by gauss_off,
 // ranges back to back from 0) and is that view's Gaussian i - gauss_off.
-// A view with a non-positive size or scale emits nothing.
-__global__ __launch_bounds__(kThreads) void project_views_kernel(
-    int num_views, const LigView *__restrict__ views, int n_total, int block_width,
-    float *__restrict__ xys, float *__restrict__ extents, float *__restrict__ conics,
-    int32_t *__restrict__ num_tiles_hit, float *__restrict__ colors) {
-    const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n_total) return;
+// A view with a non-positive size or scale emits nothing.  Colors are copied
+// for the slots whose Gaussian hits a tile (no other slot is read later).
+__device__ __forceinline__ int find_view(const LigView *__restrict__ views, int num_views,
+                                         int key) {
     int lo = 0, hi = num_views - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (views[mid].gauss_off <= i) lo = mid; else hi = mid - 1;
+        if (views[mid].gauss_off <= key) lo = mid; else hi = mid - 1;
     }
-    const LigView &v = views[lo];
+    return lo;
+}
+
+__device__ __forceinline__ void project_slot(const LigView &v, int i, int block_width,
+                                             float *__restrict__ xys,
+                                             float *__restrict__ extents,
+                                             float *__restrict__ conics,
+                                             int32_t *__restrict__ num_tiles_hit,
+                                             float *__restrict__ colors) {
     const int j = i - v.gauss_off;
     const bool ok = j < v.n && v.img_height > 0 && v.img_width > 0 && v.scale_x > 0.0f &&
                     v.scale_y > 0.0f;
+    bool hit = false;
     if (ok) {
         const int tx = (v.img_width + block_width - 1) / block_width;
         const int ty = (v.img_height + block_width - 1) / block_width;
-        project_view_one(i, j, v.cholesky, v.par, v.means2d, 1.0f, v.origin_x, v.origin_y,
-                         v.scale_x, v.scale_y, tx, ty, block_width, xys, extents, conics,
-                         num_tiles_hit);
+        hit = project_view_one(i, j, v.cholesky, v.par, v.means2d, 1.0f, v.origin_x, v.origin_y,
+                               v.scale_x, v.scale_y, tx, ty, block_width, xys, extents, conics,
+                               num_tiles_hit);
     } else {
         xys[2 * i] = xys[2 * i + 1] = 0.0f;
         extents[2 * i] = extents[2 * i + 1] = 0.0f;
         conics[3 * i] = conics[3 * i + 1] = conics[3 * i + 2] = 0.0f;
         num_tiles_hit[i] = 0;
     }
-    if (colors) {
-        const float *c = v.rgbs + 3 * (ok ? j : 0);
-        colors[3 * i] = ok ? c[0] : 0.0f;
-        colors[3 * i + 1] = ok ? c[1] : 0.0f;
-        colors[3 * i + 2] = ok ? c[2] : 0.0f;
+    if (colors && hit) {
+        const float *c = v.rgbs + 3 * j;
+        colors[3 * i] = c[0];
+        colors[3 * i + 1] = c[1];
+        colors[3 * i + 2] = c[2];
     }
+}
+
+__global__ __launch_bounds__(kThreads) void project_views_kernel(
+    int num_views, const LigView *__restrict__ views, int n_total, int block_width,
+    float *__restrict__ xys, float *__restrict__ extents, float *__restrict__ conics,
+    int32_t *__restrict__ num_tiles_hit, float *__restrict__ colors) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    // the wave's first and last slots: when one view holds both (all but the
+    // waves at a view boundary), its record is wave-uniform
+    const int w0 = __builtin_amdgcn_readfirstlane(i - (int)(threadIdx.x & (LIG_WAVE - 1)));
+    const int w1 = min(w0 + LIG_WAVE, n_total) - 1;
+    if (w0 >= n_total) return;
+    const int vf = find_view(views, num_views, w0);
+    const int vl = find_view(views, num_views, w1);
+    if (i >= n_total) return;
+    if (vf == vl)
+        project_slot(views[vf], i, block_width, xys, extents, conics, num_tiles_hit, colors);
+    else
+        project_slot(views[find_view(views, num_views, i)], i, block_width, xys, extents, conics,
+                     num_tiles_hit, colors);
 }
 
 __global__ __launch_bounds__(kThreads) void cov2d_bounds_kernel(

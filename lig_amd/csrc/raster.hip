@@ -53,6 +53,10 @@ __device__ __forceinline__ float l2_grad(float x, float t, float norm, float g) 
 // (tx, ty) has emission slot  base + ty * w + tx,  where w is the width of its
 // tile rectangle [x0, x1) x [y0, y1) (emission is row-major over it, from slot
 // cum[g-1]) and base = cum[g-1] - y0 * w - x0.
+// SKIP_EMPTY (render-only callers, cum given): Gaussians with no tile
+// intersections are left unwritten -- no bin names them -- except record 0,
+// which a zero-extent slot's (tile 0, gid 0) entry may name.
+template <bool SKIP_EMPTY = false>
 __global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restrict__ xys,
                                                    const float *__restrict__ conics,
                                                    const float *__restrict__ colors,
@@ -63,6 +67,7 @@ __global__ __launch_bounds__(256) void pack_kernel(int n, const float *__restric
                                                    int nimg, GaussPack *__restrict__ pack) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    if (SKIP_EMPTY && i > 0 && cum[i] == cum[i - 1]) return;
     const float a = conics[3 * i], b = conics[3 * i + 1], c = conics[3 * i + 2];
     const float o = opacities ? opacities[i] : 1.0f;
     GaussPack g;
@@ -1687,7 +1692,7 @@ extern "C" int lig_pack_gaussians(int n, const float *xys, const float *conics,
     if (n < 0 || (n > 0 && (!xys || !conics || !colors || !packed)))
         return LIG_ERR_INVALID_ARGUMENT;
     if (n == 0) return LIG_OK;
-    hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), n,
+    hipLaunchKernelGGL(pack_kernel<>, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), n,
                        xys, conics, colors, opacities, nullptr, nullptr, 0, 0, nullptr, 0,
                        reinterpret_cast<GaussPack *>(packed));
     return launch_status();
@@ -1872,7 +1877,7 @@ int rasterize_forward_impl(int n, int img_height, int img_width, int block_width
     const int tiles_x = (img_width + block_width - 1) / block_width;
     const int tiles_y = (img_height + block_width - 1) / block_width;
     if (n > 0)
-        hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
+        hipLaunchKernelGGL(pack_kernel<>, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
                            colors, opacities, extents, cum_tiles_hit, tiles_x, tiles_y, nullptr,
                            0, pk);
     const dim3 grid((unsigned)(tiles_x * tiles_y));
@@ -1979,7 +1984,7 @@ extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
     const int tiles_x = (img_width + 15) / 16;
     const int tiles_y = (img_height + 15) / 16;
     const int tiles = tiles_x * tiles_y;
-    hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
+    hipLaunchKernelGGL(pack_kernel<>, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
                        colors, nullptr, extents, cum_tiles_hit, tiles_x, tiles_y, nullptr, 0, pk);
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
     const int nsum = (tiles + kSumChunk - 1) / kSumChunk;
@@ -2026,7 +2031,7 @@ int fit_render_batched(int n, const int32_t *gaussian_ids_sorted, const int32_t 
     hipStream_t st = as_stream(stream);
     const BatchImage *tab = reinterpret_cast<const BatchImage *>(batch_table);
     GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
-    hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
+    hipLaunchKernelGGL(pack_kernel<>, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
                        colors, nullptr, extents, cum_tiles_hit, 0, 0, tab, num_images, pk);
     float *partial = reinterpret_cast<float *>(workspace);
     char *after_partials =
@@ -2091,26 +2096,28 @@ extern "C" int lig_render_batched(int n, const int32_t *gaussian_ids_sorted,
                                   const float *extents, const int32_t *cum_tiles_hit,
                                   const float *conics, const float *colors, void *packed,
                                   const void *batch_table, int num_images, int num_tiles,
-                                  float *out_img, void *workspace, size_t workspace_bytes,
-                                  void *stream) {
+                                  float *out_img, int batch_depth, void *workspace,
+                                  size_t workspace_bytes, void *stream) {
     if (n <= 0 || num_images < 1 || num_tiles < 1 || !gaussian_ids_sorted || !tile_bins ||
         !xys || !extents || !cum_tiles_hit || !conics || !colors || !packed || !batch_table ||
-        !out_img)
+        !out_img || !depth_ok(batch_depth))
         return LIG_ERR_INVALID_ARGUMENT;
     if (!workspace || workspace_bytes < lig_fit_batched_workspace_size(num_tiles))
         return LIG_ERR_WORKSPACE;
     hipStream_t st = as_stream(stream);
     const BatchImage *tab = reinterpret_cast<const BatchImage *>(batch_table);
     GaussPack *pk = reinterpret_cast<GaussPack *>(packed);
-    hipLaunchKernelGGL(pack_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys, conics,
-                       colors, nullptr, extents, cum_tiles_hit, 0, 0, tab, num_images, pk);
+    hipLaunchKernelGGL(pack_kernel<true>, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, xys,
+                       conics, colors, nullptr, extents, cum_tiles_hit, 0, 0, tab, num_images,
+                       pk);
     char *after_partials = reinterpret_cast<char *>(workspace) +
                            (((size_t)num_tiles * sizeof(float) + 255) & ~size_t(255));
     const int2 *bins = reinterpret_cast<const int2 *>(tile_bins);
-    const int32_t *order = launch_tile_order(num_tiles, bins, num_tiles, fit_forward_resident(256),
-                                             after_partials + kQueueInts * sizeof(int), st);
-    launch_fit_forward(256, dim3((unsigned)num_tiles), st, 1, 1, 1, gaussian_ids_sorted, bins, pk,
-                       out_img, nullptr, nullptr, tab, num_images, cum_tiles_hit,
+    const int32_t *order =
+        launch_tile_order(num_tiles, bins, num_tiles, fit_forward_resident(batch_depth),
+                          after_partials + kQueueInts * sizeof(int), st);
+    launch_fit_forward(batch_depth, dim3((unsigned)num_tiles), st, 1, 1, 1, gaussian_ids_sorted,
+                       bins, pk, out_img, nullptr, nullptr, tab, num_images, cum_tiles_hit,
                        TileQueue{nullptr, num_tiles}, order);
     return launch_status();
 }

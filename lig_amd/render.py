@@ -400,10 +400,20 @@ _CROP_DTYPE = np.dtype([("level2", "<u8"), ("f", "<u8"), ("fx", "<u8"), ("fy", "
 assert _VIEW_DTYPE.itemsize == 64 and _CROP_DTYPE.itemsize == 128
 
 
-def _to_device(arr: np.ndarray, dev) -> torch.Tensor:
-    """A host table copied to the device on the current stream (pinned
-    staging, so the copy is asynchronous)."""
-    return torch.from_numpy(arr.view(np.uint8).copy()).pin_memory().to(dev, non_blocking=True)
+def _to_device(parts, dev) -> list:
+    """Host tables (numpy arrays) copied to the device in one transfer on the
+    current stream (pinned staging, so the copy is asynchronous): uint8
+    device views, each starting on a 256-byte boundary."""
+    offs, o = [], 0
+    for a in parts:
+        offs.append(o)
+        o += -(-a.nbytes // 256) * 256
+    host = torch.empty(max(o, 1), dtype=torch.uint8, pin_memory=True)
+    hv = host.numpy()
+    for a, off in zip(parts, offs):
+        hv[off:off + a.nbytes] = np.ascontiguousarray(a).view(np.uint8).reshape(-1)
+    d = host.to(dev, non_blocking=True)
+    return [d[off:off + a.nbytes] for a, off in zip(parts, offs)]
 
 
 class CropBatch:
@@ -418,131 +428,129 @@ class CropBatch:
     finish_crop: construction projects and reads the counts, `need` is the
     HBM the rest allocates, `finish()` renders."""
 
+    depth = 64   # LDS batch depth of the render (lig_render_batched's batch_depth)
+
     def __init__(self, items):
         """items: (TwoLevel, x0, y0, w, h, scale) per crop."""
+        from .batch import batch_table
         if not items:
             raise ValueError("an empty crop batch")
         self.items = [(rep, float(x0), float(y0), int(w), int(h), float(scale))
                       for rep, x0, y0, w, h, scale in items]
         self.geo = [rep.crop_geometry(x0, y0, w, h, scale)
                     for rep, x0, y0, w, h, scale in self.items]
-        self.dev = self.items[0][0].l2.means.device
-        views = np.zeros(len(self.items), _VIEW_DTYPE)
-        goff = 0
-        self.gauss_off = []
+        dev = self.dev = self.items[0][0].l2.means.device
+        K = len(self.items)
+        views = np.zeros(K, _VIEW_DTYPE)
+        sizes, counts_n, goff = [], [], 0
         for k, ((p, vx, vy, vw, vh, vs), _, _) in enumerate(self.geo):
-            if p.means.device != self.dev:
+            if p.means.device != dev:
                 raise ValueError("a crop batch renders on one device")
             n = int(p.means.shape[0])
             views[k] = (p.par.data_ptr(), p.means.data_ptr(), p.rgbs.data_ptr(), n,
                         int(p.kind == "cholesky"), vx, vy, vs, vs, vh, vw, goff, 0)
-            self.gauss_off.append(goff)
+            sizes.append((vh, vw))
+            counts_n.append(n)
             goff += n
         if goff >= 2 ** 31:
             raise ValueError(f"a crop batch of {goff} Gaussians exceeds int32 slots")
-        self.n = goff
-        dev = self.dev
-        self.xys = torch.empty(self.n, 2, device=dev)
-        self.ext = torch.empty(self.n, 2, device=dev)
-        self.con = torch.empty(self.n, 3, device=dev)
-        self.nt = torch.empty(self.n, dtype=torch.int32, device=dev)
-        self.colors = torch.empty(self.n, 3, device=dev)
-        self._views = _to_device(views, dev)   # kept until the launches are done
-        s = L.stream_of(self.xys)
+        self.n = n_tot = goff
+        self.counts_n = counts_n
+        tab, self.num_tiles, self.view_pix = batch_table(sizes, counts_n)
+        ends = np.cumsum(counts_n, dtype=np.int64) - 1
+        self._views, self._tab, ends_b = _to_device([views, tab, np.maximum(ends, 0)], dev)
+        # projection outputs in one allocation: xys, extents [n,2], conics,
+        # colors [n,3], tiles hit [n] int32
+        buf = torch.empty(max(n_tot, 1) * 11, device=dev)
+        self.xys = buf[:2 * n_tot].view(n_tot, 2)
+        self.ext = buf[2 * n_tot:4 * n_tot].view(n_tot, 2)
+        self.con = buf[4 * n_tot:7 * n_tot].view(n_tot, 3)
+        self.colors = buf[7 * n_tot:10 * n_tot].view(n_tot, 3)
+        self.nt = buf[10 * n_tot:11 * n_tot].view(torch.int32)
+        self._buf = buf
+        s = L.stream_of(buf)
         with torch.cuda.device(dev):
-            L.call("lig_project_views_batched", len(self.items), L.ptr(self._views), self.n, 16,
+            L.call("lig_project_views_batched", K, L.ptr(self._views), n_tot, 16,
                    L.ptr(self.xys), L.ptr(self.ext), L.ptr(self.con), L.ptr(self.nt),
                    L.ptr(self.colors), s)
-        if self.n:
+        if n_tot:
             self.cum = _C.inclusive_cumsum_i32(self.nt)
-            ends = torch.tensor([self.gauss_off[k] + int(gm[0][0].means.shape[0]) - 1
-                                 for k, gm in enumerate(self.geo)], dtype=torch.int64)
-            ends_d = ends.clamp(min=0).pin_memory().to(dev, non_blocking=True)
-            upto = self.cum.index_select(0, ends_d).cpu().tolist()
+            upto = self.cum.index_select(0, ends_b.view(torch.int64)).cpu().tolist()
         else:
             self.cum = torch.zeros(0, dtype=torch.int32, device=dev)
-            upto = [0] * len(self.items)
+            upto = [0] * K
         before = 0
         self.counts = []
-        for k, gm in enumerate(self.geo):
-            c = upto[k] if gm[0][0].means.shape[0] > 0 else before
+        for k in range(K):
+            c = upto[k] if counts_n[k] > 0 else before
             self.counts.append(int(c) - before)
             before = int(c)
         self.num_intersects = before
-        sizes = [(gm[0][4], gm[0][3]) for gm in self.geo]
-        self.num_tiles = sum(((h + 15) // 16) * ((w + 15) // 16) for h, w in sizes)
-        self.view_pix = sum(h * w for h, w in sizes)
         self.out_pix = sum(w * h for _, _, _, w, h, _ in self.items)
         m = self.num_intersects
         self.need = int((L.size_query("lig_bin_workspace_size", m, self.num_tiles) if m else 0) +
-                        4 * m + 8 * self.num_tiles + 64 * self.n +
+                        4 * m + 8 * self.num_tiles + 64 * n_tot +
                         L.size_query("lig_fit_batched_workspace_size", self.num_tiles) +
-                        12 * self.view_pix + 12 * self.out_pix + 128 * len(self.items))
+                        12 * self.view_pix + 12 * self.out_pix + 128 * K)
 
     @torch.no_grad()
     def finish(self) -> list:
         """The crops, h x w x 3 each, in [0, 1] (views of one output buffer).
         Afterwards `view_images` holds each crop's level-2 view render (when
         any view has intersections)."""
-        from .batch import batch_table
         dev = self.dev
         s = L.stream_of(self.xys)
+        geo, K = self.geo, len(self.items)
+        pix_off = np.cumsum([0] + [gm[0][4] * gm[0][3] for gm in geo])
         img = None
-        geo = self.geo
         if self.num_intersects > 0:
-            counts_n = [int(gm[0][0].means.shape[0]) for gm in geo]
-            tab, tiles, pix = batch_table([(gm[0][4], gm[0][3]) for gm in geo], counts_n)
-            tab_d = _to_device(tab, dev)
-            m = self.num_intersects
+            m, tiles = self.num_intersects, self.num_tiles
             gids = torch.empty(m, dtype=torch.int32, device=dev)
             bins = torch.empty(tiles, 2, dtype=torch.int32, device=dev)
             ws = torch.empty(max(L.size_query("lig_bin_workspace_size", m, tiles), 1),
                              dtype=torch.uint8, device=dev)
-            img = torch.empty(pix * 3, device=dev)
+            img = torch.empty(self.view_pix * 3, device=dev)
             packed = torch.empty(L.size_query("lig_packed_gaussians_bytes", self.n),
                                  dtype=torch.uint8, device=dev)
             rws = torch.empty(L.size_query("lig_fit_batched_workspace_size", tiles),
                               dtype=torch.uint8, device=dev)
             with torch.cuda.device(dev):
                 L.call("lig_bin_batched", self.n, m, L.ptr(self.xys), L.ptr(self.ext),
-                       L.ptr(self.cum), L.ptr(tab_d), len(geo), tiles, L.ptr(gids), L.ptr(bins),
+                       L.ptr(self.cum), L.ptr(self._tab), K, tiles, L.ptr(gids), L.ptr(bins),
                        L.ptr(ws), ws.numel(), s)
                 L.call("lig_render_batched", self.n, L.ptr(gids), L.ptr(bins), L.ptr(self.xys),
                        L.ptr(self.ext), L.ptr(self.cum), L.ptr(self.con), L.ptr(self.colors),
-                       L.ptr(packed), L.ptr(tab_d), len(geo), tiles, L.ptr(img), L.ptr(rws),
-                       rws.numel(), s)
-            pix_off = np.cumsum([0] + [gm[0][4] * gm[0][3] for gm in geo])
+                       L.ptr(packed), L.ptr(self._tab), K, tiles, L.ptr(img), self.depth,
+                       L.ptr(rws), rws.numel(), s)
             self.view_images = [img[3 * int(pix_off[k]):3 * int(pix_off[k + 1])]
                                 .view(gm[0][4], gm[0][3], 3) for k, gm in enumerate(geo)]
         out = torch.empty(self.out_pix * 3, device=dev)
-        crops = np.zeros(len(self.items), _CROP_DTYPE)
+        crops = np.zeros(K, _CROP_DTYPE)
         results, o, blk = [], 0, 0
-        keep = []
+        base = out.data_ptr()
         for k, ((rep, x0, y0, w, h, scale), ((p, vx, vy, vw, vh, vs), (ox, oy), src)) in \
                 enumerate(zip(self.items, geo)):
             l2 = 0 if img is None or self.counts[k] == 0 else \
                 img.data_ptr() + 12 * int(pix_off[k])
             l1 = rep.level1_render()
-            keep.append(l1)
-            rec = dict(level2=l2, out=out.data_ptr() + 12 * o, view_w=vw, off_x=ox, off_y=oy,
-                       w=w, h=h, src_h=rep.l1.H, src_w=rep.l1.W, span=rep.hi - rep.lo,
-                       lo=rep.lo, block_off=blk)   # span in FP64, rounded once, as torch does
+            span = rep.hi - rep.lo   # in FP64, rounded once to FP32, as torch does
             if src[0] == "hermite":
                 f, fx, fy, fxy = l1
-                rec.update(mode=0, f=f.data_ptr(), fx=fx.data_ptr(), fy=fy.data_ptr(),
-                           fxy=fxy.data_ptr(), roi_x1=src[1][0], roi_y1=src[1][1],
-                           roi_x2=src[1][2], roi_y2=src[1][3])
+                r = src[1]
+                crops[k] = (l2, f.data_ptr(), fx.data_ptr(), fy.data_ptr(), fxy.data_ptr(),
+                            base + 12 * o, vw, ox, oy, w, h, 0, rep.l1.H, rep.l1.W, r[0], r[1],
+                            r[2], r[3], 0, 0, 0, 0, span, rep.lo, blk, 0)
             else:
                 _, fh, fw, wx, wy = src
-                rec.update(mode=1, f=l1.data_ptr(), full_h=fh, full_w=fw, win_x=wx, win_y=wy)
-            for key, val in rec.items():
-                crops[key][k] = val
+                crops[k] = (l2, l1.data_ptr(), 0, 0, 0, base + 12 * o, vw, ox, oy, w, h, 1,
+                            rep.l1.H, rep.l1.W, 0.0, 0.0, 0.0, 0.0, fh, fw, wx, wy, span, rep.lo,
+                            blk, 0)
             results.append(out[3 * o:3 * (o + w * h)].view(h, w, 3))
             o += w * h
             blk += -(-(w * h) // 256)
-        crops_d = _to_device(crops, dev)
+        (crops_d,) = _to_device([crops], dev)
         with torch.cuda.device(dev):
-            L.call("lig_compose_crops", len(self.items), L.ptr(crops_d), blk, s)
+            L.call("lig_compose_crops", K, L.ptr(crops_d), blk, s)
         return results
 
 
