@@ -844,14 +844,39 @@ __global__ __launch_bounds__(LIG_WAVE) void row_count_kernel(
 // Per row (one block): each tile's column of segment counts becomes its
 // exclusive prefix over segments (scol, same layout); the tile totals go to
 // tt, their exclusive prefix within the row to te, the row total to
-// rowtot[r] (rowtot[rows] = 0 for the scan that follows).
+// rowtot[r] (rowtot[rows] = 0 for the scan that follows).  Threads take
+// consecutive columns (coalesced) and load four segments ahead; the totals
+// meet in LDS for the scan in column order.
 constexpr int kPrefixThreads = 256;
 __global__ __launch_bounds__(kPrefixThreads) void row_prefix_kernel(
-    int P, int tiles_x, int rows, const int32_t *__restrict__ counts, int32_t *__restrict__ scol,
-    int32_t *__restrict__ tt, int32_t *__restrict__ te, int32_t *__restrict__ rowtot) {
+    int P, int tiles_x, int rows, const int32_t *__restrict__ counts,
+    int32_t *__restrict__ scol, int32_t *__restrict__ tt, int32_t *__restrict__ te,
+    int32_t *__restrict__ rowtot) {
     __shared__ int wsum[kPrefixThreads / LIG_WAVE];
+    __shared__ int s_tot[kRowMaxTiles];
     const int r = blockIdx.x;
     const size_t rbase = (size_t)r * P * tiles_x;
+    for (int t = threadIdx.x; t < tiles_x; t += kPrefixThreads) {
+        const int32_t *c = counts + rbase + t;
+        int32_t *o = scol + rbase + t;
+        int run = 0, s = 0;
+        for (; s + 4 <= P; s += 4) {
+            const int v0 = c[(size_t)s * tiles_x], v1 = c[(size_t)(s + 1) * tiles_x];
+            const int v2 = c[(size_t)(s + 2) * tiles_x], v3 = c[(size_t)(s + 3) * tiles_x];
+            o[(size_t)s * tiles_x] = run;
+            o[(size_t)(s + 1) * tiles_x] = run + v0;
+            o[(size_t)(s + 2) * tiles_x] = run + v0 + v1;
+            o[(size_t)(s + 3) * tiles_x] = run + v0 + v1 + v2;
+            run += v0 + v1 + v2 + v3;
+        }
+        for (; s < P; ++s) {
+            const int v = c[(size_t)s * tiles_x];
+            o[(size_t)s * tiles_x] = run;
+            run += v;
+        }
+        s_tot[t] = run;
+    }
+    __syncthreads();
     constexpr int kPer = kRowMaxTiles / kPrefixThreads;
     int tot[kPer];
     int mine = 0;
@@ -859,17 +884,8 @@ __global__ __launch_bounds__(kPrefixThreads) void row_prefix_kernel(
     for (int k = 0; k < kPer; ++k) {
         // thread i takes columns i * kPer .. i * kPer + kPer - 1 (row order for the scan)
         const int t = threadIdx.x * kPer + k;
-        int run = 0;
-        if (t < tiles_x) {
-            for (int s = 0; s < P; ++s) {
-                const size_t i = rbase + (size_t)s * tiles_x + t;
-                const int v = counts[i];
-                scol[i] = run;
-                run += v;
-            }
-        }
-        tot[k] = run;
-        mine += run;
+        tot[k] = t < tiles_x ? s_tot[t] : 0;
+        mine += tot[k];
     }
     int total;
     int excl = block_exclusive_scan_n<kPrefixThreads>(mine, wsum, &total);
@@ -954,18 +970,33 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
     const size_t seg_base = ((size_t)r * P + s) * tiles_x;
     const size_t row_tiles = (size_t)r * tiles_x;
     const int roff = rowoff[r];
-    int carry = 0;
-    for (int t0 = 0; t0 < tiles_x; t0 += LIG_WAVE) {
-        const int t = t0 + lane;
-        int c = 0;
+    // every chunk's loads are issued before the first is used (the wave is
+    // alone on its segment; the loads' latency, not their bytes, is the cost)
+    constexpr int kChunks = kRowMaxTiles / LIG_WAVE;
+    int cv[kChunks], sv[kChunks];
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k) {
+        const int t = k * LIG_WAVE + lane;
+        cv[k] = 0;
+        sv[k] = 0;
         if (t < tiles_x) {
-            c = counts[seg_base + t];
-            segst[t] = roff + te[row_tiles + t] + scol[seg_base + t];
+            cv[k] = counts[seg_base + t];
+            sv[k] = te[row_tiles + t] + scol[seg_base + t];
+        }
+    }
+    int carry = 0;
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k) {
+        if (k * LIG_WAVE >= tiles_x) break;  // wave-uniform
+        const int t = k * LIG_WAVE + lane;
+        const int c = cv[k];
+        const int ex = wave_exclusive_sum(c);
+        if (t < tiles_x) {
+            segst[t] = roff + sv[k];
             mask[t] = 0ull;
             cur[t] = 0;
+            lstart[t] = carry + ex;
         }
-        const int ex = wave_exclusive_sum(c);
-        if (t < tiles_x) lstart[t] = carry + ex;
         carry += __shfl(ex + c, LIG_WAVE - 1, LIG_WAVE);
     }
     if (lane == 0) lstart[tiles_x] = carry;
@@ -984,14 +1015,25 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
         const int nq = lstart[wb] - base_q;
         const bool direct = nq > bufn;
         if (nq > 0) {
+            // entries are fetched ahead: the row entry two rounds ahead and
+            // its rectangle one round ahead, so a round's LDS work covers the
+            // latency of the next rounds' loads
+            uint32_t v_cur = lo + lane < hi ? rowval[lo + lane] : 0u;
+            uint32_t v_nxt = lo + LIG_WAVE + lane < hi ? rowval[lo + LIG_WAVE + lane] : 0u;
+            uint2 rc_cur = lo + lane < hi ? rect[v_cur & ~kPhantomBit] : make_uint2(0u, 0u);
             for (int base = lo; base < hi; base += LIG_WAVE) {
                 const int j = base + lane;
+                const uint32_t v = v_cur;
+                const uint2 rc = rc_cur;
+                const int j1 = j + LIG_WAVE, j2 = j + 2 * LIG_WAVE;
+                rc_cur = j1 < hi ? rect[v_nxt & ~kPhantomBit] : make_uint2(0u, 0u);
+                v_cur = v_nxt;
+                v_nxt = j2 < hi ? rowval[j2] : 0u;
                 int x0 = 0, y0 = 0, x1 = 0, y1 = 0, g = 0;
                 bool ph = false;
                 if (j < hi) {
-                    const uint32_t v = rowval[j];
                     g = (int)(v & ~kPhantomBit);
-                    unpack_rect(rect[g], x0, y0, x1, y1, ph);
+                    unpack_rect(rc, x0, y0, x1, y1, ph);
                 }
                 const int t0 = max(x0, wa), t1 = min(x1, wb);
                 for (int t = t0; t < t1; ++t) atomicOr(&mask[t], bit);
