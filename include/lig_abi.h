@@ -583,6 +583,18 @@ int lig_gradient_aware_upscale_backward(int src_h, int src_w, int dst_h, int dst
 
 /* ---- image-quality metrics --------------------------------------------------- */
 
+/* ---- bounds-checked build ---------------------------------------------------- */
+
+/* No reference counterpart: a debugging aid.  A library built with LIG_CHECKED
+ * (lig_amd/liblig_hip_checked.so) evaluates bounds checks at its index-heavy
+ * device code (the sort's and the row expansion's LDS ranks and outputs, the
+ * emission's owner lookups, the backward's row-group sort and record slots,
+ * the launch order, the batched crop kernels); a failed check does not trap,
+ * it is counted.  lig_check_status waits for the device, then returns the
+ * failures since the last call, the first failing line and its source file,
+ * and clears them; a normal build returns LIG_ERR_UNSUPPORTED. */
+int lig_check_status(unsigned long long *failures, int *first_line, const char **first_file);
+
 /* SSIM / MS-SSIM of two image batches, replacing the third-party
  * pytorch_msssim.ms_ssim / ssim the reference's evaluation and loss menu call
  * (train.py:200, utils.py:26-34).  x, y: [n, c, h, w] FP32 addressed through

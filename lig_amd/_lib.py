@@ -12,7 +12,12 @@ import os
 import torch
 
 _PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG_DIR, "liblig_hip.so")
+# LIG_LIB_VARIANT=checked loads the bounds-checked build (liblig_hip_checked.so,
+# lig_check_status reports its failed checks); anything else the normal one
+VARIANT = os.environ.get("LIG_LIB_VARIANT", "")
+LIB_PATH = os.path.join(_PKG_DIR, "liblig_hip_checked.so" if VARIANT == "checked"
+                        else "liblig_hip.so")
+CHECKED_LIB_PATH = os.path.join(_PKG_DIR, "liblig_hip_checked.so")
 
 _c_int = ctypes.c_int
 _c_float = ctypes.c_float
@@ -105,6 +110,7 @@ _SIGNATURES = {
         _c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_int,
                  _vp, _c_size, _vp]),
     "lig_compose_crops": (_c_int, [_c_int, _vp, _c_int, _vp]),
+    "lig_check_status": (_c_int, [_vp, _vp, _vp]),
     "lig_bilinear_resize": (
         _c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "lig_gradient_aware_upscale_forward": (
@@ -204,3 +210,19 @@ def check_device(name: str, t, dtype=None):
         raise RuntimeError(f"{name} must be contiguous")
     if dtype is not None and t.dtype != dtype:
         raise RuntimeError(f"{name} must have dtype {dtype}, got {t.dtype}")
+
+
+def check_status():
+    """(failures, first line, first file) of the bounds-checked build's failed
+    checks since the last call (then cleared); None for a normal build."""
+    lib = load()
+    failures = ctypes.c_ulonglong(0)
+    line = ctypes.c_int(0)
+    path = ctypes.c_char_p(None)
+    rc = lib.lig_check_status(ctypes.byref(failures), ctypes.byref(line), ctypes.byref(path))
+    if rc == -4:   # LIG_ERR_UNSUPPORTED: built without checks
+        return None
+    if rc != 0:
+        raise LigError(f"lig_check_status: {lib.lig_strerror(rc).decode()}")
+    return (int(failures.value), int(line.value),
+            path.value.decode() if path.value is not None else None)

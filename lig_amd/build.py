@@ -1,4 +1,5 @@
-"""Build the gfx950 HIP library ``lig_amd/liblig_hip.so`` with hipcc.
+"""Build the gfx950 HIP library ``lig_amd/liblig_hip.so`` (and its
+bounds-checked variant ``liblig_hip_checked.so``) with hipcc.
 
 Run ``python -m lig_amd.build`` (or ``__graft_entry__.build()``).  The library is
 built in-tree so that it travels with the repository snapshot.
@@ -13,6 +14,7 @@ import sys
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "liblig_hip.so")
+CHECKED_LIB_PATH = os.path.join(PKG_DIR, "liblig_hip_checked.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -32,15 +34,26 @@ def sources():
 
 
 def build(verbose: bool = False) -> str:
-    tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *sources()]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building liblig_hip.so")
-    os.replace(tmp, LIB_PATH)
+    """Build the library and its bounds-checked variant (-DLIG_CHECKED,
+    ``liblig_hip_checked.so``) side by side; returns the library's path."""
+    jobs = []
+    for path, extra in ((LIB_PATH, []), (CHECKED_LIB_PATH, ["-DLIG_CHECKED"])):
+        tmp = path + ".tmp"
+        cmd = [HIPCC, *FLAGS, *extra, "-o", tmp, *sources()]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        jobs.append((path, tmp, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                 stderr=subprocess.PIPE, text=True)))
+    failed = []
+    for path, tmp, proc in jobs:
+        out, err = proc.communicate()
+        if proc.returncode != 0:
+            sys.stderr.write(out + err)
+            failed.append(os.path.basename(path))
+        else:
+            os.replace(tmp, path)
+    if failed:
+        raise RuntimeError(f"hipcc failed building {', '.join(failed)}")
     return LIB_PATH
 
 

@@ -340,6 +340,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     for (int r = 0; r < kSortRounds; ++r) {
         if (wbase + r * LIG_WAVE + lane < count) {
             const int pos = cnt_w[digit_of<K, RB>(key[r], shift, flip)] + lpos[r];
+            LIG_CHECK(pos >= 0 && pos < count);
             skey[pos] = key[r];
             sval[pos] = val[r];
         }
@@ -351,6 +352,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     for (int j = tid; j < count; j += kSortThreads) {
         const K k = skey[j];
         const int pos = lstart[digit_of<K, RB>(k, shift, flip)] + j;
+        LIG_CHECK(pos >= 0 && pos < n);
         keys_out[pos] = k;
         vals_out[pos] = sval[j];
     }
@@ -568,9 +570,11 @@ __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
         const int4 hi4 = *reinterpret_cast<const int4 *>(&s_start[wv][j8 + 4]);
         const int j = j8 + (lo4.y <= pos) + (lo4.z <= pos) + (lo4.w <= pos) + (hi4.x <= pos) +
                       (hi4.y <= pos) + (hi4.z <= pos) + (hi4.w <= pos);
+        LIG_CHECK(j >= 0 && j < LIG_WAVE);
         const int4 rc = s_rect[wv][j];
         const int4 gr = s_grid[wv][j];
         if (pos < wend) {
+            LIG_CHECK(pos >= rc.x);
             uint32_t key = 0;
             int gid = 0;
             const int rw = rc.w;
@@ -594,6 +598,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_tile_keys_kernel(
                 }
                 key = (uint32_t)gr.y + (uint32_t)(rc.z + row) * (uint32_t)gr.x +
                       (uint32_t)(rc.y + col);
+                LIG_CHECK(col >= 0 && col < rw && row >= 0);
                 gid = g0 + j;
             }
             keys[pos] = key;
@@ -774,6 +779,7 @@ __global__ __launch_bounds__(256) void emit_rows_kernel(int n, int cap,
                 else
                     hi = mid - 1;
             }
+            LIG_CHECK(s_start[wv][lo] <= pos && pos < cap);
             const uint32_t k = s_key[wv][lo];
             const bool ph = (k & kPhantomBit) != 0;
             rowkey[pos] = ph ? 0u : k + (uint32_t)(pos - s_start[wv][lo]);
@@ -825,6 +831,7 @@ __global__ __launch_bounds__(LIG_WAVE) void row_count_kernel(
         int x0, y0, x1, y1;
         bool ph;
         unpack_rect(rc, x0, y0, x1, y1, ph);
+        LIG_CHECK(x0 >= 0 && x0 <= x1 && x1 <= tiles_x);
         atomicAdd(&diff[x0], 1);
         atomicAdd(&diff[x1], -1);
     }
@@ -1036,6 +1043,7 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
                     unpack_rect(rc, x0, y0, x1, y1, ph);
                 }
                 const int t0 = max(x0, wa), t1 = min(x1, wb);
+                LIG_CHECK(j >= hi || (x0 >= 0 && x0 <= x1 && x1 <= tiles_x && y0 <= r));
                 for (int t = t0; t < t1; ++t) atomicOr(&mask[t], bit);
                 wave_lds_sync();
                 int slot0 = 0;
@@ -1049,12 +1057,14 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
                     const int slot = ph ? slot0 : slot0 + t;
                     if (direct) {
                         const int pos = segst[t] + rank;
+                        LIG_CHECK(pos >= 0 && pos < m);
                         if (pos < m) {
                             gids_out[pos] = gid;
                             if (SLOTS) slots_out[pos] = slot;
                         }
                     } else {
                         const int q = lstart[t] - base_q + rank;
+                        LIG_CHECK(q >= 0 && q < nq && q < bufn && rank < lstart[t + 1] - lstart[t]);
                         bg[q] = gid;
                         if (SLOTS) bs[q] = slot;
                         bt[q] = (uint16_t)t;
@@ -1074,6 +1084,7 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
                 for (int q = lane; q < nq; q += LIG_WAVE) {
                     const int t = bt[q];
                     const int pos = segst[t] + (q + base_q - lstart[t]);
+                    LIG_CHECK(t >= wa && t < wb && pos >= 0 && pos < m);
                     if (pos < m) {
                         gids_out[pos] = bg[q];
                         if (SLOTS) slots_out[pos] = bs[q];

@@ -22,6 +22,48 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Bounds checks.  In a library built with LIG_CHECKED, LIG_CHECK(cond)
+// evaluates cond in device code and, when it is false, counts the failure
+// and records the first failing line in this translation unit's device state
+// (no trap: the kernel carries on); lig_check_status() collects and clears the
+// state of every unit.  In a normal build LIG_CHECK compiles to nothing.
+typedef int (*CheckReader)(unsigned *count, unsigned *line, int clear);
+int register_check_reader(const char *file, CheckReader reader);
+
+}  // namespace lig
+
+#ifdef LIG_CHECKED
+static __device__ unsigned int lig_check_state[2];  // failures, first failing line
+#define LIG_CHECK(cond)                                                   \
+    do {                                                                  \
+        if (!(cond)) {                                                    \
+            atomicAdd(&lig_check_state[0], 1u);                           \
+            atomicCAS(&lig_check_state[1], 0u, (unsigned)__LINE__);       \
+        }                                                                 \
+    } while (0)
+static int lig_check_read(unsigned *count, unsigned *line, int clear) {
+    unsigned st[2] = {0u, 0u};
+    if (hipMemcpyFromSymbol(st, HIP_SYMBOL(lig_check_state), sizeof(st), 0,
+                            hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    *count = st[0];
+    *line = st[1];
+    if (clear && st[0]) {
+        const unsigned zero[2] = {0u, 0u};
+        if (hipMemcpyToSymbol(HIP_SYMBOL(lig_check_state), zero, sizeof(zero), 0,
+                              hipMemcpyHostToDevice) != hipSuccess)
+            return -1;
+    }
+    return 0;
+}
+static const int lig_check_registered = lig::register_check_reader(__BASE_FILE__,
+                                                                   lig_check_read);
+#else
+#define LIG_CHECK(cond) ((void)0)
+#endif
+
+namespace lig {
+
 // Status of the last launch on this thread, mapped to the ABI codes.
 inline int launch_status() {
     hipError_t e = hipGetLastError();

@@ -182,6 +182,7 @@ __device__ __forceinline__ void project_slot(const LigView &v, int i, int block_
                                              int32_t *__restrict__ num_tiles_hit,
                                              float *__restrict__ colors) {
     const int j = i - v.gauss_off;
+    LIG_CHECK(j >= 0);
     const bool ok = j < v.n && v.img_height > 0 && v.img_width > 0 && v.scale_x > 0.0f &&
                     v.scale_y > 0.0f;
     bool hit = false;

@@ -636,6 +636,7 @@ __device__ __forceinline__ void emit_grads(bool det, GradRec *__restrict__ rec, 
         r.r0 = make_float4(vr, vg, vb, c0);
         r.r1 = make_float4(c1, c2, gx, gy);
         r.r2 = make_float4(ax, ay, vo, __int_as_float(g));
+        LIG_CHECK(slot >= 0);
         rec[slot] = r;
     } else {
         unsafeAtomicAdd(&g_rgb[3 * g + 0], vr);
@@ -991,10 +992,12 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
                                           (float)y_org + 0.5f);
             if (m) {
                 key = ((uint32_t)__builtin_ctz(m) << 4) | (31u - (uint32_t)__builtin_clz(m));
+                LIG_CHECK((key >> 1) < (uint32_t)kFitCntWords && (key >> 4) <= (key & 15u));
                 atomicAdd(&cnt[key >> 1], 1u << (16 * (key & 1u)));
             } else {
                 const float4 p3 = pack[g].p3;
                 const int slot = __float_as_int(p3.z) + ty * __float_as_int(p3.w) + tx;
+                LIG_CHECK(slot >= 0);
                 FitRec z;
                 z.r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 z.r1 = z.r0;
@@ -1037,6 +1040,7 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
         }
         const int rank = __popcll(mm & lt);
         const int before = (int)((cnt[key >> 1] >> (16 * (key & 1u))) & 0xFFFFu);
+        LIG_CHECK(!valid || (before + rank >= 0 && before + rank < cn));
         if (valid) ord[before + rank] = (uint16_t)(lane + kThreads * k);
         if (valid && rank == 0)
             atomicAdd(&cnt[key >> 1], (uint32_t)__popcll(mm) << (16 * (key & 1u)));
@@ -1056,8 +1060,10 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
         if (start < ends[q]) {
             const int key = 4 * lane + q, r0 = key >> 4, r1 = key & 15;
             const uint32_t rows = ((2u << r1) - 1u) & ~((1u << r0) - 1u);
-            for (int j = start / kThreads; j <= (ends[q] - 1) / kThreads; ++j)
+            for (int j = start / kThreads; j <= (ends[q] - 1) / kThreads; ++j) {
+                LIG_CHECK(j >= 0 && j < kRounds);
                 atomicOr(&cnt[j], rows);
+            }
         }
         start = ends[q];
     }
@@ -1204,6 +1210,7 @@ __device__ __forceinline__ void bwd16_tile(
                 r.r0 = make_float4(A.r, A.g, A.b, 0.5f * A.sxx);
                 r.r1 = make_float4(A.sxy, 0.5f * A.syy, __builtin_fmaf(a, A.mx, b * A.my),
                                    __builtin_fmaf(b, A.mx, c * A.my));
+                LIG_CHECK(slot >= 0);
                 reinterpret_cast<FitRec *>(rec)[slot] = r;
             }
         } else {
@@ -1599,7 +1606,10 @@ __global__ __launch_bounds__(kOrderThreads) void order_scatter_kernel(
 #pragma unroll
     for (int k = 0; k < kOrderPer; ++k) {
         const int t = (blockIdx.x * kOrderPer + k) * kOrderThreads + threadIdx.x;
-        if (cls[k] >= 0) order[pre[cls[k]] + lbase[cls[k]] + rank[k]] = t;
+        if (cls[k] >= 0) {
+            LIG_CHECK(cls[k] < kOrderClasses && pre[cls[k]] + lbase[cls[k]] + rank[k] < ntiles);
+            order[pre[cls[k]] + lbase[cls[k]] + rank[k]] = t;
+        }
     }
 }
 

@@ -271,9 +271,11 @@ __global__ __launch_bounds__(256) void compose_crops_kernel(int num_crops,
         if (crops[mid].block_off <= b) lo = mid; else hi = mid - 1;
     }
     const LigCrop &c = crops[lo];
+    LIG_CHECK(b >= c.block_off && (long long)(b - c.block_off) * 256 < (long long)c.w * c.h);
     const long long p = (long long)(b - c.block_off) * 256 + threadIdx.x;
     if (p >= (long long)c.w * c.h) return;
     const int y = (int)(p / c.w), x = (int)(p - (long long)y * c.w);
+    LIG_CHECK(x + c.off_x < c.view_w && x >= 0 && y >= 0);
     float up[3];
     if (c.mode == 0) {
         const Axis ax{c.w, c.src_w, c.roi_x1, c.roi_x2};

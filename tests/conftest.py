@@ -27,3 +27,20 @@ def device():
     import lig_amd._lib as L
     L.load()
     return torch.device("cuda:0")
+
+
+@pytest.fixture(autouse=True)
+def _bounds_checks():
+    """With LIG_CHECK_ASSERT=1 (and LIG_LIB_VARIANT=checked), every test that
+    used the library must leave the bounds-checked build's failure count at
+    zero."""
+    yield
+    if os.environ.get("LIG_CHECK_ASSERT") != "1":
+        return
+    import lig_amd._lib as L
+    if L._lib is None:   # the test did not load the library
+        return
+    st = L.check_status()
+    assert st is not None, f"{L.LIB_PATH} is not the bounds-checked build"
+    failures, line, path = st
+    assert failures == 0, f"{failures} failed bounds checks, first at {path}:{line}"

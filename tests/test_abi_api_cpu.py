@@ -172,3 +172,16 @@ def test_crop_batch_tables_match_the_c_structs(tmp_path):
         for name in dt.names:
             if name != "reserved":
                 assert got[(st, name)] == dt.fields[name][1], (st, name)
+
+
+def test_checked_build_exports_the_same_symbols():
+    """The bounds-checked variant exports every declared entry point too, and
+    the normal build reports that it has no checks (LIG_ERR_UNSUPPORTED)."""
+    import ctypes
+    import lig_amd._lib as L
+    if not os.path.exists(L.CHECKED_LIB_PATH):
+        pytest.skip("bounds-checked build not present")
+    lib = ctypes.CDLL(L.CHECKED_LIB_PATH)
+    for n in _declared():
+        assert hasattr(lib, n), n
+    assert L.check_status() is None or os.environ.get("LIG_LIB_VARIANT") == "checked"
