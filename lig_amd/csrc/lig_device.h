@@ -145,11 +145,14 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
 
 // Tile work queue of a persistent launch: one counter per XCD (zeroed before
 // the launch, kQueueStride ints apart so that each sits in its own 256-byte
-// line) over that XCD's contiguous share of the tiles, the split of
-// xcd_remap.  A wave (blocks are dispatched to XCDs round-robin) takes
-// kQueueGrab tiles at a time from its own XCD's share while any is left,
-// then from the other shares in turn, so an XCD whose share runs long is
-// helped by the others.  The next grab is issued before the current tiles
+// line) over that XCD's share of the tiles.  Without a launch order (order
+// null, or order[0] = -1) the shares are the contiguous split of xcd_remap;
+// with one, the tiles are dealt round-robin in that order (heaviest first):
+// item i of XCD x's share is order[8 i + x], so every XCD starts on its part
+// of the heaviest tiles.  A wave (blocks are dispatched to XCDs round-robin)
+// takes kQueueGrab items at a time from its own XCD's share while any is
+// left, then from the other shares in turn, so an XCD whose share runs long
+// is helped by the others.  The next grab is issued before the current tiles
 // are processed.
 constexpr int kQueueStride = 64;
 constexpr int kQueueGrab = 2;
@@ -158,6 +161,7 @@ constexpr int kQueueInts = 8 * kQueueStride;
 struct TileQueue {
     int *ctr;  // kQueueInts ints
     int ntiles;
+    const int32_t *order;  // launch order or null
 };
 
 __device__ __forceinline__ void xcd_share(int x, int nt, int &start, int &size) {
@@ -175,18 +179,19 @@ __device__ __forceinline__ int queue_issue(const TileQueue &q) {
     return i;
 }
 
-// First tile and tile count of the grab `own` (from queue_issue), or of a
-// grab taken from another XCD's share when the own share is done; -1 when
-// every share is taken.  Wave-uniform results.
-__device__ __forceinline__ int queue_resolve(const TileQueue &q, int own, int &count) {
-    int t = -1, c = 0;
+// First item and item count of the grab `own` (from queue_issue), or of a
+// grab taken from another XCD's share when the own share is done, and the
+// share's XCD; -1 when every share is taken.  Wave-uniform results.
+__device__ __forceinline__ int queue_resolve(const TileQueue &q, int own, int &count, int &xcd) {
+    int t = -1, c = 0, xs = 0;
     if ((threadIdx.x & (LIG_WAVE - 1)) == 0) {
         const int x0 = blockIdx.x & 7;
         int start, size;
         xcd_share(x0, q.ntiles, start, size);
         if (own < size) {
-            t = start + own;
+            t = own;
             c = min(kQueueGrab, size - own);
+            xs = x0;
         } else {
             for (int k = 1; k < 8; ++k) {
                 const int x = (x0 + k) & 7;
@@ -196,15 +201,25 @@ __device__ __forceinline__ int queue_resolve(const TileQueue &q, int own, int &c
                     continue;  // share already taken
                 const int i = atomicAdd(ctr, kQueueGrab);
                 if (i < size) {
-                    t = start + i;
+                    t = i;
                     c = min(kQueueGrab, size - i);
+                    xs = x;
                     break;
                 }
             }
         }
     }
     count = __builtin_amdgcn_readfirstlane(c);
+    xcd = __builtin_amdgcn_readfirstlane(xs);
     return __builtin_amdgcn_readfirstlane(t);
+}
+
+// Tile of item i of XCD x's share.
+__device__ __forceinline__ int queue_tile(const TileQueue &q, int x, int i) {
+    if (q.order && q.order[0] >= 0) return q.order[8 * i + x];
+    int start, size;
+    xcd_share(x, q.ntiles, start, size);
+    return start + i;
 }
 
 // Per-Gaussian record the rasteriser kernels read (64 bytes, 16-byte aligned):

@@ -526,15 +526,15 @@ __global__ __launch_bounds__(kThreads) void raster_fwd16_queue_kernel(
     __shared__ float4 s0[DEPTH];
     __shared__ float4 s1[DEPTH];
     __shared__ float s2[HAS_OPAC ? DEPTH : 1];
-    int cnt;
-    int t = queue_resolve(queue, queue_issue(queue), cnt);
-    while (t >= 0) {
+    int cnt, x;
+    int i = queue_resolve(queue, queue_issue(queue), cnt, x);
+    while (i >= 0) {
         const int pending = queue_issue(queue);
         for (int k = 0; k < cnt; ++k)
-            fwd16_tile<HAS_OPAC, TRAIN, DEPTH>(t + k, tiles_x, img_w, img_h, gids, bins, pack,
-                                               out_img, out_wsum, final_idx, target,
-                                               loss_partial, tab, nimg, cum, s0, s1, s2);
-        t = queue_resolve(queue, pending, cnt);
+            fwd16_tile<HAS_OPAC, TRAIN, DEPTH>(queue_tile(queue, x, i + k), tiles_x, img_w, img_h,
+                                               gids, bins, pack, out_img, out_wsum, final_idx,
+                                               target, loss_partial, tab, nimg, cum, s0, s1, s2);
+        i = queue_resolve(queue, pending, cnt, x);
     }
 }
 
@@ -1287,18 +1287,18 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_fit_queue_kernel(
     GradRec *__restrict__ rec, const BatchImage *__restrict__ tab, int nimg, TileQueue queue) {
     __shared__ float4 sv[kMaxPix];
     __shared__ int sfin[kFitLdsInts];  // row-group sort
-    int cnt;
-    int t = queue_resolve(queue, queue_issue(queue), cnt);
-    while (t >= 0) {
+    int cnt, x;
+    int i = queue_resolve(queue, queue_issue(queue), cnt, x);
+    while (i >= 0) {
         const int pending = queue_issue(queue);
         for (int k = 0; k < cnt; ++k) {
-            bwd16_tile<false, true, true>(t + k, 0, 1, 1, 1, 1, gids, nullptr, bins, pack, nullptr,
-                                          nullptr, nullptr, img, target, grad_loss, 0.0f, rec,
-                                          nullptr, nullptr, nullptr, nullptr, nullptr, tab, nimg,
-                                          sv, sfin);
+            bwd16_tile<false, true, true>(queue_tile(queue, x, i + k), 0, 1, 1, 1, 1, gids,
+                                          nullptr, bins, pack, nullptr, nullptr, nullptr, img,
+                                          target, grad_loss, 0.0f, rec, nullptr, nullptr, nullptr,
+                                          nullptr, nullptr, tab, nimg, sv, sfin);
             __syncthreads();  // the next tile restages sv
         }
-        t = queue_resolve(queue, pending, cnt);
+        i = queue_resolve(queue, pending, cnt, x);
     }
 }
 
@@ -1828,7 +1828,7 @@ void launch_fit_forward(int depth, dim3 grid, hipStream_t st, int tiles_x, int i
                         const int32_t *gids, const int2 *bins, const GaussPack *pk,
                         float *out_img, const float *target, float *partial,
                         const BatchImage *tab, int nimg, const int32_t *cum = nullptr,
-                        TileQueue queue = TileQueue{nullptr, 0},
+                        TileQueue queue = TileQueue{nullptr, 0, nullptr},
                         const int32_t *order = nullptr) {
 #define LIG_FIT_FWD(D)                                                                        \
     if (queue.ctr) {                                                                          \
@@ -2006,7 +2006,7 @@ extern "C" int lig_fit_render_l2(int n, int img_height, int img_width,
                                              reinterpret_cast<char *>(workspace) + ooff, st);
     launch_fit_forward(batch_depth, dim3((unsigned)tiles), st, tiles_x, img_width, img_height,
                        gaussian_ids_sorted, bins, pk, out_img, target, partial, nullptr, 0,
-                       nullptr, TileQueue{nullptr, 0}, order);
+                       nullptr, TileQueue{nullptr, 0, nullptr}, order);
     const double numel = 3.0 * (double)img_height * (double)img_width;
     hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)nsum), dim3(kSumThreads), 0, st, tiles,
                        (const float *)partial, bsum);
@@ -2046,16 +2046,19 @@ int fit_render_batched(int n, const int32_t *gaussian_ids_sorted, const int32_t 
     float *partial = reinterpret_cast<float *>(workspace);
     char *after_partials =
         reinterpret_cast<char *>(workspace) + (((size_t)num_tiles * sizeof(float) + 255) & ~size_t(255));
-    TileQueue q{nullptr, num_tiles};
-    const int32_t *order = nullptr;
+    TileQueue q{nullptr, num_tiles, nullptr};
+    // launch order (heaviest tiles first where it matters): the grid's block
+    // order, or the order in which the queue deals tiles to the XCDs
+    const int32_t *order =
+        launch_tile_order(num_tiles, reinterpret_cast<const int2 *>(tile_bins), num_tiles,
+                          fit_forward_resident(batch_depth),
+                          after_partials + kQueueInts * sizeof(int), st);
     if (persistent) {
         q.ctr = reinterpret_cast<int *>(after_partials);
+        q.order = order;
+        order = nullptr;
         hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(kQueueInts, 256)), dim3(256), 0, st,
                            reinterpret_cast<float *>(q.ctr), (size_t)kQueueInts);
-    } else {
-        order = launch_tile_order(num_tiles, reinterpret_cast<const int2 *>(tile_bins),
-                                  num_tiles, fit_forward_resident(batch_depth),
-                                  after_partials + kQueueInts * sizeof(int), st);
     }
     launch_fit_forward(batch_depth, dim3((unsigned)num_tiles), st, 1, 1, 1, gaussian_ids_sorted,
                        reinterpret_cast<const int2 *>(tile_bins), pk, out_img, target, partial,
@@ -2128,7 +2131,7 @@ extern "C" int lig_render_batched(int n, const int32_t *gaussian_ids_sorted,
                           after_partials + kQueueInts * sizeof(int), st);
     launch_fit_forward(batch_depth, dim3((unsigned)num_tiles), st, 1, 1, 1, gaussian_ids_sorted,
                        bins, pk, out_img, nullptr, nullptr, tab, num_images, cum_tiles_hit,
-                       TileQueue{nullptr, num_tiles}, order);
+                       TileQueue{nullptr, num_tiles, nullptr}, order);
     return launch_status();
 }
 
@@ -2322,8 +2325,14 @@ int fit_backward_batched(int n, const int32_t *gaussian_ids_sorted, const int32_
     // when that slack holds them (else the one-tile-per-block launch runs)
     const size_t qoff = (256 + (size_t)num_intersects * sizeof(FitRec) + 255) & ~size_t(255);
     if (persistent && qoff + kQueueInts * sizeof(int) <= workspace_bytes) {
-        TileQueue q{reinterpret_cast<int *>(reinterpret_cast<char *>(workspace) + qoff),
-                    num_tiles};
+        char *ctr = reinterpret_cast<char *>(workspace) + qoff;
+        TileQueue q{reinterpret_cast<int *>(ctr), num_tiles, nullptr};
+        // the queue's dealing order after the counters, when the slack holds it
+        const size_t ooff = qoff + kQueueInts * sizeof(int);
+        if (ooff + order_ws_bytes(num_tiles) <= workspace_bytes)
+            q.order = launch_tile_order(num_tiles, bins, num_tiles,
+                                        resident_blocks(raster_bwd16_fit_queue_kernel),
+                                        ctr + kQueueInts * sizeof(int), st);
         hipLaunchKernelGGL(zero_f32_kernel, dim3(ceil_div(kQueueInts, 256)), dim3(256), 0, st,
                            reinterpret_cast<float *>(q.ctr), (size_t)kQueueInts);
         hipLaunchKernelGGL(raster_bwd16_fit_queue_kernel,
