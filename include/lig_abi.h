@@ -195,7 +195,8 @@ int lig_pack_gaussians(int n, const float *xys, const float *conics, const float
  * `packed` also carries each Gaussian's record addressing, and the backward
  * (lig_rasterize_backward_records) then needs no emission ids.
  * out_wsum and final_idx may both be NULL (block_width 16, no out_dx): an
- * image-only render, e.g. for serving. */
+ * image-only render, e.g. for serving; final_idx alone may be NULL (the same
+ * conditions) for a backward that takes none (lig_rasterize_backward). */
 int lig_rasterize_forward(int n, int img_height, int img_width, int block_width,
                           const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,
                           const float *xys, const float *conics, const float *colors,
@@ -239,7 +240,14 @@ int lig_rasterize_forward_ws(int n, int img_height, int img_width, int block_wid
  *     workspace (lig_rasterize_backward_workspace_size(num_intersects) bytes)
  *     and reduced per Gaussian in a fixed order that depends only on its own
  *     records (chunks of 16 slots from its first, each summed by a fixed
- *     tree, chunk sums in order): bit-reproducible, no atomics. */
+ *     tree, chunk sums in order): bit-reproducible, no atomics.
+ * final_idx may be NULL (block_width 16, no out_dx gradients) when it would
+ * be the forward's: in sum mode the final index only marks the last pair the
+ * forward accepted, and the backward evaluates the same accept test on the
+ * same FP32 values, so no pixel needs gating; each tile's Gaussians are then
+ * grouped by the tile rows they reach (as in the fused fit step).  The
+ * results equal those given the forward's final index (bit for bit with
+ * records).  The same holds for lig_rasterize_backward_records. */
 size_t lig_rasterize_backward_workspace_size(int num_intersects);
 int lig_rasterize_backward(int n, int img_height, int img_width, int block_width,
                            const int32_t *gaussian_ids_sorted, const int32_t *tile_bins,

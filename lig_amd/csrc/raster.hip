@@ -319,7 +319,9 @@ __device__ __forceinline__ uint32_t reach_rows(float mx, float my, float ap, flo
 // DEPTH (64, 128, 256): Gaussians staged in LDS per batch, DEPTH / 64 per lane;
 // deeper batches keep more gathers in flight per wave and halve or quarter the
 // barriers (chosen per workload by lig_amd.autotune).
-template <bool HAS_OPAC, bool TRAIN, int DEPTH>
+// FIN (API mode): write the final index; without it (a backward that needs
+// none) the forward writes the image and wsum only.
+template <bool HAS_OPAC, bool TRAIN, int DEPTH, bool FIN = true>
 __device__ __forceinline__ void fwd16_tile(
     int gtile, int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
@@ -455,7 +457,7 @@ __device__ __forceinline__ void fwd16_tile(
                     ab[q] = __builtin_fmaf(al, blue, ab[q]);
                     if constexpr (!TRAIN) {
                         aw[q] += al;
-                        fin[q] = ok ? kidx : fin[q];
+                        if constexpr (FIN) fin[q] = ok ? kidx : fin[q];
                     }
                 }
             }
@@ -480,7 +482,7 @@ __device__ __forceinline__ void fwd16_tile(
                             clamped_sq_err(ab[q], target[3 * pid + 2]);
             } else {
                 out_wsum[pid] = aw[q];
-                final_idx[pid] = fin[q];
+                if constexpr (FIN) final_idx[pid] = fin[q];
             }
         }
     }
@@ -495,7 +497,7 @@ __device__ __forceinline__ void fwd16_tile(
 
 // One tile per block: the tile order[blockIdx.x], or without an order (none
 // given, or order[0] = -1) the tile xcd_remap(blockIdx.x).
-template <bool HAS_OPAC, bool TRAIN, int DEPTH = kThreads>
+template <bool HAS_OPAC, bool TRAIN, int DEPTH = kThreads, bool FIN = true>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(TRAIN && DEPTH <= 128 ? 8 : 1))) void raster_fwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
@@ -508,9 +510,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(TRAIN 
     __shared__ float s2[HAS_OPAC ? DEPTH : 1];  // blue (with opacities)
     const int gtile = order && order[0] >= 0 ? order[blockIdx.x]
                                              : (int)xcd_remap(blockIdx.x, gridDim.x);
-    fwd16_tile<HAS_OPAC, TRAIN, DEPTH>(gtile, tiles_x, img_w,
-                                       img_h, gids, bins, pack, out_img, out_wsum, final_idx,
-                                       target, loss_partial, tab, nimg, cum, s0, s1, s2);
+    fwd16_tile<HAS_OPAC, TRAIN, DEPTH, FIN>(gtile, tiles_x, img_w, img_h, gids, bins, pack,
+                                            out_img, out_wsum, final_idx, target, loss_partial,
+                                            tab, nimg, cum, s0, s1, s2);
 }
 
 // Persistent form: each wave takes tiles from the per-XCD queue until none is
@@ -557,7 +559,7 @@ __device__ __forceinline__ void stage_tile_grads(
             const size_t pid = (size_t)i * img_w + j;
             sv[p] = make_float4(v_out[3 * pid], v_out[3 * pid + 1], v_out[3 * pid + 2],
                                 v_wsum ? v_wsum[pid] : 0.0f);
-            sfin[p] = final_idx[pid];
+            if (sfin) sfin[p] = final_idx[pid];
             if constexpr (GA) {
                 sa[p] = make_float4(v_dx[3 * pid], v_dx[3 * pid + 1], v_dx[3 * pid + 2],
                                     v_dy[3 * pid]);
@@ -567,7 +569,7 @@ __device__ __forceinline__ void stage_tile_grads(
             }
         } else {
             sv[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            sfin[p] = INT32_MIN;
+            if (sfin) sfin[p] = INT32_MIN;
             if constexpr (GA) {
                 sa[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 sb[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -972,7 +974,9 @@ template <bool HAS_OPAC>
 __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const int32_t *__restrict__ gids,
                                               const GaussPack *__restrict__ pack, int x_org,
                                               int y_org, int tx, int ty,
-                                              FitRec *__restrict__ rec, uint32_t *cnt,
+                                              FitRec *__restrict__ rec,
+                                              GradRec *__restrict__ grec,
+                                              const int32_t *__restrict__ slots, uint32_t *cnt,
                                               uint16_t *ord) {
     const int lane = threadIdx.x;
     for (int w = lane; w < kFitCntWords; w += kThreads) cnt[w] = 0u;
@@ -994,14 +998,26 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
                 key = ((uint32_t)__builtin_ctz(m) << 4) | (31u - (uint32_t)__builtin_clz(m));
                 LIG_CHECK((key >> 1) < (uint32_t)kFitCntWords && (key >> 4) <= (key & 15u));
                 atomicAdd(&cnt[key >> 1], 1u << (16 * (key & 1u)));
-            } else {
+            } else if (rec || grec) {
+                // the Gaussian's record for this tile, all zeros (the API
+                // record carries its gid: a zero-extent slot's is gid 0)
                 const float4 p3 = pack[g].p3;
-                const int slot = __float_as_int(p3.z) + ty * __float_as_int(p3.w) + tx;
+                const int slot = slots ? slots[c0 + i]
+                                       : __float_as_int(p3.z) + ty * __float_as_int(p3.w) + tx;
                 LIG_CHECK(slot >= 0);
-                FitRec z;
-                z.r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                z.r1 = z.r0;
-                rec[slot] = z;
+                const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (rec) {
+                    FitRec z;
+                    z.r0 = z4;
+                    z.r1 = z4;
+                    rec[slot] = z;
+                } else {
+                    GradRec z;
+                    z.r0 = z4;
+                    z.r1 = z4;
+                    z.r2 = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(g));
+                    grec[slot] = z;
+                }
             }
         }
         kw |= (uint64_t)key << (8 * k);
@@ -1085,7 +1101,13 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
 // rejects anyway (the forward and this kernel evaluate the same test on the
 // same FP32 values), so every in-image pixel is evaluated and the records are
 // those of the gated path.
-template <bool HAS_OPAC, bool DET, bool TRAIN>
+// GROUP (API backward without a final index, !TRAIN): the pairs the forward
+// accepted are exactly the ones the accept test accepts here -- in sum mode
+// the forward's final index only marks the last accepted pair, and both
+// kernels evaluate the same test on the same FP32 values -- so no pixel is
+// gated, and each chunk of the tile's Gaussians is grouped by the rows they
+// reach, as in the fit step; every record keeps its bits.
+template <bool HAS_OPAC, bool DET, bool TRAIN, bool GROUP = false>
 __device__ __forceinline__ void bwd16_tile(
     int gtile, int part, int nparts, int tiles_x, int img_w, int img_h,
     const int32_t *__restrict__ gids,
@@ -1128,9 +1150,11 @@ __device__ __forceinline__ void bwd16_tile(
         stage_tile_loss_grads(x_org, y_org, img_w, img_h, img, target, grad_loss[0], norm, sv);
     else
         stage_tile_grads<false>(16, x_org, y_org, img_w, img_h, final_idx, v_out, v_wsum,
-                                nullptr, nullptr, nullptr, sv, sfin, nullptr, nullptr, nullptr);
-    // in-image rows and columns of the tile (TRAIN)
-    const int nrows = TRAIN ? min(16, img_h - y_org) : 16;
+                                nullptr, nullptr, nullptr, sv, GROUP ? nullptr : sfin, nullptr,
+                                nullptr, nullptr);
+    // in-image rows and columns of the tile (TRAIN, GROUP; the final index
+    // excludes the others)
+    const int nrows = (TRAIN || GROUP) ? min(16, img_h - y_org) : 16;
     const uint32_t cols = img_w - x_org >= 16 ? 0xffffu : ((1u << (img_w - x_org)) - 1u);
 
     // one batch: lanes act (Gaussian g, list position kidx) over the rows in
@@ -1169,7 +1193,7 @@ __device__ __forceinline__ void bwd16_tile(
             rows &= rows - 1u;
             int fcol = 0;
             uint32_t any = cols, full = cols;
-            if constexpr (!TRAIN) {
+            if constexpr (!TRAIN && !GROUP) {
                 fcol = sfin[ly * 16 + (lane & 15)];
                 any = (uint32_t)__ballot(fcol >= base) & 0xffffu;
                 if (any == 0) continue;
@@ -1214,25 +1238,29 @@ __device__ __forceinline__ void bwd16_tile(
                 reinterpret_cast<FitRec *>(rec)[slot] = r;
             }
         } else {
-            const int slot = DET && act ? record_slot(slots, base + lane, gp, tx, ty) : 0;
+            // kidx: the lane's list position (its emission slot's index)
+            const int slot = DET && act ? record_slot(slots, kidx, gp, tx, ty) : 0;
             plain_emit<HAS_OPAC, DET>(A, gp, act, slot, g, rec, g_xy, g_xy_abs, g_conic, g_rgb,
                                       g_opac);
         }
     };
     const uint32_t all_rows = (1u << nrows) - 1u;
-    if constexpr (TRAIN) {
+    if constexpr (TRAIN || GROUP) {
         uint32_t *cnt = reinterpret_cast<uint32_t *>(sfin);
         uint16_t *ord = reinterpret_cast<uint16_t *>(sfin + kFitCntWords);
         for (int c0 = range.x; c0 < range.y; c0 += kFitChunk) {
             const int cn = min(kFitChunk, range.y - c0);
-            const int m = fit_chunk_sort<HAS_OPAC>(c0, cn, gids, pack, x_org, y_org, tx, ty,
-                                                   reinterpret_cast<FitRec *>(rec), cnt, ord);
+            const int m = fit_chunk_sort<HAS_OPAC>(
+                c0, cn, gids, pack, x_org, y_org, tx, ty,
+                TRAIN ? reinterpret_cast<FitRec *>(rec) : nullptr,
+                !TRAIN && DET ? rec : nullptr, TRAIN ? nullptr : slots, cnt, ord);
             for (int b = 0; b < m; b += kThreads) {
                 const bool act = b + lane < m;
-                const int g = act ? gids[c0 + (int)ord[b + lane]] : 0;
+                const int k = act ? c0 + (int)ord[b + lane] : 0;
+                const int g = act ? gids[k] : 0;
                 const uint32_t rows = (uint32_t)__builtin_amdgcn_readfirstlane(
                                           (int)cnt[b / kThreads]) & all_rows;
-                run_batch(b, act, INT32_MAX, g, rows);
+                run_batch(b, act, TRAIN ? INT32_MAX : k, g, rows);
             }
             wave_lds_sync();  // the next chunk rewrites cnt and ord
         }
@@ -1249,7 +1277,7 @@ __device__ __forceinline__ void bwd16_tile(
 // b takes share b % split of tile order[b / split], or without an order (none
 // given, or order[0] = -1), with u = xcd_remap(b), share u % split of tile
 // u / split.
-template <bool HAS_OPAC, bool DET, bool TRAIN>
+template <bool HAS_OPAC, bool DET, bool TRAIN, bool GROUP = false>
 __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
@@ -1261,7 +1289,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     float *__restrict__ g_rgb, float *__restrict__ g_opac, const BatchImage *__restrict__ tab,
     int nimg, int split, const int32_t *__restrict__ order) {
     __shared__ float4 sv[kMaxPix];
-    __shared__ int sfin[TRAIN ? kFitLdsInts : kMaxPix];  // TRAIN: row-group sort
+    __shared__ int sfin[(TRAIN || GROUP) ? kFitLdsInts : kMaxPix];  // row-group sort
     int gtile, part;
     if (order && order[0] >= 0) {  // the shares of a tile are consecutive blocks
         const int k = (int)blockIdx.x / split;
@@ -1272,7 +1300,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
         gtile = u / split;
         part = u - gtile * split;
     }
-    bwd16_tile<HAS_OPAC, DET, TRAIN>(gtile, part, split, tiles_x, img_w, img_h,
+    bwd16_tile<HAS_OPAC, DET, TRAIN, GROUP>(gtile, part, split, tiles_x, img_w, img_h,
                                      gids, slots, bins, pack, final_idx, v_out, v_wsum, img,
                                      target, grad_loss, norm, rec, g_xy, g_xy_abs, g_conic, g_rgb,
                                      g_opac, tab, nimg, sv, sfin);
@@ -1869,10 +1897,12 @@ int rasterize_forward_impl(int n, int img_height, int img_width, int block_width
     if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
         return LIG_ERR_INVALID_ARGUMENT;
     const bool ga = out_dx != nullptr;
-    // image only (no wsum / final_idx): 16 x 16 tiles, plain forward
+    // image only (no wsum / final_idx): 16 x 16 tiles, plain forward; wsum
+    // without the final index: 16 x 16, plain
     const bool image_only = !out_wsum && !final_idx;
-    if (!tile_bins || !out_img || (!image_only && (!out_wsum || !final_idx)) ||
-        (image_only && (ga || block_width != 16)))
+    const bool no_final = out_wsum && !final_idx;
+    if (!tile_bins || !out_img || (!out_wsum && final_idx) ||
+        ((image_only || no_final) && (ga || block_width != 16)))
         return LIG_ERR_INVALID_ARGUMENT;
     if (ga && (!out_dy || !out_dxy)) return LIG_ERR_INVALID_ARGUMENT;
     if (n > 0 && (!gaussian_ids_sorted || !xys || !conics || !colors || !packed))
@@ -1916,6 +1946,16 @@ int rasterize_forward_impl(int n, int img_height, int img_width, int block_width
         hipLaunchKernelGGL((raster_fwd16_kernel<false, true>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
                            nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, order);
+    else if (no_final && opacities)
+        hipLaunchKernelGGL((raster_fwd16_kernel<true, false, kThreads, false>), grid,
+                           dim3(kThreads), 0, st, tiles_x, img_width, img_height,
+                           gaussian_ids_sorted, bins, pk, out_img, out_wsum, nullptr, nullptr,
+                           nullptr, nullptr, 0, nullptr, order);
+    else if (no_final)
+        hipLaunchKernelGGL((raster_fwd16_kernel<false, false, kThreads, false>), grid,
+                           dim3(kThreads), 0, st, tiles_x, img_width, img_height,
+                           gaussian_ids_sorted, bins, pk, out_img, out_wsum, nullptr, nullptr,
+                           nullptr, nullptr, 0, nullptr, order);
     else if (block_width == 16 && opacities)
         hipLaunchKernelGGL((raster_fwd16_kernel<true, false>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
@@ -2148,7 +2188,15 @@ namespace lig {
 namespace {
 
 // bwd16_split for the API backward's instantiation.
-int bwd16_split_for(int has_opacities, bool det, int ntiles, long long nisect) {
+int bwd16_split_for(int has_opacities, bool det, bool group, int ntiles, long long nisect) {
+    if (group) {
+        if (has_opacities)
+            return det ? bwd16_split(raster_bwd16_kernel<true, true, false, true>, ntiles, nisect)
+                       : bwd16_split(raster_bwd16_kernel<true, false, false, true>, ntiles,
+                                     nisect);
+        return det ? bwd16_split(raster_bwd16_kernel<false, true, false, true>, ntiles, nisect)
+                   : bwd16_split(raster_bwd16_kernel<false, false, false, true>, ntiles, nisect);
+    }
     if (has_opacities)
         return det ? bwd16_split(raster_bwd16_kernel<true, true, false>, ntiles, nisect)
                    : bwd16_split(raster_bwd16_kernel<true, false, false>, ntiles, nisect);
@@ -2167,11 +2215,14 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
                           hipStream_t st, void *order_ws = nullptr, size_t order_ws_len = 0) {
     const bool ga = v_output_dx != nullptr;
     const bool det = rec != nullptr;
+    // no final index (16 x 16, plain): row-grouped, ungated
+    const bool group = final_idx == nullptr;
     const int tiles_x = (img_width + block_width - 1) / block_width;
     const int tiles_y = (img_height + block_width - 1) / block_width;
     const dim3 grid((unsigned)(tiles_x * tiles_y));
     const int split = block_width == 16 && !ga
-                          ? bwd16_split_for(has_opacities, det, tiles_x * tiles_y, num_intersects)
+                          ? bwd16_split_for(has_opacities, det, group, tiles_x * tiles_y,
+                                            num_intersects)
                           : 1;
     const dim3 grid16((unsigned)(tiles_x * tiles_y * split));
     // 16 x 16 plain backward: launch order in the caller's spare workspace
@@ -2179,10 +2230,15 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
     if (block_width == 16 && !ga && order_ws &&
         order_ws_len >= order_ws_bytes(tiles_x * tiles_y)) {
         const long long resident =
-            has_opacities ? (det ? resident_blocks(raster_bwd16_kernel<true, true, false>)
-                                 : resident_blocks(raster_bwd16_kernel<true, false, false>))
-                          : (det ? resident_blocks(raster_bwd16_kernel<false, true, false>)
-                                 : resident_blocks(raster_bwd16_kernel<false, false, false>));
+            group ? (has_opacities
+                         ? (det ? resident_blocks(raster_bwd16_kernel<true, true, false, true>)
+                                : resident_blocks(raster_bwd16_kernel<true, false, false, true>))
+                         : (det ? resident_blocks(raster_bwd16_kernel<false, true, false, true>)
+                                : resident_blocks(raster_bwd16_kernel<false, false, false, true>)))
+            : has_opacities ? (det ? resident_blocks(raster_bwd16_kernel<true, true, false>)
+                                   : resident_blocks(raster_bwd16_kernel<true, false, false>))
+                            : (det ? resident_blocks(raster_bwd16_kernel<false, true, false>)
+                                   : resident_blocks(raster_bwd16_kernel<false, false, false>));
         order = launch_tile_order(tiles_x * tiles_y, reinterpret_cast<const int2 *>(tile_bins),
                                   (long long)tiles_x * tiles_y * split, resident, order_ws, st);
     }
@@ -2206,6 +2262,15 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
         else
             hipLaunchKernelGGL(raster_bwd_ga_kernel<false>, grid, dim3(kThreads), 0, st,
                                LIG_BWD_ARGS_GA);
+    } else if (group) {
+        // 16 x 16 without a final index: row-grouped, ungated
+        const int sel = (has_opacities ? 2 : 0) | (det ? 1 : 0);
+        switch (sel) {
+            case 0: hipLaunchKernelGGL((raster_bwd16_kernel<false, false, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 1: hipLaunchKernelGGL((raster_bwd16_kernel<false, true, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 2: hipLaunchKernelGGL((raster_bwd16_kernel<true, false, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            default: hipLaunchKernelGGL((raster_bwd16_kernel<true, true, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+        }
     } else {
         // plain: (opacities, deterministic, 16x16) -> one of eight instantiations
         const int sel = (has_opacities ? 4 : 0) | (det ? 2 : 0) | (block_width == 16 ? 1 : 0);
@@ -2233,7 +2298,9 @@ bool bwd_args_ok(int n, int img_height, int img_width, int block_width, const in
     if (n < 0 || img_height <= 0 || img_width <= 0 || block_width < 1 || block_width > 16)
         return false;
     if (v_output_dx && (!v_output_dy || !v_output_dxy)) return false;
-    if (n > 0 && (!gids || !tile_bins || !packed || !final_idx || !v_output)) return false;
+    if (n > 0 && (!gids || !tile_bins || !packed || !v_output)) return false;
+    // no final index: 16 x 16 plain only
+    if (n > 0 && !final_idx && (block_width != 16 || v_output_dx)) return false;
     return true;
 }
 

@@ -382,13 +382,15 @@ def pack_gaussians(xys, conics, colors, opacities):
 
 def rasterize_forward_packed(img_height, img_width, block_width, gaussian_ids_sorted, tile_bins,
                              xys, conics, colors, opacities, compute_upscale_gradients,
-                             extents=None, cum_tiles_hit=None, ordered=True):
+                             extents=None, cum_tiles_hit=None, ordered=True, final_index=True):
     """Forward that also returns the packed records for the backward:
     (out_img, out_wsum, out_dx, out_dy, out_dxy, final_idx, packed).  With
     extents and cum_tiles_hit (16 x 16 tiles, no opacities) the pack also
     addresses the backward's records, which then need no emission ids.
     ordered=False launches the tiles in the fixed per-XCD order (same
-    results; the launch order only changes speed)."""
+    results; the launch order only changes speed).  final_index=False (16 x
+    16, no upscale gradients) skips the final index (final_idx is None): the
+    backward given None re-derives the accepted pairs with the same test."""
     for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),
                       ("tile_bins", tile_bins, _i32), ("xys", xys, _f32),
                       ("conics", conics, _f32), ("colors", colors, _f32),
@@ -400,7 +402,9 @@ def rasterize_forward_packed(img_height, img_width, block_width, gaussian_ids_so
     H, W = int(img_height), int(img_width)
     out_img = _empty((H, W, 3), xys)
     out_wsum = _empty((H, W), xys)
-    final_idx = _empty((H, W), xys, _i32)
+    if not final_index and (int(block_width) != 16 or compute_upscale_gradients):
+        raise ValueError("final_index=False needs 16 x 16 tiles without upscale gradients")
+    final_idx = _empty((H, W), xys, _i32) if final_index else None
     dx = dy = dxy = None
     if compute_upscale_gradients:
         dx, dy, dxy = (_empty((H, W, 3), xys) for _ in range(3))
@@ -426,7 +430,10 @@ def rasterize_backward_packed(img_height, img_width, block_width, gaussian_ids_s
     slots_in_pack (packed by a forward given extents and cum_tiles_hit) the
     per-Gaussian sums are reduced deterministically; otherwise by atomics.
     The deterministic form orders its tiles' launch in spare workspace unless
-    ordered=False (same results)."""
+    ordered=False (same results).  final_idx None (16 x 16, no upscale
+    gradients, after a forward with final_index=False): no pixel is gated by
+    a final index and each tile's Gaussians are grouped by the rows they
+    reach; the results equal those with the forward's final index."""
     v_output, v_render_wsum, v_output_dx, v_output_dy, v_output_dxy = map(
         _contig, (v_output, v_render_wsum, v_output_dx, v_output_dy, v_output_dxy))
     for n_, t, dt in (("gaussian_ids_sorted", gaussian_ids_sorted, _i32),

@@ -86,12 +86,16 @@ class _RasterizeGaussians(Function):
                 gaussian_ids_sorted, tile_bins, emission_ids = _C.bin_and_sort_fast(
                     num_points, num_intersects, xys, extents, cum_tiles_hit, img_height,
                     img_width, block_width, with_emission_ids=True)
+            # 16 x 16 plain: no final index (the backward re-derives the
+            # forward's accepted pairs with the same test and groups each
+            # tile's Gaussians by the rows they reach)
             (out_img, out_wsum, out_dx, out_dy, out_dxy, final_idx,
              packed) = _C.rasterize_forward_packed(
                 img_height, img_width, block_width, gaussian_ids_sorted, tile_bins, xys, conics,
                 colors, opacities, compute_upscale_gradients,
                 extents=extents if slots_in_pack else None,
-                cum_tiles_hit=cum_tiles_hit if slots_in_pack else None)
+                cum_tiles_hit=cum_tiles_hit if slots_in_pack else None,
+                final_index=not (block_width == 16 and not compute_upscale_gradients))
 
         ctx.img_width = img_width
         ctx.img_height = img_height

@@ -352,3 +352,71 @@ def test_fused_l2_loss_matches_torch(device, shape):
     (lr * 3.0).backward()
     assert abs(loss.item() - lr.item()) <= 1e-6 * max(1.0, lr.item())
     np.testing.assert_allclose(x.grad.cpu().numpy(), xr.grad.numpy(), rtol=1e-5, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["pack", "eids_opac", "dense", "skew_opac"])
+def test_api_backward_without_final_index(device, case):
+    """16 x 16 plain API path without a final index: the forward writes the
+    same image and weight sum without it, and the backward -- no pixel gated,
+    each tile's Gaussians grouped by the rows they reach -- gives the
+    gradients of the backward gated by the forward's final index bit for bit,
+    with records addressed from the pack or through emission ids, with and
+    without opacities, on tiles deeper than one 512-Gaussian sort chunk and on
+    sizes that are not multiples of 16; the atomic reduction agrees within
+    its summation-order rounding."""
+    import lig_amd.gsplat2d as g
+    from lig_amd import fit
+    from lig_amd.gsplat2d import cuda as C
+    H, W, n, scale = {"pack": (100, 130, 900, 6.0), "eids_opac": (96, 80, 1200, 4.0),
+                      "dense": (52, 70, 6000, 8.0), "skew_opac": (200, 150, 8000, 5.0)}[case]
+    op = case.endswith("opac")
+    lv = fit.Level(n, H, W, 31, scale, device=device)
+    with torch.no_grad():
+        if case == "skew_opac":
+            lv.means[: n // 2, 1] *= 0.25
+        xys, ext, con, nt = lv.project()
+    opac = torch.rand(n, device=device, generator=torch.Generator(device=device).manual_seed(4)
+                      ) * 0.9 + 0.05 if op else None
+    num, cum = g.compute_cumulative_intersects(nt)
+    col = lv.rgbs.detach()
+    pack_mode = case in ("pack", "dense")
+    if pack_mode:
+        gids, bins = C.bin_and_sort_fast(n, num, xys, ext, cum, H, W, 16)
+        kw = dict(extents=ext, cum_tiles_hit=cum)
+        bk = dict(cum_tiles_hit=cum, slots_in_pack=True)
+    else:
+        gids, bins, eids = C.bin_and_sort_fast(n, num, xys, ext, cum, H, W, 16,
+                                               with_emission_ids=True)
+        kw = {}
+        bk = dict(emission_ids_sorted=eids, cum_tiles_hit=cum)
+    if case == "dense":
+        depth = int((bins[:, 1] - bins[:, 0]).max())
+        print(f"deepest tile: {depth} Gaussians", flush=True)
+        assert depth > 512
+    f_gate = C.rasterize_forward_packed(H, W, 16, gids, bins, xys, con, col, opac, False, **kw)
+    f_free = C.rasterize_forward_packed(H, W, 16, gids, bins, xys, con, col, opac, False,
+                                        final_index=False, **kw)
+    assert f_free[5] is None
+    assert torch.equal(f_gate[0], f_free[0]) and torch.equal(f_gate[1], f_free[1])
+    gen = torch.Generator(device=device).manual_seed(7)
+    v_out = torch.randn(H, W, 3, device=device, generator=gen)
+    v_w = torch.randn(H, W, device=device, generator=gen)
+    r_gate = C.rasterize_backward_packed(H, W, 16, gids, bins, f_gate[6], n, f_gate[5], v_out,
+                                         v_w, op, **bk)
+    r_free = C.rasterize_backward_packed(H, W, 16, gids, bins, f_free[6], n, None, v_out, v_w,
+                                         op, **bk)
+    names = ("v_xy", "v_xy_abs", "v_conic", "v_colors", "v_opacity")
+    for name, a, b in zip(names, r_gate, r_free):
+        assert (a is None) == (b is None), name
+        if a is not None:
+            assert torch.equal(a, b), (name, float((a - b).abs().max()))
+    assert torch.count_nonzero(r_gate[3]) > 0
+    # atomics (no records): the same sums in another order
+    a_gate = C.rasterize_backward_packed(H, W, 16, gids, bins, f_gate[6], n, f_gate[5], v_out,
+                                         v_w, op)
+    a_free = C.rasterize_backward_packed(H, W, 16, gids, bins, f_free[6], n, None, v_out, v_w,
+                                         op)
+    for name, a, b in zip(names, a_gate, a_free):
+        if a is not None:
+            tol = 1e-4 * max(1.0, float(a.abs().max()))
+            assert float((a - b).abs().max()) <= tol, name
