@@ -244,6 +244,23 @@ constexpr float kAlphaMin = 1.0f / 255.0f;
 // log2(255): alpha >= 1/255 without opacity needs exp2(-s) >= 1/255
 constexpr float kAcceptLog2 = 7.994353436858858f;
 
+// Accept test of a pair without per-Gaussian opacity.  The reference keeps a
+// pair unless sigma < 0 or alpha = min(exp(-sigma), clamp) < 1/255
+// (forward.cu:254-263, backward.cu:105-111); with s = sigma * log2(e) and a
+// clamp above 1/255 that is 0 <= s <= log2(255), decided here on s itself: s
+// in [+0, kAcceptS] with kAcceptS = 0x40ffd1be = 7.99435329..., the largest
+// FP32 value not above log2(255) (exp2(-kAcceptS) * 255 = 1 + 9.9e-8, the
+// next value gives 1 - 2.3e-7), so the cut sits within 2.3e-7 of alpha =
+// 1/255 instead of within the device exp2's rounding of it.  Comparing the bit
+// patterns as unsigned integers takes one instruction and also rejects -0 and
+// NaN (the reference, through fminf(NaN, clamp) = clamp, would keep a NaN
+// pair at full alpha).  Every kernel without opacities uses this test, so the
+// forward and every backward accept the same pairs.
+constexpr uint32_t kAcceptSBits = 0x40ffd1beu;
+__device__ __forceinline__ bool accept_s(float s) {
+    return __float_as_uint(s) <= kAcceptSBits;
+}
+
 // sigma*log2(e) of a pair from the scaled conic (a', b', c'), evaluated with
 // explicit FMAs so that every kernel computes bit-identical values for the
 // same pair (the forward's accept test and the backward's must agree):

@@ -116,7 +116,8 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
     int tiles_x, int bw, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int2 *__restrict__ bins, const GaussPack *__restrict__ pack,
     float *__restrict__ out_img, float *__restrict__ out_wsum, int32_t *__restrict__ final_idx,
-    float *__restrict__ out_dx, float *__restrict__ out_dy, float *__restrict__ out_dxy) {
+    float *__restrict__ out_dx, float *__restrict__ out_dy, float *__restrict__ out_dxy,
+    int has_opac) {
     __shared__ float4 s0[kThreads];
     __shared__ float4 s1[kThreads];
     __shared__ float4 s2[kThreads];
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
                 const float s = pair_s(a0.z, row_terms(a0.w, a1.x, dy), dx);
                 const float e = fast_exp2(-s);
                 float al = fminf(a1.y * e, 0.999f);
-                const bool ok = !(s < 0.0f) && !(al < kAlphaMin);
+                const bool ok = has_opac ? !(s < 0.0f) && !(al < kAlphaMin) : accept_s(s);
                 al = ok ? al : 0.0f;
                 acc_r[k] = __builtin_fmaf(al, a1.z, acc_r[k]);
                 acc_g[k] = __builtin_fmaf(al, a1.w, acc_g[k]);
@@ -361,7 +362,7 @@ __device__ __forceinline__ void fwd16_tile(
 
     // the clamp is a literal operand: fminf against a VGPR the compiler cannot
     // prove canonical would add a quieting v_max before every min
-    const float amin = in_vgpr(kAlphaMin);
+    const float amin = HAS_OPAC ? in_vgpr(kAlphaMin) : kAlphaMin;
     float ar[4], ag[4], ab[4], aw[4];
     int fin[4];
 #pragma unroll
@@ -450,7 +451,7 @@ __device__ __forceinline__ void fwd16_tile(
                     const float s = pair_s(a0.z, rw, a0.x - X[q]);
                     const float e = fast_exp2(-s);
                     float al = fminf(HAS_OPAC ? a1.y * e : e, 0.999f);
-                    const bool ok = !(s < 0.0f) && !(al < amin);
+                    const bool ok = HAS_OPAC ? !(s < 0.0f) && !(al < amin) : accept_s(s);
                     al = ok ? al : 0.0f;
                     ar[q] = __builtin_fmaf(al, a1.z, ar[q]);
                     ag[q] = __builtin_fmaf(al, a1.w, ag[q]);
@@ -816,62 +817,91 @@ __global__ void add_phantom_kernel(const float *__restrict__ phantom, int has_op
 //   v_xy += v_sigma (a dx + b dy, b dx + c dy),  v_xy_abs += |that|,
 //   v_opacity += vis * v_alpha.
 // Moments: within a pixel row dy is fixed and dx = D - lx, so per pixel only
-// q_k = sum v_sigma lx^k (k = 0, 1, 2; lx the column in the tile, a constant of
-// the unrolled pixel loop) are accumulated; at the end of the row they fold
-// into sum v_sigma dx (mx), v_sigma dx^2 (sxx), v_sigma dy (my), v_sigma dy^2
-// (syy) and v_sigma dx dy (sxy).  v_xy is formed once at the end as (a mx + b my,
+// q_k = sum v_sigma lx^k (k = 1, 2; lx the column relative to the tile's
+// centre column c0 = bw / 2, a constant of the unrolled pixel loop, and D =
+// mean.x - (x_org + c0 + 0.5): about the centre the terms D^2 q0, 2 D q1 and
+// q2 that make up sum v_sigma dx^2 stay small next to it, which keeps FP32
+// cancellation down) are accumulated, beside the row's sums of alpha *
+// dL/dI per channel (and of alpha * dL/dW when that gradient can be nonzero,
+// WS).  At the end of the row q0 = sum v_sigma = -(c . those sums + the dL/dW
+// sum) and the moments fold into sum v_sigma dx (mx), v_sigma dx^2 (sxx),
+// v_sigma dy (my), v_sigma dy^2 (syy) and v_sigma dx dy (sxy); the row's
+// colour sums join v_rgb.  v_xy is formed once at the end as (a mx + b my,
 // b mx + c my); v_xy_abs accumulates |v_sigma| |u'| with the scaled conic's
 // u' = K (a dx + b dy) and is divided by K once at the end.
 struct PlainAcc {
     float r, g, b, sxx, sxy, syy, mx, my, ax, ay, o;
-    float q0, q1, q2;  // current row: sum v_sigma * lx^k, k = 0, 1, 2
+    float rr, rg, rb, rw;  // current row: sum alpha * dL/dI (rgb), alpha * dL/dW
+    float q1, q2;          // current row: sum v_sigma * lx^k, k = 1, 2
 };
 
 __device__ __forceinline__ void plain_acc_zero(PlainAcc &A) {
     A.r = A.g = A.b = A.sxx = A.sxy = A.syy = A.mx = A.my = A.ax = A.ay = A.o = 0.0f;
-    A.q0 = A.q1 = A.q2 = 0.0f;
+    A.rr = A.rg = A.rb = A.rw = A.q1 = A.q2 = 0.0f;
 }
 
-// End of a pixel row: with dx = D - lx (D = mean.x - (x_org + 0.5), lx the
-// pixel's column in the tile), sum v_sigma dx = D q0 - q1 and
+// End of a pixel row: q0 = sum v_sigma from the row's sums (v_sigma = -alpha
+// (c . dL/dI + dL/dW)); with dx = D - lx (D = mean.x - (x_org + c0 + 0.5), lx
+// the pixel's column relative to c0), sum v_sigma dx = D q0 - q1 and
 // sum v_sigma dx^2 = D (D q0 - 2 q1) + q2; the dy moments take dy once.
-__device__ __forceinline__ void plain_row_end(PlainAcc &A, float D, float dy) {
-    const float s1 = __builtin_fmaf(D, A.q0, -A.q1);
-    const float s2 = __builtin_fmaf(D, __builtin_fmaf(D, A.q0, -2.0f * A.q1), A.q2);
+template <bool WS>
+__device__ __forceinline__ void plain_row_end(PlainAcc &A, float D, float dy, float cr,
+                                              float cg, float cb) {
+    const float cz = WS ? __builtin_fmaf(cb, A.rb, A.rw) : cb * A.rb;
+    const float q0 = -__builtin_fmaf(cr, A.rr, __builtin_fmaf(cg, A.rg, cz));
+    A.r += A.rr;
+    A.g += A.rg;
+    A.b += A.rb;
+    const float s1 = __builtin_fmaf(D, q0, -A.q1);
+    const float s2 = __builtin_fmaf(D, __builtin_fmaf(D, q0, -2.0f * A.q1), A.q2);
     A.mx += s1;
     A.sxx += s2;
-    A.my = __builtin_fmaf(dy, A.q0, A.my);
+    A.my = __builtin_fmaf(dy, q0, A.my);
     A.sxy = __builtin_fmaf(dy, s1, A.sxy);
-    A.syy = __builtin_fmaf(dy * A.q0, dy, A.syy);
-    A.q0 = A.q1 = A.q2 = 0.0f;
+    A.syy = __builtin_fmaf(dy * q0, dy, A.syy);
+    A.rr = A.rg = A.rb = A.q1 = A.q2 = 0.0f;
+    if constexpr (WS) A.rw = 0.0f;
 }
 
-// One pair of the plain backward.  Rejected pairs (lane test, sigma < 0,
-// alpha < 1/255) leave the accumulators untouched, as the reference's
-// `continue`: the accumulation runs under the lane mask of accepted pairs and
-// is skipped when no lane accepts.  amin = 1/255 comes in a VGPR; clampv =
-// 0.99 is passed as a literal (a VGPR clamp the compiler cannot prove
-// canonical costs a quieting v_max before the min).
-template <bool HAS_OPAC, bool ABS = true>
+// One pair of the plain backward.  Rejected pairs (lane test, the accept
+// test) leave the accumulators untouched, as the reference's `continue`: the
+// accumulation runs under the lane mask of accepted pairs and is skipped when
+// no lane accepts.  Without opacities the test is accept_s on s alone, and the
+// exponential is evaluated inside the branch.  amin = 1/255 comes in a VGPR;
+// clampv = 0.99 is passed as a literal (a VGPR clamp the compiler cannot prove
+// canonical costs a quieting v_max before the min).  WS: dL/dW (w.w) may be
+// nonzero (the API backward); the fit step's is zero.
+template <bool HAS_OPAC, bool ABS = true, bool WS = true>
 __device__ __forceinline__ void plain_pair(PlainAcc &A, bool lane_ok, float ap, float bp,
                                            const RowTerms &rt, float op, float cr, float cg,
                                            float cb, float dx, float lx, float amin,
                                            float clampv, const float4 *__restrict__ wp,
                                            bool col0 = false) {
-    const float4 w = *wp;
     const float s = pair_s(ap, rt, dx);
-    const float vis = fast_exp2(-s);
-    const float al = fminf(HAS_OPAC ? op * vis : vis, clampv);
-    if (lane_ok && !(s < 0.0f) && !(al < amin)) {
+    float vis = 0.0f, al = 0.0f;
+    bool ok;
+    if constexpr (HAS_OPAC) {
+        vis = fast_exp2(-s);
+        al = fminf(op * vis, clampv);
+        ok = lane_ok && !(s < 0.0f) && !(al < amin);
+    } else {
+        ok = lane_ok && accept_s(s);
+    }
+    if (ok) {
+        if constexpr (!HAS_OPAC) {
+            vis = fast_exp2(-s);
+            al = fminf(vis, clampv);
+        }
+        const float4 w = *wp;
         const float va =
             __builtin_fmaf(cr, w.x, __builtin_fmaf(cg, w.y, __builtin_fmaf(cb, w.z, w.w)));
-        A.r = __builtin_fmaf(al, w.x, A.r);
-        A.g = __builtin_fmaf(al, w.y, A.g);
-        A.b = __builtin_fmaf(al, w.z, A.b);
+        A.rr = __builtin_fmaf(al, w.x, A.rr);
+        A.rg = __builtin_fmaf(al, w.y, A.rg);
+        A.rb = __builtin_fmaf(al, w.z, A.rb);
+        if constexpr (WS) A.rw = __builtin_fmaf(al, w.w, A.rw);
         if constexpr (HAS_OPAC) A.o = __builtin_fmaf(vis, va, A.o);
         const float vsig = -al * va;
-        A.q0 += vsig;
-        if (!col0) {  // col0: lx = 0 (known at compile time), no lx moments
+        if (!col0) {  // col0: the centre column (known at compile time), no lx moments
             A.q1 = __builtin_fmaf(vsig, lx, A.q1);
             A.q2 = __builtin_fmaf(vsig, lx * lx, A.q2);
         }
@@ -923,7 +953,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
     stage_tile_grads<false>(bw, x_org, y_org, img_w, img_h, final_idx, v_out, v_wsum, nullptr,
                             nullptr, nullptr, sv, sfin, nullptr, nullptr, nullptr);
 
-    const float amin = in_vgpr(kAlphaMin), clampv = 0.99f;
+    const float amin = HAS_OPAC ? in_vgpr(kAlphaMin) : kAlphaMin, clampv = 0.99f;
     for (int base = range.x; base < range.y; base += kThreads) {
         const bool act = base + lane < range.y;
         const int kidx = act ? base + lane : INT32_MAX;
@@ -931,7 +961,8 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
         const GaussPack gp = pack[g];
         PlainAcc A;
         plain_acc_zero(A);
-        const float D = gp.p0.x - ((float)x_org + 0.5f);
+        const int c0 = bw >> 1;  // moments about the tile's centre column
+        const float D = gp.p0.x - ((float)(x_org + c0) + 0.5f);
         for (int ly = 0; ly < bw; ++ly) {
             const float dy = gp.p0.y - ((float)(y_org + ly) + 0.5f);
             const RowTerms rt = row_terms(gp.p0.w, gp.p1.x, dy);
@@ -941,10 +972,10 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
                 const int fp = sfin[p];
                 if (fp < base) continue;  // wave-uniform: nothing in this batch reaches p
                 plain_pair<HAS_OPAC>(A, kidx <= fp, gp.p0.z, 0.5f * gp.p0.w, rt, gp.p1.y, gp.p1.z,
-                                     gp.p1.w, gp.p2.x, gp.p0.x - pxf, (float)lx, amin, clampv,
-                                     sv + p);
+                                     gp.p1.w, gp.p2.x, gp.p0.x - pxf, (float)(lx - c0), amin,
+                                     clampv, sv + p);
             }
-            plain_row_end(A, D, dy);
+            plain_row_end<true>(A, D, dy, gp.p1.z, gp.p1.w, gp.p2.x);
         }
         plain_emit<HAS_OPAC, DET>(A, gp, act, DET && act ? slots[base + lane] : 0, g, rec, g_xy,
                                   g_xy_abs, g_conic, g_rgb, g_opac);
@@ -1187,7 +1218,7 @@ __device__ __forceinline__ void bwd16_tile(
         PlainAcc A;
         plain_acc_zero(A);
         const int full_from = base + kThreads - 1;
-        const float amin = in_vgpr(kAlphaMin), clampv = 0.99f;
+        const float amin = HAS_OPAC ? in_vgpr(kAlphaMin) : kAlphaMin, clampv = 0.99f;
         while (rows) {
             const int ly = __builtin_ctz(rows);
             rows &= rows - 1u;
@@ -1207,24 +1238,24 @@ __device__ __forceinline__ void bwd16_tile(
             if (full == 0xffffu) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
-                    plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb, dx_of(lx),
-                                                 (float)lx, amin, clampv, row + lx, lx == 0);
+                    plain_pair<HAS_OPAC, !TRAIN, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb, dx_of(lx),
+                                                 (float)(lx - 8), amin, clampv, row + lx, lx == 8);
             } else if (full == any) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
-                        plain_pair<HAS_OPAC, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb,
-                                                     dx_of(lx), (float)lx, amin, clampv, row + lx,
-                                                     lx == 0);
+                        plain_pair<HAS_OPAC, !TRAIN, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb,
+                                                     dx_of(lx), (float)(lx - 8), amin, clampv,
+                                                     row + lx, lx == 8);
             } else {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
                         plain_pair<HAS_OPAC>(A, kidx <= __builtin_amdgcn_readlane(fcol, lx), ap,
-                                             bp, rt, op, cr, cg, cb, dx_of(lx), (float)lx, amin,
-                                             clampv, row + lx, lx == 0);
+                                             bp, rt, op, cr, cg, cb, dx_of(lx), (float)(lx - 8),
+                                             amin, clampv, row + lx, lx == 8);
             }
-            plain_row_end(A, dx_of(0), dy);
+            plain_row_end<!TRAIN>(A, dx_of(8), dy, cr, cg, cb);
         }
         if constexpr (TRAIN) {
             if (act) {
@@ -1396,7 +1427,8 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_ga_kernel(
                 const float s = pair_s(ap, rt, dx);
                 const float vis = fast_exp2(-s);
                 const float al0 = fminf(op * vis, kClamp);
-                const bool ok = kidx <= fp && !(s < 0.0f) && !(al0 < kAlphaMin);
+                const bool ok =
+                    kidx <= fp && (has_opac ? !(s < 0.0f) && !(al0 < kAlphaMin) : accept_s(s));
                 const float al = ok ? al0 : 0.0f;
                 const float vs_ = ok ? vis : 0.0f;
                 const float4 w = sv[p];
@@ -1694,23 +1726,27 @@ __global__ void zero_u64_kernel(unsigned long long *p, int count) {
 template <bool GA>
 void launch_fwd(int px, dim3 grid, hipStream_t st, int tiles_x, int bw, int img_w, int img_h,
                 const int32_t *gids, const int2 *bins, const GaussPack *pack, float *img,
-                float *wsum, int32_t *fin, float *dx, float *dy, float *dxy) {
+                float *wsum, int32_t *fin, float *dx, float *dy, float *dxy, int has_opac) {
     switch (px) {
         case 1:
             hipLaunchKernelGGL((raster_fwd_kernel<1, GA>), grid, dim3(kThreads), 0, st, tiles_x,
-                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy);
+                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy,
+                               has_opac);
             break;
         case 2:
             hipLaunchKernelGGL((raster_fwd_kernel<2, GA>), grid, dim3(kThreads), 0, st, tiles_x,
-                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy);
+                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy,
+                               has_opac);
             break;
         case 3:
             hipLaunchKernelGGL((raster_fwd_kernel<3, GA>), grid, dim3(kThreads), 0, st, tiles_x,
-                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy);
+                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy,
+                               has_opac);
             break;
         default:
             hipLaunchKernelGGL((raster_fwd_kernel<4, GA>), grid, dim3(kThreads), 0, st, tiles_x,
-                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy);
+                               bw, img_w, img_h, gids, bins, pack, img, wsum, fin, dx, dy, dxy,
+                               has_opac);
             break;
     }
 }
@@ -1937,7 +1973,7 @@ int rasterize_forward_impl(int n, int img_height, int img_width, int block_width
     if (ga)
         launch_fwd<true>(px, grid, st, tiles_x, block_width, img_width, img_height,
                          gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, out_dx,
-                         out_dy, out_dxy);
+                         out_dy, out_dxy, opacities ? 1 : 0);
     else if (image_only && opacities)
         hipLaunchKernelGGL((raster_fwd16_kernel<true, true>), grid, dim3(kThreads), 0, st,
                            tiles_x, img_width, img_height, gaussian_ids_sorted, bins, pk, out_img,
@@ -1967,7 +2003,7 @@ int rasterize_forward_impl(int n, int img_height, int img_width, int block_width
     else
         launch_fwd<false>(px, grid, st, tiles_x, block_width, img_width, img_height,
                           gaussian_ids_sorted, bins, pk, out_img, out_wsum, final_idx, nullptr,
-                          nullptr, nullptr);
+                          nullptr, nullptr, opacities ? 1 : 0);
     return launch_status();
 }
 
