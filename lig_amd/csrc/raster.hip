@@ -232,22 +232,46 @@ __global__ __launch_bounds__(kThreads) void raster_fwd_kernel(
     }
 }
 
+// Interval of offsets u along one axis over which the region {q <= lim} of
+// q(u, v) = Au u^2 + 2 B u v + Av v^2 (positive definite, det = Au Av - B^2)
+// meets the band v0 <= v <= v1 of the other axis: false when the band misses
+// the region.  The region's extreme u = +-uE, uE = sqrt(Av lim / det), lies at
+// v = -+B uE / Av; clamped into the band that v gives the band's extreme u,
+// from q(u, vc) = lim: u = (-B vc +- sqrt(Au lim - det vc^2)) / Au (equal to
+// +-uE when no clamp applies).  Reciprocals and square roots are the fast
+// hardware forms; callers widen the interval (their lim carries a margin).
+__device__ __forceinline__ bool reach_span(float Au, float Av, float B, float det, float lim,
+                                           float v0, float v1, float &ulo, float &uhi) {
+    const float idet = __builtin_amdgcn_rcpf(det);
+    const float aul = Au * lim;
+    const float vE = __builtin_sqrtf(aul * idet);  // the region's half-extent along v
+    if (v1 < -vE || v0 > vE) return false;
+    const float uE = __builtin_sqrtf(Av * lim * idet);
+    const float vt = -B * uE * __builtin_amdgcn_rcpf(Av);
+    const float vc = fminf(fmaxf(vt, v0), v1);
+    const float vd = fminf(fmaxf(-vt, v0), v1);
+    const float hc = __builtin_sqrtf(fmaxf(0.0f, __builtin_fmaf(-det * vc, vc, aul)));
+    const float hd = __builtin_sqrtf(fmaxf(0.0f, __builtin_fmaf(-det * vd, vd, aul)));
+    const float iau = __builtin_amdgcn_rcpf(Au);
+    uhi = (hc - B * vc) * iau;
+    ulo = (-hd - B * vd) * iau;
+    return true;
+}
+
 // Column strips of a 16x16 tile (bit k: columns 4k .. 4k + 3, all 16 rows)
 // that the accept region of a Gaussian can reach.  In the forward's own terms
 // the region is {u : s(u) <= T}, s(u) = a' u_x^2 + b2' u_x u_y + c' u_y^2 with
-// the staged conic (a', b2', c') = (a, 2b, c) log2(e) / 2, u = mean - pixel
-// centre, and T = log2(255 o); (bx0, by0) is the tile's first pixel centre.
-// The form's minimum over a strip's rectangle of centres is zero when the mean
-// is inside it and otherwise the least of its four edges' minima (along
-// u_x = const the form is least at u_y = -b2' u_x / (2 c'), clamped to the
-// edge; likewise across).  A strip is dropped when that minimum exceeds T by
-// more than 0.36 + 2^-10 T (0.5 + 2^-10 t in units of the unscaled
+// the staged conic (a', b2', c') = (a, 2b, c) log2(e) / 2, u = pixel centre -
+// mean, and T = log2(255 o); (bx0, by0) is the tile's first pixel centre.  The
+// columns whose centres the region {s <= lim} reaches within the tile's rows
+// form one run (reach_span along x over the band of row centres), widened by
+// 1/64 of a column each way; a strip is kept when it holds a column of that
+// run.  lim = T + 0.36 + 2^-10 T (0.5 + 2^-10 t in units of the unscaled
 // d^T Q d <= t).  Where the conic's condition number is below 1e4
 // (tr^2 <= 1e4 det), a' + c' <= 46 (a + c <= 64) and the mean lies within
-// 65536 px, the FP32 error of this minimum and of the per-pixel s together stay
-// below 0.22 (terms at most 1e4 T; centre differences off by at most 2^-8 px),
-// so a dropped strip holds no pixel the accept test passes.  Other Gaussians
-// keep all four strips.
+// 65536 px, the FP32 error of the run's ends and of the per-pixel s together
+// stay well inside that margin, so a dropped strip holds no pixel the accept
+// test passes.  Other Gaussians keep all four strips.
 __device__ __forceinline__ int reach_strips(float mx, float my, float ap, float bp2, float cp,
                                             float T, float bx0, float by0) {
     const float bp = 0.5f * bp2;
@@ -256,53 +280,41 @@ __device__ __forceinline__ int reach_strips(float mx, float my, float ap, float 
           T > 0.0f && T < 1e4f && fabsf(mx) <= 65536.0f && fabsf(my) <= 65536.0f))
         return 0xF;
     const float lim = __builtin_fmaf(T, 0x1p-10f, T) + 0.36f;
-    const float ra = -bp / ap, rc = -bp / cp;
-    auto form = [&](float x, float y) {
-        return __builtin_fmaf(ap * x, x, y * __builtin_fmaf(bp2, x, cp * y));
-    };
-    // rows of every strip: centre offsets [y0, y1]
-    const float y0 = by0 - my, y1 = y0 + 15.0f;
-    const bool in_y = y0 <= 0.0f && y1 >= 0.0f;
-    int m = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const float x0 = (bx0 + (float)(4 * k)) - mx, x1 = x0 + 3.0f;
-        const bool inside = in_y && x0 <= 0.0f && x1 >= 0.0f;
-        const float e0 = form(x0, fminf(fmaxf(rc * x0, y0), y1));
-        const float e1 = form(x1, fminf(fmaxf(rc * x1, y0), y1));
-        const float e2 = form(fminf(fmaxf(ra * y0, x0), x1), y0);
-        const float e3 = form(fminf(fmaxf(ra * y1, x0), x1), y1);
-        const float least = fminf(fminf(e0, e1), fminf(e2, e3));
-        if (inside || !(least > lim)) m |= 1 << k;
-    }
-    return m;
+    const float x0 = bx0 - mx, y0 = by0 - my;
+    float xlo, xhi;
+    if (!reach_span(ap, cp, bp, det, lim, y0, y0 + 15.0f, xlo, xhi)) return 0;
+    // clamped in float first: the conversions stay in range
+    const int c0 = (int)fminf(fmaxf(ceilf(xlo - x0 - 0x1p-6f), 0.0f), 16.0f);
+    const int c1 = (int)fmaxf(fminf(floorf(xhi - x0 + 0x1p-6f), 15.0f), -1.0f);
+    if (c0 > c1) return 0;
+    return (int)(((2u << (c1 >> 2)) - 1u) & ~((1u << (c0 >> 2)) - 1u));
 }
 
-// Rows of a 16 x 16 tile that a Gaussian can reach: bit r set unless the form
-// stays above the accept limit along the whole row r (pixel centres bx0 ..
-// bx0 + 15 at height by0 + r), the least value taken at the clamped
-// minimiser along the row, with the margin and the guards of reach_strips
-// (all rows when a guard fails).  The set bits form one run, the form's
-// minimum over a segment being convex in the row offset.
-__device__ __forceinline__ uint32_t reach_rows(float mx, float my, float ap, float bp2, float cp,
-                                               float T, float bx0, float by0) {
+// Rows of a 16 x 16 tile that a Gaussian can reach, as the run [r0, r1]
+// (false: none).  A row r is kept when the form a'x^2 + 2b'xy + c'y^2 of
+// (pixel centre - mean) can come within the accept limit anywhere on it
+// (centres bx0 .. bx0 + 15 at height by0 + r): the region {form <= lim} met
+// with the tile's column band is one interval of heights (reach_span), widened
+// by 1/64 of a row each way, lim = T (1 + 2^-10) + 0.36 (the margin of
+// reach_strips, above the FP32 error of the per-pixel s).  All rows when the
+// guards of reach_strips fail.
+__device__ __forceinline__ bool reach_row_span(float mx, float my, float ap, float bp2, float cp,
+                                               float T, float bx0, float by0, int &r0, int &r1) {
     const float bp = 0.5f * bp2;
     const float det = ap * cp - bp * bp, tr = ap + cp;
+    r0 = 0;
+    r1 = 15;
     if (!(ap > 0.0f && cp > 0.0f && det > 0.0f && tr <= 46.0f && tr * tr <= 1e4f * det &&
           T > 0.0f && T < 1e4f && fabsf(mx) <= 65536.0f && fabsf(my) <= 65536.0f))
-        return 0xFFFFu;
+        return true;
     const float lim = __builtin_fmaf(T, 0x1p-10f, T) + 0.36f;
-    const float ra = -bp / ap;
-    const float x0 = bx0 - mx, x1 = x0 + 15.0f, y0 = by0 - my;
-    uint32_t m = 0;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const float y = y0 + (float)r;
-        const float x = fminf(fmaxf(ra * y, x0), x1);
-        const float e = __builtin_fmaf(ap * x, x, y * __builtin_fmaf(bp2, x, cp * y));
-        if (!(e > lim)) m |= 1u << r;
-    }
-    return m;
+    const float x0 = bx0 - mx, y0 = by0 - my;
+    float ylo, yhi;
+    if (!reach_span(cp, ap, bp, det, lim, x0, x0 + 15.0f, ylo, yhi)) return false;
+    // clamped in float first: the conversions stay in range
+    r0 = (int)fminf(fmaxf(ceilf(ylo - y0 - 0x1p-6f), 0.0f), 16.0f);
+    r1 = (int)fmaxf(fminf(floorf(yhi - y0 + 0x1p-6f), 15.0f), -1.0f);
+    return r0 <= r1;
 }
 
 // Forward for 16x16 tiles.  Lane l owns the four pixels (l % 4 + 4k, l / 4),
@@ -1023,10 +1035,10 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
             const int g = gids[c0 + i];
             const float4 p0 = pack[g].p0, p1 = pack[g].p1;
             const float T = HAS_OPAC ? __log2f(255.0f * p1.y) : kAcceptLog2;
-            const uint32_t m = reach_rows(p0.x, p0.y, p0.z, p0.w, p1.x, T, (float)x_org + 0.5f,
-                                          (float)y_org + 0.5f);
-            if (m) {
-                key = ((uint32_t)__builtin_ctz(m) << 4) | (31u - (uint32_t)__builtin_clz(m));
+            int rf, rl;
+            if (reach_row_span(p0.x, p0.y, p0.z, p0.w, p1.x, T, (float)x_org + 0.5f,
+                               (float)y_org + 0.5f, rf, rl)) {
+                key = ((uint32_t)rf << 4) | (uint32_t)rl;
                 LIG_CHECK((key >> 1) < (uint32_t)kFitCntWords && (key >> 4) <= (key & 15u));
                 atomicAdd(&cnt[key >> 1], 1u << (16 * (key & 1u)));
             } else if (rec || grec) {
