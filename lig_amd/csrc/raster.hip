@@ -836,43 +836,53 @@ __global__ void add_phantom_kernel(const float *__restrict__ phantom, int has_op
 // cancellation down) are accumulated, beside the row's sums of alpha *
 // dL/dI per channel (and of alpha * dL/dW when that gradient can be nonzero,
 // WS).  At the end of the row q0 = sum v_sigma = -(c . those sums + the dL/dW
-// sum) and the moments fold into sum v_sigma dx (mx), v_sigma dx^2 (sxx),
-// v_sigma dy (my), v_sigma dy^2 (syy) and v_sigma dx dy (sxy); the row's
-// colour sums join v_rgb.  v_xy is formed once at the end as (a mx + b my,
-// b mx + c my); v_xy_abs accumulates |v_sigma| |u'| with the scaled conic's
-// u' = K (a dx + b dy) and is divided by K once at the end.
+// sum) gives the dy moments sum v_sigma dy (my) and dy^2 (syy) and joins its
+// total over rows; after the last row the totals give sum v_sigma dx (mx),
+// dx^2 (sxx) and dx dy (sxy); the row's colour sums join v_rgb.  v_xy is
+// formed once at the end as (a mx + b my, b mx + c my); v_xy_abs accumulates
+// |v_sigma| |u'| with the scaled conic's u' = K (a dx + b dy) and is divided
+// by K once at the end.
 struct PlainAcc {
     float r, g, b, sxx, sxy, syy, mx, my, ax, ay, o;
     float rr, rg, rb, rw;  // current row: sum alpha * dL/dI (rgb), alpha * dL/dW
-    float q1, q2;          // current row: sum v_sigma * lx^k, k = 1, 2
+    float q1;              // current row: sum v_sigma * lx
+    float q2;              // all rows: sum v_sigma * lx^2
+    float Q0, Q1, Y1;      // sums over rows of q0, q1 and dy * q1
 };
 
 __device__ __forceinline__ void plain_acc_zero(PlainAcc &A) {
     A.r = A.g = A.b = A.sxx = A.sxy = A.syy = A.mx = A.my = A.ax = A.ay = A.o = 0.0f;
-    A.rr = A.rg = A.rb = A.rw = A.q1 = A.q2 = 0.0f;
+    A.rr = A.rg = A.rb = A.rw = A.q1 = A.q2 = A.Q0 = A.Q1 = A.Y1 = 0.0f;
 }
 
 // End of a pixel row: q0 = sum v_sigma from the row's sums (v_sigma = -alpha
-// (c . dL/dI + dL/dW)); with dx = D - lx (D = mean.x - (x_org + c0 + 0.5), lx
-// the pixel's column relative to c0), sum v_sigma dx = D q0 - q1 and
-// sum v_sigma dx^2 = D (D q0 - 2 q1) + q2; the dy moments take dy once.
+// (c . dL/dI + dL/dW)); the dy moments take the row's dy once (sum v_sigma dy
+// and dy^2 in my and syy, sum v_sigma lx dy in Y1), the dx moments only need
+// totals over rows (plain_fold).
 template <bool WS>
-__device__ __forceinline__ void plain_row_end(PlainAcc &A, float D, float dy, float cr,
-                                              float cg, float cb) {
+__device__ __forceinline__ void plain_row_end(PlainAcc &A, float dy, float cr, float cg,
+                                              float cb) {
     const float cz = WS ? __builtin_fmaf(cb, A.rb, A.rw) : cb * A.rb;
     const float q0 = -__builtin_fmaf(cr, A.rr, __builtin_fmaf(cg, A.rg, cz));
     A.r += A.rr;
     A.g += A.rg;
     A.b += A.rb;
-    const float s1 = __builtin_fmaf(D, q0, -A.q1);
-    const float s2 = __builtin_fmaf(D, __builtin_fmaf(D, q0, -2.0f * A.q1), A.q2);
-    A.mx += s1;
-    A.sxx += s2;
+    A.Q0 += q0;
+    A.Q1 += A.q1;
     A.my = __builtin_fmaf(dy, q0, A.my);
-    A.sxy = __builtin_fmaf(dy, s1, A.sxy);
     A.syy = __builtin_fmaf(dy * q0, dy, A.syy);
-    A.rr = A.rg = A.rb = A.q1 = A.q2 = 0.0f;
+    A.Y1 = __builtin_fmaf(dy, A.q1, A.Y1);
+    A.rr = A.rg = A.rb = A.q1 = 0.0f;
     if constexpr (WS) A.rw = 0.0f;
+}
+
+// After the last row: with dx = D - lx (D = mean.x - (x_org + c0 + 0.5), lx the
+// pixel's column relative to c0), sum v_sigma dx = D Q0 - Q1,
+// sum v_sigma dx^2 = D (D Q0 - 2 Q1) + q2 and sum v_sigma dx dy = D my - Y1.
+__device__ __forceinline__ void plain_fold(PlainAcc &A, float D) {
+    A.mx = __builtin_fmaf(D, A.Q0, -A.Q1);
+    A.sxx = __builtin_fmaf(D, __builtin_fmaf(D, A.Q0, -2.0f * A.Q1), A.q2);
+    A.sxy = __builtin_fmaf(D, A.my, -A.Y1);
 }
 
 // One pair of the plain backward.  Rejected pairs (lane test, the accept
@@ -987,8 +997,9 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_plain_kernel(
                                      gp.p1.w, gp.p2.x, gp.p0.x - pxf, (float)(lx - c0), amin,
                                      clampv, sv + p);
             }
-            plain_row_end<true>(A, D, dy, gp.p1.z, gp.p1.w, gp.p2.x);
+            plain_row_end<true>(A, dy, gp.p1.z, gp.p1.w, gp.p2.x);
         }
+        plain_fold(A, D);
         plain_emit<HAS_OPAC, DET>(A, gp, act, DET && act ? slots[base + lane] : 0, g, rec, g_xy,
                                   g_xy_abs, g_conic, g_rgb, g_opac);
     }
@@ -1267,8 +1278,9 @@ __device__ __forceinline__ void bwd16_tile(
                                              bp, rt, op, cr, cg, cb, dx_of(lx), (float)(lx - 8),
                                              amin, clampv, row + lx, lx == 8);
             }
-            plain_row_end<!TRAIN>(A, dx_of(8), dy, cr, cg, cb);
+            plain_row_end<!TRAIN>(A, dy, cr, cg, cb);
         }
+        plain_fold(A, dx_of(8));
         if constexpr (TRAIN) {
             if (act) {
                 const int slot = __float_as_int(gp.p3.z) + ty * __float_as_int(gp.p3.w) + tx;
