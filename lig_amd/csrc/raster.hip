@@ -461,6 +461,17 @@ __device__ __forceinline__ void fwd16_tile(
                 for (int q = 0; q < 4; ++q) {
                     if (!((qbits[k][q] >> bit) & 1)) continue;
                     const float s = pair_s(a0.z, rw, a0.x - X[q]);
+                    if constexpr (TRAIN && !HAS_OPAC) {
+                        // accumulation under the lane mask of accepted pairs (a
+                        // rejected pair adds nothing), skipped when no lane accepts
+                        if (accept_s(s)) {
+                            const float al = fminf(fast_exp2(-s), 0.999f);
+                            ar[q] = __builtin_fmaf(al, a1.z, ar[q]);
+                            ag[q] = __builtin_fmaf(al, a1.w, ag[q]);
+                            ab[q] = __builtin_fmaf(al, blue, ab[q]);
+                        }
+                        continue;
+                    }
                     const float e = fast_exp2(-s);
                     float al = fminf(HAS_OPAC ? a1.y * e : e, 0.999f);
                     const bool ok = HAS_OPAC ? !(s < 0.0f) && !(al < amin) : accept_s(s);
