@@ -180,7 +180,7 @@ def valu_issue(path: str, top: int = 16, cus: int = 256, xcds: int = 8) -> str:
 
 # kernels whose HBM traffic bench.py reports: (name prefix, launch threads)
 TRAFFIC_KERNELS = {
-    "raster_bwd": ("raster_bwd16_kernel<false, true, true>", 16777216,
+    "raster_bwd": ("raster_bwd16_kernel<false, true, true", 16777216,
                    "fit-step backward, 8192^2"),
     "raster_fwd": ("raster_fwd16_kernel<false, true", 16777216, "fit-step forward, 8192^2"),
 }
