@@ -519,6 +519,21 @@ __device__ __forceinline__ void fwd16_tile(
     }
 }
 
+// Tile of the u-th launch slot when no launch order is given: on a grid of
+// whole 8 x 8 super-tiles (both tile counts multiples of 8) slots run through
+// each super-tile, super-tiles row-major, so the tiles an XCD has in flight at
+// once (a contiguous run of slots, xcd_remap) form a compact patch whose
+// Gaussians its L2 holds, instead of a sliver of one tile row; other grids
+// row-major.  Images of a batched launch keep their own order (one_image
+// false).
+__device__ __forceinline__ int tile_of_slot(int u, int tiles_x, int tiles_y, bool one_image) {
+    if (!one_image || (tiles_x & 7) || (tiles_y & 7)) return u;
+    const int w = u & 63, su = u >> 6;
+    const int sxn = tiles_x >> 3;
+    const int sy = su / sxn, sx = su - sy * sxn;
+    return ((sy << 3) + (w >> 3)) * tiles_x + (sx << 3) + (w & 7);
+}
+
 // One tile per block: the tile order[blockIdx.x], or without an order (none
 // given, or order[0] = -1) the tile xcd_remap(blockIdx.x).
 template <bool HAS_OPAC, bool TRAIN, int DEPTH = kThreads, bool FIN = true>
@@ -532,8 +547,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(TRAIN 
     __shared__ float4 s0[DEPTH];
     __shared__ float4 s1[DEPTH];
     __shared__ float s2[HAS_OPAC ? DEPTH : 1];  // blue (with opacities)
-    const int gtile = order && order[0] >= 0 ? order[blockIdx.x]
-                                             : (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int gtile = order && order[0] >= 0
+                          ? order[blockIdx.x]
+                          : tile_of_slot((int)xcd_remap(blockIdx.x, gridDim.x), tiles_x,
+                                         (img_h + 15) >> 4, tab == nullptr);
     fwd16_tile<HAS_OPAC, TRAIN, DEPTH, FIN>(gtile, tiles_x, img_w, img_h, gids, bins, pack,
                                             out_img, out_wsum, final_idx, target, loss_partial,
                                             tab, nimg, cum, s0, s1, s2);
@@ -1363,8 +1380,9 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
         gtile = order[k];
     } else {
         const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
-        gtile = u / split;
-        part = u - gtile * split;
+        const int k = u / split;
+        part = u - k * split;
+        gtile = tile_of_slot(k, tiles_x, (img_h + 15) >> 4, tab == nullptr);
     }
     bwd16_tile<HAS_OPAC, DET, TRAIN, GROUP>(gtile, part, split, tiles_x, img_w, img_h,
                                      gids, slots, bins, pack, final_idx, v_out, v_wsum, img,
