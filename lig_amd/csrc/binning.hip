@@ -646,18 +646,19 @@ __device__ __forceinline__ void bin_edge_at(int i, int m, int prev, int cur, int
     }
 }
 
+// Bin of each key value (key & kmask) >> SHIFT in a sorted run.
 template <typename K, int SHIFT>
 __global__ void bin_edges_kernel(int cap, const int32_t *m_dev, const K *__restrict__ sorted,
-                                 int num_tiles, int32_t *__restrict__ bins) {
+                                 int num_tiles, int32_t *__restrict__ bins, K kmask) {
     const int m = live_count(cap, m_dev);
     if (m < 0) return;
     const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (i0 >= (size_t)m) return;
     int t[5];
-    t[0] = i0 > 0 ? (int)(sorted[i0 - 1] >> SHIFT) : -1;
+    t[0] = i0 > 0 ? (int)((sorted[i0 - 1] & kmask) >> SHIFT) : -1;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        t[k + 1] = i0 + k < (size_t)m ? (int)(sorted[i0 + k] >> SHIFT) : -1;
+        t[k + 1] = i0 + k < (size_t)m ? (int)((sorted[i0 + k] & kmask) >> SHIFT) : -1;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (i0 + k < (size_t)m) bin_edge_at((int)(i0 + k), m, t[k], t[k + 1], num_tiles, bins);
@@ -676,19 +677,22 @@ int bits_for(int num_tiles) {
 // tile rows: one row entry (row, gid) per tile row a Gaussian spans, about
 // sqrt(tiles per Gaussian) times fewer than intersections (12.6 M against
 // 79 M at 8192^2 x 2 M).  The entries are emitted gid-major and sorted
-// stably by row, which puts each row's Gaussians in gid order.  Each row's
-// entries are split into P segments (P chosen by the host from the
-// intersection count); a counting pass writes, for every (row, segment,
-// tile), how many of the segment's entries cover the tile.  Within a row, a
-// prefix over segments per tile and a scan of the tile totals, plus a scan of
-// the row totals, give every segment's first output position in every tile
-// and the bins.  The scatter pass expands each segment's entries (rounds of
-// 64, one per lane) into their tiles; a lane's rank in a tile is the tile's
-// cursor plus the number of lower lanes covering it (a 64-bit lane mask per
-// tile, built with LDS atomics).  The output is the stable (tile, emission)
-// order of the radix path, bit for bit.
+// stably by row, which puts each row's Gaussians in gid order; each entry's
+// key carries its Gaussian's column range above the sorted row bits, so no
+// later pass gathers rectangles.  Each row's entries are split into P
+// segments (P chosen by the host from the grid size); a counting pass writes,
+// for every (row, segment, tile), how many of the segment's entries cover
+// the tile.  Within a row, a prefix over segments per tile and a scan of the
+// tile totals, plus a scan of the row totals, give every segment's first
+// output position in every tile and the bins.  The scatter pass gives every
+// (row, segment, window of kRowWin tiles) one wave: it filters the segment's
+// entries down to those that reach the window, in gid order, and takes them
+// 64 at a time, one per lane, through the window's tiles; per tile a ballot
+// of the lanes covering it ranks them, and they store their gids at
+// consecutive positions of the tile's bin.  The output is the stable
+// (tile, emission) order of the radix path, bit for bit.
 // ---------------------------------------------------------------------------
-constexpr int kRowMaxTiles = 1024;     // tiles per row the LDS tables hold
+constexpr int kRowMaxTiles = 1024;     // tiles per row (10-bit column fields)
 constexpr int kRowMaxRows = 32767;     // rows per image (15-bit row fields)
 constexpr uint32_t kPhantomBit = 0x80000000u;
 
@@ -702,6 +706,16 @@ __device__ __forceinline__ void unpack_rect(uint2 rc, int &x0, int &y0, int &x1,
     y0 = (int)(rc.y & 0xFFFFu);
     y1 = (int)((rc.y >> 16) & 0x7FFFu);
     ph = (rc.y & kPhantomBit) != 0;
+}
+
+// Row entry key: the row in the low `kshift` bits (the bits the row sort
+// orders by), then the first column x0 (10 bits) and the width - 1 (10 bits).
+__device__ __forceinline__ uint32_t row_key(int row, int x0, int x1, int kshift) {
+    return (uint32_t)row | ((uint32_t)x0 << kshift) | ((uint32_t)(x1 - x0 - 1) << (kshift + 10));
+}
+__device__ __forceinline__ void rowkey_cols(uint32_t k, int kshift, int &x0, int &x1) {
+    x0 = (int)((k >> kshift) & 1023u);
+    x1 = x0 + (int)((k >> (kshift + 10)) & 1023u) + 1;
 }
 
 // Per Gaussian: its tile rectangle and the number of tile rows it spans (the
@@ -735,15 +749,16 @@ __global__ __launch_bounds__(256) void rect_rows_kernel(int n, const float *__re
     }
 }
 
-// Row entries in gid-major order: key = tile row, value = gid (phantom bit
-// set for a phantom slot, whose key is row 0).  A wave covers its 64
-// Gaussians' joint entry range 64 entries at a time; a lane finds the owner of
-// its entry by binary search over the wave's starts, so the stores are
-// coalesced.  `cap` bounds the entries written (they never exceed the
-// intersection count the buffers hold).
+// Row entries in gid-major order: key = row_key(tile row, x0, x1) (a phantom
+// slot: row 0, columns [0, 1)), value = gid (phantom bit set for a phantom
+// slot).  A wave covers its 64 Gaussians' joint entry range 64 entries at a
+// time; a lane finds the owner of its entry by binary search over the wave's
+// starts, so the stores are coalesced.  `cap` bounds the entries written
+// (they never exceed the intersection count the buffers hold).
 __global__ __launch_bounds__(256) void emit_rows_kernel(int n, int cap,
                                                         const int32_t *__restrict__ ecum,
                                                         const uint2 *__restrict__ rect,
+                                                        int kshift,
                                                         uint32_t *__restrict__ rowkey,
                                                         uint32_t *__restrict__ rowval) {
     const int lane = threadIdx.x & (LIG_WAVE - 1);
@@ -755,17 +770,21 @@ __global__ __launch_bounds__(256) void emit_rows_kernel(int n, int cap,
     const int wend = min(ecum[min(g0 + LIG_WAVE, n) - 1], cap);
     __shared__ int s_start[256 / LIG_WAVE][LIG_WAVE];
     __shared__ uint32_t s_key[256 / LIG_WAVE][LIG_WAVE];
+    __shared__ uint32_t s_ph[256 / LIG_WAVE][LIG_WAVE];
     int start = wend;
-    uint32_t key = 0u;
+    uint32_t key = 0u, ph = 0u;
     if (g < n) {
         start = g == 0 ? 0 : ecum[g - 1];
         int x0, y0, x1, y1;
-        bool ph;
-        unpack_rect(rect[g], x0, y0, x1, y1, ph);
-        key = ph ? kPhantomBit : (uint32_t)y0;
+        bool p;
+        unpack_rect(rect[g], x0, y0, x1, y1, p);
+        // a phantom's rectangle is row 0, columns [0, 1): key 0
+        key = x1 > x0 ? row_key(y0, x0, x1, kshift) : 0u;
+        ph = p ? kPhantomBit : 0u;
     }
     s_start[wv][lane] = start;
     s_key[wv][lane] = key;
+    s_ph[wv][lane] = ph;
     wave_lds_sync();
     for (int base = wstart; base < wend; base += LIG_WAVE) {
         const int pos = base + lane;
@@ -780,10 +799,9 @@ __global__ __launch_bounds__(256) void emit_rows_kernel(int n, int cap,
                     hi = mid - 1;
             }
             LIG_CHECK(s_start[wv][lo] <= pos && pos < cap);
-            const uint32_t k = s_key[wv][lo];
-            const bool ph = (k & kPhantomBit) != 0;
-            rowkey[pos] = ph ? 0u : k + (uint32_t)(pos - s_start[wv][lo]);
-            rowval[pos] = (uint32_t)(g0 + lo) | (ph ? kPhantomBit : 0u);
+            // the row sits in the low bits: key + k is row y0 + k (y0 + k < 2^kshift)
+            rowkey[pos] = s_key[wv][lo] + (uint32_t)(pos - s_start[wv][lo]);
+            rowval[pos] = (uint32_t)(g0 + lo) | s_ph[wv][lo];
         }
     }
 }
@@ -809,42 +827,49 @@ __device__ __forceinline__ int wave_exclusive_sum(int v) {
     return x - v;
 }
 
-// Counting pass: one wave per (row, segment), a row's segments on one XCD.
+// Counting pass: one block per (row, segment), a row's segments on one XCD.
 // counts[(r P + s) tiles_x + t] = the segment's entries whose column range
 // covers tile t (a difference array over the row's columns in LDS, then its
 // prefix sum).
-__global__ __launch_bounds__(LIG_WAVE) void row_count_kernel(
-    int P, int tiles_x, const int2 *__restrict__ rowbins, const uint32_t *__restrict__ rowval,
-    const uint2 *__restrict__ rect, int32_t *__restrict__ counts) {
-    extern __shared__ int diff[];  // tiles_x + 1
-    const int lane = threadIdx.x;
+constexpr int kCountThreads = 256;
+__global__ __launch_bounds__(kCountThreads) void row_count_kernel(
+    int P, int tiles_x, int kshift, const int2 *__restrict__ rowbins,
+    const uint32_t *__restrict__ rowkey, int32_t *__restrict__ counts) {
+    __shared__ int diff[kRowMaxTiles + 1];
+    __shared__ int wsum[kCountThreads / LIG_WAVE];
     const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int r = u / P;
     const int s = u - r * P;
     int lo, hi;
     row_segment(rowbins, r, s, P, lo, hi);
-    for (int t = lane; t <= tiles_x; t += LIG_WAVE) diff[t] = 0;
-    wave_lds_sync();
-    for (int j = lo + lane; j < hi; j += LIG_WAVE) {
-        const uint32_t v = rowval[j];
-        const uint2 rc = rect[v & ~kPhantomBit];
-        int x0, y0, x1, y1;
-        bool ph;
-        unpack_rect(rc, x0, y0, x1, y1, ph);
-        LIG_CHECK(x0 >= 0 && x0 <= x1 && x1 <= tiles_x);
+    for (int t = threadIdx.x; t <= tiles_x; t += kCountThreads) diff[t] = 0;
+    __syncthreads();
+    for (int j = lo + threadIdx.x; j < hi; j += kCountThreads) {
+        int x0, x1;
+        rowkey_cols(rowkey[j], kshift, x0, x1);
+        LIG_CHECK(x0 >= 0 && x0 < x1 && x1 <= tiles_x);
         atomicAdd(&diff[x0], 1);
-        atomicAdd(&diff[x1], -1);
+        atomicAdd(&diff[min(x1, tiles_x)], -1);
     }
-    wave_lds_sync();
-    // prefix: chunks of LIG_WAVE consecutive columns, lanes on consecutive columns
-    int carry = 0;
+    __syncthreads();
+    // prefix: thread i takes columns i kPer .. i kPer + kPer - 1
+    constexpr int kPer = kRowMaxTiles / kCountThreads;
+    int d[kPer];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int t = threadIdx.x * kPer + k;
+        d[k] = t < tiles_x ? diff[t] : 0;
+        mine += d[k];
+    }
+    int total;
+    int run = block_exclusive_scan_n<kCountThreads>(mine, wsum, &total);
     int32_t *out = counts + ((size_t)r * P + s) * tiles_x;
-    for (int t0 = 0; t0 < tiles_x; t0 += LIG_WAVE) {
-        const int t = t0 + lane;
-        const int d = t < tiles_x ? diff[t] : 0;
-        const int incl = wave_exclusive_sum(d) + d;
-        if (t < tiles_x) out[t] = carry + incl;
-        carry += __shfl(incl, LIG_WAVE - 1, LIG_WAVE);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int t = threadIdx.x * kPer + k;
+        run += d[k];
+        if (t < tiles_x) out[t] = run;
     }
 }
 
@@ -923,186 +948,169 @@ __global__ void row_bins_kernel(int num_tiles, int tiles_x, const int32_t *__res
     bins[2 * t + 1] = c > 0 ? st + c : 0;
 }
 
-// Scatter pass: one wave per (row, segment), a row's segments on one XCD.
-// The segment's output for tile t is the run [S(t, s), S(t, s) + c(t, s)) of
-// the tile's bin (c from consecutive scanned counts); the runs are numbered
-// consecutively in LDS (lstart).  The wave takes the segment's entries in
-// rounds of 64 (gid order): each lane marks its tiles in per-tile 64-bit lane
-// masks, then puts its gid (0 for a phantom) -- and its emission slot when
-// asked -- at the tile's run start + cursor + popcount(mask & lower lanes) in
-// an LDS buffer, with the tile's index beside it; the lowest lane of each mask
-// advances the cursor and clears the mask.  The buffer is then written out in
-// order, so consecutive lanes store consecutive addresses of each run
-// (scattered 4-byte stores straight to memory cost 1.1 ms of the 8192^2
-// level's 1.5).  Runs beyond the buffer are taken in further windows; a
-// single run larger than the buffer is stored directly.
-constexpr int kScatterLds = 65536;     // LDS bytes per block, at most
-#ifndef LIG_SCATTER_BUF
-#define LIG_SCATTER_BUF 2048
-#endif
-constexpr int kScatterBufMax = LIG_SCATTER_BUF;  // buffer entries, at most
-
-// Buffer entries of the scatter for a row of tiles_x tiles.
-int scatter_buf(int tiles_x, bool slots) {
-    const int fixed = tiles_x * 20 + 4;        // masks, cursors, segment starts, run starts
-    const int per = 4 + 2 + (slots ? 4 : 0);   // gid, tile index, slot
-    return std::min(kScatterBufMax, (kScatterLds - fixed) / per);
-}
+// Scatter pass: one wave per (row, segment, window of kRowWin tiles), a row's
+// waves on one XCD (its entries stay in that L2).  Lane k < kRowWin loads the
+// segment's first output position in the window's tile k; the 32 positions
+// then sit in scalar registers as the tiles' cursors.  The wave reads the
+// segment's row entries 64 at a time (keys fetched four rounds ahead) and
+// queues those whose column range meets the window, in order, in an LDS ring
+// of 128; every 64 queued entries (and the rest at the end) make a round: one
+// entry per lane, and for each tile k of the window a ballot of the lanes
+// covering it gives each such lane its rank (mbcnt) and the cursor's advance
+// (popcount), so the covering lanes store their gids -- and emission slots
+// when asked -- at consecutive positions of the tile's bin.  Positions past
+// the output's capacity m are not written.
+constexpr int kRowWin = 32;          // tiles per scatter window
+constexpr int kRowQueue = 128;       // LDS ring of queued entries (two rounds)
+constexpr int kScatterWaves = 16384; // scatter waves aimed for: segments per row follow
 
 template <bool SLOTS>
 __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
-    int P, int tiles_x, int m, int bufn, const int2 *__restrict__ rowbins,
-    const uint32_t *__restrict__ rowval, const uint2 *__restrict__ rect,
-    const int32_t *__restrict__ counts, const int32_t *__restrict__ scol,
-    const int32_t *__restrict__ te, const int32_t *__restrict__ rowoff,
-    const int32_t *__restrict__ cum, int32_t *__restrict__ gids_out,
-    int32_t *__restrict__ slots_out) {
-    extern __shared__ unsigned long long smem[];
-    unsigned long long *mask = smem;                       // tiles_x
-    int *cur = reinterpret_cast<int *>(smem + tiles_x);    // tiles_x
-    int *segst = cur + tiles_x;                            // tiles_x
-    int *lstart = segst + tiles_x;                         // tiles_x + 1
-    int *bg = lstart + tiles_x + 1;                        // bufn
-    int *bs = bg + bufn;                                   // bufn (SLOTS)
-    uint16_t *bt = reinterpret_cast<uint16_t *>(bs + (SLOTS ? bufn : 0));  // bufn
+    int P, int nwin, int tiles_x, int kshift, int m, const int2 *__restrict__ rowbins,
+    const uint32_t *__restrict__ rowkey, const uint32_t *__restrict__ rowval,
+    const uint2 *__restrict__ rect, const int32_t *__restrict__ counts,
+    const int32_t *__restrict__ scol, const int32_t *__restrict__ te,
+    const int32_t *__restrict__ rowoff, const int32_t *__restrict__ cum,
+    int32_t *__restrict__ gids_out, int32_t *__restrict__ slots_out) {
+    __shared__ int qj[kRowQueue];       // queued row entries: index
+    __shared__ uint32_t qx[kRowQueue];  // and column range x0 | x1 << 16
     const int lane = threadIdx.x;
     const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
-    const int r = u / P;
-    const int s = u - r * P;
+    const int rs = u / nwin;  // r P + s
+    const int w = u - rs * nwin;
+    const int r = rs / P;
+    const int s = rs - r * P;
     int lo, hi;
     row_segment(rowbins, r, s, P, lo, hi);
     if (lo >= hi) return;  // wave-uniform
-    // run lengths and starts of the segment, lanes on consecutive tiles; then
-    // the runs numbered consecutively (lstart)
-    const size_t seg_base = ((size_t)r * P + s) * tiles_x;
-    const size_t row_tiles = (size_t)r * tiles_x;
-    const int roff = rowoff[r];
-    // every chunk's loads are issued before the first is used (the wave is
-    // alone on its segment; the loads' latency, not their bytes, is the cost)
-    constexpr int kChunks = kRowMaxTiles / LIG_WAVE;
-    int cv[kChunks], sv[kChunks];
-#pragma unroll
-    for (int k = 0; k < kChunks; ++k) {
-        const int t = k * LIG_WAVE + lane;
-        cv[k] = 0;
-        sv[k] = 0;
-        if (t < tiles_x) {
-            cv[k] = counts[seg_base + t];
-            sv[k] = te[row_tiles + t] + scol[seg_base + t];
-        }
+    const int wlo = w * kRowWin;
+    const int wn = min(kRowWin, tiles_x - wlo);
+    const int whi = wlo + wn;
+    int start = 0;
+    if (lane < wn) {
+        const int t = wlo + lane;
+        start = rowoff[r] + te[(size_t)r * tiles_x + t] + scol[(size_t)rs * tiles_x + t];
     }
-    int carry = 0;
+    // fast rounds: a full window whose positions all stay below m (and
+    // below 2^30, so 32-bit byte offsets address them); cursors in scalar
+    // registers.  Otherwise the cursors stay in lane k of `start` and every
+    // store is checked against m.
+    int end = 0;
+    if (lane < wn) end = start + counts[(size_t)rs * tiles_x + wlo + lane];
+    bool fast = wn == kRowWin && m <= (1 << 30);
 #pragma unroll
-    for (int k = 0; k < kChunks; ++k) {
-        if (k * LIG_WAVE >= tiles_x) break;  // wave-uniform
-        const int t = k * LIG_WAVE + lane;
-        const int c = cv[k];
-        const int ex = wave_exclusive_sum(c);
-        if (t < tiles_x) {
-            segst[t] = roff + sv[k];
-            mask[t] = 0ull;
-            cur[t] = 0;
-            lstart[t] = carry + ex;
+    for (int off = LIG_WAVE / 2; off > 0; off >>= 1) end = max(end, __shfl_xor(end, off, LIG_WAVE));
+    fast = fast && end <= m;
+    int cur[kRowWin];
+#pragma unroll
+    for (int k = 0; k < kRowWin; ++k) cur[k] = __builtin_amdgcn_readlane(start, k);
+
+    // one round: queue slots head .. head + cnt - 1, one per lane
+    auto round = [&](int head, int cnt) {
+        wave_lds_sync();
+        const bool valid = lane < cnt;
+        const int q = (head + lane) & (kRowQueue - 1);
+        const int j = valid ? qj[q] : 0;
+        const uint32_t xx = valid ? qx[q] : 0u;
+        wave_lds_sync();  // the slots just read are refilled after the round
+        const int x0 = (int)(xx & 0xFFFFu), x1 = (int)(xx >> 16);
+        const uint32_t v = valid ? rowval[j] : 0u;
+        const bool ph = (v & kPhantomBit) != 0;
+        const int g = (int)(v & ~kPhantomBit);
+        const int gid = ph ? 0 : g;
+        int slot0 = 0;
+        if (SLOTS && valid) {
+            slot0 = g == 0 ? 0 : cum[g - 1];
+            if (!ph) slot0 += (r - (int)(rect[g].y & 0xFFFFu)) * (x1 - x0) - x0;
         }
-        carry += __shfl(ex + c, LIG_WAVE - 1, LIG_WAVE);
-    }
-    if (lane == 0) lstart[tiles_x] = carry;
-    wave_lds_sync();
-    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (LIG_WAVE - lane));
-    const unsigned long long bit = 1ull << lane;
-    for (int wa = 0; wa < tiles_x;) {
-        // window [wa, wb): the most tiles whose runs fit the buffer (one at least)
-        int a = wa + 1, b = tiles_x;
-        while (a < b) {
-            const int mid = (a + b + 1) >> 1;
-            if (lstart[mid] - lstart[wa] <= bufn) a = mid; else b = mid - 1;
-        }
-        const int wb = a;
-        const int base_q = lstart[wa];
-        const int nq = lstart[wb] - base_q;
-        const bool direct = nq > bufn;
-        if (nq > 0) {
-            // entries are fetched ahead: the row entry two rounds ahead and
-            // its rectangle one round ahead, so a round's LDS work covers the
-            // latency of the next rounds' loads
-            uint32_t v_cur = lo + lane < hi ? rowval[lo + lane] : 0u;
-            uint32_t v_nxt = lo + LIG_WAVE + lane < hi ? rowval[lo + LIG_WAVE + lane] : 0u;
-            uint2 rc_cur = lo + lane < hi ? rect[v_cur & ~kPhantomBit] : make_uint2(0u, 0u);
-            for (int base = lo; base < hi; base += LIG_WAVE) {
-                const int j = base + lane;
-                const uint32_t v = v_cur;
-                const uint2 rc = rc_cur;
-                const int j1 = j + LIG_WAVE, j2 = j + 2 * LIG_WAVE;
-                rc_cur = j1 < hi ? rect[v_nxt & ~kPhantomBit] : make_uint2(0u, 0u);
-                v_cur = v_nxt;
-                v_nxt = j2 < hi ? rowval[j2] : 0u;
-                int x0 = 0, y0 = 0, x1 = 0, y1 = 0, g = 0;
-                bool ph = false;
-                if (j < hi) {
-                    g = (int)(v & ~kPhantomBit);
-                    unpack_rect(rc, x0, y0, x1, y1, ph);
+        const unsigned wd = (unsigned)(x1 - x0);  // 0 on an idle lane
+        const int d = wlo - x0;
+        if (fast) {
+#pragma unroll
+            for (int k = 0; k < kRowWin; ++k) {
+                const bool c = (unsigned)(d + k) < wd;
+                const uint64_t mk = __ballot(c);
+                // the cursor joins the rank as mbcnt's addend
+                const uint32_t pos = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, (uint32_t)cur[k]));
+                if (c) {
+                    LIG_CHECK((int)pos >= 0 && (int)pos < m);
+                    *reinterpret_cast<int32_t *>(reinterpret_cast<char *>(gids_out) +
+                                                 (size_t)(pos * 4u)) = gid;
+                    if (SLOTS)
+                        *reinterpret_cast<int32_t *>(reinterpret_cast<char *>(slots_out) +
+                                                     (size_t)(pos * 4u)) =
+                            ph ? slot0 : slot0 + wlo + k;
                 }
-                const int t0 = max(x0, wa), t1 = min(x1, wb);
-                LIG_CHECK(j >= hi || (x0 >= 0 && x0 <= x1 && x1 <= tiles_x && y0 <= r));
-                for (int t = t0; t < t1; ++t) atomicOr(&mask[t], bit);
-                wave_lds_sync();
-                int slot0 = 0;
-                if (SLOTS && t1 > t0) {
-                    slot0 = g == 0 ? 0 : cum[g - 1];
-                    if (!ph) slot0 += (r - y0) * (x1 - x0) - x0;
-                }
-                const int gid = ph ? 0 : g;
-                for (int t = t0; t < t1; ++t) {
-                    const int rank = cur[t] + __popcll(mask[t] & lt);
-                    const int slot = ph ? slot0 : slot0 + t;
-                    if (direct) {
-                        const int pos = segst[t] + rank;
-                        LIG_CHECK(pos >= 0 && pos < m);
-                        if (pos < m) {
-                            gids_out[pos] = gid;
-                            if (SLOTS) slots_out[pos] = slot;
-                        }
-                    } else {
-                        const int q = lstart[t] - base_q + rank;
-                        LIG_CHECK(q >= 0 && q < nq && q < bufn && rank < lstart[t + 1] - lstart[t]);
-                        bg[q] = gid;
-                        if (SLOTS) bs[q] = slot;
-                        bt[q] = (uint16_t)t;
-                    }
-                }
-                wave_lds_sync();
-                for (int t = t0; t < t1; ++t) {
-                    const unsigned long long mt = mask[t];
-                    if ((mt & lt) == 0ull) {  // the lowest lane covering t
-                        cur[t] += __popcll(mt);
-                        mask[t] = 0ull;
-                    }
-                }
-                wave_lds_sync();
+                cur[k] += __popcll(mk);
             }
-            if (!direct) {  // the window's runs in order: consecutive lanes, consecutive slots
-                for (int q = lane; q < nq; q += LIG_WAVE) {
-                    const int t = bt[q];
-                    const int pos = segst[t] + (q + base_q - lstart[t]);
-                    LIG_CHECK(t >= wa && t < wb && pos >= 0 && pos < m);
-                    if (pos < m) {
-                        gids_out[pos] = bg[q];
-                        if (SLOTS) slots_out[pos] = bs[q];
-                    }
+        } else {
+            for (int k = 0; k < wn; ++k) {
+                const bool c = (unsigned)(d + k) < wd;
+                const uint64_t mk = __ballot(c);
+                const int ck = __builtin_amdgcn_readlane(start, k);
+                const int pos = ck + (int)__builtin_amdgcn_mbcnt_hi(
+                                         (uint32_t)(mk >> 32),
+                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
+                if (c && pos < m) {
+                    gids_out[pos] = gid;
+                    if (SLOTS) slots_out[pos] = ph ? slot0 : slot0 + wlo + k;
                 }
-                wave_lds_sync();  // the next window reuses the buffer
+                start = lane == k ? ck + __popcll(mk) : start;
             }
         }
-        wa = wb;
+    };
+
+    int head = 0, tail = 0;  // queue positions (wave-uniform)
+    auto key_at = [&](int jj) { return jj < hi ? rowkey[jj] : 0u; };
+    uint32_t k0 = key_at(lo + lane), k1 = key_at(lo + LIG_WAVE + lane);
+    uint32_t k2 = key_at(lo + 2 * LIG_WAVE + lane), k3 = key_at(lo + 3 * LIG_WAVE + lane);
+    for (int base = lo; base < hi; base += LIG_WAVE) {
+        const uint32_t kc = k0;
+        k0 = k1;
+        k1 = k2;
+        k2 = k3;
+        k3 = key_at(base + 4 * LIG_WAVE + lane);
+        const int j = base + lane;
+        int x0 = 0, x1 = 0;
+        rowkey_cols(kc, kshift, x0, x1);
+        const bool ov = j < hi && x0 < whi && x1 > wlo;
+        const uint64_t mo = __ballot(ov);
+        if (ov) {
+            const int q = (tail + (int)__builtin_amdgcn_mbcnt_hi(
+                                      (uint32_t)(mo >> 32),
+                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mo, 0u))) &
+                          (kRowQueue - 1);
+            qj[q] = j;
+            qx[q] = (uint32_t)x0 | ((uint32_t)x1 << 16);
+        }
+        tail += __popcll(mo);
+        const bool last = base + LIG_WAVE >= hi;
+        while (tail - head >= LIG_WAVE || (last && tail > head)) {
+            const int cnt = min(LIG_WAVE, tail - head);
+            round(head, cnt);
+            head += cnt;
+        }
     }
 }
 
-// Segments per row: enough that a segment's output (its entries' tiles) fills
-// about three quarters of the scatter's buffer on an average row.
+// Bits the row sort orders by for `rows` rows (its passes x digit width),
+// the shift of the column fields in a row entry's key.
+int row_key_shift(int rows) {
+    int passes, rb;
+    digit_plan(bits_for(rows), &passes, &rb);
+    return passes * rb;
+}
+
+int row_windows(int tiles_x) { return (tiles_x + kRowWin - 1) / kRowWin; }
+
+// Segments per row: enough for about kScatterWaves scatter waves, but a
+// segment keeps about 1024 intersections on an average row.
 int row_segments(int m, int tiles_x, int tiles_y) {
-    const double target = 0.75 * scatter_buf(tiles_x, false);
-    const double p = std::ceil((double)m / ((double)tiles_y * target));
-    return (int)std::min(4096.0, std::max(1.0, p));
+    const long long rw = (long long)tiles_y * row_windows(tiles_x);
+    const long long by_waves = (kScatterWaves + rw - 1) / rw;
+    const long long by_len = std::max(1LL, (long long)m / ((long long)tiles_y * 1024));
+    return (int)std::max(1LL, std::min(std::min(by_waves, by_len), 4096LL));
 }
 
 // Workspace of the row path (0 when it does not apply): rectangles, row
@@ -1112,7 +1120,7 @@ int row_segments(int m, int tiles_x, int tiles_y) {
 // scan.
 size_t rows_ws_bytes(int n, int m, int tiles_x, int tiles_y, int *P_out) {
     if (n <= 0 || m <= 0 || tiles_x < 1 || tiles_y < 1 || tiles_x > kRowMaxTiles ||
-        tiles_y > kRowMaxRows)
+        tiles_y > kRowMaxRows || row_key_shift(tiles_y) + 20 > 32)
         return 0;
     const int P = row_segments(m, tiles_x, tiles_y);
     const size_t T = (size_t)tiles_x * tiles_y;
@@ -1160,42 +1168,42 @@ int bin_rows(int n, int m, const float *xys, const float *extents, const int32_t
                        extents, cum, tiles_x, tiles_y, block_width, rect, ecum);
     scan_i32(ecum, ecum, nn, true, scan_ws, st);
     const int32_t *r_dev = ecum + (n - 1);  // row entries (at most m)
+    const int ks = row_key_shift(tiles_y);
     hipLaunchKernelGGL(emit_rows_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, n, m,
-                       (const int32_t *)ecum, (const uint2 *)rect, rowkey, rowval);
+                       (const int32_t *)ecum, (const uint2 *)rect, ks, rowkey, rowval);
     int rc = radix_sort<uint32_t>(rowkey, rowval, rowkey_s, rowval_s, mm, bits_for(tiles_y), 0u,
                                   sort_ws, st, r_dev);
     if (rc != LIG_OK) return rc;
     hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * ty, 256)), dim3(256), 0, st, rowbins,
                        2 * ty, 0);
     hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(mm, 1024)), dim3(256), 0,
-                       st, m, r_dev, (const uint32_t *)rowkey_s, tiles_y, rowbins);
+                       st, m, r_dev, (const uint32_t *)rowkey_s, tiles_y, rowbins,
+                       (uint32_t)((1ull << ks) - 1));
     const unsigned segs = (unsigned)tiles_y * (unsigned)P;
-    hipLaunchKernelGGL(row_count_kernel, dim3(segs), dim3(LIG_WAVE), (tiles_x + 1) * 4, st, P,
-                       tiles_x, (const int2 *)rowbins, (const uint32_t *)rowval_s,
-                       (const uint2 *)rect, counts);
+    hipLaunchKernelGGL(row_count_kernel, dim3(segs), dim3(kCountThreads), 0, st, P, tiles_x, ks,
+                       (const int2 *)rowbins, (const uint32_t *)rowkey_s, counts);
     hipLaunchKernelGGL(row_prefix_kernel, dim3(tiles_y), dim3(kPrefixThreads), 0, st, P, tiles_x,
                        tiles_y, (const int32_t *)counts, scol, tt, te, rowoff);
     scan_i32(rowoff, rowoff, ty + 1, false, row_ws, st);
     hipLaunchKernelGGL(row_bins_kernel, dim3(ceil_div(T, 256)), dim3(256), 0, st, (int)T,
                        tiles_x, (const int32_t *)rowoff, (const int32_t *)tt,
                        (const int32_t *)te, tile_bins);
-    const bool slots = emission_ids_sorted != nullptr;
-    const int bufn = scatter_buf(tiles_x, slots);
-    const size_t lds = (size_t)tiles_x * 20 + 4 + (size_t)bufn * (6 + (slots ? 4 : 0));
-    if (slots)
-        hipLaunchKernelGGL(row_scatter_kernel<true>, dim3(segs), dim3(LIG_WAVE), lds, st, P,
-                           tiles_x, m, bufn, (const int2 *)rowbins,
+    const int nwin = row_windows(tiles_x);
+    const unsigned waves = segs * (unsigned)nwin;
+    if (emission_ids_sorted)
+        hipLaunchKernelGGL(row_scatter_kernel<true>, dim3(waves), dim3(LIG_WAVE), 0, st, P, nwin,
+                           tiles_x, ks, m, (const int2 *)rowbins, (const uint32_t *)rowkey_s,
                            (const uint32_t *)rowval_s, (const uint2 *)rect,
                            (const int32_t *)counts, (const int32_t *)scol, (const int32_t *)te,
-                           (const int32_t *)rowoff, cum, gaussian_ids_sorted,
-                           emission_ids_sorted);
+                           (const int32_t *)rowoff,
+                           cum, gaussian_ids_sorted, emission_ids_sorted);
     else
-        hipLaunchKernelGGL(row_scatter_kernel<false>, dim3(segs), dim3(LIG_WAVE), lds, st, P,
-                           tiles_x, m, bufn, (const int2 *)rowbins,
+        hipLaunchKernelGGL(row_scatter_kernel<false>, dim3(waves), dim3(LIG_WAVE), 0, st, P, nwin,
+                           tiles_x, ks, m, (const int2 *)rowbins, (const uint32_t *)rowkey_s,
                            (const uint32_t *)rowval_s, (const uint2 *)rect,
                            (const int32_t *)counts, (const int32_t *)scol, (const int32_t *)te,
-                           (const int32_t *)rowoff, cum, gaussian_ids_sorted,
-                           emission_ids_sorted);
+                           (const int32_t *)rowoff,
+                           cum, gaussian_ids_sorted, emission_ids_sorted);
     return launch_status();
 }
 
@@ -1272,7 +1280,7 @@ extern "C" int lig_get_tile_bin_edges(int num_intersects, const int64_t *isect_i
         if (!isect_ids_sorted) return LIG_ERR_INVALID_ARGUMENT;
         hipLaunchKernelGGL((bin_edges_kernel<int64_t, 32>), dim3(ceil_div(num_intersects, 1024)),
                            dim3(256), 0, st, num_intersects, (const int32_t *)nullptr,
-                           isect_ids_sorted, (int)nt, tile_bins);
+                           isect_ids_sorted, (int)nt, tile_bins, ~(int64_t)0);
     }
     return launch_status();
 }
@@ -1343,7 +1351,7 @@ int bin_and_sort(int n, int num_intersects, const float *xys, const float *exten
         if (rc != LIG_OK) return rc;
         hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(m, 1024)), dim3(256), 0,
                            st, num_intersects, (const int32_t *)nullptr,
-                           (const uint32_t *)keys_sorted, nt, tile_bins);
+                           (const uint32_t *)keys_sorted, nt, tile_bins, ~0u);
         return launch_status();
     }
     slots_sorted = reinterpret_cast<uint32_t *>(emission_ids_sorted);
@@ -1481,7 +1489,7 @@ int bin_batched(int n, int m, const int32_t *m_dev, const float *xys, const floa
                                   bits_for(num_tiles), 0u, p, st, m_dev);
     if (rc != LIG_OK) return rc;
     hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(mm, 1024)), dim3(256), 0,
-                       st, m, m_dev, (const uint32_t *)keys_sorted, num_tiles, tile_bins);
+                       st, m, m_dev, (const uint32_t *)keys_sorted, num_tiles, tile_bins, ~0u);
     return launch_status();
 }
 
