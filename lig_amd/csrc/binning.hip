@@ -951,17 +951,18 @@ __global__ void row_bins_kernel(int num_tiles, int tiles_x, const int32_t *__res
 // Scatter pass: one wave per (row, segment, window of kRowWin tiles), a row's
 // waves on one XCD (its entries stay in that L2).  Lane k < kRowWin loads the
 // segment's first output position in the window's tile k; the 32 positions
-// then sit in scalar registers as the tiles' cursors.  The wave reads the
-// segment's row entries 64 at a time (keys fetched four rounds ahead) and
+// then serve as the tiles' cursors.  The wave reads the segment's row
+// entries four rounds of 64 at a time (the next four requested before these
+// are used) and
 // queues those whose column range meets the window, in order, in an LDS ring
-// of 128; every 64 queued entries (and the rest at the end) make a round: one
+// of 512; every 64 queued entries (and the rest at the end) make a round: one
 // entry per lane, and for each tile k of the window a ballot of the lanes
 // covering it gives each such lane its rank (mbcnt) and the cursor's advance
 // (popcount), so the covering lanes store their gids -- and emission slots
 // when asked -- at consecutive positions of the tile's bin.  Positions past
 // the output's capacity m are not written.
 constexpr int kRowWin = 32;          // tiles per scatter window
-constexpr int kRowQueue = 128;       // LDS ring of queued entries (two rounds)
+constexpr int kRowQueue = 512;       // LDS ring of queued entries
 constexpr int kScatterWaves = 16384; // scatter waves aimed for: segments per row follow
 
 template <bool SLOTS>
@@ -972,7 +973,7 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
     const int32_t *__restrict__ scol, const int32_t *__restrict__ te,
     const int32_t *__restrict__ rowoff, const int32_t *__restrict__ cum,
     int32_t *__restrict__ gids_out, int32_t *__restrict__ slots_out) {
-    __shared__ int qj[kRowQueue];       // queued row entries: index
+    __shared__ uint32_t qv[kRowQueue];  // queued row entries: value (gid | phantom bit)
     __shared__ uint32_t qx[kRowQueue];  // and column range x0 | x1 << 16
     const int lane = threadIdx.x;
     const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
@@ -1010,11 +1011,10 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
         wave_lds_sync();
         const bool valid = lane < cnt;
         const int q = (head + lane) & (kRowQueue - 1);
-        const int j = valid ? qj[q] : 0;
+        const uint32_t v = valid ? qv[q] : 0u;
         const uint32_t xx = valid ? qx[q] : 0u;
         wave_lds_sync();  // the slots just read are refilled after the round
         const int x0 = (int)(xx & 0xFFFFu), x1 = (int)(xx >> 16);
-        const uint32_t v = valid ? rowval[j] : 0u;
         const bool ph = (v & kPhantomBit) != 0;
         const int g = (int)(v & ~kPhantomBit);
         const int gid = ph ? 0 : g;
@@ -1062,16 +1062,13 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
     };
 
     int head = 0, tail = 0;  // queue positions (wave-uniform)
-    auto key_at = [&](int jj) { return jj < hi ? rowkey[jj] : 0u; };
-    uint32_t k0 = key_at(lo + lane), k1 = key_at(lo + LIG_WAVE + lane);
-    uint32_t k2 = key_at(lo + 2 * LIG_WAVE + lane), k3 = key_at(lo + 3 * LIG_WAVE + lane);
-    for (int base = lo; base < hi; base += LIG_WAVE) {
-        const uint32_t kc = k0;
-        k0 = k1;
-        k1 = k2;
-        k2 = k3;
-        k3 = key_at(base + 4 * LIG_WAVE + lane);
-        const int j = base + lane;
+    // unconditional loads (clamped to the segment's last entry; lanes past
+    // hi are masked by the overlap test): the next four rounds' keys and
+    // values are requested before this iteration's are filtered
+    auto key_at = [&](int jj) { return rowkey[min(jj, hi - 1)]; };
+    auto val_at = [&](int jj) { return rowval[min(jj, hi - 1)]; };
+    auto filter = [&](uint32_t kc, uint32_t vc, int b) {
+        const int j = b + lane;
         int x0 = 0, x1 = 0;
         rowkey_cols(kc, kshift, x0, x1);
         const bool ov = j < hi && x0 < whi && x1 > wlo;
@@ -1081,16 +1078,40 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
                                       (uint32_t)(mo >> 32),
                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mo, 0u))) &
                           (kRowQueue - 1);
-            qj[q] = j;
+            qv[q] = vc;
             qx[q] = (uint32_t)x0 | ((uint32_t)x1 << 16);
         }
         tail += __popcll(mo);
-        const bool last = base + LIG_WAVE >= hi;
+    };
+    uint32_t k0 = key_at(lo + lane), k1 = key_at(lo + LIG_WAVE + lane);
+    uint32_t k2 = key_at(lo + 2 * LIG_WAVE + lane), k3 = key_at(lo + 3 * LIG_WAVE + lane);
+    uint32_t v0 = val_at(lo + lane), v1 = val_at(lo + LIG_WAVE + lane);
+    uint32_t v2 = val_at(lo + 2 * LIG_WAVE + lane), v3 = val_at(lo + 3 * LIG_WAVE + lane);
+    for (int base = lo; base < hi; base += 4 * LIG_WAVE) {
+        const int nb = base + 4 * LIG_WAVE + lane;
+        const uint32_t n0 = key_at(nb), n1 = key_at(nb + LIG_WAVE);
+        const uint32_t n2 = key_at(nb + 2 * LIG_WAVE), n3 = key_at(nb + 3 * LIG_WAVE);
+        const uint32_t w0 = val_at(nb), w1 = val_at(nb + LIG_WAVE);
+        const uint32_t w2 = val_at(nb + 2 * LIG_WAVE), w3 = val_at(nb + 3 * LIG_WAVE);
+        // up to 4 x 64 entries join the (at most 63) queued: 319 < kRowQueue
+        filter(k0, v0, base);
+        filter(k1, v1, base + LIG_WAVE);
+        filter(k2, v2, base + 2 * LIG_WAVE);
+        filter(k3, v3, base + 3 * LIG_WAVE);
+        const bool last = base + 4 * LIG_WAVE >= hi;
         while (tail - head >= LIG_WAVE || (last && tail > head)) {
             const int cnt = min(LIG_WAVE, tail - head);
             round(head, cnt);
             head += cnt;
         }
+        k0 = n0;
+        k1 = n1;
+        k2 = n2;
+        k3 = n3;
+        v0 = w0;
+        v1 = w1;
+        v2 = w2;
+        v3 = w3;
     }
 }
 
