@@ -718,21 +718,19 @@ __device__ __forceinline__ void rowkey_cols(uint32_t k, int kshift, int &x0, int
     x1 = x0 + (int)((k >> (kshift + 10)) & 1023u) + 1;
 }
 
-// Per Gaussian: its tile rectangle and the number of tile rows it spans (the
-// rows of its emission slots; 1 for a phantom slot, 0 without a slot).
+// Per Gaussian: its tile rectangle ((0, 0) without a slot; row 0, columns
+// [0, 1) for a phantom slot).
 __global__ __launch_bounds__(256) void rect_rows_kernel(int n, const float *__restrict__ xys,
                                                         const float *__restrict__ extents,
                                                         const int32_t *__restrict__ cum,
                                                         int tiles_x, int tiles_y,
                                                         int block_width,
-                                                        uint2 *__restrict__ rect,
-                                                        int32_t *__restrict__ nrows) {
+                                                        uint2 *__restrict__ rect) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const int cnt = cum[g] - (g == 0 ? 0 : cum[g - 1]);
     if (cnt <= 0) {
         rect[g] = make_uint2(0u, 0u);
-        nrows[g] = 0;
         return;
     }
     const float ex = extents[2 * g], ey = extents[2 * g + 1];
@@ -742,67 +740,246 @@ __global__ __launch_bounds__(256) void rect_rows_kernel(int n, const float *__re
                   y1);
         rect[g] = make_uint2((uint32_t)x0 | ((uint32_t)x1 << 16),
                              (uint32_t)y0 | ((uint32_t)y1 << 16));
-        nrows[g] = y1 - y0;
     } else {
         rect[g] = make_uint2(1u << 16, (1u << 16) | kPhantomBit);
-        nrows[g] = 1;
     }
 }
 
-// Row entries in gid-major order: key = row_key(tile row, x0, x1) (a phantom
-// slot: row 0, columns [0, 1)), value = gid (phantom bit set for a phantom
-// slot).  A wave covers its 64 Gaussians' joint entry range 64 entries at a
-// time; a lane finds the owner of its entry by binary search over the wave's
-// starts, so the stores are coalesced.  `cap` bounds the entries written
-// (they never exceed the intersection count the buffers hold).
-__global__ __launch_bounds__(256) void emit_rows_kernel(int n, int cap,
-                                                        const int32_t *__restrict__ ecum,
-                                                        const uint2 *__restrict__ rect,
-                                                        int kshift,
-                                                        uint32_t *__restrict__ rowkey,
-                                                        uint32_t *__restrict__ rowval) {
-    const int lane = threadIdx.x & (LIG_WAVE - 1);
-    const int wv = threadIdx.x / LIG_WAVE;
-    const int g0 = blockIdx.x * blockDim.x + wv * LIG_WAVE;
-    if (g0 >= n) return;  // wave-uniform
-    const int g = g0 + lane;
-    const int wstart = g0 == 0 ? 0 : ecum[g0 - 1];
-    const int wend = min(ecum[min(g0 + LIG_WAVE, n) - 1], cap);
-    __shared__ int s_start[256 / LIG_WAVE][LIG_WAVE];
-    __shared__ uint32_t s_key[256 / LIG_WAVE][LIG_WAVE];
-    __shared__ uint32_t s_ph[256 / LIG_WAVE][LIG_WAVE];
-    int start = wend;
-    uint32_t key = 0u, ph = 0u;
-    if (g < n) {
-        start = g == 0 ? 0 : ecum[g - 1];
+// Row entries straight from the rectangles, in (row, gid) order: the
+// Gaussians are cut into segments of `gseg` consecutive gids; a counting pass
+// gives every (segment, row) the number of the segment's Gaussians spanning
+// the row, a prefix over segments per row and a scan of the row totals give
+// each segment's first position in each row (and the row bins), and a
+// scatter pass gives every (segment, window of kRowWin rows) one wave, which
+// filters the segment's Gaussians down to those spanning a row of the window
+// and, row by row, ranks them by ballot and writes their entries at
+// consecutive positions.  An entry is key = row_key(row, x0, x1), value = gid
+// (phantom bit set for a phantom slot, whose key is 0: row 0, columns
+// [0, 1)); the order is that of a stable sort by row of the gid-major entries.
+constexpr int kRowWin = 32;           // rows (tiles) per window of the scatter passes
+constexpr int kRowQueue = 512;        // LDS ring of queued entries
+constexpr int kScatterWaves = 16384;  // scatter waves aimed for: segment counts follow
+constexpr int kMaxRowsPacked = 4096;  // rows the key's row field holds (12 bits)
+constexpr int kRowCountThreads = 256;
+
+// Gaussians per segment of the direct row pass.
+int gauss_segment(int n, int tiles_y) {
+    const long long nrw = (tiles_y + kRowWin - 1) / kRowWin;
+    const long long target = std::max(1LL, (kScatterWaves + nrw - 1) / nrw);
+    return (int)std::max(256LL, ((long long)n + target - 1) / target);
+}
+
+__global__ __launch_bounds__(kRowCountThreads) void gauss_row_count_kernel(
+    int n, int gseg, int tiles_y, const uint2 *__restrict__ rect, int32_t *__restrict__ counts) {
+    __shared__ int diff[kMaxRowsPacked + 1];
+    __shared__ int wsum[kRowCountThreads / LIG_WAVE];
+    const int s = blockIdx.x;
+    const int g0 = s * gseg, g1 = min(n, g0 + gseg);
+    for (int r = threadIdx.x; r <= tiles_y; r += kRowCountThreads) diff[r] = 0;
+    __syncthreads();
+    for (int g = g0 + threadIdx.x; g < g1; g += kRowCountThreads) {
         int x0, y0, x1, y1;
-        bool p;
-        unpack_rect(rect[g], x0, y0, x1, y1, p);
-        // a phantom's rectangle is row 0, columns [0, 1): key 0
-        key = x1 > x0 ? row_key(y0, x0, x1, kshift) : 0u;
-        ph = p ? kPhantomBit : 0u;
-    }
-    s_start[wv][lane] = start;
-    s_key[wv][lane] = key;
-    s_ph[wv][lane] = ph;
-    wave_lds_sync();
-    for (int base = wstart; base < wend; base += LIG_WAVE) {
-        const int pos = base + lane;
-        if (pos < wend) {
-            // owner: the last lane whose range starts at or before pos
-            int lo = 0, hi = LIG_WAVE - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_start[wv][mid] <= pos)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            LIG_CHECK(s_start[wv][lo] <= pos && pos < cap);
-            // the row sits in the low bits: key + k is row y0 + k (y0 + k < 2^kshift)
-            rowkey[pos] = s_key[wv][lo] + (uint32_t)(pos - s_start[wv][lo]);
-            rowval[pos] = (uint32_t)(g0 + lo) | s_ph[wv][lo];
+        bool ph;
+        unpack_rect(rect[g], x0, y0, x1, y1, ph);
+        if (y1 > y0) {
+            LIG_CHECK(y1 <= tiles_y);
+            atomicAdd(&diff[y0], 1);
+            atomicAdd(&diff[y1], -1);
         }
+    }
+    __syncthreads();
+    // prefix over rows: thread i takes rows i kPer .. i kPer + kPer - 1
+    constexpr int kPer = kMaxRowsPacked / kRowCountThreads;
+    int d[kPer];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int r = threadIdx.x * kPer + k;
+        d[k] = r < tiles_y ? diff[r] : 0;
+        mine += d[k];
+    }
+    int total;
+    int run = block_exclusive_scan_n<kRowCountThreads>(mine, wsum, &total);
+    int32_t *out = counts + (size_t)s * tiles_y;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int r = threadIdx.x * kPer + k;
+        run += d[k];
+        if (r < tiles_y) out[r] = run;
+    }
+}
+
+// Per row (one block): the exclusive prefix of the segments' counts (pre,
+// same layout as counts) and the row total (rowtot[r]; rowtot[rows] = 0 for
+// the scan that follows).
+__global__ __launch_bounds__(kRowCountThreads) void gauss_row_prefix_kernel(
+    int nseg, int tiles_y, const int32_t *__restrict__ counts, int32_t *__restrict__ pre,
+    int32_t *__restrict__ rowtot) {
+    __shared__ int wsum[kRowCountThreads / LIG_WAVE];
+    const int r = blockIdx.x;
+    const int per = (nseg + kRowCountThreads - 1) / kRowCountThreads;
+    const int s0 = min(nseg, (int)threadIdx.x * per), s1 = min(nseg, s0 + per);
+    int mine = 0;
+    for (int q = s0; q < s1; ++q) mine += counts[(size_t)q * tiles_y + r];
+    int total;
+    int run = block_exclusive_scan_n<kRowCountThreads>(mine, wsum, &total);
+    for (int q = s0; q < s1; ++q) {
+        pre[(size_t)q * tiles_y + r] = run;
+        run += counts[(size_t)q * tiles_y + r];
+    }
+    if (threadIdx.x == 0) {
+        rowtot[r] = total;
+        if (r == 0) rowtot[tiles_y] = 0;
+    }
+}
+
+// Row bins from the scanned row totals, clamped to the entry buffer's m.
+__global__ void entry_bins_kernel(int rows, int m, const int32_t *__restrict__ off,
+                                  int2 *__restrict__ rowbins) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows) rowbins[r] = make_int2(min(off[r], m), min(off[r + 1], m));
+}
+
+// The ballot rounds of a window: lane l holds queue entry head + l (cnt of
+// them); for each of the window's wn lines k (rows or tiles) the lanes with
+// (unsigned)(d + k) < wd cover it and write at consecutive positions from
+// the line's cursor.  `cur` holds the cursors (fast: all 32 lines present and
+// every position below m, below 2^30) or, in lane k of `curv`, the cursor of
+// line k (checked against m).  put(pos, k) stores one lane's output.
+template <typename Put>
+__device__ __forceinline__ void ballot_lines(bool fast, int wn, int m, int d, unsigned wd,
+                                             int (&cur)[kRowWin], int &curv, Put put) {
+    const int lane = threadIdx.x & (LIG_WAVE - 1);
+    if (fast) {
+#pragma unroll
+        for (int k = 0; k < kRowWin; ++k) {
+            const bool c = (unsigned)(d + k) < wd;
+            const uint64_t mk = __ballot(c);
+            // the cursor joins the rank as mbcnt's addend
+            const uint32_t pos = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, (uint32_t)cur[k]));
+            if (c) {
+                LIG_CHECK((int)pos >= 0 && (int)pos < m);
+                put(pos, k);
+            }
+            cur[k] += __popcll(mk);
+        }
+    } else {
+        for (int k = 0; k < wn; ++k) {
+            const bool c = (unsigned)(d + k) < wd;
+            const uint64_t mk = __ballot(c);
+            const int ck = __builtin_amdgcn_readlane(curv, k);
+            const int pos = ck + (int)__builtin_amdgcn_mbcnt_hi(
+                                     (uint32_t)(mk >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
+            if (c && pos < m) put((uint32_t)pos, k);
+            curv = lane == k ? ck + __popcll(mk) : curv;
+        }
+    }
+}
+
+// The window's cursors: lane k < wn holds line k's first position (start)
+// and its end; fast when the window is full and every end is at most m <= 2^30.
+__device__ __forceinline__ bool window_cursors(int start, int end, int wn, int m,
+                                               int (&cur)[kRowWin]) {
+    bool fast = wn == kRowWin && m <= (1 << 30);
+#pragma unroll
+    for (int off = LIG_WAVE / 2; off > 0; off >>= 1) end = max(end, __shfl_xor(end, off, LIG_WAVE));
+    fast = fast && end <= m;
+#pragma unroll
+    for (int k = 0; k < kRowWin; ++k) cur[k] = __builtin_amdgcn_readlane(start, k);
+    return fast;
+}
+
+__device__ __forceinline__ void put32(int32_t *__restrict__ base, uint32_t pos, int32_t v) {
+    *reinterpret_cast<int32_t *>(reinterpret_cast<char *>(base) + (size_t)(pos * 4u)) = v;
+}
+
+// Scatter of the direct row pass: one wave per (segment, window of kRowWin
+// rows), a segment's windows on one XCD.
+__global__ __launch_bounds__(LIG_WAVE) void gauss_row_scatter_kernel(
+    int n, int gseg, int nrw, int tiles_y, int kshift, int m, const uint2 *__restrict__ rect,
+    const int32_t *__restrict__ counts, const int32_t *__restrict__ pre,
+    const int32_t *__restrict__ off, uint32_t *__restrict__ rowkey,
+    uint32_t *__restrict__ rowval) {
+    __shared__ uint32_t qv[kRowQueue];  // gid | phantom bit
+    __shared__ uint32_t qk[kRowQueue];  // key of the Gaussian's row 0
+    __shared__ uint32_t qy[kRowQueue];  // y0 | y1 << 16
+    const int lane = threadIdx.x;
+    const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int s = u / nrw;
+    const int w = u - s * nrw;
+    const int g0 = s * gseg, g1 = min(n, g0 + gseg);
+    if (g0 >= g1) return;  // wave-uniform
+    const int r0 = w * kRowWin;
+    const int rn = min(kRowWin, tiles_y - r0);
+    const int r1 = r0 + rn;
+    int start = 0, end = 0;
+    if (lane < rn) {
+        const size_t c = (size_t)s * tiles_y + r0 + lane;
+        start = off[r0 + lane] + pre[c];
+        end = start + counts[c];
+    }
+    int cur[kRowWin];
+    const bool fast = window_cursors(start, end, rn, m, cur);
+
+    auto round = [&](int head, int cnt) {
+        wave_lds_sync();
+        const bool valid = lane < cnt;
+        const int q = (head + lane) & (kRowQueue - 1);
+        const uint32_t v = valid ? qv[q] : 0u;
+        const uint32_t kb = valid ? qk[q] : 0u;
+        const uint32_t yy = valid ? qy[q] : 0u;
+        wave_lds_sync();  // the slots just read are refilled after the round
+        const int y0 = (int)(yy & 0xFFFFu), y1 = (int)(yy >> 16);
+        ballot_lines(fast, rn, m, r0 - y0, (unsigned)(y1 - y0), cur, start,
+                     [&](uint32_t pos, int k) {
+                         put32(reinterpret_cast<int32_t *>(rowkey), pos,
+                               (int32_t)(kb + (uint32_t)(r0 + k)));
+                         put32(reinterpret_cast<int32_t *>(rowval), pos, (int32_t)v);
+                     });
+    };
+
+    int head = 0, tail = 0;
+    auto rect_at = [&](int g) { return rect[min(g, g1 - 1)]; };
+    auto filter = [&](uint2 rc, int b) {
+        const int g = b + lane;
+        int x0, y0, x1, y1;
+        bool ph;
+        unpack_rect(rc, x0, y0, x1, y1, ph);
+        const bool ov = g < g1 && y1 > y0 && y0 < r1 && y1 > r0;
+        const uint64_t mo = __ballot(ov);
+        if (ov) {
+            const int q = (tail + (int)__builtin_amdgcn_mbcnt_hi(
+                                      (uint32_t)(mo >> 32),
+                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mo, 0u))) &
+                          (kRowQueue - 1);
+            qv[q] = (uint32_t)g | (ph ? kPhantomBit : 0u);
+            qk[q] = row_key(0, x0, x1, kshift);  // a phantom's: row_key(0, 0, 1) = 0
+            qy[q] = (uint32_t)y0 | ((uint32_t)y1 << 16);
+        }
+        tail += __popcll(mo);
+    };
+    uint2 c0 = rect_at(g0 + lane), c1 = rect_at(g0 + LIG_WAVE + lane);
+    uint2 c2 = rect_at(g0 + 2 * LIG_WAVE + lane), c3 = rect_at(g0 + 3 * LIG_WAVE + lane);
+    for (int base = g0; base < g1; base += 4 * LIG_WAVE) {
+        const int nb = base + 4 * LIG_WAVE + lane;
+        const uint2 n0 = rect_at(nb), n1 = rect_at(nb + LIG_WAVE);
+        const uint2 n2 = rect_at(nb + 2 * LIG_WAVE), n3 = rect_at(nb + 3 * LIG_WAVE);
+        filter(c0, base);
+        filter(c1, base + LIG_WAVE);
+        filter(c2, base + 2 * LIG_WAVE);
+        filter(c3, base + 3 * LIG_WAVE);
+        const bool last = base + 4 * LIG_WAVE >= g1;
+        while (tail - head >= LIG_WAVE || (last && tail > head)) {
+            const int cnt = min(LIG_WAVE, tail - head);
+            round(head, cnt);
+            head += cnt;
+        }
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
     }
 }
 
@@ -950,21 +1127,16 @@ __global__ void row_bins_kernel(int num_tiles, int tiles_x, const int32_t *__res
 
 // Scatter pass: one wave per (row, segment, window of kRowWin tiles), a row's
 // waves on one XCD (its entries stay in that L2).  Lane k < kRowWin loads the
-// segment's first output position in the window's tile k; the 32 positions
-// then serve as the tiles' cursors.  The wave reads the segment's row
-// entries four rounds of 64 at a time (the next four requested before these
-// are used) and
-// queues those whose column range meets the window, in order, in an LDS ring
-// of 512; every 64 queued entries (and the rest at the end) make a round: one
-// entry per lane, and for each tile k of the window a ballot of the lanes
-// covering it gives each such lane its rank (mbcnt) and the cursor's advance
-// (popcount), so the covering lanes store their gids -- and emission slots
-// when asked -- at consecutive positions of the tile's bin.  Positions past
-// the output's capacity m are not written.
-constexpr int kRowWin = 32;          // tiles per scatter window
-constexpr int kRowQueue = 512;       // LDS ring of queued entries
-constexpr int kScatterWaves = 16384; // scatter waves aimed for: segments per row follow
-
+// segment's first output position in the window's tile k, the tile's cursor.
+// The wave reads the segment's row entries four rounds of 64 at a time (the
+// next four requested before these are used) and queues those whose column
+// range meets the window, in order, in an LDS ring of 512; every 64 queued
+// entries (and the rest at the end) make a round: one entry per lane, and
+// for each tile k of the window a ballot of the lanes covering it gives each
+// such lane its rank (mbcnt) and the cursor's advance (popcount), so the
+// covering lanes store their gids -- and emission slots when asked -- at
+// consecutive positions of the tile's bin.  Positions past the output's
+// capacity m are not written.
 template <bool SLOTS>
 __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
     int P, int nwin, int tiles_x, int kshift, int m, const int2 *__restrict__ rowbins,
@@ -987,24 +1159,14 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
     const int wlo = w * kRowWin;
     const int wn = min(kRowWin, tiles_x - wlo);
     const int whi = wlo + wn;
-    int start = 0;
+    int start = 0, end = 0;
     if (lane < wn) {
         const int t = wlo + lane;
         start = rowoff[r] + te[(size_t)r * tiles_x + t] + scol[(size_t)rs * tiles_x + t];
+        end = start + counts[(size_t)rs * tiles_x + t];
     }
-    // fast rounds: a full window whose positions all stay below m (and
-    // below 2^30, so 32-bit byte offsets address them); cursors in scalar
-    // registers.  Otherwise the cursors stay in lane k of `start` and every
-    // store is checked against m.
-    int end = 0;
-    if (lane < wn) end = start + counts[(size_t)rs * tiles_x + wlo + lane];
-    bool fast = wn == kRowWin && m <= (1 << 30);
-#pragma unroll
-    for (int off = LIG_WAVE / 2; off > 0; off >>= 1) end = max(end, __shfl_xor(end, off, LIG_WAVE));
-    fast = fast && end <= m;
     int cur[kRowWin];
-#pragma unroll
-    for (int k = 0; k < kRowWin; ++k) cur[k] = __builtin_amdgcn_readlane(start, k);
+    const bool fast = window_cursors(start, end, wn, m, cur);
 
     // one round: queue slots head .. head + cnt - 1, one per lane
     auto round = [&](int head, int cnt) {
@@ -1023,42 +1185,11 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
             slot0 = g == 0 ? 0 : cum[g - 1];
             if (!ph) slot0 += (r - (int)(rect[g].y & 0xFFFFu)) * (x1 - x0) - x0;
         }
-        const unsigned wd = (unsigned)(x1 - x0);  // 0 on an idle lane
-        const int d = wlo - x0;
-        if (fast) {
-#pragma unroll
-            for (int k = 0; k < kRowWin; ++k) {
-                const bool c = (unsigned)(d + k) < wd;
-                const uint64_t mk = __ballot(c);
-                // the cursor joins the rank as mbcnt's addend
-                const uint32_t pos = __builtin_amdgcn_mbcnt_hi(
-                    (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, (uint32_t)cur[k]));
-                if (c) {
-                    LIG_CHECK((int)pos >= 0 && (int)pos < m);
-                    *reinterpret_cast<int32_t *>(reinterpret_cast<char *>(gids_out) +
-                                                 (size_t)(pos * 4u)) = gid;
-                    if (SLOTS)
-                        *reinterpret_cast<int32_t *>(reinterpret_cast<char *>(slots_out) +
-                                                     (size_t)(pos * 4u)) =
-                            ph ? slot0 : slot0 + wlo + k;
-                }
-                cur[k] += __popcll(mk);
-            }
-        } else {
-            for (int k = 0; k < wn; ++k) {
-                const bool c = (unsigned)(d + k) < wd;
-                const uint64_t mk = __ballot(c);
-                const int ck = __builtin_amdgcn_readlane(start, k);
-                const int pos = ck + (int)__builtin_amdgcn_mbcnt_hi(
-                                         (uint32_t)(mk >> 32),
-                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
-                if (c && pos < m) {
-                    gids_out[pos] = gid;
-                    if (SLOTS) slots_out[pos] = ph ? slot0 : slot0 + wlo + k;
-                }
-                start = lane == k ? ck + __popcll(mk) : start;
-            }
-        }
+        ballot_lines(fast, wn, m, wlo - x0, (unsigned)(x1 - x0), cur, start,
+                     [&](uint32_t pos, int k) {
+                         put32(gids_out, pos, gid);
+                         if (SLOTS) put32(slots_out, pos, ph ? slot0 : slot0 + wlo + k);
+                     });
     };
 
     int head = 0, tail = 0;  // queue positions (wave-uniform)
@@ -1115,13 +1246,8 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
     }
 }
 
-// Bits the row sort orders by for `rows` rows (its passes x digit width),
-// the shift of the column fields in a row entry's key.
-int row_key_shift(int rows) {
-    int passes, rb;
-    digit_plan(bits_for(rows), &passes, &rb);
-    return passes * rb;
-}
+// Bits of a row entry key's row field (the column fields sit above them).
+int row_key_shift(int rows) { return bits_for(rows); }
 
 int row_windows(int tiles_x) { return (tiles_x + kRowWin - 1) / kRowWin; }
 
@@ -1134,25 +1260,28 @@ int row_segments(int m, int tiles_x, int tiles_y) {
     return (int)std::max(1LL, std::min(std::min(by_waves, by_len), 4096LL));
 }
 
-// Workspace of the row path (0 when it does not apply): rectangles, row
-// counts and their scan, the row entries and their sorted copies, the row
-// sort's workspace, the row bins, the (row, segment, tile) counts and their
-// segment prefixes, tile totals and their row prefixes, row totals and their
-// scan.
+// Workspace of the row path (0 when it does not apply): rectangles, the
+// direct row pass's (segment, row) counts and their prefixes, row totals and
+// their scan and its workspace, the row entries (keys, values), the row bins,
+// the (row, segment, tile) counts and their segment prefixes, tile totals and
+// their row prefixes, row totals and their scan.
 size_t rows_ws_bytes(int n, int m, int tiles_x, int tiles_y, int *P_out) {
     if (n <= 0 || m <= 0 || tiles_x < 1 || tiles_y < 1 || tiles_x > kRowMaxTiles ||
-        tiles_y > kRowMaxRows || row_key_shift(tiles_y) + 20 > 32)
+        tiles_y > kMaxRowsPacked || row_key_shift(tiles_y) + 20 > 32)
         return 0;
     const int P = row_segments(m, tiles_x, tiles_y);
     const size_t T = (size_t)tiles_x * tiles_y;
     const size_t nctr = (size_t)P * T;
-    if (nctr > (size_t)INT32_MAX) return 0;
+    const int gseg = gauss_segment(n, tiles_y);
+    const size_t nseg = ((size_t)n + gseg - 1) / gseg;
+    const size_t gctr = nseg * (size_t)tiles_y;
+    if (nctr > (size_t)INT32_MAX || gctr > (size_t)INT32_MAX) return 0;
     if (P_out) *P_out = P;
     const size_t nn = (size_t)n, mm = (size_t)m, ty = (size_t)tiles_y;
-    return align_up(8 * nn) + align_up(4 * nn) + align_up(scan_ws_elems(nn) * 4) +
-           4 * align_up(4 * mm) + sort_ws_bytes<uint32_t>(mm, bits_for(tiles_y)) +
-           align_up(8 * ty) + 2 * align_up(4 * nctr) + 2 * align_up(4 * T) +
-           align_up(4 * (ty + 1)) + align_up(scan_ws_elems(ty + 1) * 4) + 256;
+    return align_up(8 * nn) + 2 * align_up(4 * gctr) + align_up(4 * (ty + 1)) +
+           align_up(scan_ws_elems(ty + 1) * 4) + 2 * align_up(4 * mm) + align_up(8 * ty) +
+           2 * align_up(4 * nctr) + 2 * align_up(4 * T) + align_up(4 * (ty + 1)) +
+           align_up(scan_ws_elems(ty + 1) * 4) + 256;
 }
 
 // The row path of bin_and_sort (workspace checked by the caller).
@@ -1163,6 +1292,9 @@ int bin_rows(int n, int m, const float *xys, const float *extents, const int32_t
     rows_ws_bytes(n, m, tiles_x, tiles_y, &P);
     const size_t nn = (size_t)n, mm = (size_t)m, T = (size_t)tiles_x * tiles_y;
     const size_t nctr = (size_t)P * T, ty = (size_t)tiles_y;
+    const int gseg = gauss_segment(n, tiles_y);
+    const int nseg = (int)((nn + gseg - 1) / gseg);
+    const size_t gctr = (size_t)nseg * ty;
     char *p = ws;
     auto take = [&p](size_t bytes) {
         char *q = p;
@@ -1170,14 +1302,13 @@ int bin_rows(int n, int m, const float *xys, const float *extents, const int32_t
         return q;
     };
     uint2 *rect = reinterpret_cast<uint2 *>(take(8 * nn));
-    int32_t *ecum = reinterpret_cast<int32_t *>(take(4 * nn));
-    int32_t *scan_ws = reinterpret_cast<int32_t *>(take(scan_ws_elems(nn) * 4));
+    int32_t *gcounts = reinterpret_cast<int32_t *>(take(4 * gctr));
+    int32_t *gpre = reinterpret_cast<int32_t *>(take(4 * gctr));
+    int32_t *eoff = reinterpret_cast<int32_t *>(take(4 * (ty + 1)));
+    int32_t *eoff_ws = reinterpret_cast<int32_t *>(take(scan_ws_elems(ty + 1) * 4));
     uint32_t *rowkey = reinterpret_cast<uint32_t *>(take(4 * mm));
     uint32_t *rowval = reinterpret_cast<uint32_t *>(take(4 * mm));
-    uint32_t *rowkey_s = reinterpret_cast<uint32_t *>(take(4 * mm));
-    uint32_t *rowval_s = reinterpret_cast<uint32_t *>(take(4 * mm));
-    char *sort_ws = take(sort_ws_bytes<uint32_t>(mm, bits_for(tiles_y)));
-    int32_t *rowbins = reinterpret_cast<int32_t *>(take(8 * ty));
+    int2 *rowbins = reinterpret_cast<int2 *>(take(8 * ty));
     int32_t *counts = reinterpret_cast<int32_t *>(take(4 * nctr));
     int32_t *scol = reinterpret_cast<int32_t *>(take(4 * nctr));
     int32_t *tt = reinterpret_cast<int32_t *>(take(4 * T));
@@ -1185,24 +1316,26 @@ int bin_rows(int n, int m, const float *xys, const float *extents, const int32_t
     int32_t *rowoff = reinterpret_cast<int32_t *>(take(4 * (ty + 1)));
     int32_t *row_ws = reinterpret_cast<int32_t *>(take(scan_ws_elems(ty + 1) * 4));
 
-    hipLaunchKernelGGL(rect_rows_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, n, xys,
-                       extents, cum, tiles_x, tiles_y, block_width, rect, ecum);
-    scan_i32(ecum, ecum, nn, true, scan_ws, st);
-    const int32_t *r_dev = ecum + (n - 1);  // row entries (at most m)
     const int ks = row_key_shift(tiles_y);
-    hipLaunchKernelGGL(emit_rows_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, n, m,
-                       (const int32_t *)ecum, (const uint2 *)rect, ks, rowkey, rowval);
-    int rc = radix_sort<uint32_t>(rowkey, rowval, rowkey_s, rowval_s, mm, bits_for(tiles_y), 0u,
-                                  sort_ws, st, r_dev);
-    if (rc != LIG_OK) return rc;
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * ty, 256)), dim3(256), 0, st, rowbins,
-                       2 * ty, 0);
-    hipLaunchKernelGGL((bin_edges_kernel<uint32_t, 0>), dim3(ceil_div(mm, 1024)), dim3(256), 0,
-                       st, m, r_dev, (const uint32_t *)rowkey_s, tiles_y, rowbins,
-                       (uint32_t)((1ull << ks) - 1));
+    hipLaunchKernelGGL(rect_rows_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, st, n, xys,
+                       extents, cum, tiles_x, tiles_y, block_width, rect);
+    // row entries in (row, gid) order, straight from the rectangles
+    hipLaunchKernelGGL(gauss_row_count_kernel, dim3(nseg), dim3(kRowCountThreads), 0, st, n,
+                       gseg, tiles_y, (const uint2 *)rect, gcounts);
+    hipLaunchKernelGGL(gauss_row_prefix_kernel, dim3(tiles_y), dim3(kRowCountThreads), 0, st,
+                       nseg, tiles_y, (const int32_t *)gcounts, gpre, eoff);
+    scan_i32(eoff, eoff, ty + 1, false, eoff_ws, st);
+    hipLaunchKernelGGL(entry_bins_kernel, dim3(ceil_div(ty, 256)), dim3(256), 0, st, tiles_y, m,
+                       (const int32_t *)eoff, rowbins);
+    const int nrw = (tiles_y + kRowWin - 1) / kRowWin;
+    hipLaunchKernelGGL(gauss_row_scatter_kernel, dim3((unsigned)nseg * (unsigned)nrw),
+                       dim3(LIG_WAVE), 0, st, n, gseg, nrw, tiles_y, ks, m, (const uint2 *)rect,
+                       (const int32_t *)gcounts, (const int32_t *)gpre, (const int32_t *)eoff,
+                       rowkey, rowval);
+    // tiles from the row entries
     const unsigned segs = (unsigned)tiles_y * (unsigned)P;
     hipLaunchKernelGGL(row_count_kernel, dim3(segs), dim3(kCountThreads), 0, st, P, tiles_x, ks,
-                       (const int2 *)rowbins, (const uint32_t *)rowkey_s, counts);
+                       (const int2 *)rowbins, (const uint32_t *)rowkey, counts);
     hipLaunchKernelGGL(row_prefix_kernel, dim3(tiles_y), dim3(kPrefixThreads), 0, st, P, tiles_x,
                        tiles_y, (const int32_t *)counts, scol, tt, te, rowoff);
     scan_i32(rowoff, rowoff, ty + 1, false, row_ws, st);
@@ -1213,18 +1346,18 @@ int bin_rows(int n, int m, const float *xys, const float *extents, const int32_t
     const unsigned waves = segs * (unsigned)nwin;
     if (emission_ids_sorted)
         hipLaunchKernelGGL(row_scatter_kernel<true>, dim3(waves), dim3(LIG_WAVE), 0, st, P, nwin,
-                           tiles_x, ks, m, (const int2 *)rowbins, (const uint32_t *)rowkey_s,
-                           (const uint32_t *)rowval_s, (const uint2 *)rect,
+                           tiles_x, ks, m, (const int2 *)rowbins, (const uint32_t *)rowkey,
+                           (const uint32_t *)rowval, (const uint2 *)rect,
                            (const int32_t *)counts, (const int32_t *)scol, (const int32_t *)te,
-                           (const int32_t *)rowoff,
-                           cum, gaussian_ids_sorted, emission_ids_sorted);
+                           (const int32_t *)rowoff, cum, gaussian_ids_sorted,
+                           emission_ids_sorted);
     else
         hipLaunchKernelGGL(row_scatter_kernel<false>, dim3(waves), dim3(LIG_WAVE), 0, st, P, nwin,
-                           tiles_x, ks, m, (const int2 *)rowbins, (const uint32_t *)rowkey_s,
-                           (const uint32_t *)rowval_s, (const uint2 *)rect,
+                           tiles_x, ks, m, (const int2 *)rowbins, (const uint32_t *)rowkey,
+                           (const uint32_t *)rowval, (const uint2 *)rect,
                            (const int32_t *)counts, (const int32_t *)scol, (const int32_t *)te,
-                           (const int32_t *)rowoff,
-                           cum, gaussian_ids_sorted, emission_ids_sorted);
+                           (const int32_t *)rowoff, cum, gaussian_ids_sorted,
+                           emission_ids_sorted);
     return launch_status();
 }
 
