@@ -897,11 +897,13 @@ __device__ __forceinline__ void put32(int32_t *__restrict__ base, uint32_t pos, 
 
 // Scatter of the direct row pass: one wave per (segment, window of kRowWin
 // rows), a segment's windows on one XCD.
+template <bool SLOTS>
 __global__ __launch_bounds__(LIG_WAVE) void gauss_row_scatter_kernel(
     int n, int gseg, int nrw, int tiles_y, int kshift, int m, const uint2 *__restrict__ rect,
-    const int32_t *__restrict__ counts, const int32_t *__restrict__ pre,
-    const int32_t *__restrict__ off, uint32_t *__restrict__ rowkey,
-    uint32_t *__restrict__ rowval) {
+    const int32_t *__restrict__ cum, const int32_t *__restrict__ counts,
+    const int32_t *__restrict__ pre, const int32_t *__restrict__ off,
+    uint32_t *__restrict__ rowkey, uint32_t *__restrict__ rowval,
+    int32_t *__restrict__ rowslot) {
     __shared__ uint32_t qv[kRowQueue];  // gid | phantom bit
     __shared__ uint32_t qk[kRowQueue];  // key of the Gaussian's row 0
     __shared__ uint32_t qy[kRowQueue];  // y0 | y1 << 16
@@ -932,11 +934,23 @@ __global__ __launch_bounds__(LIG_WAVE) void gauss_row_scatter_kernel(
         const uint32_t yy = valid ? qy[q] : 0u;
         wave_lds_sync();  // the slots just read are refilled after the round
         const int y0 = (int)(yy & 0xFFFFu), y1 = (int)(yy >> 16);
+        // emission slot of the entry's row at column 0: cum[g - 1] + (row - y0)
+        // (x1 - x0) - x0 (a phantom's single slot: cum[g - 1])
+        int sb = 0, sw = 0;
+        if (SLOTS && valid) {
+            const int g = (int)(v & ~kPhantomBit);
+            int x0, x1;
+            rowkey_cols(kb, kshift, x0, x1);
+            const bool ph = (v & kPhantomBit) != 0;
+            sb = (g == 0 ? 0 : cum[g - 1]) + (ph ? 0 : -y0 * (x1 - x0) - x0);
+            sw = ph ? 0 : x1 - x0;
+        }
         ballot_lines(fast, rn, m, r0 - y0, (unsigned)(y1 - y0), cur, start,
                      [&](uint32_t pos, int k) {
                          put32(reinterpret_cast<int32_t *>(rowkey), pos,
                                (int32_t)(kb + (uint32_t)(r0 + k)));
                          put32(reinterpret_cast<int32_t *>(rowval), pos, (int32_t)v);
+                         if (SLOTS) put32(rowslot, pos, sb + (r0 + k) * sw);
                      });
     };
 
@@ -1134,19 +1148,20 @@ __global__ void row_bins_kernel(int num_tiles, int tiles_x, const int32_t *__res
 // entries (and the rest at the end) make a round: one entry per lane, and
 // for each tile k of the window a ballot of the lanes covering it gives each
 // such lane its rank (mbcnt) and the cursor's advance (popcount), so the
-// covering lanes store their gids -- and emission slots when asked -- at
-// consecutive positions of the tile's bin.  Positions past the output's
-// capacity m are not written.
+// covering lanes store their gids -- and emission slots when asked, from
+// each entry's slot at column 0 -- at consecutive positions of the tile's
+// bin.  Positions past the output's capacity m are not written.
 template <bool SLOTS>
 __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
     int P, int nwin, int tiles_x, int kshift, int m, const int2 *__restrict__ rowbins,
     const uint32_t *__restrict__ rowkey, const uint32_t *__restrict__ rowval,
-    const uint2 *__restrict__ rect, const int32_t *__restrict__ counts,
+    const int32_t *__restrict__ rowslot, const int32_t *__restrict__ counts,
     const int32_t *__restrict__ scol, const int32_t *__restrict__ te,
-    const int32_t *__restrict__ rowoff, const int32_t *__restrict__ cum,
-    int32_t *__restrict__ gids_out, int32_t *__restrict__ slots_out) {
+    const int32_t *__restrict__ rowoff, int32_t *__restrict__ gids_out,
+    int32_t *__restrict__ slots_out) {
     __shared__ uint32_t qv[kRowQueue];  // queued row entries: value (gid | phantom bit)
     __shared__ uint32_t qx[kRowQueue];  // and column range x0 | x1 << 16
+    __shared__ int qs[SLOTS ? kRowQueue : 1];  // and slot at column 0 (emission slots)
     const int lane = threadIdx.x;
     const int u = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int rs = u / nwin;  // r P + s
@@ -1175,20 +1190,16 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
         const int q = (head + lane) & (kRowQueue - 1);
         const uint32_t v = valid ? qv[q] : 0u;
         const uint32_t xx = valid ? qx[q] : 0u;
+        const int slot0 = SLOTS && valid ? qs[q] : 0;
         wave_lds_sync();  // the slots just read are refilled after the round
         const int x0 = (int)(xx & 0xFFFFu), x1 = (int)(xx >> 16);
         const bool ph = (v & kPhantomBit) != 0;
-        const int g = (int)(v & ~kPhantomBit);
-        const int gid = ph ? 0 : g;
-        int slot0 = 0;
-        if (SLOTS && valid) {
-            slot0 = g == 0 ? 0 : cum[g - 1];
-            if (!ph) slot0 += (r - (int)(rect[g].y & 0xFFFFu)) * (x1 - x0) - x0;
-        }
+        const int gid = ph ? 0 : (int)(v & ~kPhantomBit);
         ballot_lines(fast, wn, m, wlo - x0, (unsigned)(x1 - x0), cur, start,
                      [&](uint32_t pos, int k) {
                          put32(gids_out, pos, gid);
-                         if (SLOTS) put32(slots_out, pos, ph ? slot0 : slot0 + wlo + k);
+                         // a phantom's only tile is tile 0 (wlo + k = 0)
+                         if (SLOTS) put32(slots_out, pos, slot0 + wlo + k);
                      });
     };
 
@@ -1198,7 +1209,8 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
     // values are requested before this iteration's are filtered
     auto key_at = [&](int jj) { return rowkey[min(jj, hi - 1)]; };
     auto val_at = [&](int jj) { return rowval[min(jj, hi - 1)]; };
-    auto filter = [&](uint32_t kc, uint32_t vc, int b) {
+    auto slot_at = [&](int jj) { return SLOTS ? rowslot[min(jj, hi - 1)] : 0; };
+    auto filter = [&](uint32_t kc, uint32_t vc, int sc, int b) {
         const int j = b + lane;
         int x0 = 0, x1 = 0;
         rowkey_cols(kc, kshift, x0, x1);
@@ -1211,6 +1223,7 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
                           (kRowQueue - 1);
             qv[q] = vc;
             qx[q] = (uint32_t)x0 | ((uint32_t)x1 << 16);
+            if (SLOTS) qs[q] = sc;
         }
         tail += __popcll(mo);
     };
@@ -1218,17 +1231,21 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
     uint32_t k2 = key_at(lo + 2 * LIG_WAVE + lane), k3 = key_at(lo + 3 * LIG_WAVE + lane);
     uint32_t v0 = val_at(lo + lane), v1 = val_at(lo + LIG_WAVE + lane);
     uint32_t v2 = val_at(lo + 2 * LIG_WAVE + lane), v3 = val_at(lo + 3 * LIG_WAVE + lane);
+    int s0 = slot_at(lo + lane), s1 = slot_at(lo + LIG_WAVE + lane);
+    int s2 = slot_at(lo + 2 * LIG_WAVE + lane), s3 = slot_at(lo + 3 * LIG_WAVE + lane);
     for (int base = lo; base < hi; base += 4 * LIG_WAVE) {
         const int nb = base + 4 * LIG_WAVE + lane;
         const uint32_t n0 = key_at(nb), n1 = key_at(nb + LIG_WAVE);
         const uint32_t n2 = key_at(nb + 2 * LIG_WAVE), n3 = key_at(nb + 3 * LIG_WAVE);
         const uint32_t w0 = val_at(nb), w1 = val_at(nb + LIG_WAVE);
         const uint32_t w2 = val_at(nb + 2 * LIG_WAVE), w3 = val_at(nb + 3 * LIG_WAVE);
+        const int t0 = slot_at(nb), t1 = slot_at(nb + LIG_WAVE);
+        const int t2 = slot_at(nb + 2 * LIG_WAVE), t3 = slot_at(nb + 3 * LIG_WAVE);
         // up to 4 x 64 entries join the (at most 63) queued: 319 < kRowQueue
-        filter(k0, v0, base);
-        filter(k1, v1, base + LIG_WAVE);
-        filter(k2, v2, base + 2 * LIG_WAVE);
-        filter(k3, v3, base + 3 * LIG_WAVE);
+        filter(k0, v0, s0, base);
+        filter(k1, v1, s1, base + LIG_WAVE);
+        filter(k2, v2, s2, base + 2 * LIG_WAVE);
+        filter(k3, v3, s3, base + 3 * LIG_WAVE);
         const bool last = base + 4 * LIG_WAVE >= hi;
         while (tail - head >= LIG_WAVE || (last && tail > head)) {
             const int cnt = min(LIG_WAVE, tail - head);
@@ -1243,6 +1260,10 @@ __global__ __launch_bounds__(LIG_WAVE) void row_scatter_kernel(
         v1 = w1;
         v2 = w2;
         v3 = w3;
+        s0 = t0;
+        s1 = t1;
+        s2 = t2;
+        s3 = t3;
     }
 }
 
@@ -1262,7 +1283,8 @@ int row_segments(int m, int tiles_x, int tiles_y) {
 
 // Workspace of the row path (0 when it does not apply): rectangles, the
 // direct row pass's (segment, row) counts and their prefixes, row totals and
-// their scan and its workspace, the row entries (keys, values), the row bins,
+// their scan and its workspace, the row entries (keys, values, slots at
+// column 0 for the emission slots), the row bins,
 // the (row, segment, tile) counts and their segment prefixes, tile totals and
 // their row prefixes, row totals and their scan.
 size_t rows_ws_bytes(int n, int m, int tiles_x, int tiles_y, int *P_out) {
@@ -1279,7 +1301,7 @@ size_t rows_ws_bytes(int n, int m, int tiles_x, int tiles_y, int *P_out) {
     if (P_out) *P_out = P;
     const size_t nn = (size_t)n, mm = (size_t)m, ty = (size_t)tiles_y;
     return align_up(8 * nn) + 2 * align_up(4 * gctr) + align_up(4 * (ty + 1)) +
-           align_up(scan_ws_elems(ty + 1) * 4) + 2 * align_up(4 * mm) + align_up(8 * ty) +
+           align_up(scan_ws_elems(ty + 1) * 4) + 3 * align_up(4 * mm) + align_up(8 * ty) +
            2 * align_up(4 * nctr) + 2 * align_up(4 * T) + align_up(4 * (ty + 1)) +
            align_up(scan_ws_elems(ty + 1) * 4) + 256;
 }
@@ -1308,6 +1330,7 @@ int bin_rows(int n, int m, const float *xys, const float *extents, const int32_t
     int32_t *eoff_ws = reinterpret_cast<int32_t *>(take(scan_ws_elems(ty + 1) * 4));
     uint32_t *rowkey = reinterpret_cast<uint32_t *>(take(4 * mm));
     uint32_t *rowval = reinterpret_cast<uint32_t *>(take(4 * mm));
+    int32_t *rowslot = reinterpret_cast<int32_t *>(take(4 * mm));
     int2 *rowbins = reinterpret_cast<int2 *>(take(8 * ty));
     int32_t *counts = reinterpret_cast<int32_t *>(take(4 * nctr));
     int32_t *scol = reinterpret_cast<int32_t *>(take(4 * nctr));
@@ -1328,10 +1351,17 @@ int bin_rows(int n, int m, const float *xys, const float *extents, const int32_t
     hipLaunchKernelGGL(entry_bins_kernel, dim3(ceil_div(ty, 256)), dim3(256), 0, st, tiles_y, m,
                        (const int32_t *)eoff, rowbins);
     const int nrw = (tiles_y + kRowWin - 1) / kRowWin;
-    hipLaunchKernelGGL(gauss_row_scatter_kernel, dim3((unsigned)nseg * (unsigned)nrw),
-                       dim3(LIG_WAVE), 0, st, n, gseg, nrw, tiles_y, ks, m, (const uint2 *)rect,
-                       (const int32_t *)gcounts, (const int32_t *)gpre, (const int32_t *)eoff,
-                       rowkey, rowval);
+    const bool slots = emission_ids_sorted != nullptr;
+    if (slots)
+        hipLaunchKernelGGL(gauss_row_scatter_kernel<true>, dim3((unsigned)nseg * (unsigned)nrw),
+                           dim3(LIG_WAVE), 0, st, n, gseg, nrw, tiles_y, ks, m,
+                           (const uint2 *)rect, cum, (const int32_t *)gcounts,
+                           (const int32_t *)gpre, (const int32_t *)eoff, rowkey, rowval, rowslot);
+    else
+        hipLaunchKernelGGL(gauss_row_scatter_kernel<false>, dim3((unsigned)nseg * (unsigned)nrw),
+                           dim3(LIG_WAVE), 0, st, n, gseg, nrw, tiles_y, ks, m,
+                           (const uint2 *)rect, cum, (const int32_t *)gcounts,
+                           (const int32_t *)gpre, (const int32_t *)eoff, rowkey, rowval, rowslot);
     // tiles from the row entries
     const unsigned segs = (unsigned)tiles_y * (unsigned)P;
     hipLaunchKernelGGL(row_count_kernel, dim3(segs), dim3(kCountThreads), 0, st, P, tiles_x, ks,
@@ -1344,20 +1374,18 @@ int bin_rows(int n, int m, const float *xys, const float *extents, const int32_t
                        (const int32_t *)te, tile_bins);
     const int nwin = row_windows(tiles_x);
     const unsigned waves = segs * (unsigned)nwin;
-    if (emission_ids_sorted)
+    if (slots)
         hipLaunchKernelGGL(row_scatter_kernel<true>, dim3(waves), dim3(LIG_WAVE), 0, st, P, nwin,
                            tiles_x, ks, m, (const int2 *)rowbins, (const uint32_t *)rowkey,
-                           (const uint32_t *)rowval, (const uint2 *)rect,
+                           (const uint32_t *)rowval, (const int32_t *)rowslot,
                            (const int32_t *)counts, (const int32_t *)scol, (const int32_t *)te,
-                           (const int32_t *)rowoff, cum, gaussian_ids_sorted,
-                           emission_ids_sorted);
+                           (const int32_t *)rowoff, gaussian_ids_sorted, emission_ids_sorted);
     else
         hipLaunchKernelGGL(row_scatter_kernel<false>, dim3(waves), dim3(LIG_WAVE), 0, st, P, nwin,
                            tiles_x, ks, m, (const int2 *)rowbins, (const uint32_t *)rowkey,
-                           (const uint32_t *)rowval, (const uint2 *)rect,
+                           (const uint32_t *)rowval, (const int32_t *)rowslot,
                            (const int32_t *)counts, (const int32_t *)scol, (const int32_t *)te,
-                           (const int32_t *)rowoff, cum, gaussian_ids_sorted,
-                           emission_ids_sorted);
+                           (const int32_t *)rowoff, gaussian_ids_sorted, emission_ids_sorted);
     return launch_status();
 }
 

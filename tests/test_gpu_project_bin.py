@@ -153,12 +153,12 @@ def test_binning_row_path_matches_radix_path(device, oracle_lib, n, H, W, bw, op
 
 @pytest.mark.parametrize("kind", ["windows", "direct"])
 def test_binning_row_path_buffer_overflow(device, oracle_lib, kind):
-    """The row path's scatter stages a segment's output in an LDS buffer of
-    2048 entries: a segment whose output exceeds it is taken in several
-    windows ("windows": 3000 Gaussians each covering two tiles of one tile
-    row), and a single tile run larger than the buffer is stored directly
-    ("direct": 5000 Gaussians on one tile).  Same gids, bins and emission
-    slots as the oracle and the radix path."""
+    """Dense corners of the row path: 3000 Gaussians each covering two tiles
+    of one tile row ("windows": one row's entries, every window of the row
+    scatter queueing many rounds) and 5000 Gaussians on one tile ("direct":
+    one tile's cursor advancing over dozens of ballot rounds, the queue
+    wrapping many times).  Same gids, bins and emission slots as the oracle
+    and the radix path."""
     import lig_amd._lib as L
     import lig_amd.gsplat2d as g
     from lig_amd.gsplat2d import cuda as C
@@ -250,3 +250,39 @@ def test_scan_sizes(device):
         out = C.inclusive_cumsum_i32(_t(x, device, torch.int32))
         np.testing.assert_array_equal(_np(out), np.cumsum(x, dtype=np.int64).astype(np.int32),
                                       err_msg=f"n={n}")
+
+
+@pytest.mark.parametrize("H,W,rows_path", [
+    (65536, 16384, True),        # 4096 x 1024 tiles: the row key's 12-bit row, 10-bit columns
+    (65552, 1024, False),        # 4097 tile rows: past the row key, the radix path
+    (512, 16400, False),         # 1025 tiles per row: past the column fields, the radix path
+])
+def test_binning_row_key_limits(device, oracle_lib, H, W, rows_path):
+    """Tile grids at and just past what the row path's packed entry key holds
+    (rows below 2^12, columns below 2^10, widths up to 1024): Gaussians
+    spanning whole tile rows, the last row and the last column, plus small
+    ones; the binning equals the oracle's whichever path runs, and the row
+    path's workspace size says which one that is."""
+    import lig_amd._lib as L
+    import lig_amd.gsplat2d as g
+    from lig_amd.gsplat2d import cuda as C
+    bw = 16
+    n = 600
+    s = random_scene(n, H, W, 31, cholesky=True, scale=(0.5, 3.0), offdiag=0.0)
+    means, par = s["means"], s["par"]
+    means[:4] = [[W / 2, 24.0], [W / 2, H - 9.0], [W - 3.0, H / 2], [W / 2, H / 2]]
+    par[:4] = [[0.6 * W, 0.0, 0.5], [0.6 * W, 0.0, 0.5], [0.5, 0.0, 40.0], [300.0, 0.0, 300.0]]
+    print(f"row key limits {W}x{H} (tiles {(W + 15) // 16} x {(H + 15) // 16})", flush=True)
+    xys, ext, con, nt = oracle_lib.project_gaussians_forward(par, means, H, W, bw, None,
+                                                             cholesky=True)
+    m, _, _, _, _, gs, bins = oracle_lib.bin_and_sort(xys, ext, nt, H, W, bw)
+    assert int(nt[0]) == (W + 15) // 16 * 2 or int(nt[0]) == (W + 15) // 16  # whole tile rows
+    rows_b = L.size_query("lig_bin_rows_workspace_size", n, m, H, W, bw)
+    assert (rows_b > 0) == rows_path
+    num, cum = g.compute_cumulative_intersects(_t(nt, device, torch.int32))
+    assert num == m
+    for eids in (False, True):
+        a = C.bin_and_sort_fast(n, m, _t(xys, device), _t(ext, device), cum, H, W, bw,
+                                with_emission_ids=eids)
+        np.testing.assert_array_equal(_np(a[0]), gs)
+        np.testing.assert_array_equal(_np(a[1]), bins)
