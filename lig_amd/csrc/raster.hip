@@ -1189,7 +1189,9 @@ __device__ __attribute__((noinline)) int fit_chunk_sort(int c0, int cn, const in
 // kernels evaluate the same test on the same FP32 values -- so no pixel is
 // gated, and each chunk of the tile's Gaussians is grouped by the rows they
 // reach, as in the fit step; every record keeps its bits.
-template <bool HAS_OPAC, bool DET, bool TRAIN, bool GROUP = false>
+// WSUM: the caller passes a wsum gradient (API mode); without one its terms
+// are dropped from the pair and row-end arithmetic (they would add zeros).
+template <bool HAS_OPAC, bool DET, bool TRAIN, bool GROUP = false, bool WSUM = true>
 __device__ __forceinline__ void bwd16_tile(
     int gtile, int part, int nparts, int tiles_x, int img_w, int img_h,
     const int32_t *__restrict__ gids,
@@ -1289,24 +1291,24 @@ __device__ __forceinline__ void bwd16_tile(
             if (full == 0xffffu) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
-                    plain_pair<HAS_OPAC, !TRAIN, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb, dx_of(lx),
+                    plain_pair<HAS_OPAC, !TRAIN, !TRAIN && WSUM>(A, true, ap, bp, rt, op, cr, cg, cb, dx_of(lx),
                                                  (float)(lx - 8), amin, clampv, row + lx, lx == 8);
             } else if (full == any) {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
-                        plain_pair<HAS_OPAC, !TRAIN, !TRAIN>(A, true, ap, bp, rt, op, cr, cg, cb,
+                        plain_pair<HAS_OPAC, !TRAIN, !TRAIN && WSUM>(A, true, ap, bp, rt, op, cr, cg, cb,
                                                      dx_of(lx), (float)(lx - 8), amin, clampv,
                                                      row + lx, lx == 8);
             } else {
 #pragma unroll
                 for (int lx = 0; lx < 16; ++lx)
                     if (any & (1u << lx))
-                        plain_pair<HAS_OPAC>(A, kidx <= __builtin_amdgcn_readlane(fcol, lx), ap,
+                        plain_pair<HAS_OPAC, true, WSUM>(A, kidx <= __builtin_amdgcn_readlane(fcol, lx), ap,
                                              bp, rt, op, cr, cg, cb, dx_of(lx), (float)(lx - 8),
                                              amin, clampv, row + lx, lx == 8);
             }
-            plain_row_end<!TRAIN>(A, dy, cr, cg, cb);
+            plain_row_end<!TRAIN && WSUM>(A, dy, cr, cg, cb);
         }
         plain_fold(A, dx_of(8));
         if constexpr (TRAIN) {
@@ -1360,7 +1362,7 @@ __device__ __forceinline__ void bwd16_tile(
 // b takes share b % split of tile order[b / split], or without an order (none
 // given, or order[0] = -1), with u = xcd_remap(b), share u % split of tile
 // u / split.
-template <bool HAS_OPAC, bool DET, bool TRAIN, bool GROUP = false>
+template <bool HAS_OPAC, bool DET, bool TRAIN, bool GROUP = false, bool WSUM = true>
 __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
     int tiles_x, int img_w, int img_h, const int32_t *__restrict__ gids,
     const int32_t *__restrict__ slots, const int2 *__restrict__ bins,
@@ -1384,7 +1386,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd16_kernel(
         part = u - k * split;
         gtile = tile_of_slot(k, tiles_x, (img_h + 15) >> 4, tab == nullptr);
     }
-    bwd16_tile<HAS_OPAC, DET, TRAIN, GROUP>(gtile, part, split, tiles_x, img_w, img_h,
+    bwd16_tile<HAS_OPAC, DET, TRAIN, GROUP, WSUM>(gtile, part, split, tiles_x, img_w, img_h,
                                      gids, slots, bins, pack, final_idx, v_out, v_wsum, img,
                                      target, grad_loss, norm, rec, g_xy, g_xy_abs, g_conic, g_rgb,
                                      g_opac, tab, nimg, sv, sfin);
@@ -2352,12 +2354,17 @@ int launch_backward_tiles(int n, int img_height, int img_width, int block_width,
             hipLaunchKernelGGL(raster_bwd_ga_kernel<false>, grid, dim3(kThreads), 0, st,
                                LIG_BWD_ARGS_GA);
     } else if (group) {
-        // 16 x 16 without a final index: row-grouped, ungated
-        const int sel = (has_opacities ? 2 : 0) | (det ? 1 : 0);
+        // 16 x 16 without a final index: row-grouped, ungated; without a wsum
+        // gradient its terms are left out
+        const int sel = (v_wsum ? 4 : 0) | (has_opacities ? 2 : 0) | (det ? 1 : 0);
         switch (sel) {
-            case 0: hipLaunchKernelGGL((raster_bwd16_kernel<false, false, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
-            case 1: hipLaunchKernelGGL((raster_bwd16_kernel<false, true, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
-            case 2: hipLaunchKernelGGL((raster_bwd16_kernel<true, false, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 0: hipLaunchKernelGGL((raster_bwd16_kernel<false, false, false, true, false>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 1: hipLaunchKernelGGL((raster_bwd16_kernel<false, true, false, true, false>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 2: hipLaunchKernelGGL((raster_bwd16_kernel<true, false, false, true, false>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 3: hipLaunchKernelGGL((raster_bwd16_kernel<true, true, false, true, false>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 4: hipLaunchKernelGGL((raster_bwd16_kernel<false, false, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 5: hipLaunchKernelGGL((raster_bwd16_kernel<false, true, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
+            case 6: hipLaunchKernelGGL((raster_bwd16_kernel<true, false, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
             default: hipLaunchKernelGGL((raster_bwd16_kernel<true, true, false, true>), grid16, dim3(kThreads), 0, st, LIG_BWD_ARGS_16); break;
         }
     } else {

@@ -362,8 +362,9 @@ def test_api_backward_without_final_index(device, case):
     gradients of the backward gated by the forward's final index bit for bit,
     with records addressed from the pack or through emission ids, with and
     without opacities, on tiles deeper than one 512-Gaussian sort chunk and on
-    sizes that are not multiples of 16; the atomic reduction agrees within
-    its summation-order rounding."""
+    sizes that are not multiples of 16; without a wsum gradient the same bits
+    as with a zero one; the atomic reduction agrees within its
+    summation-order rounding."""
     import lig_amd.gsplat2d as g
     from lig_amd import fit
     from lig_amd.gsplat2d import cuda as C
@@ -411,6 +412,15 @@ def test_api_backward_without_final_index(device, case):
         if a is not None:
             assert torch.equal(a, b), (name, float((a - b).abs().max()))
     assert torch.count_nonzero(r_gate[3]) > 0
+    # no wsum gradient (a loss of the image alone): the variant without the
+    # wsum terms gives the bits of a zero wsum gradient
+    r_none = C.rasterize_backward_packed(H, W, 16, gids, bins, f_free[6], n, None, v_out, None,
+                                         op, **bk)
+    r_zero = C.rasterize_backward_packed(H, W, 16, gids, bins, f_free[6], n, None, v_out,
+                                         torch.zeros_like(v_w), op, **bk)
+    for name, a, b in zip(names, r_none, r_zero):
+        if a is not None:
+            assert torch.equal(a, b), (name, float((a - b).abs().max()))
     # atomics (no records): the same sums in another order
     a_gate = C.rasterize_backward_packed(H, W, 16, gids, bins, f_gate[6], n, f_gate[5], v_out,
                                          v_w, op)
