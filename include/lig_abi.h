@@ -136,11 +136,13 @@ int lig_get_tile_bin_edges(int num_intersects, const int64_t *isect_ids_sorted, 
  * Workspace: lig_bin_workspace_size(num_intersects, tiles). */
 size_t lig_bin_workspace_size(int num_intersects, int num_tiles);
 /* No reference counterpart: the workspace of lig_bin_and_sort_gaussians's row
- * path (0 when it does not apply: more than 1024 tiles per row).  The row path
- * emits one entry per (tile row, Gaussian), sorts those by row and expands
- * each row's entries into its tiles with per-tile counts and ranks; it gives
- * the same outputs bit for bit and runs when the workspace passed holds this
- * many bytes, the radix path otherwise (which needs lig_bin_workspace_size). */
+ * path (0 when it does not apply: more than 1024 tiles per row or more than
+ * 4096 tile rows).  The row path writes one entry per (tile row, Gaussian) in
+ * (row, gid) order straight from each Gaussian's tile rectangle and expands
+ * each row's entries into its tiles with per-tile counts and ballot ranks; it
+ * gives the same outputs bit for bit and runs when the workspace passed holds
+ * this many bytes, the radix path otherwise (which needs
+ * lig_bin_workspace_size). */
 size_t lig_bin_rows_workspace_size(int n, int num_intersects, int img_height, int img_width,
                                    int block_width);
 int lig_bin_and_sort_gaussians(int n, int num_intersects, const float *xys,
