@@ -671,29 +671,29 @@ int bits_for(int num_tiles) {
 }
 
 // ---------------------------------------------------------------------------
-// Row-expansion binning (one image, at most kRowMaxTiles tiles per row).
+// Row-expansion binning (one image, at most kRowMaxTiles tiles per row and
+// kMaxRowsPacked rows).
 //
 // A Gaussian's tiles form a rectangle, so its intersections are laid out by
 // tile rows: one row entry (row, gid) per tile row a Gaussian spans, about
 // sqrt(tiles per Gaussian) times fewer than intersections (12.6 M against
-// 79 M at 8192^2 x 2 M).  The entries are emitted gid-major and sorted
-// stably by row, which puts each row's Gaussians in gid order; each entry's
-// key carries its Gaussian's column range above the sorted row bits, so no
-// later pass gathers rectangles.  Each row's entries are split into P
-// segments (P chosen by the host from the grid size); a counting pass writes,
-// for every (row, segment, tile), how many of the segment's entries cover
-// the tile.  Within a row, a prefix over segments per tile and a scan of the
-// tile totals, plus a scan of the row totals, give every segment's first
-// output position in every tile and the bins.  The scatter pass gives every
-// (row, segment, window of kRowWin tiles) one wave: it filters the segment's
-// entries down to those that reach the window, in gid order, and takes them
-// 64 at a time, one per lane, through the window's tiles; per tile a ballot
-// of the lanes covering it ranks them, and they store their gids at
-// consecutive positions of the tile's bin.  The output is the stable
-// (tile, emission) order of the radix path, bit for bit.
+// 79 M at 8192^2 x 2 M).  The entries are written straight from the
+// rectangles in (row, gid) order (the direct row pass below, a counting
+// sort by row); each entry's key carries its Gaussian's column range above
+// the row bits, so no later pass gathers rectangles.  Each row's entries are
+// split into P segments (P chosen by the host from the grid size); a
+// counting pass writes, for every (row, segment, tile), how many of the
+// segment's entries cover the tile.  Within a row, a prefix over segments per
+// tile and a scan of the tile totals, plus a scan of the row totals, give
+// every segment's first output position in every tile and the bins.  The
+// scatter pass gives every (row, segment, window of kRowWin tiles) one wave:
+// it filters the segment's entries down to those that reach the window, in
+// gid order, and takes them 64 at a time, one per lane, through the window's
+// tiles; per tile a ballot of the lanes covering it ranks them, and they
+// store their gids at consecutive positions of the tile's bin.  The output is
+// the stable (tile, emission) order of the radix path, bit for bit.
 // ---------------------------------------------------------------------------
 constexpr int kRowMaxTiles = 1024;     // tiles per row (10-bit column fields)
-constexpr int kRowMaxRows = 32767;     // rows per image (15-bit row fields)
 constexpr uint32_t kPhantomBit = 0x80000000u;
 
 // Tile rectangle of a Gaussian: x = x0 | x1 << 16, y = y0 | y1 << 16, bit 31
@@ -708,8 +708,8 @@ __device__ __forceinline__ void unpack_rect(uint2 rc, int &x0, int &y0, int &x1,
     ph = (rc.y & kPhantomBit) != 0;
 }
 
-// Row entry key: the row in the low `kshift` bits (the bits the row sort
-// orders by), then the first column x0 (10 bits) and the width - 1 (10 bits).
+// Row entry key: the row in the low `kshift` bits (enough for the image's
+// rows), then the first column x0 (10 bits) and the width - 1 (10 bits).
 __device__ __forceinline__ uint32_t row_key(int row, int x0, int x1, int kshift) {
     return (uint32_t)row | ((uint32_t)x0 << kshift) | ((uint32_t)(x1 - x0 - 1) << (kshift + 10));
 }
