@@ -368,6 +368,9 @@ def main():
                     help="steps of the batched-vs-plain comparison at a fitted PSNR (0 skips)")
     ap.add_argument("--api-steps", type=int, default=5,
                     help="timed steps of the reference-API chain at the headline workload")
+    ap.add_argument("--captured-steps", type=int, default=-1,
+                    help="steps of each level's fused step replayed as a hipGraph (0 skips; "
+                         "default: 50 for the launch-bound c1/c2 sizes, 0 for c3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -442,6 +445,29 @@ def main():
               "l2": sum(ev[1].elapsed_time(ev[2]) for ev in evs)}
     steps_per_s = args.steps / elapsed
     log(f"timed {args.steps} steps in {elapsed:.3f}s")
+
+    # launch-bound sizes: each level's fused step recorded once as a hipGraph
+    # (Level.graph) and replayed; the levels are independent, so k two-level
+    # steps take k replays of each level's graph
+    captured = None
+    ncap = args.captured_steps if args.captured_steps >= 0 else (
+        0 if args.config == "c3" else 50)
+    if ncap > 0:
+        cap_s, caps = 0.0, 0
+        for lv, tt in ((l1, t1), (l2, t2)):
+            gph = lv.graph(tt)
+            gph.run(2)
+            torch.cuda.synchronize()
+            c0 = time.perf_counter()
+            gph.run(ncap)
+            torch.cuda.synchronize()
+            cap_s += time.perf_counter() - c0
+            caps += gph.captures
+            del gph
+        captured = {"steps_s": round(ncap / cap_s, 3), "ms_per_step": round(cap_s / ncap * 1e3, 4),
+                    "steps": ncap, "captures": caps,
+                    "path": "each level's fused step replayed as a hipGraph (Level.graph)"}
+        log(f"captured {captured}")
 
     # render throughput of the level-2 representation: the image-only renderer
     # (lig_amd.render.render_view: view projection, binning, image-only raster)
@@ -604,6 +630,7 @@ def main():
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
         "api_chain": api_chain,
+        "captured": captured,
         "autotune": {k: {"tile/depth": v.get("measured_with", v["choice"]), "ms": v["ms"]}
                      for k, v in tuned.items()},
         "batched": batched,
